@@ -1,0 +1,208 @@
+/* unidock_amd.h -- C ABI of the MI355X (gfx950) batched pose-search library.
+ *
+ * The reference has no formal plugin API; its GPU seam is C++:
+ *   monte_carlo_template::do_docking<Config>(models, outputs, precalculate_byatom,
+ *       m_data_list_gpu, igrid, corner1, corner2, rng, verbosity, seed, bias)
+ *       -- unidock/src/lib/monte_carlo.h:121-126, body unidock/src/cuda/monte_carlo.cu:2397-2974
+ *   precalculate_parallel(m_data_list, precalculate_byatom, sf, models, rs, thread)
+ *       -- unidock/src/lib/precalculate.h:315-318, body unidock/src/cuda/precalculate.cu:169-326
+ * called from Vina::global_search_gpu<Config> (unidock/src/lib/vina.h:154-358) and
+ * Vina::set_ligand_from_object_gpu (unidock/src/lib/vina.cpp:246-303).
+ * Each entry point below cites the reference interface it replaces.
+ *
+ * Conventions: plain pointers and sizes, no exceptions, no exit().  Every
+ * call returns UD_OK (0) or an error code; ud_last_error() describes the last
+ * failure on a context.  Host pointers are caller-owned and only read/written
+ * during the call; device buffers are owned by the context and reused across
+ * calls.  A context is bound to one device and one HIP stream and must not be
+ * used from two threads at once; separate contexts are independent (the
+ * reference's paired mode runs one Vina object per std::thread,
+ * unidock/src/main/simulation_container.h:419-449).
+ */
+#ifndef UNIDOCK_AMD_H
+#define UNIDOCK_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+enum {
+    UD_OK = 0,
+    UD_ERR_INVALID = 1,   /* bad argument or descriptor */
+    UD_ERR_CAPACITY = 2,  /* a ligand/grid exceeds a documented capacity */
+    UD_ERR_HIP = 3,       /* HIP runtime failure */
+    UD_ERR_STATE = 4,     /* call order violated (e.g. dock before maps) */
+    UD_ERR_NOMEM = 5      /* device allocation failed */
+};
+
+/* scoring functions (scoring_function.h:34) */
+enum { UD_SF_VINA = 0, UD_SF_VINARDO = 1 };
+/* grid-map storage: FP32, or FP16 with a power-of-two scale per map; all
+ * interpolation and accumulation is FP32 either way */
+enum { UD_GRID_F32 = 0, UD_GRID_F16 = 1 };
+
+/* capacities (the reference's largest bucket is 300 atoms / 48 torsions,
+ * unidock/src/cuda/kernel.h:242-277; exceeding ours returns UD_ERR_CAPACITY,
+ * never undefined behaviour) */
+#define UD_MAX_ATOMS 128
+#define UD_MAX_TORSIONS 32
+#define UD_MAX_PAIRS 8128
+#define UD_MAX_MAPS 32
+#define UD_MAX_GRID_POINTS (1 << 24)
+
+/* A pose is UD_POSE_FLOATS floats: position[3], orientation quaternion
+ * (w, x, y, z)[4], torsions[UD_MAX_TORSIONS] (pre-order of the torsion tree);
+ * the layout of output_type_cuda_t minus its coordinates (kernel.h:296-306). */
+#define UD_POSE_FLOATS (7 + UD_MAX_TORSIONS)
+/* A gradient is UD_GRAD_FLOATS floats: force[3], torque[3], d/dtorsion[T]
+ * (the change vector of conf.h:182-227). */
+#define UD_GRAD_FLOATS (6 + UD_MAX_TORSIONS)
+
+typedef struct ud_ctx ud_ctx;
+
+/* A batch of ligands in flat arrays (offsets have n_ligands + 1 entries).
+ * Per ligand, atoms are grouped by rigid node and nodes are in depth-first
+ * pre-order with the root first; node k > 0 carries torsion k - 1.
+ *   atom_local[3a]   coordinates relative to the owning node's origin in the
+ *                    input frame (parse_pdbqt.cpp:206-211)
+ *   atom_xs[a]       X-Score type (0..35) or -1 for untyped atoms (hydrogens)
+ *   atom_heavy[a]    1 for heavy atoms (gyration radius)
+ *   node_parent[k]   parent node (local index), -1 for the root
+ *   node_abegin/aend atom range of the node (local indices)
+ *   node_origin[3k]  node origin in the input frame; node_axis[3k] unit
+ *                    rotation axis (ignored for the root) (tree.h:101-131)
+ *   pairs[2p]        intramolecular pairs (local atom indices)
+ *                    (model.cpp:592-630, ligand.pairs)
+ *   num_tors[l]      torsion count of conf_independent (conf_independent.cpp:79-103)
+ * Replaces the per-ligand model / m_cuda_t marshalling of do_docking_base
+ * (monte_carlo.cu:2487-2631). */
+typedef struct {
+    int32_t n_ligands;
+    const int32_t* atom_offset;
+    const int32_t* node_offset;
+    const int32_t* pair_offset;
+    const float* atom_local;
+    const int32_t* atom_xs;
+    const int32_t* atom_heavy;
+    const int32_t* node_parent;
+    const int32_t* node_abegin;
+    const int32_t* node_aend;
+    const float* node_origin;
+    const float* node_axis;
+    const int32_t* pairs;
+    const float* num_tors;
+} ud_ligand_batch;
+
+/* Monte Carlo search parameters: the public fields of monte_carlo_template
+ * (monte_carlo.h:75-98) as set by Vina::global_search_gpu (vina.h:189-212). */
+typedef struct {
+    int32_t exhaustiveness;      /* chains per ligand */
+    int32_t global_steps;        /* MC steps per chain (min(heuristic, max_step)) */
+    int32_t local_steps;         /* BFGS iterations; <= 0: (25 + n_atoms) / 3 per ligand */
+    int32_t lanes_per_chain;     /* 0: autotuned per size bucket, else 8/16/32/64 */
+    uint64_t seed;
+    float hunt_cap[3];           /* curl caps during the search (10, 10, 10) */
+    float authentic_v[3];        /* caps of the refinement BFGS (1000, 1000, 1000) */
+    float temperature;           /* Metropolis temperature (1.2) */
+    float mutation_amplitude;    /* 2 */
+} ud_search_params;
+
+/* fill *p with the reference defaults (exhaustiveness 8, global_steps 20) */
+void ud_default_search_params(ud_search_params* p);
+
+/* ---- context ---------------------------------------------------------- */
+/* Create a context on device_id using HIP stream `stream` (NULL = the null
+ * stream).  Replaces the implicit CUDA device/stream use of do_docking_base
+ * (monte_carlo.cu:2414-2480) and mc_stream's own stream (monte_carlo.cu:367-368). */
+int ud_create(int device_id, void* stream, ud_ctx** out);
+int ud_destroy(ud_ctx* ctx);
+const char* ud_last_error(const ud_ctx* ctx);
+
+/* Scoring function and weights.  Vina: 7 weights (gauss1, gauss2, repulsion,
+ * hydrophobic, hydrogen, glue, rot); Vinardo: 6 (gauss1, repulsion,
+ * hydrophobic, hydrogen, glue, rot).  Replaces Vina::set_vina_weights /
+ * set_vinardo_weights + set_forcefield (vina.cpp:393-433) and builds the
+ * per-type-pair tables that precalculate_parallel builds per atom pair
+ * (precalculate.cu:14-167, precalculate.h:203-241). */
+int ud_set_scoring(ud_ctx* ctx, int sf, const double* weights, int n_weights);
+
+/* Search box: n_voxels = ceil(size / granularity), begin = center - span / 2
+ * (Vina::compute_vina_maps geometry, vina.cpp:527-539).  n_points_out (3) may
+ * be NULL. */
+int ud_set_box(ud_ctx* ctx, const double* center, const double* size, double granularity, int32_t* n_points_out);
+
+/* Build grid maps on the GPU for the listed XS types from the receptor atoms
+ * (x, y, z per atom; types -1 are skipped).  Replaces cache::populate_no_bias
+ * (cache.cpp:471-549) called from Vina::compute_vina_maps (vina.cpp:491-567).
+ * precision: UD_GRID_F32 or UD_GRID_F16. */
+int ud_compute_maps(ud_ctx* ctx, int n_rec, const float* rec_xyz, const int32_t* rec_xs, int n_types,
+                    const int32_t* types, int precision);
+
+/* Load precomputed FP32 map values (x fastest, one map per listed type) and
+ * store them at `precision` (Vina::load_maps, vina.cpp:569-607). */
+int ud_set_maps(ud_ctx* ctx, int n_types, const int32_t* types, const float* values, int precision);
+
+/* Read back one map as FP32 (dequantised for FP16 storage). */
+int ud_get_map(ud_ctx* ctx, int xs_type, float* out);
+
+/* Set the ligand batch (copied to the device; replaces
+ * Vina::set_ligand_from_object_gpu, vina.cpp:246-303). */
+int ud_set_ligands(ud_ctx* ctx, const ud_ligand_batch* batch);
+
+/* Energy and gradient of n_poses poses (pose_ligand[i] = ligand index).
+ * v = curl caps (intra, inter, other) as in model::eval_deriv (model.cpp:954-981).
+ * energy_out[n_poses], grad_out[n_poses * UD_GRAD_FLOATS]; host pointers.
+ * The device path of the reference is m_eval_deriv_warp (warp_ops.cuh:836-860). */
+int ud_score_grad(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* poses, const float* v,
+                  float* energy_out, float* grad_out);
+/* Same with device pointers, asynchronous on the context's stream. */
+int ud_score_grad_device(ud_ctx* ctx, int n_poses, const int32_t* d_pose_ligand, const float* d_poses,
+                         const float* v, float* d_energy, float* d_grad);
+
+/* BFGS local optimisation from given starts (poses in/out), quasi_newton +
+ * bfgs (quasi_newton.cpp:43-53, bfgs.h:97-147; device: bfgs_warp,
+ * warp_ops.cuh:1051-1119).  max_steps <= 0: (25 + n_atoms) / 3 per ligand.
+ * energy_out and evals_out may be NULL. */
+int ud_optimize(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, float* poses, const float* v, int max_steps,
+                float* energy_out, int32_t* evals_out);
+
+/* Monte Carlo docking of every ligand of the batch, exhaustiveness chains
+ * each, in one launch per size bucket (kernel<32, Config>,
+ * monte_carlo.cu:236-356, launched at monte_carlo.cu:2877-2882).
+ * Outputs per chain (ligand-major, chain-minor): best_poses
+ * [n_ligands * exh * UD_POSE_FLOATS], best_e [n_ligands * exh], evals
+ * [n_ligands * exh] (energy+gradient evaluations).  Any output may be NULL:
+ * results then stay on the device for ud_fetch_dock_results. */
+int ud_dock(ud_ctx* ctx, const ud_search_params* params, float* best_poses, float* best_e, int64_t* evals);
+int ud_fetch_dock_results(ud_ctx* ctx, float* best_poses, float* best_e, int64_t* evals);
+
+/* Final-score components of poses with authentic caps (v = 1000): inter =
+ * grid energy (cache::eval, cache.cpp:84-114), intra = intramolecular pairs
+ * with the piecewise-constant table (eval_interacting_pairs,
+ * model.cpp:885-905).  Used by the host post-processing (Vina::score_gpu,
+ * vina.cpp:1245-1327). */
+int ud_score_terms(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* poses, float* inter_out,
+                   float* intra_out);
+
+/* Heavy-atom coordinates of poses (model::set + get_heavy_atom_movable_coords,
+ * model.cpp:699-706), coords_out[n_poses * UD_MAX_ATOMS * 3], all atoms in
+ * ligand order. */
+int ud_pose_coords(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* poses, float* coords_out);
+
+/* Kernel timing of the last ud_dock / ud_score_grad call on the context's
+ * stream (HIP events around the launches): total milliseconds and number of
+ * launches. */
+int ud_last_kernel_time(const ud_ctx* ctx, double* ms, int32_t* launches);
+
+/* Seeded random sequential addition of n_atoms points in [0, edge)^3 with a
+ * minimum spacing (synthetic receptors); host-only. */
+int ud_synth_receptor(int n_atoms, double edge, double min_dist, uint64_t seed, long long max_attempts,
+                      double* xyz_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UNIDOCK_AMD_H */

@@ -1,0 +1,1138 @@
+// oracle.cpp -- CPU restatement of Uni-Dock's batched pose-search path.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load this library, and only as the checker
+// (or as the timed CPU baseline).  The product (uni-dock_amd/) never links,
+// loads or calls it.
+//
+// Every function is templated on the real type R and instantiated twice:
+//   R = double : the FP64 oracle (parity target for energies/gradients/minima)
+//   R = float  : a restatement of the reference's CPU path, which is FP32
+//                throughout (`typedef float fl`, unidock/src/lib/common.h:45);
+//                this is the "reference analytic derivative" and the CPU
+//                baseline that bench.py times.
+// Citations are to files of the reference tree (unidock/src/...).
+//
+// Pinning (what checks each function, see tests/test_oracle.py):
+//   potentials        closed-form invariants (gauss(0)=1 at optimum, repulsion 0
+//                     for d>0, slope_step endpoints, cutoff), hand values
+//   pair tables       recomputation from the closed form at the sample radii,
+//                     dor = central difference, fast = midpoint
+//   grid build        brute force definition; numpy recomputation on tiny receptors
+//   grid eval         affine fields reproduced exactly, central FD inside cells,
+//                     out-of-box penalty slope*miss/factor
+//   FK                bond lengths / rigid distances preserved, zero pose
+//                     reproduces the input, rotation equivariance
+//   eval_deriv        central finite differences (h=1e-3 .. 1e-6) in FP64
+//   BFGS/line search  quadratic-free checks: monotone decrease, stationarity,
+//                     FP32 vs FP64 agreement from identical starts
+//   Philox-4x32-10    Random123 published known-answer vectors
+//   MC chain          seeded determinism; FP32/FP64 agreement of early steps
+//   final score       closed form of num_tors_div
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------- constants
+// epsilon_fl / max_fl are the float limits in the reference (common.h:227-230);
+// the FP64 oracle keeps the same thresholds because they are part of the
+// algorithm (BFGS curvature test, gradient cut-off, curl guard).
+static const double kEpsFl = 1.1920928955078125e-07;
+static const double kMaxFl = 3.4028234663852886e+38;
+static const double kPi = 3.1415926535897931;  // common.h:204 (stored as float there)
+
+enum { XS_C_H = 0, XS_C_P, XS_N_P, XS_N_D, XS_N_A, XS_N_DA, XS_O_P, XS_O_D, XS_O_A, XS_O_DA,
+       XS_S_P, XS_P_P, XS_F_H, XS_Cl_H, XS_Br_H, XS_I_H, XS_Si, XS_At, XS_Met_D,
+       XS_C_H_CG0, XS_C_P_CG0, XS_G0, XS_C_H_CG1, XS_C_P_CG1, XS_G1, XS_C_H_CG2, XS_C_P_CG2,
+       XS_G2, XS_C_H_CG3, XS_C_P_CG3, XS_G3, XS_W, XS_O_XA, XS_N_XA, XS_O_XD, XS_N_XD,
+       XS_SIZE };  // atom_constants.h:84-120
+
+// xs_vdw_radii / xs_vinardo_vdw_radii, looked up BY INDEX exactly like
+// xs_radius() (atom_constants.h:313-396).  The table's order after Met_D is
+// C_H_CG0, C_P_CG0, C_H_CG1, ... G0..G3, W, which differs from the enum; the
+// index lookup is what the reference computes, so it is kept.
+static const double kXsRadius[XS_SIZE] = {
+    1.9, 1.9, 1.8, 1.8, 1.8, 1.8, 1.7, 1.7, 1.7, 1.7, 2.0, 2.1, 1.5, 1.8, 2.0, 2.2, 2.2, 2.3,
+    1.2, 1.9, 1.9, 1.9, 1.9, 1.9, 1.9, 1.9, 1.9, 0.0, 0.0, 0.0, 0.0, 0.0, 1.7, 1.8, 1.7, 1.8};
+static const double kXsVinardoRadius[XS_SIZE] = {
+    2.0, 2.0, 1.7, 1.7, 1.7, 1.7, 1.6, 1.6, 1.6, 1.6, 2.0, 2.1, 1.5, 1.8, 2.0, 2.2, 2.2, 2.3,
+    1.2, 2.0, 2.0, 2.0, 2.0, 2.0, 2.0, 2.0, 2.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.6, 1.7, 1.6, 1.7};
+
+// atom_constants.h:420-444
+static bool xs_is_hydrophobic(int t) {
+    return t == XS_C_H || t == XS_F_H || t == XS_Cl_H || t == XS_Br_H || t == XS_I_H;
+}
+static bool xs_is_acceptor(int t) {
+    return t == XS_N_A || t == XS_N_DA || t == XS_O_A || t == XS_O_DA || t == XS_O_XA ||
+           t == XS_N_XA;
+}
+static bool xs_is_donor(int t) {
+    return t == XS_N_D || t == XS_N_DA || t == XS_O_D || t == XS_O_DA || t == XS_Met_D ||
+           t == XS_N_XD || t == XS_O_XD;
+}
+static bool xs_h_bond_possible(int a, int b) {
+    return (xs_is_donor(a) && xs_is_acceptor(b)) || (xs_is_donor(b) && xs_is_acceptor(a));
+}
+// potentials.h:41-46, 106-126
+static bool is_glue_type(int t) { return t == XS_G0 || t == XS_G1 || t == XS_G2 || t == XS_G3; }
+static bool is_glued(int a, int b) {
+    const int G[4] = {XS_G0, XS_G1, XS_G2, XS_G3};
+    const int CH[4] = {XS_C_H_CG0, XS_C_H_CG1, XS_C_H_CG2, XS_C_H_CG3};
+    const int CP[4] = {XS_C_P_CG0, XS_C_P_CG1, XS_C_P_CG2, XS_C_P_CG3};
+    for (int k = 0; k < 4; ++k) {
+        if (a == G[k] && (b == CH[k] || b == CP[k])) return true;
+        if (b == G[k] && (a == CH[k] || a == CP[k])) return true;
+    }
+    return false;
+}
+
+enum { SF_VINA = 0, SF_VINARDO = 1 };
+
+// ---------------------------------------------------------------- potentials
+// slope_step, potentials.h:30-39
+template <class R> R slope_step(R x_bad, R x_good, R x) {
+    if (x_bad < x_good) {
+        if (x <= x_bad) return R(0);
+        if (x >= x_good) return R(1);
+    } else {
+        if (x >= x_bad) return R(0);
+        if (x <= x_good) return R(1);
+    }
+    return (x - x_bad) / (x_good - x_bad);
+}
+
+template <class R> R optimal_distance(int sf, int t1, int t2) {
+    // potentials.h:48-51 (vina), 61-64 (vinardo)
+    if (is_glue_type(t1) || is_glue_type(t2)) return R(0);
+    const double* rad = (sf == SF_VINA) ? kXsRadius : kXsVinardoRadius;
+    return R(rad[t1]) + R(rad[t2]);
+}
+
+template <class R> R sqr(R x) { return x * x; }
+
+// ScoringFunction::eval(t1, t2, r) = sum_i w_i * term_i(t1, t2, r)
+// (scoring_function.h:114-118) with the term list of scoring_function.h:47-70.
+// The atom-based overload used for intramolecular tables (precalculate_byatom,
+// precalculate.h:211-228) reduces to the same function of the two XS types for
+// Vina and Vinardo (potentials.h:137-343), so one routine serves both tables.
+template <class R> R sf_eval(int sf, int t1, int t2, R r, const R* w) {
+    R acc = 0;
+    if (sf == SF_VINA) {
+        const R od = optimal_distance<R>(SF_VINA, t1, t2);
+        // vina_gaussian(0, 0.5, 8) potentials.h:137-158
+        R term = 0;
+        if (r < R(8)) term = std::exp(-sqr((r - (od + R(0))) / R(0.5)));
+        acc += w[0] * term;
+        // vina_gaussian(3, 2.0, 8)
+        term = 0;
+        if (r < R(8)) term = std::exp(-sqr((r - (od + R(3))) / R(2.0)));
+        acc += w[1] * term;
+        // vina_repulsion(0, 8) potentials.h:160-181
+        term = 0;
+        if (r < R(8)) {
+            R d = r - (od + R(0));
+            term = (d > R(0)) ? R(0) : d * d;
+        }
+        acc += w[2] * term;
+        // vina_hydrophobic(0.5, 1.5, 8) potentials.h:183-207
+        term = 0;
+        if (r < R(8) && xs_is_hydrophobic(t1) && xs_is_hydrophobic(t2))
+            term = slope_step<R>(R(1.5), R(0.5), r - od);
+        acc += w[3] * term;
+        // vina_non_dir_h_bond(-0.7, 0, 8) potentials.h:209-243
+        term = 0;
+        if (r < R(8) && xs_h_bond_possible(t1, t2)) {
+            bool x_type = (t1 >= XS_O_XA && t1 <= XS_N_XD) || (t2 >= XS_O_XA && t2 <= XS_N_XD);
+            term = slope_step<R>(R(0), R(-0.7), r - od);
+            if (x_type) term = R(10.0) * term;
+        }
+        acc += w[4] * term;
+        // linearattraction(20) potentials.h:487-507
+        term = 0;
+        if (r < R(20) && is_glued(t1, t2)) term = r;
+        acc += w[5] * term;
+    } else {
+        const R od = optimal_distance<R>(SF_VINARDO, t1, t2);
+        // vinardo_gaussian(0, 0.8, 8) potentials.h:246-269
+        R term = 0;
+        if (r < R(8)) term = std::exp(-sqr((r - (od + R(0))) / R(0.8)));
+        acc += w[0] * term;
+        // vinardo_repulsion(0, 8) potentials.h:271-292
+        term = 0;
+        if (r < R(8)) {
+            R d = r - (od + R(0));
+            term = (d > R(0)) ? R(0) : d * d;
+        }
+        acc += w[1] * term;
+        // vinardo_hydrophobic(0, 2.5, 8) potentials.h:294-318
+        term = 0;
+        if (r < R(8) && xs_is_hydrophobic(t1) && xs_is_hydrophobic(t2))
+            term = slope_step<R>(R(2.5), R(0), r - od);
+        acc += w[2] * term;
+        // vinardo_non_dir_h_bond(-0.6, 0, 8) potentials.h:320-343
+        term = 0;
+        if (r < R(8) && xs_h_bond_possible(t1, t2)) term = slope_step<R>(R(0), R(-0.6), r - od);
+        acc += w[3] * term;
+        // linearattraction(20)
+        term = 0;
+        if (r < R(20) && is_glued(t1, t2)) term = r;
+        acc += w[4] * term;
+    }
+    return acc;
+}
+
+// Number of distance samples: sz(factor * max_cutoff_sqr) + 3 (precalculate.h:142-145)
+static const int kFactor = 32;
+static const int kTableN = 32 * 64 + 3;  // 2051
+
+// precalculate_element::init_from_smooth_fst (precalculate.h:66-85) applied to
+// smooth[].first = min(v, sf.eval(t1, t2, rs[i])) with v = max_fl
+// (precalculate.h:160-171).  rs[i] = sqrt(i / factor) (precalculate.h:191-196).
+template <class R> struct PairTable {
+    std::vector<R> e, dor, fast;
+    void build(int sf, int t1, int t2, const R* w) {
+        const int n = kTableN;
+        e.assign(n, 0); dor.assign(n, 0); fast.assign(n, 0);
+        std::vector<R> rs(n);
+        for (int i = 0; i < n; ++i) rs[i] = std::sqrt(R(i) / R(kFactor));
+        for (int i = 0; i < n; ++i) {
+            R v = sf_eval<R>(sf, t1, t2, rs[i], w);
+            e[i] = std::min<R>(R(kMaxFl), v);
+        }
+        for (int i = 0; i < n; ++i) {
+            if (i == 0 || i == n - 1)
+                dor[i] = 0;
+            else {
+                R delta = rs[i + 1] - rs[i - 1];
+                R r = rs[i];
+                dor[i] = (e[i + 1] - e[i - 1]) / (delta * r);
+            }
+            R f1 = e[i];
+            R f2 = (i + 1 >= n) ? R(0) : e[i + 1];
+            fast[i] = (f2 + f1) / R(2);
+        }
+    }
+    // precalculate_element::eval_fast, precalculate.h:40-47
+    R eval_fast(R r2) const { return fast[int(R(kFactor) * r2)]; }
+    // precalculate_element::eval_deriv, precalculate.h:48-65
+    void eval_deriv(R r2, R& oe, R& odor) const {
+        R rf = R(kFactor) * r2;
+        int i1 = int(rf);
+        int i2 = i1 + 1;
+        R rem = rf - R(i1);
+        oe = e[i1] + rem * (e[i2] - e[i1]);
+        odor = dor[i1] + rem * (dor[i2] - dor[i1]);
+    }
+};
+
+// triangular_matrix_index_permissive (triangular_matrix_index.h:28-37)
+static int tri_index(int i, int j) { return (i <= j) ? i + j * (j + 1) / 2 : j + i * (i + 1) / 2; }
+
+// ---------------------------------------------------------------- vectors
+template <class R> struct V3 {
+    R x[3];
+    V3() { x[0] = x[1] = x[2] = 0; }
+    V3(R a, R b, R c) { x[0] = a; x[1] = b; x[2] = c; }
+    R& operator[](int i) { return x[i]; }
+    const R& operator[](int i) const { return x[i]; }
+};
+template <class R> V3<R> operator+(const V3<R>& a, const V3<R>& b) { return V3<R>(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+template <class R> V3<R> operator-(const V3<R>& a, const V3<R>& b) { return V3<R>(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+template <class R> V3<R> operator*(R s, const V3<R>& a) { return V3<R>(s * a[0], s * a[1], s * a[2]); }
+template <class R> R dot(const V3<R>& a, const V3<R>& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+template <class R> V3<R> cross(const V3<R>& a, const V3<R>& b) {
+    return V3<R>(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
+}
+template <class R> R norm_sqr(const V3<R>& a) { return sqr(a[0]) + sqr(a[1]) + sqr(a[2]); }
+
+// quaternion (a, b, c, d) = a + bi + cj + dk, Hamilton product.
+template <class R> struct Q {
+    R a, b, c, d;
+};
+template <class R> Q<R> qmul(const Q<R>& p, const Q<R>& q) {
+    return Q<R>{p.a * q.a - p.b * q.b - p.c * q.c - p.d * q.d,
+                p.a * q.b + p.b * q.a + p.c * q.d - p.d * q.c,
+                p.a * q.c - p.b * q.d + p.c * q.a + p.d * q.b,
+                p.a * q.d + p.b * q.c - p.c * q.b + p.d * q.a};
+}
+// Matrix stored row-major m[3*i + j].
+// quaternion_to_r3, quaternion.cpp:77-114
+template <class R> void q_to_mat(const Q<R>& q, R* m) {
+    const R aa = q.a * q.a, ab = q.a * q.b, ac = q.a * q.c, ad = q.a * q.d;
+    const R bb = q.b * q.b, bc = q.b * q.c, bd = q.b * q.d;
+    const R cc = q.c * q.c, cd = q.c * q.d, dd = q.d * q.d;
+    m[0] = (aa + bb - cc - dd); m[1] = R(2) * (-ad + bc); m[2] = R(2) * (ac + bd);
+    m[3] = R(2) * (ad + bc); m[4] = (aa - bb + cc - dd); m[5] = R(2) * (-ab + cd);
+    m[6] = R(2) * (-ac + bd); m[7] = R(2) * (ab + cd); m[8] = (aa - bb - cc + dd);
+}
+template <class R> V3<R> matvec(const R* m, const V3<R>& v) {
+    return V3<R>(m[0] * v[0] + m[1] * v[1] + m[2] * v[2], m[3] * v[0] + m[4] * v[1] + m[5] * v[2],
+                 m[6] * v[0] + m[7] * v[1] + m[8] * v[2]);
+}
+// normalize_angle, common.h:260-281
+template <class R> void normalize_angle(R& x) {
+    const R pi = R(kPi);
+    if (x > R(3) * pi) {
+        R n = (x - pi) / (R(2) * pi);
+        x -= R(2) * pi * std::ceil(n);
+        normalize_angle(x);
+    } else if (x < -R(3) * pi) {
+        R n = (-x - pi) / (R(2) * pi);
+        x += R(2) * pi * std::ceil(n);
+        normalize_angle(x);
+    } else if (x > pi) {
+        x -= R(2) * pi;
+    } else if (x < -pi) {
+        x += R(2) * pi;
+    }
+}
+template <class R> R normalized_angle(R x) { normalize_angle(x); return x; }
+// angle_to_quaternion(axis, angle), quaternion.cpp:37-47
+template <class R> Q<R> axis_angle_q(const V3<R>& axis, R angle) {
+    normalize_angle(angle);
+    R c = std::cos(angle / R(2));
+    R s = std::sin(angle / R(2));
+    return Q<R>{c, s * axis[0], s * axis[1], s * axis[2]};
+}
+// angle_to_quaternion(rotation), quaternion.cpp:49-59
+template <class R> Q<R> rotation_q(const V3<R>& rot) {
+    R angle = std::sqrt(norm_sqr(rot));
+    if (angle > R(kEpsFl)) {
+        V3<R> axis(rot[0] / angle, rot[1] / angle, rot[2] / angle);
+        return axis_angle_q(axis, angle);
+    }
+    return Q<R>{1, 0, 0, 0};
+}
+// quaternion_normalize_approx, quaternion.h:105-117
+template <class R> void q_normalize_approx(Q<R>& q) {
+    const R s = q.a * q.a + q.b * q.b + q.c * q.c + q.d * q.d;
+    if (std::abs(s - R(1)) < R(1e-6)) return;
+    const R a = std::sqrt(s);
+    const R inv = R(1) / a;
+    q.a *= inv; q.b *= inv; q.c *= inv; q.d *= inv;
+}
+// quaternion_increment, quaternion.cpp:128-133
+template <class R> void q_increment(Q<R>& q, const V3<R>& rot) {
+    q = qmul(rotation_q(rot), q);
+    q_normalize_approx(q);
+}
+
+// ---------------------------------------------------------------- grid
+// grid::init (grid.cpp:24-37), compute_vina_maps geometry (vina.cpp:527-539)
+template <class R> struct Grid {
+    int n[3] = {0, 0, 0};   // sample points per axis (= n_voxels + 1)
+    R init[3], factor[3], factor_inv[3], dim_minus_1[3];
+    std::vector<std::vector<R>> maps;  // one per XS type (index = type), empty if absent
+    void set_geometry(const double center[3], const double size[3], double granularity) {
+        for (int i = 0; i < 3; ++i) {
+            long nv = long(std::ceil(size[i] / granularity));
+            R real_span = R(granularity * double(nv));
+            R begin = R(center[i]) - real_span / R(2);
+            R end = begin + real_span;
+            n[i] = int(nv) + 1;
+            init[i] = begin;
+            R range = end - begin;
+            dim_minus_1[i] = R(n[i] - 1);
+            factor[i] = dim_minus_1[i] / range;
+            factor_inv[i] = R(1) / factor[i];
+        }
+        maps.assign(XS_SIZE, std::vector<R>());
+    }
+    R& at(int t, int x, int y, int z) { return maps[t][x + n[0] * (y + n[1] * size_t(z))]; }
+    R at(int t, int x, int y, int z) const { return maps[t][x + n[0] * (y + n[1] * size_t(z))]; }
+    // grid::index_to_argument (grid.h:45-48)
+    V3<R> point(int x, int y, int z) const {
+        return V3<R>(init[0] + factor_inv[0] * R(x), init[1] + factor_inv[1] * R(y),
+                     init[2] + factor_inv[2] * R(z));
+    }
+};
+
+// curl(e, deriv, v) and curl(e, v), curl.h:28-43
+template <class R> void curl(R& e, V3<R>& deriv, R v) {
+    if (e > R(0) && v < R(0.1) * R(kMaxFl)) {
+        R tmp = (v < R(kEpsFl)) ? R(0) : (v / (v + e));
+        e *= tmp;
+        R t2 = tmp * tmp;
+        deriv = t2 * deriv;
+    }
+}
+template <class R> void curl(R& e, R v) {
+    if (e > R(0) && v < R(0.1) * R(kMaxFl)) {
+        R tmp = (v < R(kEpsFl)) ? R(0) : (v / (v + e));
+        e *= tmp;
+    }
+}
+
+// grid::evaluate_aux (grid.cpp:39-129).  deriv == nullptr -> energy only.
+template <class R> R grid_eval(const Grid<R>& g, int t, const V3<R>& loc, R slope, R v, V3<R>* deriv) {
+    R s[3], miss[3];
+    int region[3], a[3];
+    for (int i = 0; i < 3; ++i) {
+        s[i] = (loc[i] - g.init[i]) * g.factor[i];
+        miss[i] = 0;
+        if (s[i] < R(0)) {
+            miss[i] = -s[i]; region[i] = -1; a[i] = 0; s[i] = 0;
+        } else if (s[i] >= g.dim_minus_1[i]) {
+            miss[i] = s[i] - g.dim_minus_1[i]; region[i] = 1; a[i] = g.n[i] - 2; s[i] = 1;
+        } else {
+            region[i] = 0; a[i] = int(s[i]); s[i] -= R(a[i]);
+        }
+    }
+    const R penalty = slope * (miss[0] * g.factor_inv[0] + miss[1] * g.factor_inv[1] + miss[2] * g.factor_inv[2]);
+    const int x0 = a[0], y0 = a[1], z0 = a[2], x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+    const R f000 = g.at(t, x0, y0, z0), f100 = g.at(t, x1, y0, z0), f010 = g.at(t, x0, y1, z0),
+            f110 = g.at(t, x1, y1, z0), f001 = g.at(t, x0, y0, z1), f101 = g.at(t, x1, y0, z1),
+            f011 = g.at(t, x0, y1, z1), f111 = g.at(t, x1, y1, z1);
+    const R x = s[0], y = s[1], z = s[2];
+    const R mx = R(1) - x, my = R(1) - y, mz = R(1) - z;
+    R f = f000 * mx * my * mz + f100 * x * my * mz + f010 * mx * y * mz + f110 * x * y * mz +
+          f001 * mx * my * z + f101 * x * my * z + f011 * mx * y * z + f111 * x * y * z;
+    if (deriv) {
+        const R x_g = f000 * (-1) * my * mz + f100 * 1 * my * mz + f010 * (-1) * y * mz + f110 * 1 * y * mz +
+                      f001 * (-1) * my * z + f101 * 1 * my * z + f011 * (-1) * y * z + f111 * 1 * y * z;
+        const R y_g = f000 * mx * (-1) * mz + f100 * x * (-1) * mz + f010 * mx * 1 * mz + f110 * x * 1 * mz +
+                      f001 * mx * (-1) * z + f101 * x * (-1) * z + f011 * mx * 1 * z + f111 * x * 1 * z;
+        const R z_g = f000 * mx * my * (-1) + f100 * x * my * (-1) + f010 * mx * y * (-1) + f110 * x * y * (-1) +
+                      f001 * mx * my * 1 + f101 * x * my * 1 + f011 * mx * y * 1 + f111 * x * y * 1;
+        V3<R> grad(x_g, y_g, z_g);
+        curl(f, grad, v);
+        for (int i = 0; i < 3; ++i) {
+            R ge = (region[i] == 0) ? grad[i] : R(0);
+            (*deriv)[i] = g.factor[i] * ge + slope * R(region[i]);
+        }
+        return f + penalty;
+    }
+    curl(f, v);
+    return f + penalty;
+}
+
+// ---------------------------------------------------------------- ligand
+// A ligand in the flat layout shared with the C-ABI: atoms grouped by rigid
+// node (contiguous ranges), nodes in depth-first pre-order (parents first),
+// local atom coordinates relative to the owning node's origin in the input
+// frame (parse_pdbqt.cpp:206-211), relative origin/axis per node
+// (tree.h:121-131), intramolecular pairs (model.cpp:592-630).
+template <class R> struct Ligand {
+    int n_atoms = 0, n_nodes = 0, n_pairs = 0;
+    std::vector<V3<R>> local;       // per atom
+    std::vector<int> xs;            // per atom, -1 = untyped (hydrogen etc.)
+    std::vector<int> heavy;         // per atom
+    std::vector<int> node_of;       // per atom
+    std::vector<int> parent, abegin, aend;  // per node
+    std::vector<V3<R>> rel_origin, rel_axis, origin0;  // per node (origin0: input-frame origin)
+    std::vector<int> pa, pb, ptp;   // per pair
+    R num_tors = 0;
+    int n_tors() const { return n_nodes - 1; }
+};
+
+// Pose: position[3], orientation q[4] (a,b,c,d), torsions[T] -> flat 7+T.
+template <class R> struct Pose {
+    V3<R> pos;
+    Q<R> q{1, 0, 0, 0};
+    std::vector<R> tors;
+};
+
+template <class R> struct Ctx {
+    int sf = SF_VINA;
+    R w[7];
+    std::vector<PairTable<R>> tables;  // indexed by tri_index(t1, t2), built lazily
+    std::vector<char> have;
+    Grid<R> grid;
+    R slope = R(1e6);  // vina.cpp:515
+    std::vector<Ligand<R>> ligs;
+    const PairTable<R>& table(int t1, int t2) {
+        int k = tri_index(t1, t2);
+        if (!have[k]) { tables[k].build(sf, t1, t2, w); have[k] = 1; }
+        return tables[k];
+    }
+    void init(int sf_, const double* weights) {
+        sf = sf_;
+        int nw = (sf == SF_VINA) ? 7 : 6;
+        for (int i = 0; i < 7; ++i) w[i] = 0;
+        // weights as given to set_vina_weights/set_vinardo_weights; the last
+        // is weight_rot, turned into the internal 5*w_rot/0.1 - 1 (vina.cpp:404, 425)
+        for (int i = 0; i < nw - 1; ++i) w[i] = R(weights[i]);
+        w[nw - 1] = R(5) * R(weights[nw - 1]) / R(0.1) - R(1);
+        if (sf == SF_VINARDO) { w[6] = w[5]; w[5] = 0; }
+        int ntp = XS_SIZE * (XS_SIZE + 1) / 2;
+        tables.assign(ntp, PairTable<R>());
+        have.assign(ntp, 0);
+    }
+    R rot_weight_internal() const { return (sf == SF_VINA) ? w[6] : w[6]; }
+};
+
+// Grid type remapping of cache::eval_deriv (cache.cpp:160-199): G types carry
+// no grid term, C_*_CG* map to C_H / C_P.
+static int grid_type(int t) {
+    if (t < 0 || t >= XS_SIZE) return -1;
+    switch (t) {
+        case XS_G0: case XS_G1: case XS_G2: case XS_G3: return -1;
+        case XS_C_H_CG0: case XS_C_H_CG1: case XS_C_H_CG2: case XS_C_H_CG3: return XS_C_H;
+        case XS_C_P_CG0: case XS_C_P_CG1: case XS_C_P_CG2: case XS_C_P_CG3: return XS_C_P;
+        default: return t;
+    }
+}
+
+// model::set -> heterotree/tree set_conf (tree.h:121-144, 176-214): root
+// frame from (position, orientation), each child in pre-order from its parent.
+template <class R> void forward_kinematics(const Ligand<R>& L, const Pose<R>& P, std::vector<V3<R>>& coords,
+                                           std::vector<V3<R>>& origins, std::vector<V3<R>>& axes,
+                                           std::vector<std::array<R, 9>>* mats_out = nullptr) {
+    const int nn = L.n_nodes;
+    origins.assign(nn, V3<R>());
+    axes.assign(nn, V3<R>());
+    std::vector<Q<R>> qs(nn);
+    std::vector<std::array<R, 9>> ms(nn);
+    coords.assign(L.n_atoms, V3<R>());
+    for (int k = 0; k < nn; ++k) {
+        if (k == 0) {
+            origins[0] = P.pos;
+            qs[0] = P.q;  // rigid_body::set_conf does not normalise (tree.h:95-99)
+        } else {
+            int p = L.parent[k];
+            origins[k] = origins[p] + matvec(ms[p].data(), L.rel_origin[k]);
+            axes[k] = matvec(ms[p].data(), L.rel_axis[k]);
+            Q<R> t = qmul(axis_angle_q(axes[k], P.tors[k - 1]), qs[p]);
+            q_normalize_approx(t);
+            qs[k] = t;
+        }
+        q_to_mat(qs[k], ms[k].data());
+        for (int i = L.abegin[k]; i < L.aend[k]; ++i) coords[i] = origins[k] + matvec(ms[k].data(), L.local[i]);
+    }
+    if (mats_out) *mats_out = ms;
+}
+
+// model::eval_deriv (model.cpp:954-981) for a single ligand without flex
+// residues: grid term (cache::eval_deriv), intramolecular pairs
+// (eval_interacting_pairs_deriv, model.cpp:907-934), then the torsion-tree
+// derivative (tree.h:75-84, 164-193, 216-222).  g has 6 + T entries:
+// [force(3), torque(3), torsion derivatives].
+template <class R> R eval_deriv(Ctx<R>& C, const Ligand<R>& L, const Pose<R>& P, const R v[3], R* g,
+                                std::vector<V3<R>>* coords_out = nullptr, R* e_inter = nullptr, R* e_intra = nullptr) {
+    std::vector<V3<R>> coords, origins, axes;
+    forward_kinematics(L, P, coords, origins, axes);
+    std::vector<V3<R>> mf(L.n_atoms);
+    R e = 0;
+    for (int i = 0; i < L.n_atoms; ++i) {
+        int t = grid_type(L.xs[i]);
+        if (t < 0 || C.grid.maps[t].empty()) { mf[i] = V3<R>(); continue; }
+        V3<R> d;
+        e += grid_eval(C.grid, t, coords[i], C.slope, v[1], &d);
+        mf[i] = d;
+    }
+    R einter = e;
+    R eintra = 0;
+    const R cutoff_sqr = R(64);  // sqr(sf.get_cutoff()), scoring_function.h:56,66
+    for (int k = 0; k < L.n_pairs; ++k) {
+        const int a = L.pa[k], b = L.pb[k];
+        V3<R> r = coords[b] - coords[a];
+        R r2 = norm_sqr(r);
+        if (r2 < cutoff_sqr) {
+            R pe, dor;
+            C.table(L.xs[a], L.xs[b]).eval_deriv(r2, pe, dor);
+            V3<R> force = dor * r;
+            curl(pe, force, v[0]);
+            eintra += pe;
+            mf[a] = mf[a] - force;
+            mf[b] = mf[b] + force;
+        }
+    }
+    e += eintra;
+    // tree derivative: post-order accumulation of force and torque
+    const int nn = L.n_nodes;
+    std::vector<V3<R>> F(nn), T(nn);
+    for (int k = 0; k < nn; ++k) {
+        for (int i = L.abegin[k]; i < L.aend[k]; ++i) {
+            F[k] = F[k] + mf[i];
+            T[k] = T[k] + cross(coords[i] - origins[k], mf[i]);
+        }
+    }
+    // children appear after their parent in pre-order, so a reverse sweep
+    // visits every child before its parent; within one parent the children
+    // are added in reverse order, which only reorders a floating-point sum
+    // relative to branches_derivative (tree.h:152-162).
+    for (int k = nn - 1; k >= 1; --k) {
+        int p = L.parent[k];
+        F[p] = F[p] + F[k];
+        T[p] = T[p] + (cross(origins[k] - origins[p], F[k]) + T[k]);
+    }
+    g[0] = F[0][0]; g[1] = F[0][1]; g[2] = F[0][2];
+    g[3] = T[0][0]; g[4] = T[0][1]; g[5] = T[0][2];
+    for (int k = 1; k < nn; ++k) g[5 + k] = dot(T[k], axes[k]);
+    if (coords_out) *coords_out = coords;
+    if (e_inter) *e_inter = einter;
+    if (e_intra) *e_intra = eintra;
+    return e;
+}
+
+// conf::increment (conf.h:98-104, 50-56, 169-172)
+template <class R> void pose_increment(Pose<R>& x, const R* p, R alpha) {
+    for (int i = 0; i < 3; ++i) x.pos[i] += alpha * p[i];
+    V3<R> rot(alpha * p[3], alpha * p[4], alpha * p[5]);
+    q_increment(x.q, rot);
+    for (size_t k = 0; k < x.tors.size(); ++k) {
+        x.tors[k] += normalized_angle(alpha * p[6 + k]);
+        normalize_angle(x.tors[k]);
+    }
+}
+
+// bfgs() with line_search() and bfgs_update() (bfgs.h:30-147), quasi_newton
+// driver (quasi_newton.cpp:43-53).
+template <class R> R bfgs(Ctx<R>& C, const Ligand<R>& L, Pose<R>& x, const R v[3], int max_steps, int* evalcount) {
+    const int n = 6 + L.n_tors();
+    std::vector<R> h(n * (n + 1) / 2, R(0));
+    auto H = [&](int i, int j) -> R& { return h[tri_index(i, j)]; };
+    for (int i = 0; i < n; ++i) H(i, i) = 1;
+    std::vector<R> g(n), g_new(n), p(n), y(n), minus_hy(n);
+    Pose<R> x_new = x;
+    R f0 = eval_deriv(C, L, x, v, g.data());
+    ++*evalcount;
+    const R f_orig = f0;
+    const std::vector<R> g_orig = g;
+    const Pose<R> x_orig = x;
+    auto dotn = [&](const std::vector<R>& a, const std::vector<R>& b) {
+        R s = 0;
+        for (int i = 0; i < n; ++i) s += a[i] * b[i];
+        return s;
+    };
+    for (int step = 0; step < max_steps; ++step) {
+        for (int i = 0; i < n; ++i) {  // minus_mat_vec_product
+            R sum = 0;
+            for (int j = 0; j < n; ++j) sum += H(i, j) * g[j];
+            p[i] = -sum;
+        }
+        // line_search
+        R f1 = 0;
+        R alpha = 1;
+        const R pg = dotn(p, g);
+        for (int trial = 0; trial < 10; ++trial) {
+            x_new = x;
+            pose_increment(x_new, p.data(), alpha);
+            f1 = eval_deriv(C, L, x_new, v, g_new.data());
+            ++*evalcount;
+            if (f1 - f0 < R(0.0001) * alpha * pg) break;
+            alpha *= R(0.5);
+        }
+        for (int i = 0; i < n; ++i) y[i] = g_new[i] - g[i];
+        f0 = f1;
+        x = x_new;
+        if (!(std::sqrt(dotn(g, g)) >= R(1e-5))) break;
+        g = g_new;
+        if (step == 0) {
+            const R yy = dotn(y, y);
+            if (std::abs(yy) > R(kEpsFl)) {
+                const R d = alpha * dotn(y, p) / yy;
+                for (int i = 0; i < n; ++i) H(i, i) = d;
+            }
+        }
+        // bfgs_update (bfgs.h:46-63)
+        const R yp = dotn(y, p);
+        if (alpha * yp < R(kEpsFl)) continue;
+        for (int i = 0; i < n; ++i) {
+            R sum = 0;
+            for (int j = 0; j < n; ++j) sum += H(i, j) * y[j];
+            minus_hy[i] = -sum;
+        }
+        const R yhy = -dotn(y, minus_hy);
+        const R r = R(1) / (alpha * yp);
+        for (int i = 0; i < n; ++i)
+            for (int j = i; j < n; ++j)
+                H(i, j) += alpha * r * (minus_hy[i] * p[j] + minus_hy[j] * p[i]) +
+                           alpha * alpha * (r * r * yhy + r) * p[i] * p[j];
+    }
+    if (!(f0 <= f_orig)) {
+        f0 = f_orig;
+        x = x_orig;
+    }
+    return f0;
+}
+
+// ---------------------------------------------------------------- Philox
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key
+// increments 0x9E3779B9 / 0xBB67AE85, ten rounds.
+static void philox(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+        uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+        uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+        uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+// Stream layout used by the search (identical in the HIP kernels):
+//   counter = (step, chain, purpose, 0), key = (seed_lo, seed_hi)
+//   step 0 is the random start; MC step s uses counter step s + 1.
+//   purpose 1: start position / mutation, 2: start orientation / Metropolis,
+//   3 + j: start torsions 4j .. 4j+3.
+static inline double u01(uint32_t x) { return double(x >> 8) * (1.0 / 16777216.0); }
+
+template <class R> void random_start(const Ligand<R>& L, const R lo[3], const R hi[3], uint64_t seed, uint32_t chain, Pose<R>& P) {
+    uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    uint32_t o[4];
+    uint32_t c[4] = {0, chain, 1, 0};
+    philox(c, key, o);
+    // random_in_box (random.cpp:78-84): uniform per axis between the corners
+    for (int i = 0; i < 3; ++i) P.pos[i] = lo[i] + R(u01(o[i])) * (hi[i] - lo[i]);
+    // random_orientation (quaternion.cpp:116-126): normalised 4-D Gaussian
+    c[2] = 2;
+    philox(c, key, o);
+    R nrm4[4];
+    for (int j = 0; j < 2; ++j) {
+        R u1 = R(1.0 - u01(o[2 * j]));  // (0, 1]
+        R u2 = R(u01(o[2 * j + 1]));
+        R rad = std::sqrt(R(-2) * std::log(u1));
+        R ang = R(2 * kPi) * u2;
+        nrm4[2 * j] = rad * std::cos(ang);
+        nrm4[2 * j + 1] = rad * std::sin(ang);
+    }
+    R nq = std::sqrt(nrm4[0] * nrm4[0] + nrm4[1] * nrm4[1] + nrm4[2] * nrm4[2] + nrm4[3] * nrm4[3]);
+    if (!(nq > R(kEpsFl))) { nrm4[0] = 1; nrm4[1] = nrm4[2] = nrm4[3] = 0; nq = 1; }
+    P.q = Q<R>{nrm4[0] / nq, nrm4[1] / nq, nrm4[2] / nq, nrm4[3] / nq};
+    // torsions_randomize (conf.h:58-61): U(-pi, pi)
+    P.tors.assign(L.n_tors(), R(0));
+    for (int k = 0; k < L.n_tors(); ++k) {
+        if (k % 4 == 0) { c[2] = 3 + k / 4; philox(c, key, o); }
+        P.tors[k] = R(-kPi) + R(2 * kPi) * R(u01(o[k % 4]));
+    }
+}
+
+// model::gyration_radius (model.cpp:871-883) of the pose being mutated:
+// RMS distance of heavy atoms from the root origin.
+template <class R> R gyration_radius(const Ligand<R>& L, const std::vector<V3<R>>& coords, const V3<R>& root) {
+    R acc = 0;
+    int cnt = 0;
+    for (int i = 0; i < L.n_atoms; ++i)
+        if (L.heavy[i]) { acc += norm_sqr(coords[i] - root); ++cnt; }
+    return cnt > 0 ? std::sqrt(acc / R(cnt)) : R(0);
+}
+
+// mutate_conf (mutate.cpp:35-81): one of {position, orientation, one torsion}
+// chosen uniformly; position += amplitude * u, orientation incremented by
+// (amplitude / r_gyr) * u, u uniform in the unit ball; torsion := U(-pi, pi).
+template <class R> void mutate(const Ligand<R>& L, Pose<R>& P, R rgyr, R amplitude, uint64_t seed, uint32_t chain, uint32_t step) {
+    uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    uint32_t c[4] = {step + 1, chain, 1, 0};
+    uint32_t o[4];
+    philox(c, key, o);
+    const uint32_t count = 2 + L.n_tors();
+    uint32_t which = uint32_t((uint64_t(o[0]) * count) >> 32);
+    // unit ball: direction from (z, phi), radius cbrt(u)
+    R z = R(2) * R(u01(o[1])) - R(1);
+    R phi = R(2 * kPi) * R(u01(o[2]));
+    R rr = std::cbrt(R(u01(o[3])));
+    R sxy = std::sqrt(std::max(R(0), R(1) - z * z));
+    V3<R> u(rr * sxy * std::cos(phi), rr * sxy * std::sin(phi), rr * z);
+    if (which == 0) {
+        P.pos = P.pos + amplitude * u;
+        return;
+    }
+    if (which == 1) {
+        if (rgyr > R(kEpsFl)) {
+            V3<R> rot = (amplitude / rgyr) * u;
+            q_increment(P.q, rot);
+        }
+        return;
+    }
+    P.tors[which - 2] = R(-kPi) + R(2 * kPi) * R(u01(o[1]));
+}
+
+// metropolis_accept (monte_carlo.cu:92-97)
+template <class R> bool metropolis(R old_f, R new_f, R temperature, R u) {
+    if (new_f < old_f) return true;
+    const R acc = std::exp((old_f - new_f) / temperature);
+    return u < acc;
+}
+
+// One Monte Carlo chain as run per chain by kernel<32, Config>
+// (monte_carlo.cu:236-356): random start, then for each step: mutate a copy,
+// BFGS with hunt_cap, Metropolis at T = 1.2, and when the accepted pose beats
+// the chain's best, BFGS again with authentic_v = 1000 and keep it.  The BFGS
+// is the CPU semantics of bfgs.h (see bfgs above).
+template <class R> R mc_chain(Ctx<R>& C, const Ligand<R>& L, uint64_t seed, uint32_t chain, int global_steps,
+                              int local_steps, R amplitude, R temperature, Pose<R>& best, int* evalcount,
+                              R* trace = nullptr) {
+    const R hunt[3] = {10, 10, 10};     // vina.h:207
+    const R authentic[3] = {1000, 1000, 1000};
+    R lo[3], hi[3];
+    for (int i = 0; i < 3; ++i) {
+        lo[i] = C.grid.init[i];
+        hi[i] = C.grid.init[i] + C.grid.factor_inv[i] * R(C.grid.n[i] - 1);
+    }
+    Pose<R> tmp;
+    random_start(L, lo, hi, seed, chain, tmp);
+    R tmp_e = 0;
+    R best_e = std::numeric_limits<R>::infinity();
+    best = tmp;
+    std::vector<V3<R>> coords, origins, axes;
+    forward_kinematics(L, tmp, coords, origins, axes);
+    R rgyr = gyration_radius(L, coords, tmp.pos);
+    uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    for (int step = 0; step < global_steps; ++step) {
+        Pose<R> cand = tmp;
+        mutate(L, cand, rgyr, amplitude, seed, chain, uint32_t(step));
+        R cand_e = bfgs(C, L, cand, hunt, local_steps, evalcount);
+        uint32_t c[4] = {uint32_t(step + 1), chain, 2, 0}, o[4];
+        philox(c, key, o);
+        bool acc = metropolis(tmp_e, cand_e, temperature, R(u01(o[0])));
+        if (step == 0 || acc) {
+            tmp = cand;
+            tmp_e = cand_e;
+            forward_kinematics(L, tmp, coords, origins, axes);
+            rgyr = gyration_radius(L, coords, tmp.pos);
+            if (tmp_e < best_e) {
+                tmp_e = bfgs(C, L, tmp, authentic, local_steps, evalcount);
+                forward_kinematics(L, tmp, coords, origins, axes);
+                rgyr = gyration_radius(L, coords, tmp.pos);
+                if (tmp_e < best_e) {
+                    best = tmp;
+                    best_e = tmp_e;
+                }
+            }
+        }
+        if (trace) trace[step] = best_e;
+    }
+    return best_e;
+}
+
+// Final affinity of a pose (Vina::score_gpu, vina.cpp:1245-1327, no flex, no
+// refinement): inter = grid energy with authentic_v (cache::eval, energy-only
+// path, cache.cpp:84-114), intra = ligand pairs with eval_fast and curl at
+// authentic_v (eval_interacting_pairs, model.cpp:885-905), then
+// num_tors_div (conf_independent.cpp:153-156):
+//   total = (inter + intra - intra_best) / (1 + 0.1 (w_rot_internal + 1) num_tors / 5)
+template <class R> void score_terms(Ctx<R>& C, const Ligand<R>& L, const Pose<R>& P, R* inter, R* intra) {
+    std::vector<V3<R>> coords, origins, axes;
+    forward_kinematics(L, P, coords, origins, axes);
+    R e = 0;
+    for (int i = 0; i < L.n_atoms; ++i) {
+        int t = grid_type(L.xs[i]);
+        if (t < 0 || C.grid.maps[t].empty()) continue;
+        e += grid_eval(C.grid, t, coords[i], C.slope, R(1000), (V3<R>*)nullptr);
+    }
+    R ei = 0;
+    for (int k = 0; k < L.n_pairs; ++k) {
+        const int a = L.pa[k], b = L.pb[k];
+        R r2 = norm_sqr(coords[b] - coords[a]);
+        if (r2 < R(64)) {
+            R t = C.table(L.xs[a], L.xs[b]).eval_fast(r2);
+            curl(t, R(1000));
+            ei += t;
+        }
+    }
+    *inter = e;
+    *intra = ei;
+}
+template <class R> R conf_independent(const Ctx<R>& C, R num_tors, R x) {
+    const R weight = R(0.1) * (C.w[6] + R(1));
+    const R y = R(1) + weight * num_tors / R(5.0);
+    if (std::abs(x) < R(kEpsFl)) return 0;
+    if (std::abs(y) < R(kEpsFl)) return (x * y > 0) ? R(kMaxFl) : R(-kMaxFl);
+    return x / y;
+}
+
+// Grid population, cache::populate_no_bias (cache.cpp:471-549): for each
+// sample point and needed type t2, the sum over typed receptor atoms with
+// r^2 <= cutoff^2 of eval_fast(t1, t2, r^2) of the TYPE-based table
+// (precalculate, precalculate.h:130-201).  Brute force over all atoms; the
+// reference's szv_grid only prunes atoms that fail the same r^2 test.
+template <class R> void build_maps(Ctx<R>& C, int n_rec, const double* rec_xyz, const int* rec_xs, int n_types, const int* types) {
+    Grid<R>& g = C.grid;
+    std::vector<V3<R>> ra;
+    std::vector<int> rt;
+    for (int i = 0; i < n_rec; ++i) {
+        if (rec_xs[i] < 0 || rec_xs[i] >= XS_SIZE) continue;
+        ra.push_back(V3<R>(R(rec_xyz[3 * i]), R(rec_xyz[3 * i + 1]), R(rec_xyz[3 * i + 2])));
+        rt.push_back(rec_xs[i]);
+    }
+    std::vector<const PairTable<R>*> tabs;
+    for (int j = 0; j < n_types; ++j) {
+        g.maps[types[j]].assign(size_t(g.n[0]) * g.n[1] * g.n[2], R(0));
+    }
+    const R cutoff_sqr = R(64);
+    std::vector<R> acc(n_types);
+    for (int z = 0; z < g.n[2]; ++z)
+        for (int y = 0; y < g.n[1]; ++y)
+            for (int x = 0; x < g.n[0]; ++x) {
+                V3<R> p = g.point(x, y, z);
+                std::fill(acc.begin(), acc.end(), R(0));
+                for (size_t i = 0; i < ra.size(); ++i) {
+                    R r2 = norm_sqr(ra[i] - p);
+                    if (r2 <= cutoff_sqr)
+                        for (int j = 0; j < n_types; ++j) acc[j] += C.table(rt[i], types[j]).eval_fast(r2);
+                }
+                for (int j = 0; j < n_types; ++j) g.at(types[j], x, y, z) = acc[j];
+            }
+}
+
+}  // namespace orc
+
+// ====================================================================== C ABI
+// Opaque handle; precision selected at creation (32 or 64).
+using namespace orc;
+
+struct OrcHandle {
+    int prec;
+    Ctx<double> d;
+    Ctx<float> f;
+};
+
+template <class R> static Ctx<R>& ctx_of(OrcHandle* h);
+template <> Ctx<double>& ctx_of<double>(OrcHandle* h) { return h->d; }
+template <> Ctx<float>& ctx_of<float>(OrcHandle* h) { return h->f; }
+
+template <class R> static void load_pose(const Ligand<R>& L, const double* p, Pose<R>& P) {
+    P.pos = V3<R>(R(p[0]), R(p[1]), R(p[2]));
+    P.q = Q<R>{R(p[3]), R(p[4]), R(p[5]), R(p[6])};
+    P.tors.resize(L.n_tors());
+    for (int k = 0; k < L.n_tors(); ++k) P.tors[k] = R(p[7 + k]);
+}
+template <class R> static void store_pose(const Pose<R>& P, double* p) {
+    p[0] = P.pos[0]; p[1] = P.pos[1]; p[2] = P.pos[2];
+    p[3] = P.q.a; p[4] = P.q.b; p[5] = P.q.c; p[6] = P.q.d;
+    for (size_t k = 0; k < P.tors.size(); ++k) p[7 + k] = P.tors[k];
+}
+
+
+extern "C" void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox(ctr, key, out); }
+
+extern "C" OrcHandle* orc_new(int prec, int sf, const double* weights) {
+    OrcHandle* h = new OrcHandle();
+    h->prec = prec;
+    h->d.init(sf, weights);
+    h->f.init(sf, weights);
+    return h;
+}
+extern "C" void orc_free(OrcHandle* h) { delete h; }
+
+// sample tables of one type pair: e, dor, fast (each 2051 doubles)
+extern "C" void orc_pair_table(OrcHandle* h, int t1, int t2, double* e, double* dor, double* fast) {
+    if (h->prec == 64) {
+        const auto& t = h->d.table(t1, t2);
+        for (int i = 0; i < kTableN; ++i) { e[i] = t.e[i]; dor[i] = t.dor[i]; fast[i] = t.fast[i]; }
+    } else {
+        const auto& t = h->f.table(t1, t2);
+        for (int i = 0; i < kTableN; ++i) { e[i] = t.e[i]; dor[i] = t.dor[i]; fast[i] = t.fast[i]; }
+    }
+}
+extern "C" double orc_sf_eval(OrcHandle* h, int t1, int t2, double r) {
+    if (h->prec == 64) return sf_eval<double>(h->d.sf, t1, t2, r, h->d.w);
+    return sf_eval<float>(h->f.sf, t1, t2, float(r), h->f.w);
+}
+
+// grid geometry -> n[3], init[3], factor[3], factor_inv[3]
+extern "C" void orc_set_grid(OrcHandle* h, const double* center, const double* size, double granularity, int* n_out, double* geo_out) {
+    h->d.grid.set_geometry(center, size, granularity);
+    h->f.grid.set_geometry(center, size, granularity);
+    for (int i = 0; i < 3; ++i) {
+        n_out[i] = h->d.grid.n[i];
+        if (h->prec == 64) {
+            geo_out[i] = h->d.grid.init[i]; geo_out[3 + i] = h->d.grid.factor[i]; geo_out[6 + i] = h->d.grid.factor_inv[i];
+        } else {
+            geo_out[i] = h->f.grid.init[i]; geo_out[3 + i] = h->f.grid.factor[i]; geo_out[6 + i] = h->f.grid.factor_inv[i];
+        }
+    }
+}
+extern "C" void orc_build_maps(OrcHandle* h, int n_rec, const double* xyz, const int* xs, int n_types, const int* types) {
+    if (h->prec == 64) build_maps(h->d, n_rec, xyz, xs, n_types, types);
+    else build_maps(h->f, n_rec, xyz, xs, n_types, types);
+}
+// copy maps in/out (x fastest); values as double
+extern "C" void orc_get_map(OrcHandle* h, int t, double* out) {
+    if (h->prec == 64) { auto& m = h->d.grid.maps[t]; for (size_t i = 0; i < m.size(); ++i) out[i] = m[i]; }
+    else { auto& m = h->f.grid.maps[t]; for (size_t i = 0; i < m.size(); ++i) out[i] = m[i]; }
+}
+extern "C" void orc_set_map(OrcHandle* h, int t, const double* in) {
+    size_t n = size_t(h->d.grid.n[0]) * h->d.grid.n[1] * h->d.grid.n[2];
+    h->d.grid.maps[t].assign(in, in + n);
+    h->f.grid.maps[t].resize(n);
+    for (size_t i = 0; i < n; ++i) h->f.grid.maps[t][i] = float(in[i]);
+}
+extern "C" double orc_grid_eval(OrcHandle* h, int t, const double* loc, double v, double* deriv) {
+    if (h->prec == 64) {
+        V3<double> d;
+        double e = grid_eval(h->d.grid, t, V3<double>(loc[0], loc[1], loc[2]), h->d.slope, v, deriv ? &d : nullptr);
+        if (deriv) for (int i = 0; i < 3; ++i) deriv[i] = d[i];
+        return e;
+    }
+    V3<float> d;
+    float e = grid_eval(h->f.grid, t, V3<float>(float(loc[0]), float(loc[1]), float(loc[2])), h->f.slope, float(v), deriv ? &d : nullptr);
+    if (deriv) for (int i = 0; i < 3; ++i) deriv[i] = d[i];
+    return e;
+}
+
+// Add one ligand in the flat layout.  Arrays:
+//   local[3*n_atoms], xs[n_atoms], heavy[n_atoms]
+//   parent[n_nodes], abegin[n_nodes], aend[n_nodes], origin0[3*n_nodes], axis0[3*n_nodes]
+//   pairs[2*n_pairs]
+// Returns the ligand index.
+template <class R> static int add_lig(Ctx<R>& C, int na, const double* local, const int* xs, const int* heavy, int nn,
+                                      const int* parent, const int* ab, const int* ae, const double* origin0,
+                                      const double* axis0, int np, const int* pairs, double num_tors) {
+    Ligand<R> L;
+    L.n_atoms = na; L.n_nodes = nn; L.n_pairs = np;
+    L.local.resize(na); L.xs.assign(xs, xs + na); L.heavy.assign(heavy, heavy + na);
+    L.node_of.assign(na, 0);
+    for (int i = 0; i < na; ++i) L.local[i] = V3<R>(R(local[3 * i]), R(local[3 * i + 1]), R(local[3 * i + 2]));
+    L.parent.assign(parent, parent + nn); L.abegin.assign(ab, ab + nn); L.aend.assign(ae, ae + nn);
+    L.rel_origin.resize(nn); L.rel_axis.resize(nn); L.origin0.resize(nn);
+    for (int k = 0; k < nn; ++k) {
+        L.origin0[k] = V3<R>(R(origin0[3 * k]), R(origin0[3 * k + 1]), R(origin0[3 * k + 2]));
+        for (int i = ab[k]; i < ae[k]; ++i) L.node_of[i] = k;
+    }
+    for (int k = 0; k < nn; ++k) {
+        if (k == 0) continue;
+        // segment c'tor (tree.h:123-131): relative_origin = origin - parent origin,
+        // relative_axis = axis (parent orientation is the identity at set-up)
+        L.rel_origin[k] = L.origin0[k] - L.origin0[parent[k]];
+        L.rel_axis[k] = V3<R>(R(axis0[3 * k]), R(axis0[3 * k + 1]), R(axis0[3 * k + 2]));
+    }
+    L.pa.resize(np); L.pb.resize(np); L.ptp.resize(np);
+    for (int k = 0; k < np; ++k) { L.pa[k] = pairs[2 * k]; L.pb[k] = pairs[2 * k + 1]; L.ptp[k] = tri_index(xs[pairs[2 * k]], xs[pairs[2 * k + 1]]); }
+    L.num_tors = R(num_tors);
+    C.ligs.push_back(L);
+    return int(C.ligs.size()) - 1;
+}
+extern "C" int orc_add_ligand(OrcHandle* h, int na, const double* local, const int* xs, const int* heavy, int nn, const int* parent,
+                   const int* ab, const int* ae, const double* origin0, const double* axis0, int np, const int* pairs, double num_tors) {
+    add_lig(h->f, na, local, xs, heavy, nn, parent, ab, ae, origin0, axis0, np, pairs, num_tors);
+    return add_lig(h->d, na, local, xs, heavy, nn, parent, ab, ae, origin0, axis0, np, pairs, num_tors);
+}
+
+template <class R> static double eval_impl(Ctx<R>& C, int lig, const double* pose, const double* v, double* grad, double* coords_out, double* terms) {
+    const Ligand<R>& L = C.ligs[lig];
+    Pose<R> P;
+    load_pose(L, pose, P);
+    R vv[3] = {R(v[0]), R(v[1]), R(v[2])};
+    std::vector<R> g(6 + L.n_tors());
+    std::vector<V3<R>> coords;
+    R ei, ea;
+    R e = eval_deriv(C, L, P, vv, g.data(), &coords, &ei, &ea);
+    if (grad) for (size_t i = 0; i < g.size(); ++i) grad[i] = g[i];
+    if (coords_out) for (int i = 0; i < L.n_atoms; ++i) for (int k = 0; k < 3; ++k) coords_out[3 * i + k] = coords[i][k];
+    if (terms) { terms[0] = ei; terms[1] = ea; }
+    return e;
+}
+// energy + gradient (6 + T) of one pose; terms[2] = (grid part, intramolecular part)
+extern "C" double orc_eval_deriv(OrcHandle* h, int lig, const double* pose, const double* v, double* grad, double* coords, double* terms) {
+    if (h->prec == 64) return eval_impl(h->d, lig, pose, v, grad, coords, terms);
+    return eval_impl(h->f, lig, pose, v, grad, coords, terms);
+}
+
+template <class R> static double bfgs_impl(Ctx<R>& C, int lig, double* pose, const double* v, int max_steps, int* evals) {
+    const Ligand<R>& L = C.ligs[lig];
+    Pose<R> P;
+    load_pose(L, pose, P);
+    R vv[3] = {R(v[0]), R(v[1]), R(v[2])};
+    int ec = 0;
+    R e = bfgs(C, L, P, vv, max_steps, &ec);
+    store_pose(P, pose);
+    if (evals) *evals = ec;
+    return e;
+}
+extern "C" double orc_bfgs(OrcHandle* h, int lig, double* pose, const double* v, int max_steps, int* evals) {
+    if (h->prec == 64) return bfgs_impl(h->d, lig, pose, v, max_steps, evals);
+    return bfgs_impl(h->f, lig, pose, v, max_steps, evals);
+}
+
+template <class R> static double mc_impl(Ctx<R>& C, int lig, uint64_t seed, uint32_t chain, int gsteps, int lsteps, double* best_pose, int* evals, double* trace) {
+    const Ligand<R>& L = C.ligs[lig];
+    Pose<R> best;
+    int ec = 0;
+    std::vector<R> tr(gsteps);
+    R e = mc_chain(C, L, seed, chain, gsteps, lsteps, R(2), R(1.2), best, &ec, tr.data());
+    store_pose(best, best_pose);
+    if (evals) *evals = ec;
+    if (trace) for (int i = 0; i < gsteps; ++i) trace[i] = tr[i];
+    return e;
+}
+// one MC chain (global chain id `chain`); returns best energy, writes best pose
+extern "C" double orc_mc_chain(OrcHandle* h, int lig, uint64_t seed, uint32_t chain, int gsteps, int lsteps, double* best_pose, int* evals, double* trace) {
+    if (h->prec == 64) return mc_impl(h->d, lig, seed, chain, gsteps, lsteps, best_pose, evals, trace);
+    return mc_impl(h->f, lig, seed, chain, gsteps, lsteps, best_pose, evals, trace);
+}
+// random start pose of a chain (exposed to check the device generator)
+extern "C" void orc_random_start(OrcHandle* h, int lig, uint64_t seed, uint32_t chain, double* pose) {
+    const Ligand<double>& L = h->d.ligs[lig];
+    double lo[3], hi[3];
+    for (int i = 0; i < 3; ++i) { lo[i] = h->d.grid.init[i]; hi[i] = h->d.grid.init[i] + h->d.grid.factor_inv[i] * (h->d.grid.n[i] - 1); }
+    Pose<double> P;
+    random_start(L, lo, hi, seed, chain, P);
+    store_pose(P, pose);
+}
+// mutation of a given pose (exposed to check the device mutation)
+extern "C" void orc_mutate(OrcHandle* h, int lig, double* pose, double rgyr, uint64_t seed, uint32_t chain, uint32_t step) {
+    const Ligand<double>& L = h->d.ligs[lig];
+    Pose<double> P;
+    load_pose(L, pose, P);
+    mutate(L, P, rgyr, 2.0, seed, chain, step);
+    store_pose(P, pose);
+}
+// inter / intra terms with authentic caps (for the final affinity)
+extern "C" void orc_score_terms(OrcHandle* h, int lig, const double* pose, double* inter, double* intra) {
+    if (h->prec == 64) {
+        Pose<double> P; load_pose(h->d.ligs[lig], pose, P);
+        score_terms(h->d, h->d.ligs[lig], P, inter, intra);
+    } else {
+        Pose<float> P; load_pose(h->f.ligs[lig], pose, P);
+        float a, b;
+        score_terms(h->f, h->f.ligs[lig], P, &a, &b);
+        *inter = a; *intra = b;
+    }
+}
+extern "C" double orc_conf_independent(OrcHandle* h, double num_tors, double x) {
+    if (h->prec == 64) return conf_independent(h->d, num_tors, x);
+    return conf_independent(h->f, float(num_tors), float(x));
+}
+
+// CPU docking baseline: exhaustiveness chains of every listed ligand spread
+// over n_threads std::threads (parallel_mc, parallel_mc.cpp:318-329).
+
+#include <atomic>
+#include <thread>
+extern "C" double orc_dock_cpu(OrcHandle* h, int n_lig, const int* ligs, int exh, uint64_t seed, int gsteps,
+                               const int* lsteps, int n_threads, double* best_e_out, long long* evals_out) {
+    std::atomic<int> next(0);
+    const int total = n_lig * exh;
+    std::vector<long long> evals(n_threads, 0);
+    std::vector<double> be(total, 0);
+    auto work = [&](int tid) {
+        for (;;) {
+            int t = next.fetch_add(1);
+            if (t >= total) break;
+            int li = t / exh, ci = t % exh;
+            int lig = ligs[li];
+            int ec = 0;
+            std::vector<double> pose(7 + 64);
+            double e;
+            if (h->prec == 64) e = mc_impl(h->d, lig, seed, uint32_t(li * exh + ci), gsteps, lsteps[li], pose.data(), &ec, nullptr);
+            else e = mc_impl(h->f, lig, seed, uint32_t(li * exh + ci), gsteps, lsteps[li], pose.data(), &ec, nullptr);
+            be[t] = e;
+            evals[tid] += ec;
+        }
+    };
+    // tables are built lazily; build them for all pairs used before threading
+    for (int li = 0; li < n_lig; ++li) {
+        if (h->prec == 64) { auto& L = h->d.ligs[ligs[li]]; for (int k = 0; k < L.n_pairs; ++k) h->d.table(L.xs[L.pa[k]], L.xs[L.pb[k]]); }
+        else { auto& L = h->f.ligs[ligs[li]]; for (int k = 0; k < L.n_pairs; ++k) h->f.table(L.xs[L.pa[k]], L.xs[L.pb[k]]); }
+    }
+    std::vector<std::thread> th;
+    for (int i = 0; i < n_threads; ++i) th.emplace_back(work, i);
+    for (auto& t : th) t.join();
+    long long ev = 0;
+    for (auto e : evals) ev += e;
+    if (evals_out) *evals_out = ev;
+    if (best_e_out) for (int i = 0; i < total; ++i) best_e_out[i] = be[i];
+    return 0;
+}

@@ -1,0 +1,218 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so built from oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker / the timed CPU baseline,
+never by the product.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+POSE = 7 + 32  # pose stride shared with the product's C ABI
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_L = None
+
+
+def load():
+    global _L
+    if _L is not None:
+        return _L
+    if not os.path.exists(LIB):
+        build()
+    L = ctypes.CDLL(LIB)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    sig = {
+        "orc_philox": (None, [u32p, u32p, u32p]),
+        "orc_new": (vp, [ctypes.c_int, ctypes.c_int, dp]),
+        "orc_free": (None, [vp]),
+        "orc_pair_table": (None, [vp, ctypes.c_int, ctypes.c_int, dp, dp, dp]),
+        "orc_sf_eval": (ctypes.c_double, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+        "orc_set_grid": (None, [vp, dp, dp, ctypes.c_double, ip, dp]),
+        "orc_build_maps": (None, [vp, ctypes.c_int, dp, ip, ctypes.c_int, ip]),
+        "orc_get_map": (None, [vp, ctypes.c_int, dp]),
+        "orc_set_map": (None, [vp, ctypes.c_int, dp]),
+        "orc_grid_eval": (ctypes.c_double, [vp, ctypes.c_int, dp, ctypes.c_double, dp]),
+        "orc_add_ligand": (ctypes.c_int, [vp, ctypes.c_int, dp, ip, ip, ctypes.c_int, ip, ip, ip, dp, dp,
+                                          ctypes.c_int, ip, ctypes.c_double]),
+        "orc_eval_deriv": (ctypes.c_double, [vp, ctypes.c_int, dp, dp, dp, dp, dp]),
+        "orc_bfgs": (ctypes.c_double, [vp, ctypes.c_int, dp, dp, ctypes.c_int, ip]),
+        "orc_mc_chain": (ctypes.c_double, [vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
+                                           ctypes.c_int, dp, ip, dp]),
+        "orc_random_start": (None, [vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, dp]),
+        "orc_mutate": (None, [vp, ctypes.c_int, dp, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
+                              ctypes.c_uint32]),
+        "orc_score_terms": (None, [vp, ctypes.c_int, dp, dp, dp]),
+        "orc_conf_independent": (ctypes.c_double, [vp, ctypes.c_double, ctypes.c_double]),
+        "orc_dock_cpu": (ctypes.c_double, [vp, ctypes.c_int, ip, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ip,
+                                           ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)]),
+    }
+    for n, (r, a) in sig.items():
+        f = getattr(L, n)
+        f.restype = r
+        f.argtypes = a
+    _L = L
+    return L
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def philox(ctr, key):
+    L = load()
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    L.orc_philox(c, k, o)
+    return list(o)
+
+
+class Oracle:
+    """prec = 64 (FP64 oracle) or 32 (FP32 restatement of the reference CPU path)."""
+
+    def __init__(self, prec=64, sf=0, weights=None):
+        from_w = weights
+        if from_w is None:
+            from_w = (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 50.0, 0.05846) if sf == 0 else \
+                (-0.045, 0.8, -0.035, -0.600, 50.0, 0.05846)
+        self.L = load()
+        self.w = np.array(from_w, dtype=np.float64)
+        self.h = self.L.orc_new(prec, sf, _d(self.w))
+        self.prec = prec
+        self.ligs = []
+
+    def __del__(self):
+        try:
+            self.L.orc_free(self.h)
+        except Exception:
+            pass
+
+    def pair_table(self, t1, t2):
+        e = np.zeros(2051); dor = np.zeros(2051); fast = np.zeros(2051)
+        self.L.orc_pair_table(self.h, t1, t2, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              dor.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              fast.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return e, dor, fast
+
+    def sf_eval(self, t1, t2, r):
+        return self.L.orc_sf_eval(self.h, t1, t2, r)
+
+    def set_grid(self, center, size, granularity=0.375):
+        n = np.zeros(3, dtype=np.int32)
+        geo = np.zeros(9)
+        self.L.orc_set_grid(self.h, _d(center), _d(size), granularity, n.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                            geo.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        self.n = tuple(int(x) for x in n)
+        self.geo = geo
+        return self.n, geo
+
+    def build_maps(self, rec_xyz, rec_xs, types):
+        x = np.ascontiguousarray(rec_xyz, dtype=np.float64)
+        t = np.ascontiguousarray(rec_xs, dtype=np.int32)
+        ty = np.ascontiguousarray(types, dtype=np.int32)
+        self.L.orc_build_maps(self.h, len(t), _d(x), _i(t), len(ty), _i(ty))
+
+    def get_map(self, t):
+        out = np.zeros(int(np.prod(self.n)))
+        self.L.orc_get_map(self.h, t, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def set_map(self, t, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self.L.orc_set_map(self.h, t, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    def grid_eval(self, t, loc, v=1000.0, deriv=True):
+        d = np.zeros(3)
+        e = self.L.orc_grid_eval(self.h, t, _d(loc), v,
+                                 d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if deriv else None)
+        return e, d
+
+    def add_ligand(self, lig):
+        """lig: unidock_tools.synth.Ligand (or any object with the same fields)."""
+        idx = self.L.orc_add_ligand(self.h, lig.n_atoms, _d(lig.local), _i(lig.xs), _i(lig.heavy), len(lig.parent),
+                                    _i(lig.parent), _i(lig.abegin), _i(lig.aend), _d(lig.origin), _d(lig.axis),
+                                    len(lig.pairs), _i(lig.pairs), float(lig.num_tors))
+        self.ligs.append(lig)
+        return idx
+
+    def eval_deriv(self, lig, pose, v=(1000.0, 1000.0, 1000.0)):
+        n_t = self.ligs[lig].n_tors
+        p = np.zeros(POSE)
+        p[:len(pose)] = pose
+        g = np.zeros(6 + 32)
+        coords = np.zeros(3 * self.ligs[lig].n_atoms)
+        terms = np.zeros(2)
+        e = self.L.orc_eval_deriv(self.h, lig, _d(p), _d(v), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  coords.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  terms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return e, g[:6 + n_t], coords.reshape(-1, 3), terms
+
+    def bfgs(self, lig, pose, v=(1000.0, 1000.0, 1000.0), max_steps=None):
+        if max_steps is None:
+            max_steps = (25 + self.ligs[lig].n_atoms) // 3
+        p = np.zeros(POSE)
+        p[:len(pose)] = pose
+        ev = ctypes.c_int()
+        pp = np.ascontiguousarray(p)
+        e = self.L.orc_bfgs(self.h, lig, pp.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _d(v), max_steps,
+                            ctypes.byref(ev))
+        return e, pp, ev.value
+
+    def mc_chain(self, lig, seed, chain, global_steps, local_steps=None):
+        if local_steps is None:
+            local_steps = (25 + self.ligs[lig].n_atoms) // 3
+        best = np.zeros(POSE)
+        tr = np.zeros(global_steps)
+        ev = ctypes.c_int()
+        e = self.L.orc_mc_chain(self.h, lig, seed, chain, global_steps, local_steps,
+                                best.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(ev),
+                                tr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return e, best, ev.value, tr
+
+    def random_start(self, lig, seed, chain):
+        p = np.zeros(POSE)
+        self.L.orc_random_start(self.h, lig, seed, chain, p.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return p
+
+    def mutate(self, lig, pose, rgyr, seed, chain, step):
+        p = np.zeros(POSE)
+        p[:len(pose)] = pose
+        self.L.orc_mutate(self.h, lig, p.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), rgyr, seed, chain, step)
+        return p
+
+    def score_terms(self, lig, pose):
+        p = np.zeros(POSE)
+        p[:len(pose)] = pose
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        self.L.orc_score_terms(self.h, lig, _d(p), ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def conf_independent(self, num_tors, x):
+        return self.L.orc_conf_independent(self.h, num_tors, x)
+
+    def dock_cpu(self, ligs, exh, seed, global_steps, local_steps, n_threads):
+        ligs = np.ascontiguousarray(ligs, dtype=np.int32)
+        ls = np.ascontiguousarray(local_steps, dtype=np.int32)
+        be = np.zeros(len(ligs) * exh)
+        ev = ctypes.c_longlong()
+        self.L.orc_dock_cpu(self.h, len(ligs), _i(ligs), exh, seed, global_steps, _i(ls), n_threads,
+                            be.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(ev))
+        return be, ev.value

@@ -1,0 +1,919 @@
+// Host side of the C ABI (include/unidock_amd.h): context, uploads, size
+// buckets, work lists and launches.  Every entry point returns a status code;
+// nothing throws across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/unidock_amd.h"
+#include "ud_internal.h"
+#include "ud_launch.h"
+
+using namespace ud;
+
+struct ud_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // scoring
+    int sf = -1;
+    float w[7] = {0, 0, 0, 0, 0, 0, 0};  // term weights (glue at [5] for Vina, [4] for Vinardo) + internal rot
+    float rot_internal = 0.f;
+    float4* d_table_deriv = nullptr;
+    float* d_table_fast = nullptr;
+    // grid
+    bool box_set = false;
+    float init[3], factor[3], factor_inv[3], dimm1[3];
+    int n[3] = {0, 0, 0};
+    int n_points = 0;
+    int precision = UD_GRID_F32;
+    int n_maps = 0;
+    int map_types[UD_MAX_MAPS];
+    int slot_of_type[36];
+    float scale[UD_MAX_MAPS];
+    void* d_maps = nullptr;
+    // ligands
+    int n_lig = 0;
+    std::vector<LigHdr> hdr;
+    std::vector<int> lig_nat, lig_nn;  // per ligand atoms / nodes
+    std::vector<std::vector<int>> lig_xs;
+    LigHdr* d_hdr = nullptr;
+    AtomRec* d_atoms = nullptr;
+    NodeRec* d_nodes = nullptr;
+    PairRec* d_pairs = nullptr;
+    // scratch / results
+    WorkItem* d_work = nullptr;
+    size_t work_cap = 0;
+    int32_t* d_perm = nullptr;
+    size_t perm_cap = 0;
+    float* d_poses = nullptr;
+    size_t poses_cap = 0;
+    float* d_f1 = nullptr;
+    float* d_f2 = nullptr;
+    size_t f_cap = 0, f2_cap = 0;
+    std::vector<AtomRec> h_atoms;  // host copy of the atom records
+    bool slots_dirty = true;
+    int32_t* d_i1 = nullptr;
+    size_t i_cap = 0;
+    float* d_out_pose = nullptr;
+    float* d_out_e = nullptr;
+    long long* d_out_evals = nullptr;
+    size_t out_cap = 0;
+    size_t out_n = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0;
+    int last_launches = 0;
+    // autotuned lanes per bucket (atom capacity class)
+    int tuned_lanes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(ud_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+#define UD_HIP(c, call)                                                                         \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return fail(c, UD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T> int grow(ud_ctx* c, T*& p, size_t& cap, size_t n) {
+    if (n <= cap && p) return UD_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    size_t nn = std::max<size_t>(n, 1);
+    if (hipMalloc(reinterpret_cast<void**>(&p), nn * sizeof(T)) != hipSuccess) {
+        cap = 0;
+        return fail(c, UD_ERR_NOMEM, "device allocation failed");
+    }
+    cap = nn;
+    return UD_OK;
+}
+
+int tri_index(int i, int j) { return (i <= j) ? i + j * (j + 1) / 2 : j + i * (i + 1) / 2; }
+
+// cache.cpp:160-199: G types carry no grid term, CG carbons use C_H / C_P
+int grid_type_of(int t) {
+    if (t < 0 || t >= 36) return -1;
+    if (t == 21 || t == 24 || t == 27 || t == 30) return -1;
+    if (t == 19 || t == 22 || t == 25 || t == 28) return 0;
+    if (t == 20 || t == 23 || t == 26 || t == 29) return 1;
+    return t;
+}
+
+GridDev grid_dev(const ud_ctx* c) {
+    GridDev g;
+    std::memset(&g, 0, sizeof(g));
+    g.maps = c->d_maps;
+    for (int i = 0; i < 3; ++i) {
+        g.init[i] = c->init[i];
+        g.factor[i] = c->factor[i];
+        g.factor_inv[i] = c->factor_inv[i];
+        g.dimm1[i] = c->dimm1[i];
+        g.n[i] = c->n[i];
+    }
+    g.n_points = c->n_points;
+    g.slope = 1e6f;  // vina.cpp:515
+    for (int i = 0; i < UD_MAX_MAPS; ++i) g.scale[i] = c->scale[i];
+    return g;
+}
+
+LigandSetDev lig_dev(const ud_ctx* c) {
+    LigandSetDev l;
+    l.hdr = c->d_hdr;
+    l.atoms = c->d_atoms;
+    l.nodes = c->d_nodes;
+    l.pairs = c->d_pairs;
+    l.table_deriv = c->d_table_deriv;
+    l.table_fast = c->d_table_fast;
+    return l;
+}
+
+// largest lanes-per-group whose LDS footprint fits one workgroup (64 KiB)
+int fit_lanes(int lanes, int na, int nn) {
+    while (lanes < 64 && lds_bytes(lanes, na, nn) > 65536) lanes *= 2;
+    return lanes;
+}
+
+int ready_for_eval(ud_ctx* c) {
+    if (c->sf < 0) return fail(c, UD_ERR_STATE, "scoring function not set");
+    if (!c->d_maps) return fail(c, UD_ERR_STATE, "grid maps not computed");
+    if (c->n_lig == 0) return fail(c, UD_ERR_STATE, "no ligands set");
+    for (int l = 0; l < c->n_lig; ++l)
+        for (int t : c->lig_xs[l]) {
+            int gt = grid_type_of(t);
+            if (gt >= 0 && c->slot_of_type[gt] < 0)
+                return fail(c, UD_ERR_STATE, "affinity map for atom type " + std::to_string(gt) + " is not present");
+        }
+    return UD_OK;
+}
+
+// Upload atoms with their grid slots (depends on the current maps).
+int upload_atoms(ud_ctx* c, const std::vector<AtomRec>& atoms) {
+    size_t cap = 0;
+    AtomRec* p = nullptr;
+    if (c->d_atoms) (void)hipFree(c->d_atoms);
+    c->d_atoms = nullptr;
+    int rc = grow(c, p, cap, atoms.size());
+    if (rc) return rc;
+    c->d_atoms = p;
+    UD_HIP(c, hipMemcpy(c->d_atoms, atoms.data(), atoms.size() * sizeof(AtomRec), hipMemcpyHostToDevice));
+    return UD_OK;
+}
+
+// Build poses grouped by ligand: perm (sorted pose indices) and work items
+// of up to `groups` poses of one ligand each.
+void make_pose_work(int n_poses, const int32_t* pose_ligand, int groups, std::vector<int32_t>& perm,
+                    std::vector<WorkItem>& work) {
+    perm.resize(n_poses);
+    for (int i = 0; i < n_poses; ++i) perm[i] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return pose_ligand[a] < pose_ligand[b]; });
+    work.clear();
+    int i = 0;
+    while (i < n_poses) {
+        int lig = pose_ligand[perm[i]];
+        int j = i;
+        while (j < n_poses && pose_ligand[perm[j]] == lig && j - i < groups) ++j;
+        work.push_back(WorkItem{lig, i, j - i, 0});
+        i = j;
+    }
+}
+
+int caps_all(const ud_ctx* c, int& na, int& nn) {
+    na = 1;
+    nn = 1;
+    for (int l = 0; l < c->n_lig; ++l) {
+        na = std::max(na, c->lig_nat[l]);
+        nn = std::max(nn, c->lig_nn[l]);
+    }
+    return UD_OK;
+}
+
+int check_pose_args(ud_ctx* c, int n_poses, const int32_t* pose_ligand) {
+    if (n_poses < 0 || (n_poses > 0 && !pose_ligand)) return fail(c, UD_ERR_INVALID, "bad pose arguments");
+    for (int i = 0; i < n_poses; ++i)
+        if (pose_ligand[i] < 0 || pose_ligand[i] >= c->n_lig) return fail(c, UD_ERR_INVALID, "pose ligand index out of range");
+    return UD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ud_default_search_params(ud_search_params* p) {
+    if (!p) return;
+    p->exhaustiveness = 8;
+    p->global_steps = 20;
+    p->local_steps = 0;
+    p->lanes_per_chain = 0;
+    p->seed = 181129;
+    for (int i = 0; i < 3; ++i) {
+        p->hunt_cap[i] = 10.f;
+        p->authentic_v[i] = 1000.f;
+    }
+    p->temperature = 1.2f;
+    p->mutation_amplitude = 2.f;
+}
+
+int ud_create(int device_id, void* stream, ud_ctx** out) {
+    if (!out) return UD_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UD_ERR_HIP;
+    if (device_id < 0 || device_id >= ndev) return UD_ERR_INVALID;
+    if (hipSetDevice(device_id) != hipSuccess) return UD_ERR_HIP;
+    ud_ctx* c = new ud_ctx();
+    c->device = device_id;
+    c->stream = reinterpret_cast<hipStream_t>(stream);
+    for (int i = 0; i < 36; ++i) c->slot_of_type[i] = -1;
+    for (int i = 0; i < UD_MAX_MAPS; ++i) c->scale[i] = 1.f;
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return UD_ERR_HIP;
+    }
+    *out = c;
+    return UD_OK;
+}
+
+int ud_destroy(ud_ctx* c) {
+    if (!c) return UD_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    void* ptrs[] = {c->d_table_deriv, c->d_table_fast, c->d_maps, c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs,
+                    c->d_work, c->d_perm, c->d_poses, c->d_f1, c->d_f2, c->d_i1, c->d_out_pose, c->d_out_e,
+                    c->d_out_evals};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c;
+    return UD_OK;
+}
+
+const char* ud_last_error(const ud_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ud_set_scoring(ud_ctx* c, int sf, const double* weights, int n_weights) {
+    if (!c) return UD_ERR_INVALID;
+    if (sf != UD_SF_VINA && sf != UD_SF_VINARDO) return fail(c, UD_ERR_INVALID, "unknown scoring function");
+    const int need = (sf == UD_SF_VINA) ? 7 : 6;
+    if (!weights || n_weights != need) return fail(c, UD_ERR_INVALID, "wrong number of weights");
+    UD_HIP(c, hipSetDevice(c->device));
+    c->sf = sf;
+    for (int i = 0; i < 7; ++i) c->w[i] = 0.f;
+    for (int i = 0; i < need - 1; ++i) c->w[i] = float(weights[i]);
+    // internal rotational weight 5 * w_rot / 0.1 - 1 (vina.cpp:404, 425)
+    c->rot_internal = float(5.0 * weights[need - 1] / 0.1 - 1.0);
+    size_t cap1 = 0, cap2 = 0;
+    if (!c->d_table_deriv) {
+        int rc = grow(c, c->d_table_deriv, cap1, size_t(kTypePairs) * kTableStride);
+        if (rc) return rc;
+        rc = grow(c, c->d_table_fast, cap2, size_t(kTypePairs) * kTableStride);
+        if (rc) return rc;
+    }
+    UD_HIP(c, launch_tables(sf, c->w, c->d_table_deriv, c->d_table_fast, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    return UD_OK;
+}
+
+int ud_set_box(ud_ctx* c, const double* center, const double* size, double granularity, int32_t* n_out) {
+    if (!c || !center || !size) return UD_ERR_INVALID;
+    if (!(granularity > 0)) return fail(c, UD_ERR_INVALID, "granularity must be positive");
+    long long pts = 1;
+    for (int i = 0; i < 3; ++i) {
+        if (!(size[i] > 0)) return fail(c, UD_ERR_INVALID, "grid box dimensions must be greater than 0");
+        const long nv = long(std::ceil(size[i] / granularity));
+        // compute_vina_maps geometry (vina.cpp:527-539) and grid::init (grid.cpp:24-37), in float
+        const float real_span = float(granularity * double(nv));
+        const float begin = float(center[i]) - real_span / 2.f;
+        const float end = begin + real_span;
+        c->n[i] = int(nv) + 1;
+        c->init[i] = begin;
+        const float range = end - begin;
+        c->dimm1[i] = float(c->n[i] - 1);
+        c->factor[i] = c->dimm1[i] / range;
+        c->factor_inv[i] = 1.f / c->factor[i];
+        pts *= c->n[i];
+        if (n_out) n_out[i] = c->n[i];
+    }
+    if (pts > UD_MAX_GRID_POINTS) return fail(c, UD_ERR_CAPACITY, "grid has more points than UD_MAX_GRID_POINTS");
+    c->n_points = int(pts);
+    c->box_set = true;
+    if (c->d_maps) {
+        (void)hipFree(c->d_maps);
+        c->d_maps = nullptr;
+    }
+    c->n_maps = 0;
+    for (int i = 0; i < 36; ++i) c->slot_of_type[i] = -1;
+    return UD_OK;
+}
+
+static int store_maps(ud_ctx* c, float* d_f32, int n_types, const int32_t* types, int precision) {
+    // d_f32: n_types maps of n_points floats on the device (consumed)
+    if (c->d_maps) {
+        (void)hipFree(c->d_maps);
+        c->d_maps = nullptr;
+    }
+    for (int i = 0; i < 36; ++i) c->slot_of_type[i] = -1;
+    c->n_maps = n_types;
+    for (int m = 0; m < n_types; ++m) {
+        c->map_types[m] = types[m];
+        c->slot_of_type[types[m]] = m;
+        c->scale[m] = 1.f;
+    }
+    c->precision = precision;
+    c->slots_dirty = true;
+    if (precision == UD_GRID_F32) {
+        c->d_maps = d_f32;
+        return UD_OK;
+    }
+    // FP16 with a power-of-two scale per map: max |v| / scale in [16384, 32768)
+    unsigned int* d_mx = nullptr;
+    UD_HIP(c, hipMalloc(&d_mx, sizeof(unsigned int) * n_types));
+    UD_HIP(c, hipMemsetAsync(d_mx, 0, sizeof(unsigned int) * n_types, c->stream));
+    UD_HIP(c, launch_grid_absmax(d_f32, c->n_points, n_types, d_mx, c->stream));
+    std::vector<unsigned int> mx(n_types);
+    UD_HIP(c, hipMemcpyAsync(mx.data(), d_mx, sizeof(unsigned int) * n_types, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<float> inv(n_types);
+    for (int m = 0; m < n_types; ++m) {
+        float v;
+        std::memcpy(&v, &mx[m], sizeof(float));
+        int e = 0;
+        if (v > 0.f && std::isfinite(v)) {
+            std::frexp(v, &e);  // v = f * 2^e, f in [0.5, 1)
+            e -= 15;            // v / 2^e in [16384, 32768)
+        }
+        c->scale[m] = std::ldexp(1.f, e);
+        inv[m] = std::ldexp(1.f, -e);
+    }
+    float* d_inv = nullptr;
+    UD_HIP(c, hipMalloc(&d_inv, sizeof(float) * n_types));
+    UD_HIP(c, hipMemcpyAsync(d_inv, inv.data(), sizeof(float) * n_types, hipMemcpyHostToDevice, c->stream));
+    void* d_h = nullptr;
+    if (hipMalloc(&d_h, size_t(2) * n_types * c->n_points) != hipSuccess) return fail(c, UD_ERR_NOMEM, "map allocation failed");
+    UD_HIP(c, launch_grid_to_half(d_f32, c->n_points, n_types, d_inv, d_h, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(d_inv);
+    (void)hipFree(d_mx);
+    (void)hipFree(d_f32);
+    c->d_maps = d_h;
+    return UD_OK;
+}
+
+static int check_types(ud_ctx* c, int n_types, const int32_t* types) {
+    if (n_types <= 0 || n_types > UD_MAX_MAPS || !types) return fail(c, UD_ERR_INVALID, "bad map type list");
+    for (int m = 0; m < n_types; ++m) {
+        if (types[m] < 0 || types[m] >= 36 || grid_type_of(types[m]) != types[m])
+            return fail(c, UD_ERR_INVALID, "map type must be a plain XS type");
+        for (int k = 0; k < m; ++k)
+            if (types[k] == types[m]) return fail(c, UD_ERR_INVALID, "duplicate map type");
+    }
+    return UD_OK;
+}
+
+int ud_compute_maps(ud_ctx* c, int n_rec, const float* rec_xyz, const int32_t* rec_xs, int n_types,
+                    const int32_t* types, int precision) {
+    if (!c) return UD_ERR_INVALID;
+    if (!c->box_set) return fail(c, UD_ERR_STATE, "box not set");
+    if (c->sf < 0) return fail(c, UD_ERR_STATE, "scoring function not set");
+    if (n_rec < 0 || (n_rec > 0 && (!rec_xyz || !rec_xs))) return fail(c, UD_ERR_INVALID, "bad receptor arrays");
+    if (precision != UD_GRID_F32 && precision != UD_GRID_F16) return fail(c, UD_ERR_INVALID, "bad precision");
+    int rc = check_types(c, n_types, types);
+    if (rc) return rc;
+    UD_HIP(c, hipSetDevice(c->device));
+    std::vector<float4> rec;
+    rec.reserve(n_rec);
+    for (int i = 0; i < n_rec; ++i) {
+        if (rec_xs[i] < 0 || rec_xs[i] >= 36) continue;  // untyped atoms do not contribute (cache.cpp:527)
+        int32_t t = rec_xs[i];
+        float tf;
+        std::memcpy(&tf, &t, 4);
+        rec.push_back(make_float4(rec_xyz[3 * i], rec_xyz[3 * i + 1], rec_xyz[3 * i + 2], tf));
+    }
+    float4* d_rec = nullptr;
+    UD_HIP(c, hipMalloc(&d_rec, sizeof(float4) * std::max<size_t>(rec.size(), 1)));
+    if (!rec.empty())
+        UD_HIP(c, hipMemcpyAsync(d_rec, rec.data(), sizeof(float4) * rec.size(), hipMemcpyHostToDevice, c->stream));
+    float* d_f32 = nullptr;
+    if (hipMalloc(&d_f32, sizeof(float) * size_t(n_types) * c->n_points) != hipSuccess)
+        return fail(c, UD_ERR_NOMEM, "map allocation failed");
+    GridBuildArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (int i = 0; i < 3; ++i) {
+        a.init[i] = c->init[i];
+        a.factor_inv[i] = c->factor_inv[i];
+        a.n[i] = c->n[i];
+    }
+    a.n_points = c->n_points;
+    a.n_maps = n_types;
+    for (int m = 0; m < n_types; ++m) a.types[m] = types[m];
+    UD_HIP(c, hipEventRecord(c->ev0, c->stream));
+    UD_HIP(c, launch_grid_build(d_rec, int(rec.size()), a, c->d_table_fast, d_f32, c->stream));
+    UD_HIP(c, hipEventRecord(c->ev1, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    c->last_launches = 1;
+    (void)hipFree(d_rec);
+    return store_maps(c, d_f32, n_types, types, precision);
+}
+
+int ud_set_maps(ud_ctx* c, int n_types, const int32_t* types, const float* values, int precision) {
+    if (!c) return UD_ERR_INVALID;
+    if (!c->box_set) return fail(c, UD_ERR_STATE, "box not set");
+    if (!values) return fail(c, UD_ERR_INVALID, "null map values");
+    if (precision != UD_GRID_F32 && precision != UD_GRID_F16) return fail(c, UD_ERR_INVALID, "bad precision");
+    int rc = check_types(c, n_types, types);
+    if (rc) return rc;
+    UD_HIP(c, hipSetDevice(c->device));
+    float* d_f32 = nullptr;
+    if (hipMalloc(&d_f32, sizeof(float) * size_t(n_types) * c->n_points) != hipSuccess)
+        return fail(c, UD_ERR_NOMEM, "map allocation failed");
+    UD_HIP(c, hipMemcpy(d_f32, values, sizeof(float) * size_t(n_types) * c->n_points, hipMemcpyHostToDevice));
+    return store_maps(c, d_f32, n_types, types, precision);
+}
+
+int ud_get_map(ud_ctx* c, int xs_type, float* out) {
+    if (!c || !out) return UD_ERR_INVALID;
+    if (xs_type < 0 || xs_type >= 36 || c->slot_of_type[xs_type] < 0) return fail(c, UD_ERR_INVALID, "no map for type");
+    const int m = c->slot_of_type[xs_type];
+    UD_HIP(c, hipSetDevice(c->device));
+    if (c->precision == UD_GRID_F32) {
+        UD_HIP(c, hipMemcpy(out, static_cast<float*>(c->d_maps) + size_t(m) * c->n_points, sizeof(float) * c->n_points,
+                            hipMemcpyDeviceToHost));
+    } else {
+        std::vector<uint16_t> h(c->n_points);
+        UD_HIP(c, hipMemcpy(h.data(), static_cast<uint16_t*>(c->d_maps) + size_t(m) * c->n_points, 2 * size_t(c->n_points),
+                            hipMemcpyDeviceToHost));
+        for (int i = 0; i < c->n_points; ++i) {
+            // IEEE binary16 -> float
+            const uint16_t b = h[i];
+            const uint32_t sign = (b >> 15) & 1, ex = (b >> 10) & 31, man = b & 1023;
+            float v;
+            if (ex == 0) v = std::ldexp(float(man), -24);
+            else if (ex == 31) v = man ? NAN : INFINITY;
+            else v = std::ldexp(float(man | 1024), int(ex) - 25);
+            out[i] = (sign ? -v : v) * c->scale[m];
+        }
+    }
+    return UD_OK;
+}
+
+int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
+    if (!c || !b) return UD_ERR_INVALID;
+    if (b->n_ligands <= 0) return fail(c, UD_ERR_INVALID, "empty ligand batch");
+    if (!b->atom_offset || !b->node_offset || !b->pair_offset || !b->atom_local || !b->atom_xs || !b->atom_heavy ||
+        !b->node_parent || !b->node_abegin || !b->node_aend || !b->node_origin || !b->node_axis || !b->num_tors)
+        return fail(c, UD_ERR_INVALID, "null array in ligand batch");
+    UD_HIP(c, hipSetDevice(c->device));
+    const int L = b->n_ligands;
+    std::vector<LigHdr> hdr(L);
+    std::vector<AtomRec> atoms;
+    std::vector<NodeRec> nodes;
+    std::vector<PairRec> pairs;
+    std::vector<int> nat(L), nnd(L);
+    std::vector<std::vector<int>> xs(L);
+    for (int l = 0; l < L; ++l) {
+        const int a0 = b->atom_offset[l], a1 = b->atom_offset[l + 1];
+        const int k0 = b->node_offset[l], k1 = b->node_offset[l + 1];
+        const int p0 = b->pair_offset[l], p1 = b->pair_offset[l + 1];
+        const int na = a1 - a0, nn = k1 - k0, np = p1 - p0;
+        const std::string tag = "ligand " + std::to_string(l) + ": ";
+        if (na <= 0 || nn <= 0 || np < 0) return fail(c, UD_ERR_INVALID, tag + "empty or negative ranges");
+        if (na > UD_MAX_ATOMS) return fail(c, UD_ERR_CAPACITY, tag + "more than UD_MAX_ATOMS atoms");
+        if (nn - 1 > UD_MAX_TORSIONS) return fail(c, UD_ERR_CAPACITY, tag + "more than UD_MAX_TORSIONS torsions");
+        if (np > UD_MAX_PAIRS) return fail(c, UD_ERR_CAPACITY, tag + "more than UD_MAX_PAIRS pairs");
+        if (np > 0 && !b->pairs) return fail(c, UD_ERR_INVALID, tag + "null pair array");
+        // nodes: pre-order, contiguous atom ranges covering [0, na)
+        std::vector<int> depth(nn, 0), sub_end(nn, 0), node_of(na, -1);
+        int expect = 0;
+        for (int k = 0; k < nn; ++k) {
+            const int par = b->node_parent[k0 + k];
+            const int ab = b->node_abegin[k0 + k], ae = b->node_aend[k0 + k];
+            if (k == 0 ? par != -1 : (par < 0 || par >= k)) return fail(c, UD_ERR_INVALID, tag + "nodes must be in pre-order with the root first");
+            if (ab != expect || ae <= ab || ae > na) return fail(c, UD_ERR_INVALID, tag + "node atom ranges must be contiguous and non-empty");
+            expect = ae;
+            depth[k] = (k == 0) ? 0 : depth[par] + 1;
+            for (int i = ab; i < ae; ++i) node_of[i] = k;
+        }
+        if (expect != na) return fail(c, UD_ERR_INVALID, tag + "node atom ranges do not cover all atoms");
+        // subtree atom end: pre-order subtrees are contiguous
+        for (int k = nn - 1; k >= 0; --k) {
+            int e = b->node_aend[k0 + k];
+            for (int j = k + 1; j < nn; ++j) {
+                // descendants of k are the following nodes until depth returns to <= depth[k]
+                if (depth[j] <= depth[k]) break;
+                e = std::max(e, b->node_aend[k0 + j]);
+            }
+            sub_end[k] = e;
+        }
+        int maxd = 0;
+        for (int k = 0; k < nn; ++k) maxd = std::max(maxd, depth[k]);
+        // check pre-order consistency: parent of j must lie on the path
+        for (int k = 1; k < nn; ++k) {
+            const int par = b->node_parent[k0 + k];
+            if (depth[k] != depth[par] + 1) return fail(c, UD_ERR_INVALID, tag + "inconsistent tree depth");
+        }
+        for (int k = 0; k < nn; ++k) {
+            NodeRec r;
+            const int par = b->node_parent[k0 + k];
+            for (int d = 0; d < 3; ++d) {
+                // segment c'tor (tree.h:123-131): relative origin/axis at the identity parent orientation
+                r.ro[d] = (k == 0) ? 0.f : b->node_origin[3 * (k0 + k) + d] - b->node_origin[3 * (k0 + par) + d];
+                r.ax[d] = (k == 0) ? 0.f : b->node_axis[3 * (k0 + k) + d];
+            }
+            if (k > 0) {
+                const float nrm = std::sqrt(r.ax[0] * r.ax[0] + r.ax[1] * r.ax[1] + r.ax[2] * r.ax[2]);
+                if (!(std::fabs(nrm - 1.f) < 0.005f)) return fail(c, UD_ERR_INVALID, tag + "node axis is not a unit vector");
+            }
+            r.parent = (k == 0) ? 0 : par;
+            r.packed = uint32_t(b->node_abegin[k0 + k]) | (uint32_t(b->node_aend[k0 + k]) << 8) |
+                       (uint32_t(sub_end[k]) << 16) | (uint32_t(depth[k]) << 24);
+            nodes.push_back(r);
+        }
+        int n_heavy = 0;
+        for (int i = 0; i < na; ++i) {
+            const int t = b->atom_xs[a0 + i];
+            if (t < -1 || t >= 36) return fail(c, UD_ERR_INVALID, tag + "atom type out of range");
+            xs[l].push_back(t);
+            AtomRec a;
+            a.x = b->atom_local[3 * (a0 + i)];
+            a.y = b->atom_local[3 * (a0 + i) + 1];
+            a.z = b->atom_local[3 * (a0 + i) + 2];
+            const int heavy = b->atom_heavy[a0 + i] ? 1 : 0;
+            n_heavy += heavy;
+            a.meta = uint32_t(node_of[i]) | (0xFFu << 8) | (uint32_t(heavy) << 16) | (uint32_t(t < 0 ? 0xFF : t) << 24);
+            atoms.push_back(a);
+        }
+        for (int q = 0; q < np; ++q) {
+            const int pa = b->pairs[2 * (p0 + q)], pb = b->pairs[2 * (p0 + q) + 1];
+            if (pa < 0 || pb < 0 || pa >= na || pb >= na || pa == pb) return fail(c, UD_ERR_INVALID, tag + "pair index out of range");
+            const int ta = b->atom_xs[a0 + pa], tb = b->atom_xs[a0 + pb];
+            if (ta < 0 || tb < 0) return fail(c, UD_ERR_INVALID, tag + "pair with an untyped atom");
+            pairs.push_back(uint32_t(pa) | (uint32_t(pb) << 8) | (uint32_t(tri_index(ta, tb)) << 16));
+        }
+        LigHdr& h = hdr[l];
+        std::memset(&h, 0, sizeof(h));
+        h.atom_begin = a0;
+        h.n_atoms = na;
+        h.node_begin = k0;
+        h.n_nodes = nn;
+        h.pair_begin = p0;
+        h.n_pairs = np;
+        h.max_depth = maxd;
+        h.local_steps = (25 + na) / 3;  // vina.h:197
+        h.num_tors = b->num_tors[l];
+        h.n_heavy = n_heavy;
+        nat[l] = na;
+        nnd[l] = nn;
+    }
+    // commit
+    for (void* p : {static_cast<void*>(c->d_hdr), static_cast<void*>(c->d_nodes), static_cast<void*>(c->d_pairs)})
+        if (p) (void)hipFree(p);
+    c->d_hdr = nullptr;
+    c->d_nodes = nullptr;
+    c->d_pairs = nullptr;
+    size_t cap = 0;
+    int rc = grow(c, c->d_hdr, cap, hdr.size());
+    if (rc) return rc;
+    cap = 0;
+    rc = grow(c, c->d_nodes, cap, nodes.size());
+    if (rc) return rc;
+    cap = 0;
+    rc = grow(c, c->d_pairs, cap, std::max<size_t>(pairs.size(), 1));
+    if (rc) return rc;
+    UD_HIP(c, hipMemcpy(c->d_hdr, hdr.data(), hdr.size() * sizeof(LigHdr), hipMemcpyHostToDevice));
+    UD_HIP(c, hipMemcpy(c->d_nodes, nodes.data(), nodes.size() * sizeof(NodeRec), hipMemcpyHostToDevice));
+    if (!pairs.empty()) UD_HIP(c, hipMemcpy(c->d_pairs, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
+    c->hdr = hdr;
+    c->n_lig = L;
+    c->lig_nat = nat;
+    c->lig_nn = nnd;
+    c->lig_xs = xs;
+    // atoms carry grid slots; they are (re)written before each evaluation
+    rc = upload_atoms(c, atoms);
+    if (rc) return rc;
+    c->h_atoms = atoms;
+    c->slots_dirty = true;
+    return UD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// refresh the grid slots of the uploaded atoms for the current maps
+int refresh_slots(ud_ctx* c) {
+    int rc = ready_for_eval(c);
+    if (rc) return rc;
+    if (!c->slots_dirty) return UD_OK;
+    std::vector<AtomRec>& atoms = c->h_atoms;
+    const size_t total = atoms.size();
+    bool changed = false;
+    for (auto& a : atoms) {
+        const uint32_t t = a.meta >> 24;
+        const int gt = (t == 0xFF) ? -1 : grid_type_of(int(t));
+        const uint32_t slot = (gt < 0) ? 0xFFu : uint32_t(c->slot_of_type[gt]);
+        const uint32_t meta = (a.meta & ~(0xFFu << 8)) | (slot << 8);
+        if (meta != a.meta) { a.meta = meta; changed = true; }
+    }
+    if (changed) UD_HIP(c, hipMemcpy(c->d_atoms, atoms.data(), total * sizeof(AtomRec), hipMemcpyHostToDevice));
+    c->slots_dirty = false;
+    return UD_OK;
+}
+
+int pose_upload(ud_ctx* c, int n_poses, const float* poses) {
+    int rc = grow(c, c->d_poses, c->poses_cap, size_t(n_poses) * UD_POSE_FLOATS);
+    if (rc) return rc;
+    UD_HIP(c, hipMemcpyAsync(c->d_poses, poses, sizeof(float) * n_poses * UD_POSE_FLOATS, hipMemcpyHostToDevice, c->stream));
+    return UD_OK;
+}
+
+int work_upload(ud_ctx* c, const std::vector<int32_t>& perm, const std::vector<WorkItem>& work) {
+    int rc = grow(c, c->d_perm, c->perm_cap, std::max<size_t>(perm.size(), 1));
+    if (rc) return rc;
+    rc = grow(c, c->d_work, c->work_cap, std::max<size_t>(work.size(), 1));
+    if (rc) return rc;
+    if (!perm.empty()) UD_HIP(c, hipMemcpyAsync(c->d_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (!work.empty()) UD_HIP(c, hipMemcpyAsync(c->d_work, work.data(), work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, c->stream));
+    return UD_OK;
+}
+
+// default lanes per chain by atom capacity (replaced by autotuned values)
+int default_lanes(int na) {
+    if (na <= 16) return 16;
+    if (na <= 48) return 32;
+    return 64;
+}
+}  // namespace
+
+extern "C" {
+
+int ud_score_grad_device(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const float* d_poses, const float* v,
+                         float* d_energy, float* d_grad) {
+    if (!c || !v) return UD_ERR_INVALID;
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc) return rc;
+    if (n_poses == 0) return UD_OK;
+    UD_HIP(c, hipSetDevice(c->device));
+    rc = refresh_slots(c);
+    if (rc) return rc;
+    int na, nn;
+    caps_all(c, na, nn);
+    const int lanes = fit_lanes(16, na, nn);
+    std::vector<int32_t> perm;
+    std::vector<WorkItem> work;
+    make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
+    rc = work_upload(c, perm, work);
+    if (rc) return rc;
+    UD_HIP(c, hipEventRecord(c->ev0, c->stream));
+    UD_HIP(c, launch_score(lanes, c->precision, lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm,
+                           d_poses, v[0], v[1], d_energy, d_grad, c->stream));
+    UD_HIP(c, hipEventRecord(c->ev1, c->stream));
+    c->last_launches = 1;
+    return UD_OK;
+}
+
+int ud_score_grad(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const float* poses, const float* v,
+                  float* energy_out, float* grad_out) {
+    if (!c || !v || (n_poses > 0 && (!poses || !energy_out))) return UD_ERR_INVALID;
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc || n_poses == 0) return rc;
+    UD_HIP(c, hipSetDevice(c->device));
+    rc = pose_upload(c, n_poses, poses);
+    if (rc) return rc;
+    rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * UD_GRAD_FLOATS);
+    if (rc) return rc;
+    rc = grow(c, c->d_f2, c->f2_cap, size_t(n_poses));
+    if (rc) return rc;
+    float* d_e = c->d_f2;
+    float* d_g = c->d_f1;
+    rc = ud_score_grad_device(c, n_poses, pose_ligand, c->d_poses, v, d_e, d_g);
+    if (rc) return rc;
+    UD_HIP(c, hipMemcpyAsync(energy_out, d_e, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    if (grad_out)
+        UD_HIP(c, hipMemcpyAsync(grad_out, d_g, sizeof(float) * size_t(n_poses) * UD_GRAD_FLOATS, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    return UD_OK;
+}
+
+int ud_optimize(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses, const float* v, int max_steps,
+                float* energy_out, int32_t* evals_out) {
+    if (!c || !v || (n_poses > 0 && !poses)) return UD_ERR_INVALID;
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc || n_poses == 0) return rc;
+    UD_HIP(c, hipSetDevice(c->device));
+    rc = refresh_slots(c);
+    if (rc) return rc;
+    rc = pose_upload(c, n_poses, poses);
+    if (rc) return rc;
+    rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * UD_GRAD_FLOATS);
+    if (rc) return rc;
+    rc = grow(c, c->d_i1, c->i_cap, size_t(n_poses));
+    if (rc) return rc;
+    int na, nn;
+    caps_all(c, na, nn);
+    const int lanes = fit_lanes(default_lanes(na), na, nn);
+    std::vector<int32_t> perm;
+    std::vector<WorkItem> work;
+    make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
+    rc = work_upload(c, perm, work);
+    if (rc) return rc;
+    UD_HIP(c, hipEventRecord(c->ev0, c->stream));
+    UD_HIP(c, launch_optimize(lanes, c->precision, lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm,
+                              c->d_poses, v[0], v[1], max_steps, c->d_f1, c->d_i1, c->stream));
+    UD_HIP(c, hipEventRecord(c->ev1, c->stream));
+    UD_HIP(c, hipMemcpyAsync(poses, c->d_poses, sizeof(float) * size_t(n_poses) * UD_POSE_FLOATS, hipMemcpyDeviceToHost, c->stream));
+    if (energy_out) UD_HIP(c, hipMemcpyAsync(energy_out, c->d_f1, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    if (evals_out) UD_HIP(c, hipMemcpyAsync(evals_out, c->d_i1, sizeof(int32_t) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    c->last_launches = 1;
+    return UD_OK;
+}
+
+int ud_dock(ud_ctx* c, const ud_search_params* p, float* best_poses, float* best_e, int64_t* evals) {
+    if (!c || !p) return UD_ERR_INVALID;
+    if (p->exhaustiveness < 1) return fail(c, UD_ERR_INVALID, "exhaustiveness must be 1 or greater");
+    if (p->global_steps < 0) return fail(c, UD_ERR_INVALID, "global_steps must be >= 0");
+    if (p->lanes_per_chain != 0 && p->lanes_per_chain != 8 && p->lanes_per_chain != 16 && p->lanes_per_chain != 32 &&
+        p->lanes_per_chain != 64)
+        return fail(c, UD_ERR_INVALID, "lanes_per_chain must be 0, 8, 16, 32 or 64");
+    UD_HIP(c, hipSetDevice(c->device));
+    int rc = refresh_slots(c);
+    if (rc) return rc;
+    const int exh = p->exhaustiveness;
+    const size_t nch = size_t(c->n_lig) * exh;
+    if (nch > size_t(1) << 31) return fail(c, UD_ERR_CAPACITY, "too many chains");
+    if (nch > c->out_cap) {
+        for (void* q : {static_cast<void*>(c->d_out_pose), static_cast<void*>(c->d_out_e), static_cast<void*>(c->d_out_evals)})
+            if (q) (void)hipFree(q);
+        c->d_out_pose = nullptr;
+        c->d_out_e = nullptr;
+        c->d_out_evals = nullptr;
+        if (hipMalloc(&c->d_out_pose, nch * UD_POSE_FLOATS * sizeof(float)) != hipSuccess ||
+            hipMalloc(&c->d_out_e, nch * sizeof(float)) != hipSuccess ||
+            hipMalloc(&c->d_out_evals, nch * sizeof(long long)) != hipSuccess) {
+            c->out_cap = 0;
+            return fail(c, UD_ERR_NOMEM, "result allocation failed");
+        }
+        c->out_cap = nch;
+    }
+    c->out_n = nch;
+    SearchDev sp;
+    std::memset(&sp, 0, sizeof(sp));
+    sp.exh = exh;
+    sp.global_steps = p->global_steps;
+    sp.local_steps_override = p->local_steps;
+    sp.seed_lo = uint32_t(p->seed);
+    sp.seed_hi = uint32_t(p->seed >> 32);
+    for (int i = 0; i < 3; ++i) {
+        sp.hunt[i] = p->hunt_cap[i];
+        sp.authentic[i] = p->authentic_v[i];
+        // random_in_box corners: the grid's first and last sample points
+        sp.box_lo[i] = c->init[i];
+        sp.box_hi[i] = c->init[i] + c->factor_inv[i] * float(c->n[i] - 1);
+    }
+    sp.temperature = p->temperature;
+    sp.amplitude = p->mutation_amplitude;
+    // size buckets by atom capacity; each bucket is one launch, largest first
+    const int bucket_caps[] = {16, 32, 48, 64, 96, 128};
+    std::vector<std::vector<int>> bl(6);
+    for (int l = 0; l < c->n_lig; ++l) {
+        int bi = 0;
+        while (bucket_caps[bi] < c->lig_nat[l]) ++bi;
+        bl[bi].push_back(l);
+    }
+    std::vector<std::vector<WorkItem>> works;
+    std::vector<int> lanes_v, na_v, nn_v;
+    size_t total_work = 0;
+    for (int bi = 5; bi >= 0; --bi) {
+        if (bl[bi].empty()) continue;
+        int na = 1, nn = 1;
+        for (int l : bl[bi]) {
+            na = std::max(na, c->lig_nat[l]);
+            nn = std::max(nn, c->lig_nn[l]);
+        }
+        int lanes = p->lanes_per_chain ? p->lanes_per_chain : (c->tuned_lanes[bi] ? c->tuned_lanes[bi] : default_lanes(na));
+        lanes = fit_lanes(lanes, na, nn);
+        if (lds_bytes(lanes, na, nn) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
+        const int G = 64 / lanes;
+        // larger ligands first inside the bucket (tail balance)
+        std::vector<int> order = bl[bi];
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b2) {
+            return c->lig_nat[a] * c->lig_nn[a] > c->lig_nat[b2] * c->lig_nn[b2];
+        });
+        std::vector<WorkItem> w;
+        for (int l : order)
+            for (int c0 = 0; c0 < exh; c0 += G) w.push_back(WorkItem{l, l * exh + c0, std::min(G, exh - c0), 0});
+        total_work += w.size();
+        works.push_back(w);
+        lanes_v.push_back(lanes);
+        na_v.push_back(na);
+        nn_v.push_back(nn);
+    }
+    rc = grow(c, c->d_work, c->work_cap, total_work);
+    if (rc) return rc;
+    std::vector<WorkItem> all;
+    for (auto& w : works) all.insert(all.end(), w.begin(), w.end());
+    UD_HIP(c, hipMemcpyAsync(c->d_work, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice, c->stream));
+    UD_HIP(c, hipEventRecord(c->ev0, c->stream));
+    size_t off = 0;
+    for (size_t k = 0; k < works.size(); ++k) {
+        UD_HIP(c, launch_dock(lanes_v[k], c->precision, lig_dev(c), grid_dev(c), na_v[k], nn_v[k], sp, c->d_work + off,
+                              int(works[k].size()), c->d_out_pose, c->d_out_e, c->d_out_evals, c->stream));
+        off += works[k].size();
+    }
+    UD_HIP(c, hipEventRecord(c->ev1, c->stream));
+    c->last_launches = int(works.size());
+    if (best_poses || best_e || evals) return ud_fetch_dock_results(c, best_poses, best_e, evals);
+    return UD_OK;
+}
+
+int ud_fetch_dock_results(ud_ctx* c, float* best_poses, float* best_e, int64_t* evals) {
+    if (!c) return UD_ERR_INVALID;
+    if (c->out_n == 0) return fail(c, UD_ERR_STATE, "no docking results");
+    UD_HIP(c, hipSetDevice(c->device));
+    const size_t n = c->out_n;
+    if (best_poses) UD_HIP(c, hipMemcpyAsync(best_poses, c->d_out_pose, n * UD_POSE_FLOATS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (best_e) UD_HIP(c, hipMemcpyAsync(best_e, c->d_out_e, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (evals) UD_HIP(c, hipMemcpyAsync(evals, c->d_out_evals, n * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    return UD_OK;
+}
+
+int ud_score_terms(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const float* poses, float* inter_out,
+                   float* intra_out) {
+    if (!c || (n_poses > 0 && (!poses || !inter_out || !intra_out))) return UD_ERR_INVALID;
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc || n_poses == 0) return rc;
+    UD_HIP(c, hipSetDevice(c->device));
+    rc = refresh_slots(c);
+    if (rc) return rc;
+    rc = pose_upload(c, n_poses, poses);
+    if (rc) return rc;
+    rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * 2);
+    if (rc) return rc;
+    int na, nn;
+    caps_all(c, na, nn);
+    if (lds_bytes(16, na, nn) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
+    std::vector<int32_t> perm;
+    std::vector<WorkItem> work;
+    make_pose_work(n_poses, pose_ligand, 4, perm, work);
+    rc = work_upload(c, perm, work);
+    if (rc) return rc;
+    UD_HIP(c, launch_terms(c->precision, lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm,
+                           c->d_poses, 1000.f, c->d_f1, c->d_f1 + n_poses, c->stream));
+    UD_HIP(c, hipMemcpyAsync(inter_out, c->d_f1, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipMemcpyAsync(intra_out, c->d_f1 + n_poses, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    return UD_OK;
+}
+
+int ud_pose_coords(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const float* poses, float* coords_out) {
+    if (!c || (n_poses > 0 && (!poses || !coords_out))) return UD_ERR_INVALID;
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc || n_poses == 0) return rc;
+    if (c->n_lig == 0) return fail(c, UD_ERR_STATE, "no ligands set");
+    UD_HIP(c, hipSetDevice(c->device));
+    rc = pose_upload(c, n_poses, poses);
+    if (rc) return rc;
+    rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * UD_MAX_ATOMS * 3);
+    if (rc) return rc;
+    int na, nn;
+    caps_all(c, na, nn);
+    if (lds_bytes(16, na, nn) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
+    std::vector<int32_t> perm;
+    std::vector<WorkItem> work;
+    make_pose_work(n_poses, pose_ligand, 4, perm, work);
+    rc = work_upload(c, perm, work);
+    if (rc) return rc;
+    UD_HIP(c, launch_coords(lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm, c->d_poses, c->d_f1,
+                            c->stream));
+    UD_HIP(c, hipMemcpyAsync(coords_out, c->d_f1, sizeof(float) * size_t(n_poses) * UD_MAX_ATOMS * 3, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    return UD_OK;
+}
+
+int ud_last_kernel_time(const ud_ctx* c, double* ms, int32_t* launches) {
+    if (!c) return UD_ERR_INVALID;
+    if (ms) *ms = c->last_ms;
+    if (launches) *launches = c->last_launches;
+    return UD_OK;
+}
+
+}  // extern "C"

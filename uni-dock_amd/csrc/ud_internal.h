@@ -1,0 +1,103 @@
+// Internal device-side data layout shared by the kernels and the host side
+// of the library (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/unidock_amd.h"
+
+namespace ud {
+
+// Per-ligand header in HBM.
+struct LigHdr {
+    int32_t atom_begin;   // into the atom array
+    int32_t n_atoms;
+    int32_t node_begin;   // into the node array
+    int32_t n_nodes;      // torsions = n_nodes - 1
+    int32_t pair_begin;   // into the pair array
+    int32_t n_pairs;
+    int32_t max_depth;    // deepest node level (root = 0)
+    int32_t local_steps;  // BFGS iterations, (25 + n_atoms) / 3 (vina.h:197)
+    float num_tors;       // conf_independent torsion count
+    int32_t n_heavy;
+    int32_t pad[2];
+};
+static_assert(sizeof(LigHdr) == 48, "LigHdr layout");
+
+// Atom record (16 B): local coordinates + packed metadata
+//   bits 0..7   owning node
+//   bits 8..15  grid map slot (0xFF = no grid term)
+//   bit  16     heavy atom
+//   bits 24..31 XS type (0xFF = untyped)
+struct AtomRec {
+    float x, y, z;
+    uint32_t meta;
+};
+
+// Node record (32 B): relative origin / axis in the parent frame
+// (segment::relative_origin / relative_axis, tree.h:121-131)
+//   packed bits 0..7 atom begin, 8..15 atom end, 16..23 subtree atom end,
+//   24..31 depth
+struct NodeRec {
+    float ro[3];
+    int32_t parent;
+    float ax[3];
+    uint32_t packed;
+};
+
+// Pair record: a | b << 8 | table slot << 16
+typedef uint32_t PairRec;
+
+// Pair-table entry for the derivative path: e_k, dor_k, e_{k+1} - e_k,
+// dor_{k+1} - dor_k (precalculate_element::eval_deriv, precalculate.h:48-65)
+static constexpr int kTableN = 2051;     // sz(32 * 64) + 3 (precalculate.h:142-145)
+// entries 0..2047 are reachable with r^2 < 64 (2048 is kept for r^2 == 64 in map builds)
+
+struct GridDev {
+    const void* maps;      // n_maps * n_points values (float or __half bits)
+    float init[3];
+    float factor[3];
+    float factor_inv[3];
+    float dimm1[3];
+    int32_t n[3];
+    int32_t n_points;
+    float slope;
+    float scale[UD_MAX_MAPS];  // FP16 maps: value = half * scale
+};
+
+struct LigandSetDev {
+    const LigHdr* hdr;
+    const AtomRec* atoms;
+    const NodeRec* nodes;
+    const PairRec* pairs;
+    const float4* table_deriv;  // slot * kTableUsed
+    const float* table_fast;    // slot * kTableUsed
+};
+
+// Work item: one ligand and a run of consecutive poses/chains of it.
+struct WorkItem {
+    int32_t lig;
+    int32_t first;   // first pose / chain index (global)
+    int32_t count;   // <= groups per workgroup
+    int32_t pad;
+};
+
+struct SearchDev {
+    int32_t exh;
+    int32_t global_steps;
+    int32_t local_steps_override;  // > 0 overrides the per-ligand value
+    uint32_t seed_lo, seed_hi;
+    float hunt[3];
+    float authentic[3];
+    float temperature;
+    float amplitude;
+    float box_lo[3], box_hi[3];
+};
+
+// Capacity of one launch (dynamic LDS is sized from it)
+struct LaunchCap {
+    int32_t n_atoms;  // max atoms
+    int32_t n_nodes;  // max nodes
+};
+
+}  // namespace ud

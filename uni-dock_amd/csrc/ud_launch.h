@@ -1,0 +1,44 @@
+// Host-side launch wrappers of the kernels in ud_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ud_internal.h"
+
+namespace ud {
+
+static constexpr int kTableStride = 2049;  // table entries stored per type pair
+static constexpr int kTypePairs = 36 * 37 / 2;
+
+struct GridBuildArgs {
+    float init[3];
+    float factor_inv[3];
+    int32_t n[3];
+    int32_t n_points;
+    int32_t n_maps;
+    int32_t types[UD_MAX_MAPS];
+};
+
+size_t lds_bytes(int lanes, int na_cap, int nn_cap);
+
+hipError_t launch_tables(int sf, const float* w, float4* deriv, float* fast, hipStream_t st);
+hipError_t launch_grid_build(const float4* rec, int n_rec, const GridBuildArgs& a, const float* table_fast,
+                             float* maps_out, hipStream_t st);
+hipError_t launch_grid_absmax(const float* maps, int n_points, int n_maps, unsigned int* out, hipStream_t st);
+hipError_t launch_grid_to_half(const float* maps, int n_points, int n_maps, const float* inv_scale, void* out,
+                               hipStream_t st);
+hipError_t launch_score(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+                        const WorkItem* work, int n_work, const int32_t* perm, const float* poses, float v_intra,
+                        float v_inter, float* e_out, float* g_out, hipStream_t st);
+hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+                           const WorkItem* work, int n_work, const int32_t* perm, float* poses, float v_intra,
+                           float v_inter, int max_steps, float* e_out, int32_t* evals_out, hipStream_t st);
+hipError_t launch_dock(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+                       const SearchDev& sp, const WorkItem* work, int n_work, float* out_pose, float* out_e,
+                       long long* out_evals, hipStream_t st);
+hipError_t launch_terms(int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap, const WorkItem* work,
+                        int n_work, const int32_t* perm, const float* poses, float v, float* inter_out,
+                        float* intra_out, hipStream_t st);
+hipError_t launch_coords(const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap, const WorkItem* work,
+                         int n_work, const int32_t* perm, const float* poses, float* coords_out, hipStream_t st);
+
+}  // namespace ud

@@ -1,0 +1,264 @@
+"""ctypes binding of the C ABI (include/unidock_amd.h).
+
+The binding loads the in-tree libunidock_amd.so and raises if it is missing:
+there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import _build
+
+UD_OK, UD_ERR_INVALID, UD_ERR_CAPACITY, UD_ERR_HIP, UD_ERR_STATE, UD_ERR_NOMEM = range(6)
+UD_SF_VINA, UD_SF_VINARDO = 0, 1
+UD_GRID_F32, UD_GRID_F16 = 0, 1
+UD_MAX_ATOMS = 128
+UD_MAX_TORSIONS = 32
+UD_POSE_FLOATS = 7 + UD_MAX_TORSIONS
+UD_GRAD_FLOATS = 6 + UD_MAX_TORSIONS
+
+VINA_WEIGHTS = (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 50.0, 0.05846)  # vina.h:129-132
+VINARDO_WEIGHTS = (-0.045, 0.8, -0.035, -0.600, 50.0, 0.05846)                      # vina.h:133-135
+
+_c_int_p = ctypes.POINTER(ctypes.c_int32)
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class UdLigandBatch(ctypes.Structure):
+    _fields_ = [("n_ligands", ctypes.c_int32),
+                ("atom_offset", _c_int_p), ("node_offset", _c_int_p), ("pair_offset", _c_int_p),
+                ("atom_local", _c_float_p), ("atom_xs", _c_int_p), ("atom_heavy", _c_int_p),
+                ("node_parent", _c_int_p), ("node_abegin", _c_int_p), ("node_aend", _c_int_p),
+                ("node_origin", _c_float_p), ("node_axis", _c_float_p),
+                ("pairs", _c_int_p), ("num_tors", _c_float_p)]
+
+
+class UdSearchParams(ctypes.Structure):
+    _fields_ = [("exhaustiveness", ctypes.c_int32), ("global_steps", ctypes.c_int32),
+                ("local_steps", ctypes.c_int32), ("lanes_per_chain", ctypes.c_int32),
+                ("seed", ctypes.c_uint64),
+                ("hunt_cap", ctypes.c_float * 3), ("authentic_v", ctypes.c_float * 3),
+                ("temperature", ctypes.c_float), ("mutation_amplitude", ctypes.c_float)]
+
+
+_LIB = None
+
+
+def lib_path() -> str:
+    return _build.LIB
+
+
+def load() -> ctypes.CDLL:
+    """Load libunidock_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() "
+                           "(no CPU fallback exists)")
+    L = ctypes.CDLL(path)
+    vp, ip, fp, dp = ctypes.c_void_p, _c_int_p, _c_float_p, ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "ud_default_search_params": (None, [ctypes.POINTER(UdSearchParams)]),
+        "ud_create": (ctypes.c_int, [ctypes.c_int, vp, ctypes.POINTER(vp)]),
+        "ud_destroy": (ctypes.c_int, [vp]),
+        "ud_last_error": (ctypes.c_char_p, [vp]),
+        "ud_set_scoring": (ctypes.c_int, [vp, ctypes.c_int, dp, ctypes.c_int]),
+        "ud_set_box": (ctypes.c_int, [vp, dp, dp, ctypes.c_double, ip]),
+        "ud_compute_maps": (ctypes.c_int, [vp, ctypes.c_int, fp, ip, ctypes.c_int, ip, ctypes.c_int]),
+        "ud_set_maps": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, ctypes.c_int]),
+        "ud_get_map": (ctypes.c_int, [vp, ctypes.c_int, fp]),
+        "ud_set_ligands": (ctypes.c_int, [vp, ctypes.POINTER(UdLigandBatch)]),
+        "ud_score_grad": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, fp, fp]),
+        "ud_score_grad_device": (ctypes.c_int, [vp, ctypes.c_int, ip, vp, fp, vp, vp]),
+        "ud_optimize": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, ctypes.c_int, fp, ip]),
+        "ud_dock": (ctypes.c_int, [vp, ctypes.POINTER(UdSearchParams), fp, fp, ctypes.POINTER(ctypes.c_int64)]),
+        "ud_fetch_dock_results": (ctypes.c_int, [vp, fp, fp, ctypes.POINTER(ctypes.c_int64)]),
+        "ud_score_terms": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, fp]),
+        "ud_pose_coords": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp]),
+        "ud_last_kernel_time": (ctypes.c_int, [vp, dp, ip]),
+        "ud_synth_receptor": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                             ctypes.c_longlong, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class UdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"unidock_amd error {code}: {msg}")
+        self.code = code
+
+
+def synth_receptor_xyz(n_atoms: int, edge: float, min_dist: float, seed: int) -> np.ndarray:
+    L = load()
+    out = np.zeros(n_atoms * 3, dtype=np.float64)
+    rc = L.ud_synth_receptor(n_atoms, edge, min_dist, seed, 10**8, _p(out, ctypes.c_double))
+    if rc != UD_OK:
+        raise UdError(rc, "receptor sampler failed (density too high?)")
+    return out.reshape(-1, 3)
+
+
+class Context:
+    """One device context (ud_ctx) on a HIP stream."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self.L = load()
+        h = ctypes.c_void_p()
+        rc = self.L.ud_create(device, ctypes.c_void_p(stream or 0), ctypes.byref(h))
+        if rc != UD_OK:
+            raise UdError(rc, "ud_create failed (no HIP device?)")
+        self.h = h
+        self.n_ligands = 0
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            self.L.ud_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        if rc != UD_OK:
+            raise UdError(rc, self.L.ud_last_error(self.h).decode())
+
+    # ---- set-up
+    def set_scoring(self, sf: int = UD_SF_VINA, weights=None):
+        w = np.array(weights if weights is not None else (VINA_WEIGHTS if sf == UD_SF_VINA else VINARDO_WEIGHTS),
+                     dtype=np.float64)
+        self._ck(self.L.ud_set_scoring(self.h, sf, _p(w, ctypes.c_double), len(w)))
+
+    def set_box(self, center, size, granularity=0.375):
+        c = np.array(center, dtype=np.float64)
+        s = np.array(size, dtype=np.float64)
+        n = np.zeros(3, dtype=np.int32)
+        self._ck(self.L.ud_set_box(self.h, _p(c, ctypes.c_double), _p(s, ctypes.c_double), granularity,
+                                   _p(n, ctypes.c_int32)))
+        self.npts = tuple(int(x) for x in n)
+        return self.npts
+
+    def compute_maps(self, rec_xyz, rec_xs, types, precision=UD_GRID_F32):
+        x = np.ascontiguousarray(rec_xyz, dtype=np.float32).reshape(-1)
+        t = np.ascontiguousarray(rec_xs, dtype=np.int32)
+        ty = np.ascontiguousarray(types, dtype=np.int32)
+        self._ck(self.L.ud_compute_maps(self.h, len(t), _p(x, ctypes.c_float), _p(t, ctypes.c_int32), len(ty),
+                                        _p(ty, ctypes.c_int32), precision))
+        self.map_types = [int(v) for v in ty]
+
+    def set_maps(self, types, values, precision=UD_GRID_F32):
+        ty = np.ascontiguousarray(types, dtype=np.int32)
+        v = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        self._ck(self.L.ud_set_maps(self.h, len(ty), _p(ty, ctypes.c_int32), _p(v, ctypes.c_float), precision))
+        self.map_types = [int(x) for x in ty]
+
+    def get_map(self, xs_type: int) -> np.ndarray:
+        out = np.zeros(int(np.prod(self.npts)), dtype=np.float32)
+        self._ck(self.L.ud_get_map(self.h, xs_type, _p(out, ctypes.c_float)))
+        return out
+
+    def set_ligands(self, arrays: dict):
+        a = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in arrays.items()}
+        b = UdLigandBatch()
+        b.n_ligands = a["n_ligands"]
+        for k in ("atom_offset", "node_offset", "pair_offset", "atom_xs", "atom_heavy", "node_parent",
+                  "node_abegin", "node_aend", "pairs"):
+            a[k] = np.ascontiguousarray(a[k], dtype=np.int32).reshape(-1)
+            setattr(b, k, _p(a[k], ctypes.c_int32))
+        for k in ("atom_local", "node_origin", "node_axis", "num_tors"):
+            a[k] = np.ascontiguousarray(a[k], dtype=np.float32).reshape(-1)
+            setattr(b, k, _p(a[k], ctypes.c_float))
+        self._ck(self.L.ud_set_ligands(self.h, ctypes.byref(b)))
+        self._keep = a
+        self.n_ligands = a["n_ligands"]
+        self.n_tors = np.diff(a["node_offset"]) - 1
+        self.n_atoms = np.diff(a["atom_offset"])
+
+    # ---- compute
+    def score_grad(self, pose_ligand, poses, v=(1000.0, 1000.0, 1000.0)):
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        P = np.ascontiguousarray(poses, dtype=np.float32).reshape(len(pl), UD_POSE_FLOATS)
+        vv = np.array(v, dtype=np.float32)
+        e = np.zeros(len(pl), dtype=np.float32)
+        g = np.zeros((len(pl), UD_GRAD_FLOATS), dtype=np.float32)
+        self._ck(self.L.ud_score_grad(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float),
+                                      _p(vv, ctypes.c_float), _p(e, ctypes.c_float), _p(g, ctypes.c_float)))
+        return e, g
+
+    def score_grad_device(self, pose_ligand, d_poses: int, d_energy: int, d_grad: int, v=(1000.0, 1000.0, 1000.0)):
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        vv = np.array(v, dtype=np.float32)
+        self._ck(self.L.ud_score_grad_device(self.h, len(pl), _p(pl, ctypes.c_int32), ctypes.c_void_p(d_poses),
+                                             _p(vv, ctypes.c_float), ctypes.c_void_p(d_energy),
+                                             ctypes.c_void_p(d_grad)))
+
+    def optimize(self, pose_ligand, poses, v=(1000.0, 1000.0, 1000.0), max_steps=0):
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        P = np.array(poses, dtype=np.float32).reshape(len(pl), UD_POSE_FLOATS).copy()
+        vv = np.array(v, dtype=np.float32)
+        e = np.zeros(len(pl), dtype=np.float32)
+        ev = np.zeros(len(pl), dtype=np.int32)
+        self._ck(self.L.ud_optimize(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float),
+                                    _p(vv, ctypes.c_float), max_steps, _p(e, ctypes.c_float), _p(ev, ctypes.c_int32)))
+        return P, e, ev
+
+    def dock(self, exhaustiveness=8, global_steps=20, seed=181129, local_steps=0, lanes=0, fetch=True):
+        p = UdSearchParams()
+        self.L.ud_default_search_params(ctypes.byref(p))
+        p.exhaustiveness = exhaustiveness
+        p.global_steps = global_steps
+        p.seed = seed
+        p.local_steps = local_steps
+        p.lanes_per_chain = lanes
+        self._ck(self.L.ud_dock(self.h, ctypes.byref(p), None, None, None))
+        self._last_dock = (self.n_ligands, exhaustiveness)
+        if fetch:
+            return self.fetch_dock()
+
+    def fetch_dock(self):
+        n = self._last_dock[0] * self._last_dock[1]
+        P = np.zeros((n, UD_POSE_FLOATS), dtype=np.float32)
+        e = np.zeros(n, dtype=np.float32)
+        ev = np.zeros(n, dtype=np.int64)
+        self._ck(self.L.ud_fetch_dock_results(self.h, _p(P, ctypes.c_float), _p(e, ctypes.c_float),
+                                              _p(ev, ctypes.c_int64)))
+        return P, e, ev
+
+    def score_terms(self, pose_ligand, poses):
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        P = np.ascontiguousarray(poses, dtype=np.float32).reshape(len(pl), UD_POSE_FLOATS)
+        a = np.zeros(len(pl), dtype=np.float32)
+        b = np.zeros(len(pl), dtype=np.float32)
+        self._ck(self.L.ud_score_terms(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float),
+                                       _p(a, ctypes.c_float), _p(b, ctypes.c_float)))
+        return a, b
+
+    def pose_coords(self, pose_ligand, poses):
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        P = np.ascontiguousarray(poses, dtype=np.float32).reshape(len(pl), UD_POSE_FLOATS)
+        out = np.zeros((len(pl), UD_MAX_ATOMS, 3), dtype=np.float32)
+        self._ck(self.L.ud_pose_coords(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float),
+                                       _p(out, ctypes.c_float)))
+        return out
+
+    def last_kernel_time(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int32()
+        self._ck(self.L.ud_last_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
