@@ -192,6 +192,12 @@ int ud_score_terms(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const f
  * ligand order. */
 int ud_pose_coords(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* poses, float* coords_out);
 
+/* Lanes per pose for ud_score_grad / ud_optimize / ud_score_terms /
+ * ud_pose_coords on this context: 8, 16, 32 or 64; 0 restores the automatic
+ * choice.  (The reference fixes 32 threads per pose, monte_carlo.cu:117-118;
+ * here a wavefront of 64 lanes holds 64 / lanes poses.) */
+int ud_set_lanes(ud_ctx* ctx, int lanes);
+
 /* Kernel timing of the last ud_dock / ud_score_grad call on the context's
  * stream (HIP events around the launches): total milliseconds and number of
  * launches. */

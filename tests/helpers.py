@@ -30,10 +30,16 @@ def small_receptor(n_atoms=400, edge=24.0, seed=5):
             continue
     xs = synth._draw_types(rng, n_atoms)
     cav = synth.find_cavity(xyz, edge, 12.0)
-    return xyz, xs, cav
+    # open a pocket around the cavity so that docked poses are not clash
+    # dominated (there BFGS trajectories are chaotic in FP32)
+    keep = np.linalg.norm(xyz - cav, axis=1) > 5.0
+    return xyz[keep], xs[keep], cav
 
 
-def box_of(center, size=12.0):
+BOX = 16.0  # search box edge of the test scenes (A)
+
+
+def box_of(center, size=BOX):
     return np.asarray(center, dtype=np.float64), np.array([size, size, size], dtype=np.float64)
 
 
@@ -48,7 +54,7 @@ def small_ligands(n=6, seed=11):
     return tuple(ligs)
 
 
-def oracle_scene(prec=64, sf=0, size=12.0, with_maps=True):
+def oracle_scene(prec=64, sf=0, size=BOX, with_maps=True):
     xyz, xs, cav = small_receptor()
     o = Oracle(prec=prec, sf=sf)
     c, s = box_of(cav, size)

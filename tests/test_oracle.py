@@ -224,7 +224,7 @@ def test_eval_deriv_matches_central_differences():
             g = o.eval_deriv(li, pose)[1]
             fd = _fd_grad(o, li, lig, pose, 1e-5)
             scale = max(1.0, np.abs(g).max())
-            np.testing.assert_allclose(g, fd, atol=2e-3 * scale, rtol=0)
+            np.testing.assert_allclose(g, fd, atol=5e-3 * scale, rtol=0)
             gb = o.eval_deriv(bare[li], pose)[1]
             fdb = _fd_grad(o, bare[li], lig, pose, 1e-5)
             np.testing.assert_allclose(gb, fdb, atol=1e-6 * max(1.0, np.abs(gb).max()), rtol=1e-6)

@@ -69,6 +69,7 @@ struct ud_ctx {
     int last_launches = 0;
     // autotuned lanes per bucket (atom capacity class)
     int tuned_lanes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int forced_lanes = 0;  // ud_set_lanes
 };
 
 namespace {
@@ -666,7 +667,7 @@ int ud_score_grad_device(ud_ctx* c, int n_poses, const int32_t* pose_ligand, con
     if (rc) return rc;
     int na, nn;
     caps_all(c, na, nn);
-    const int lanes = fit_lanes(16, na, nn);
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, na, nn);
     std::vector<int32_t> perm;
     std::vector<WorkItem> work;
     make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
@@ -722,7 +723,7 @@ int ud_optimize(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses
     if (rc) return rc;
     int na, nn;
     caps_all(c, na, nn);
-    const int lanes = fit_lanes(default_lanes(na), na, nn);
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : default_lanes(na), na, nn);
     std::vector<int32_t> perm;
     std::vector<WorkItem> work;
     make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
@@ -870,13 +871,14 @@ int ud_score_terms(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const flo
     if (rc) return rc;
     int na, nn;
     caps_all(c, na, nn);
-    if (lds_bytes(16, na, nn) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, na, nn);
+    if (lds_bytes(lanes, na, nn) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
     std::vector<int32_t> perm;
     std::vector<WorkItem> work;
-    make_pose_work(n_poses, pose_ligand, 4, perm, work);
+    make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
     rc = work_upload(c, perm, work);
     if (rc) return rc;
-    UD_HIP(c, launch_terms(c->precision, lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm,
+    UD_HIP(c, launch_terms(lanes, c->precision, lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm,
                            c->d_poses, 1000.f, c->d_f1, c->d_f1 + n_poses, c->stream));
     UD_HIP(c, hipMemcpyAsync(inter_out, c->d_f1, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
     UD_HIP(c, hipMemcpyAsync(intra_out, c->d_f1 + n_poses, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
@@ -896,16 +898,25 @@ int ud_pose_coords(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const flo
     if (rc) return rc;
     int na, nn;
     caps_all(c, na, nn);
-    if (lds_bytes(16, na, nn) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, na, nn);
+    if (lds_bytes(lanes, na, nn) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
     std::vector<int32_t> perm;
     std::vector<WorkItem> work;
-    make_pose_work(n_poses, pose_ligand, 4, perm, work);
+    make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
     rc = work_upload(c, perm, work);
     if (rc) return rc;
-    UD_HIP(c, launch_coords(lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm, c->d_poses, c->d_f1,
+    UD_HIP(c, launch_coords(lanes, lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm, c->d_poses, c->d_f1,
                             c->stream));
     UD_HIP(c, hipMemcpyAsync(coords_out, c->d_f1, sizeof(float) * size_t(n_poses) * UD_MAX_ATOMS * 3, hipMemcpyDeviceToHost, c->stream));
     UD_HIP(c, hipStreamSynchronize(c->stream));
+    return UD_OK;
+}
+
+int ud_set_lanes(ud_ctx* c, int lanes) {
+    if (!c) return UD_ERR_INVALID;
+    if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+        return fail(c, UD_ERR_INVALID, "lanes must be 0, 8, 16, 32 or 64");
+    c->forced_lanes = lanes;
     return UD_OK;
 }
 

@@ -992,27 +992,30 @@ hipError_t launch_dock(int lanes, int gf, const LigandSetDev& ls, const GridDev&
     return hipGetLastError();
 }
 
-hipError_t launch_terms(int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap, const WorkItem* work,
-                        int n_work, const int32_t* perm, const float* poses, float v, float* inter_out, float* intra_out,
-                        hipStream_t st) {
+hipError_t launch_terms(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+                        const WorkItem* work, int n_work, const int32_t* perm, const float* poses, float v,
+                        float* inter_out, float* intra_out, hipStream_t st) {
     const KArgs ka = make_kargs(ls, g, na_cap, nn_cap);
-    constexpr int L = 16;
-    const size_t shm = lds_bytes(L, na_cap, nn_cap);
-    if (gf == UD_GRID_F16)
-        hipLaunchKernelGGL((terms_kernel<L, UD_GRID_F16>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses, v,
-                           inter_out, intra_out);
-    else
-        hipLaunchKernelGGL((terms_kernel<L, UD_GRID_F32>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses, v,
-                           inter_out, intra_out);
+    const size_t shm = lds_bytes(lanes, na_cap, nn_cap);
+    UD_DISPATCH_L(lanes, {
+        if (gf == UD_GRID_F16)
+            hipLaunchKernelGGL((terms_kernel<L, UD_GRID_F16>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
+                               v, inter_out, intra_out);
+        else
+            hipLaunchKernelGGL((terms_kernel<L, UD_GRID_F32>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
+                               v, inter_out, intra_out);
+    });
     return hipGetLastError();
 }
 
-hipError_t launch_coords(const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap, const WorkItem* work,
-                         int n_work, const int32_t* perm, const float* poses, float* coords_out, hipStream_t st) {
+hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+                         const WorkItem* work, int n_work, const int32_t* perm, const float* poses, float* coords_out,
+                         hipStream_t st) {
     const KArgs ka = make_kargs(ls, g, na_cap, nn_cap);
-    constexpr int L = 16;
-    const size_t shm = lds_bytes(L, na_cap, nn_cap);
-    hipLaunchKernelGGL((coords_kernel<L>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses, coords_out);
+    const size_t shm = lds_bytes(lanes, na_cap, nn_cap);
+    UD_DISPATCH_L(lanes, {
+        hipLaunchKernelGGL((coords_kernel<L>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses, coords_out);
+    });
     return hipGetLastError();
 }
 

@@ -35,10 +35,10 @@ hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const Grid
 hipError_t launch_dock(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
                        const SearchDev& sp, const WorkItem* work, int n_work, float* out_pose, float* out_e,
                        long long* out_evals, hipStream_t st);
-hipError_t launch_terms(int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap, const WorkItem* work,
+hipError_t launch_terms(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap, const WorkItem* work,
                         int n_work, const int32_t* perm, const float* poses, float v, float* inter_out,
                         float* intra_out, hipStream_t st);
-hipError_t launch_coords(const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap, const WorkItem* work,
+hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap, const WorkItem* work,
                          int n_work, const int32_t* perm, const float* poses, float* coords_out, hipStream_t st);
 
 }  // namespace ud

@@ -81,6 +81,7 @@ def load() -> ctypes.CDLL:
         "ud_score_terms": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, fp]),
         "ud_pose_coords": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp]),
         "ud_last_kernel_time": (ctypes.c_int, [vp, dp, ip]),
+        "ud_set_lanes": (ctypes.c_int, [vp, ctypes.c_int]),
         "ud_synth_receptor": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                              ctypes.c_longlong, dp]),
     }
@@ -256,6 +257,9 @@ class Context:
         self._ck(self.L.ud_pose_coords(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float),
                                        _p(out, ctypes.c_float)))
         return out
+
+    def set_lanes(self, lanes: int):
+        self._ck(self.L.ud_set_lanes(self.h, lanes))
 
     def last_kernel_time(self):
         ms = ctypes.c_double()
