@@ -1,0 +1,262 @@
+"""Benchmark: ligands/s of batched Monte Carlo docking on one MI355X.
+
+Workload (the metric's configuration): synthetic receptor of 2,000 heavy
+atoms (protein-like X-Score mix, >= 3.0 A spacing in a 40 A cube), 1,024
+synthetic ligands (10-40 heavy atoms, 0-10 torsions), search box 22.5 A at
+0.375 A (61^3 points per map), Vina scoring, exhaustiveness 128, max_step 20.
+
+A step is one ud_dock call over the whole batch with every input already
+resident on the device.  Two forms are timed:
+  batched  FP16 grid maps (per-map power-of-two scale, FP32 arithmetic),
+           lanes per chain chosen per size bucket           -> `value`
+  plain    FP32 grid maps, one chain per 64-lane wavefront
+The best pose of sampled ligands is re-scored by the FP64 oracle to report
+the FP16 form's energy error.  The CPU baseline is the FP32 C++ restatement
+of the reference's CPU global search on a bounded sample of the same batch.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "uni-dock_amd"))
+sys.path.insert(0, ROOT)
+
+from unidock_tools import _lib, synth  # noqa: E402
+
+HBM_PEAK = 8.0e12
+LIGAND_TYPES = [0, 1, 2, 3, 8, 9, 10]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(n_lig, seed):
+    rec = synth.make_receptor(2000, seed=seed)
+    ligs = synth.make_ligand_set(n_lig, seed=seed + 1, atoms=(10, 40), tors=(0, 10))
+    return rec, ligs
+
+
+def global_steps_for(ligs, max_step):
+    # Vina::global_search_gpu (vina.h:192-204): 70 * 3 * (50 + max(N + 10 dof)) / 2, capped by max_step
+    heur = max(l.n_atoms + 10 * (6 + l.n_tors) for l in ligs)
+    return min(int(70 * 3 * (50 + heur) / 2), max_step)
+
+
+def setup_ctx(rec, ligs, precision):
+    ctx = _lib.Context(0)
+    ctx.set_scoring(_lib.UD_SF_VINA)
+    ctx.set_box(rec.cavity, (22.5, 22.5, 22.5), 0.375)
+    ctx.compute_maps(rec.xyz, rec.xs, LIGAND_TYPES, precision)
+    ctx.set_ligands(synth.LigandBatch(ligs).arrays())
+    return ctx
+
+
+def time_form(ctx, n_lig, exh, gsteps, lanes, steps, warmup, seed):
+    for _ in range(warmup):
+        ctx.dock(exhaustiveness=exh, global_steps=gsteps, seed=seed, lanes=lanes, fetch=False)
+    if warmup:
+        ctx.fetch_dock()
+    kms = 0.0
+    per_launch = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.dock(exhaustiveness=exh, global_steps=gsteps, seed=seed, lanes=lanes, fetch=False)
+        P, e, ev = ctx.fetch_dock()
+        ms, nl = ctx.last_kernel_time()
+        kms += ms
+        li = ctx.last_launches()
+        if per_launch is None:
+            per_launch = li
+        else:
+            for a, b in zip(per_launch, li):
+                a["ms"] += b["ms"]
+    wall = time.perf_counter() - t0
+    for a in per_launch:
+        a["ms"] /= steps
+    return dict(wall=wall, ligs_per_s=n_lig * steps / wall, kernel_ms=kms / steps, launches=per_launch,
+                poses=P, energies=e, evals=ev)
+
+
+def algorithmic_bytes(ligs, exh, evals, bytes_per_value, sel=None):
+    # 8 corner gathers per typed atom per energy+gradient evaluation
+    n_typed = np.repeat([int(np.sum(l.xs >= 0)) for l in ligs], exh).astype(np.float64)
+    ev = evals.astype(np.float64)
+    if sel is not None:
+        mask = np.repeat(sel, exh)
+        n_typed, ev = n_typed[mask], ev[mask]
+    return float(np.sum(ev * n_typed) * 8 * bytes_per_value)
+
+
+def bucket_mask(ligs, atom_cap):
+    lo = max([c for c in _lib.BUCKET_ATOMS if c < atom_cap], default=0)
+    return np.array([lo < l.n_atoms <= atom_cap for l in ligs])
+
+
+def cpu_baseline(rec, ligs, exh, gsteps, n_sample, threads):
+    from oracle.oracle import Oracle
+    o = Oracle(32, 0)
+    o.set_grid(rec.cavity, np.array([22.5, 22.5, 22.5]), 0.375)
+    o.build_maps(rec.xyz, rec.xs, LIGAND_TYPES)
+    idx = []
+    for l in ligs[:n_sample]:
+        idx.append(o.add_ligand(l))
+    lsteps = [(25 + l.n_atoms) // 3 for l in ligs[:n_sample]]
+    t0 = time.perf_counter()
+    be, ev = o.dock_cpu(idx, exh, 181129, gsteps, lsteps, threads)
+    wall = time.perf_counter() - t0
+    return dict(value=n_sample / wall, unit="ligands/s", cores=threads, kind="port",
+                sample=f"{n_sample} ligands of the batch x exhaustiveness {exh} x {gsteps} MC steps, "
+                       f"FP32 C++ restatement of the reference CPU global search, {wall:.1f} s wall",
+                evals_per_s=ev / wall)
+
+
+def fp64_error(rec, ligs, ctx16, res, exh, n_sample):
+    """Best pose per sampled ligand: FP16-grid energy vs FP64 oracle (own FP64 maps)."""
+    from oracle.oracle import Oracle
+    o = Oracle(64, 0)
+    o.set_grid(rec.cavity, np.array([22.5, 22.5, 22.5]), 0.375)
+    o.build_maps(rec.xyz, rec.xs, LIGAND_TYPES)
+    sel = np.linspace(0, len(ligs) - 1, n_sample).astype(int)
+    pl, P = [], []
+    for li in sel:
+        o.add_ligand(ligs[li])
+        e = res["energies"][li * exh:(li + 1) * exh]
+        c = int(np.argmin(e))
+        pl.append(li)
+        P.append(res["poses"][li * exh + c])
+    e16, _ = ctx16.score_grad(np.array(pl, dtype=np.int32), np.array(P))
+    errs = []
+    for k, li in enumerate(sel):
+        e64 = o.eval_deriv(k, np.array(P[k], dtype=np.float64))[0]
+        errs.append(abs(float(e16[k]) - e64))
+    return float(np.max(errs)), float(np.mean(errs))
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB in one elementwise kernel: the
+    reference point for the FETCH_SIZE / WRITE_SIZE counters (it reads and
+    writes exactly mib MiB)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst[-8:], src[-8:])
+    print(json.dumps({"copy_kernel_mib": mib, "launches": 3, "bytes_each_way": n * 4}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ligands", type=int, default=1024)
+    ap.add_argument("--exhaustiveness", type=int, default=128)
+    ap.add_argument("--max-step", type=int, default=20)
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per chain of the batched form (0 = per bucket)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--copy-kernel-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
+    args = ap.parse_args()
+    if args.copy_kernel_mib:
+        copy_calibration(args.copy_kernel_mib)
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the ligand batch would be split across "
+              "GPUs by the host with no collective in the search", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    rec, ligs = make_workload(args.ligands, args.seed)
+    gsteps = global_steps_for(ligs, args.max_step)
+    log(f"[bench] workload ready in {time.perf_counter() - t0:.1f}s: {len(ligs)} ligands, "
+        f"atoms {min(l.n_atoms for l in ligs)}-{max(l.n_atoms for l in ligs)}, global_steps {gsteps}")
+    exh = args.exhaustiveness
+
+    ctx16 = setup_ctx(rec, ligs, _lib.UD_GRID_F16)
+    tuned = None
+    if args.lanes == 0:
+        t1 = time.perf_counter()
+        tuned = [int(x) for x in ctx16.autotune(exhaustiveness=exh, global_steps=2, sample_ligands=64)]
+        log(f"[bench] autotuned lanes per bucket {dict(zip(_lib.BUCKET_ATOMS, tuned))} "
+            f"in {time.perf_counter() - t1:.1f}s (untimed set-up)")
+    log("[bench] batched FP16 form")
+    b = time_form(ctx16, len(ligs), exh, gsteps, args.lanes, args.steps, args.warmup, 181129)
+    log(f"[bench] batched: {b['ligs_per_s']:.1f} ligands/s, kernel {b['kernel_ms']:.1f} ms/step, "
+        f"launches {b['launches']}")
+    plain = None
+    if not args.no_plain:
+        ctx32 = setup_ctx(rec, ligs, _lib.UD_GRID_F32)
+        log("[bench] plain FP32 form")
+        plain = time_form(ctx32, len(ligs), exh, gsteps, 64, args.steps, args.warmup, 181129)
+        log(f"[bench] plain: {plain['ligs_per_s']:.1f} ligands/s, kernel {plain['kernel_ms']:.1f} ms/step")
+        ctx32.close()
+
+    evals = float(b["evals"].sum())
+    # dominant kernel: the longest launch of the batched form (one size bucket)
+    dom = max(b["launches"], key=lambda a: a["ms"])
+    sel = bucket_mask(ligs, dom["atom_cap"])
+    bytes_dom = algorithmic_bytes(ligs, exh, b["evals"], 2, sel)
+    achieved = bytes_dom / (dom["ms"] * 1e-3)
+    roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
+                traffic=None)
+    log(f"[bench] dominant launch: bucket <= {dom['atom_cap']} atoms, {dom['lanes']} lanes, "
+        f"{dom['n_ligands']} ligands, {dom['ms']:.2f} ms, {bytes_dom / 1e6:.1f} MB algorithmic")
+    err_max, err_mean = fp64_error(rec, ligs, ctx16, b, exh, 64)
+    log(f"[bench] FP16 best-pose energy error vs FP64: max {err_max:.4f} mean {err_mean:.5f} kcal/mol")
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] CPU baseline on {threads} threads")
+        cpu = cpu_baseline(rec, ligs, exh, gsteps, args.cpu_sample, threads)
+        log(f"[bench] cpu: {cpu['value']:.3f} ligands/s")
+
+    out = {
+        "metric": "ligands/sec on 1 MI355X, batched FP16-grid vs plain, at <=0.1 kcal/mol err",
+        "value": round(b["ligs_per_s"], 3),
+        "unit": "ligands/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * b["wall"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded receptor and ligands)",
+        "config": {
+            "workload": "batched docking: 2,000-atom receptor, 1,024 ligands (10-40 atoms, 0-10 torsions), "
+                        "box 22.5 A / 0.375 A (61^3), Vina, exhaustiveness 128, max_step 20",
+            "n_ligands": len(ligs), "exhaustiveness": exh, "global_steps": gsteps, "grid": "61^3 fp16 + per-map scale",
+            "lanes_per_chain": args.lanes or "autotuned per size bucket",
+        },
+        "roofline": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in roof.items()},
+        "cpu_baseline": cpu,
+        "evals_per_s": evals / (b["kernel_ms"] * 1e-3),
+        "kernel_ms_per_step": b["kernel_ms"],
+        "launches": b["launches"],
+        "tuned_lanes": dict(zip([str(c) for c in _lib.BUCKET_ATOMS], tuned)) if tuned else None,
+        "dominant_kernel": {"name": f"search_kernel<{dom['lanes']}, fp16 maps, MC>",
+                            "bucket_atoms": dom["atom_cap"], "avg_ms": dom["ms"],
+                            "algorithmic_bytes": bytes_dom},
+        "plain_fp32_ligands_per_s": plain["ligs_per_s"] if plain else None,
+        "speedup_vs_plain": (b["ligs_per_s"] / plain["ligs_per_s"]) if plain else None,
+        "fp16_best_pose_err_vs_fp64_kcal": {"max": err_max, "mean": err_mean, "sample": 64},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

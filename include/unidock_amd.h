@@ -61,6 +61,21 @@ enum { UD_GRID_F32 = 0, UD_GRID_F16 = 1 };
  * (the change vector of conf.h:182-227). */
 #define UD_GRAD_FLOATS (6 + UD_MAX_TORSIONS)
 
+/* ud_dock groups ligands into size buckets by heavy+hydrogen atom count
+ * (<= 16, 32, 48, 64, 96, 128 atoms) and launches each bucket once; the
+ * reference groups ligands by size class too (SmallConfig / MediumConfig /
+ * LargeConfig ..., kernel.h:242-277, chosen in main.cpp:267-275). */
+#define UD_N_BUCKETS 6
+
+/* Timing of one launch of the last ud_dock (ud_last_launches). */
+typedef struct ud_launch_info {
+    int32_t atom_cap;      /* atom capacity of the bucket */
+    int32_t lanes;         /* lanes per chain */
+    int32_t n_ligands;
+    int32_t n_workgroups;  /* one wavefront each */
+    double ms;             /* kernel time, HIP events on the context's stream */
+} ud_launch_info;
+
 typedef struct ud_ctx ud_ctx;
 
 /* A batch of ligands in flat arrays (offsets have n_ligands + 1 entries).
@@ -202,6 +217,26 @@ int ud_set_lanes(ud_ctx* ctx, int lanes);
  * stream (HIP events around the launches): total milliseconds and number of
  * launches. */
 int ud_last_kernel_time(const ud_ctx* ctx, double* ms, int32_t* launches);
+
+/* Per-launch timing of the last ud_dock (resolved by ud_fetch_dock_results,
+ * or by ud_dock when it fetches): up to max_launches entries, launch order
+ * (largest bucket first); n_out receives the number of launches.  The
+ * reference times its docking loop on the host only (main.cpp:272-276). */
+int ud_last_launches(const ud_ctx* ctx, int max_launches, ud_launch_info* out, int32_t* n_out);
+
+/* Lanes per chain of each size bucket for ud_dock (0 = built-in default).
+ * The reference fixes 32 threads per chain for every size class
+ * (monte_carlo.cu:117-118, 2877-2882). */
+int ud_set_bucket_lanes(ud_ctx* ctx, const int32_t* lanes /* [UD_N_BUCKETS] */);
+
+/* Autotuner: for every size bucket of the current ligand set, docks an
+ * evenly spaced sample of up to sample_ligands ligands with params at 8, 16,
+ * 32 and 64 lanes per chain (those that fit the LDS), keeps the lane count
+ * with the most energy+gradient evaluations per second and stores it as the
+ * bucket's setting (also written to lanes_out[UD_N_BUCKETS] if non-NULL).
+ * Discards earlier docking results.  No counterpart in the reference (fixed
+ * 32 threads per chain). */
+int ud_autotune(ud_ctx* ctx, const ud_search_params* params, int sample_ligands, int32_t* lanes_out);
 
 /* Seeded random sequential addition of n_atoms points in [0, edge)^3 with a
  * minimum spacing (synthetic receptors); host-only. */

@@ -376,7 +376,9 @@ template <class R> R grid_eval(const Grid<R>& g, int t, const V3<R>& loc, R slop
     for (int i = 0; i < 3; ++i) {
         s[i] = (loc[i] - g.init[i]) * g.factor[i];
         miss[i] = 0;
-        if (s[i] < R(0)) {
+        // `!(s >= 0)` also routes a NaN coordinate (an overflowed FP32 pose)
+        // to the clamped branch instead of an undefined int(NaN) index
+        if (!(s[i] >= R(0))) {
             miss[i] = -s[i]; region[i] = -1; a[i] = 0; s[i] = 0;
         } else if (s[i] >= g.dim_minus_1[i]) {
             miss[i] = s[i] - g.dim_minus_1[i]; region[i] = 1; a[i] = g.n[i] - 2; s[i] = 1;

@@ -337,3 +337,35 @@ def test_capacity_and_state_errors(scene):
         ctx.set_ligands(arr)
     assert ei.value.code == _lib.UD_ERR_CAPACITY
     ctx.close()
+
+
+def test_autotune_and_bucket_lanes(scene):
+    """The autotuner stores one supported lane count per occupied size bucket;
+    docking with the tuned setting equals docking with the same lanes set by
+    hand, and the per-launch timings cover the whole dock."""
+    _say("[gpu] autotune")
+    ctx = scene["ctx"]
+    nat = [l.n_atoms for l in scene["ligs"]]
+    occupied = {min(b for b in _lib.BUCKET_ATOMS if n <= b) for n in nat}
+    tuned = ctx.autotune(exhaustiveness=8, global_steps=1, sample_ligands=4)
+    for cap, lanes in zip(_lib.BUCKET_ATOMS, tuned):
+        assert (lanes in (8, 16, 32, 64)) if cap in occupied else lanes == 0
+    with pytest.raises(_lib.UdError) as ei:
+        ctx.fetch_dock()
+    assert ei.value.code == _lib.UD_ERR_STATE
+    P1, e1, ev1 = ctx.dock(exhaustiveness=8, global_steps=3, seed=4)
+    total_ms, n = ctx.last_kernel_time()
+    li = ctx.last_launches()
+    assert n == len(li) == len(occupied)
+    assert [a["atom_cap"] for a in li] == sorted(occupied, reverse=True)
+    for a in li:
+        assert a["lanes"] == tuned[_lib.BUCKET_ATOMS.index(a["atom_cap"])] and a["ms"] > 0
+    assert sum(a["ms"] for a in li) == pytest.approx(total_ms, rel=1e-3, abs=1e-2)
+    ctx.set_bucket_lanes([0] * _lib.UD_N_BUCKETS)
+    ctx.set_bucket_lanes(list(tuned))
+    P2, e2, ev2 = ctx.dock(exhaustiveness=8, global_steps=3, seed=4)
+    assert np.array_equal(P1, P2) and np.array_equal(e1, e2) and np.array_equal(ev1, ev2)
+    with pytest.raises(_lib.UdError) as ei:
+        ctx.set_bucket_lanes([12, 0, 0, 0, 0, 0])
+    assert ei.value.code == _lib.UD_ERR_INVALID
+    ctx.set_bucket_lanes([0] * _lib.UD_N_BUCKETS)

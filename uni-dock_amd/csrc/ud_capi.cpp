@@ -67,8 +67,11 @@ struct ud_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
     int last_launches = 0;
-    // autotuned lanes per bucket (atom capacity class)
-    int tuned_lanes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // per-launch timing of the last ud_dock: event k marks the end of launch k
+    hipEvent_t lev[UD_N_BUCKETS] = {};
+    ud_launch_info linfo[UD_N_BUCKETS] = {};
+    // lanes per chain per size bucket (ud_autotune / ud_set_bucket_lanes; 0 = default)
+    int tuned_lanes[UD_N_BUCKETS] = {0, 0, 0, 0, 0, 0};
     int forced_lanes = 0;  // ud_set_lanes
 };
 
@@ -137,8 +140,8 @@ LigandSetDev lig_dev(const ud_ctx* c) {
 }
 
 // largest lanes-per-group whose LDS footprint fits one workgroup (64 KiB)
-int fit_lanes(int lanes, int na, int nn) {
-    while (lanes < 64 && lds_bytes(lanes, na, nn) > 65536) lanes *= 2;
+int fit_lanes(int lanes, const LaunchCap& cap) {
+    while (lanes < 64 && lds_bytes(lanes, cap) > 65536) lanes *= 2;
     return lanes;
 }
 
@@ -186,15 +189,18 @@ void make_pose_work(int n_poses, const int32_t* pose_ligand, int groups, std::ve
     }
 }
 
-int caps_all(const ud_ctx* c, int& na, int& nn) {
-    na = 1;
-    nn = 1;
-    for (int l = 0; l < c->n_lig; ++l) {
-        na = std::max(na, c->lig_nat[l]);
-        nn = std::max(nn, c->lig_nn[l]);
+LaunchCap caps_of(const ud_ctx* c, const std::vector<int>* ligs) {
+    LaunchCap cap{1, 1, 0};
+    const int n = ligs ? int(ligs->size()) : c->n_lig;
+    for (int k = 0; k < n; ++k) {
+        const int l = ligs ? (*ligs)[k] : k;
+        cap.n_atoms = std::max(cap.n_atoms, c->lig_nat[l]);
+        cap.n_nodes = std::max(cap.n_nodes, c->lig_nn[l]);
+        cap.n_pairs = std::max(cap.n_pairs, c->hdr[l].n_pairs);
     }
-    return UD_OK;
+    return cap;
 }
+LaunchCap caps_all(const ud_ctx* c) { return caps_of(c, nullptr); }
 
 int check_pose_args(ud_ctx* c, int n_poses, const int32_t* pose_ligand) {
     if (n_poses < 0 || (n_poses > 0 && !pose_ligand)) return fail(c, UD_ERR_INVALID, "bad pose arguments");
@@ -234,8 +240,10 @@ int ud_create(int device_id, void* stream, ud_ctx** out) {
     c->stream = reinterpret_cast<hipStream_t>(stream);
     for (int i = 0; i < 36; ++i) c->slot_of_type[i] = -1;
     for (int i = 0; i < UD_MAX_MAPS; ++i) c->scale[i] = 1.f;
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
+    bool ev_ok = hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
+    for (int k = 0; k < UD_N_BUCKETS && ev_ok; ++k) ev_ok = hipEventCreate(&c->lev[k]) == hipSuccess;
+    if (!ev_ok) {
+        ud_destroy(c);
         return UD_ERR_HIP;
     }
     *out = c;
@@ -253,6 +261,8 @@ int ud_destroy(ud_ctx* c) {
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->lev)
+        if (e) (void)hipEventDestroy(e);
     delete c;
     return UD_OK;
 }
@@ -647,11 +657,9 @@ int work_upload(ud_ctx* c, const std::vector<int32_t>& perm, const std::vector<W
 }
 
 // default lanes per chain by atom capacity (replaced by autotuned values)
-int default_lanes(int na) {
-    if (na <= 16) return 16;
-    if (na <= 48) return 32;
-    return 64;
-}
+// lanes per chain without tuning, from MI355X timings of the search kernel
+// over ligand size buckets (32 lanes up to 32 atoms, a full wavefront above)
+int default_lanes(int na) { return na <= 32 ? 32 : 64; }
 }  // namespace
 
 extern "C" {
@@ -665,16 +673,16 @@ int ud_score_grad_device(ud_ctx* c, int n_poses, const int32_t* pose_ligand, con
     UD_HIP(c, hipSetDevice(c->device));
     rc = refresh_slots(c);
     if (rc) return rc;
-    int na, nn;
-    caps_all(c, na, nn);
-    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, na, nn);
+    const LaunchCap cap = caps_all(c);
+    const int na = cap.n_atoms;
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, cap);
     std::vector<int32_t> perm;
     std::vector<WorkItem> work;
     make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
     rc = work_upload(c, perm, work);
     if (rc) return rc;
     UD_HIP(c, hipEventRecord(c->ev0, c->stream));
-    UD_HIP(c, launch_score(lanes, c->precision, lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm,
+    UD_HIP(c, launch_score(lanes, c->precision, lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
                            d_poses, v[0], v[1], d_energy, d_grad, c->stream));
     UD_HIP(c, hipEventRecord(c->ev1, c->stream));
     c->last_launches = 1;
@@ -721,16 +729,16 @@ int ud_optimize(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses
     if (rc) return rc;
     rc = grow(c, c->d_i1, c->i_cap, size_t(n_poses));
     if (rc) return rc;
-    int na, nn;
-    caps_all(c, na, nn);
-    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : default_lanes(na), na, nn);
+    const LaunchCap cap = caps_all(c);
+    const int na = cap.n_atoms;
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : default_lanes(na), cap);
     std::vector<int32_t> perm;
     std::vector<WorkItem> work;
     make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
     rc = work_upload(c, perm, work);
     if (rc) return rc;
     UD_HIP(c, hipEventRecord(c->ev0, c->stream));
-    UD_HIP(c, launch_optimize(lanes, c->precision, lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm,
+    UD_HIP(c, launch_optimize(lanes, c->precision, lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
                               c->d_poses, v[0], v[1], max_steps, c->d_f1, c->d_i1, c->stream));
     UD_HIP(c, hipEventRecord(c->ev1, c->stream));
     UD_HIP(c, hipMemcpyAsync(poses, c->d_poses, sizeof(float) * size_t(n_poses) * UD_POSE_FLOATS, hipMemcpyDeviceToHost, c->stream));
@@ -744,13 +752,33 @@ int ud_optimize(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses
     return UD_OK;
 }
 
-int ud_dock(ud_ctx* c, const ud_search_params* p, float* best_poses, float* best_e, int64_t* evals) {
-    if (!c || !p) return UD_ERR_INVALID;
+}  // extern "C"
+
+namespace {
+
+const int kBucketCaps[UD_N_BUCKETS] = {16, 32, 48, 64, 96, 128};
+
+int bucket_of(int n_atoms) {
+    int bi = 0;
+    while (bi < UD_N_BUCKETS - 1 && kBucketCaps[bi] < n_atoms) ++bi;
+    return bi;
+}
+
+int check_search_params(ud_ctx* c, const ud_search_params* p) {
     if (p->exhaustiveness < 1) return fail(c, UD_ERR_INVALID, "exhaustiveness must be 1 or greater");
     if (p->global_steps < 0) return fail(c, UD_ERR_INVALID, "global_steps must be >= 0");
     if (p->lanes_per_chain != 0 && p->lanes_per_chain != 8 && p->lanes_per_chain != 16 && p->lanes_per_chain != 32 &&
         p->lanes_per_chain != 64)
         return fail(c, UD_ERR_INVALID, "lanes_per_chain must be 0, 8, 16, 32 or 64");
+    return UD_OK;
+}
+
+// Monte Carlo search of the ligands in `subset` (all ligands when null): one
+// launch per size bucket, largest bucket first.  Chain results land at
+// ligand * exhaustiveness + chain of the context's output buffers, so a
+// subset run leaves the other ligands' slots untouched.  lanes_all != 0
+// forces the lanes per chain of every bucket.
+int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subset, int lanes_all) {
     UD_HIP(c, hipSetDevice(c->device));
     int rc = refresh_slots(c);
     if (rc) return rc;
@@ -789,26 +817,25 @@ int ud_dock(ud_ctx* c, const ud_search_params* p, float* best_poses, float* best
     sp.temperature = p->temperature;
     sp.amplitude = p->mutation_amplitude;
     // size buckets by atom capacity; each bucket is one launch, largest first
-    const int bucket_caps[] = {16, 32, 48, 64, 96, 128};
-    std::vector<std::vector<int>> bl(6);
-    for (int l = 0; l < c->n_lig; ++l) {
-        int bi = 0;
-        while (bucket_caps[bi] < c->lig_nat[l]) ++bi;
-        bl[bi].push_back(l);
+    std::vector<std::vector<int>> bl(UD_N_BUCKETS);
+    if (subset) {
+        for (int l : *subset) bl[bucket_of(c->lig_nat[l])].push_back(l);
+    } else {
+        for (int l = 0; l < c->n_lig; ++l) bl[bucket_of(c->lig_nat[l])].push_back(l);
     }
     std::vector<std::vector<WorkItem>> works;
-    std::vector<int> lanes_v, na_v, nn_v;
+    std::vector<int> lanes_v;
+    std::vector<LaunchCap> cap_v;
+    int nl = 0;
     size_t total_work = 0;
-    for (int bi = 5; bi >= 0; --bi) {
+    for (int bi = UD_N_BUCKETS - 1; bi >= 0; --bi) {
         if (bl[bi].empty()) continue;
-        int na = 1, nn = 1;
-        for (int l : bl[bi]) {
-            na = std::max(na, c->lig_nat[l]);
-            nn = std::max(nn, c->lig_nn[l]);
-        }
-        int lanes = p->lanes_per_chain ? p->lanes_per_chain : (c->tuned_lanes[bi] ? c->tuned_lanes[bi] : default_lanes(na));
-        lanes = fit_lanes(lanes, na, nn);
-        if (lds_bytes(lanes, na, nn) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
+        const LaunchCap cap = caps_of(c, &bl[bi]);
+        int lanes = lanes_all ? lanes_all
+                              : (p->lanes_per_chain ? p->lanes_per_chain
+                                                    : (c->tuned_lanes[bi] ? c->tuned_lanes[bi] : default_lanes(cap.n_atoms)));
+        lanes = fit_lanes(lanes, cap);
+        if (lds_bytes(lanes, cap) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
         const int G = 64 / lanes;
         // larger ligands first inside the bucket (tail balance)
         std::vector<int> order = bl[bi];
@@ -819,10 +846,15 @@ int ud_dock(ud_ctx* c, const ud_search_params* p, float* best_poses, float* best
         for (int l : order)
             for (int c0 = 0; c0 < exh; c0 += G) w.push_back(WorkItem{l, l * exh + c0, std::min(G, exh - c0), 0});
         total_work += w.size();
-        works.push_back(w);
+        ud_launch_info& li = c->linfo[nl++];
+        li.atom_cap = kBucketCaps[bi];
+        li.lanes = lanes;
+        li.n_ligands = int(order.size());
+        li.n_workgroups = int(w.size());
+        li.ms = 0.0;
+        works.push_back(std::move(w));
         lanes_v.push_back(lanes);
-        na_v.push_back(na);
-        nn_v.push_back(nn);
+        cap_v.push_back(cap);
     }
     rc = grow(c, c->d_work, c->work_cap, total_work);
     if (rc) return rc;
@@ -832,12 +864,38 @@ int ud_dock(ud_ctx* c, const ud_search_params* p, float* best_poses, float* best
     UD_HIP(c, hipEventRecord(c->ev0, c->stream));
     size_t off = 0;
     for (size_t k = 0; k < works.size(); ++k) {
-        UD_HIP(c, launch_dock(lanes_v[k], c->precision, lig_dev(c), grid_dev(c), na_v[k], nn_v[k], sp, c->d_work + off,
+        UD_HIP(c, launch_dock(lanes_v[k], c->precision, lig_dev(c), grid_dev(c), cap_v[k], sp, c->d_work + off,
                               int(works[k].size()), c->d_out_pose, c->d_out_e, c->d_out_evals, c->stream));
+        UD_HIP(c, hipEventRecord(c->lev[k], c->stream));
         off += works[k].size();
     }
     UD_HIP(c, hipEventRecord(c->ev1, c->stream));
-    c->last_launches = int(works.size());
+    c->last_launches = nl;
+    return UD_OK;
+}
+
+// Resolve the timing events of the last dock (after a stream synchronize).
+void dock_times(ud_ctx* c) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    for (int k = 0; k < c->last_launches; ++k) {
+        float m = 0;
+        (void)hipEventElapsedTime(&m, k == 0 ? c->ev0 : c->lev[k - 1], c->lev[k]);
+        c->linfo[k].ms = m;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ud_dock(ud_ctx* c, const ud_search_params* p, float* best_poses, float* best_e, int64_t* evals) {
+    if (!c || !p) return UD_ERR_INVALID;
+    int rc = check_search_params(c, p);
+    if (rc) return rc;
+    rc = dock_run(c, p, nullptr, 0);
+    if (rc) return rc;
     if (best_poses || best_e || evals) return ud_fetch_dock_results(c, best_poses, best_e, evals);
     return UD_OK;
 }
@@ -851,9 +909,72 @@ int ud_fetch_dock_results(ud_ctx* c, float* best_poses, float* best_e, int64_t* 
     if (best_e) UD_HIP(c, hipMemcpyAsync(best_e, c->d_out_e, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (evals) UD_HIP(c, hipMemcpyAsync(evals, c->d_out_evals, n * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
     UD_HIP(c, hipStreamSynchronize(c->stream));
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    c->last_ms = ms;
+    dock_times(c);
+    return UD_OK;
+}
+
+int ud_last_launches(const ud_ctx* c, int max_launches, ud_launch_info* out, int32_t* n_out) {
+    if (!c || (max_launches > 0 && !out)) return UD_ERR_INVALID;
+    const int n = std::min(max_launches, c->last_launches);
+    for (int k = 0; k < n; ++k) out[k] = c->linfo[k];
+    if (n_out) *n_out = c->last_launches;
+    return UD_OK;
+}
+
+int ud_set_bucket_lanes(ud_ctx* c, const int32_t* lanes) {
+    if (!c || !lanes) return UD_ERR_INVALID;
+    for (int b = 0; b < UD_N_BUCKETS; ++b)
+        if (lanes[b] != 0 && lanes[b] != 8 && lanes[b] != 16 && lanes[b] != 32 && lanes[b] != 64)
+            return fail(c, UD_ERR_INVALID, "bucket lanes must be 0, 8, 16, 32 or 64");
+    for (int b = 0; b < UD_N_BUCKETS; ++b) c->tuned_lanes[b] = lanes[b];
+    return UD_OK;
+}
+
+int ud_autotune(ud_ctx* c, const ud_search_params* p, int sample_ligands, int32_t* lanes_out) {
+    if (!c || !p || sample_ligands < 1) return UD_ERR_INVALID;
+    int rc = check_search_params(c, p);
+    if (rc) return rc;
+    rc = ready_for_eval(c);
+    if (rc) return rc;
+    const int exh = p->exhaustiveness;
+    std::vector<std::vector<int>> bl(UD_N_BUCKETS);
+    for (int l = 0; l < c->n_lig; ++l) bl[bucket_of(c->lig_nat[l])].push_back(l);
+    std::vector<long long> ev;
+    for (int bi = 0; bi < UD_N_BUCKETS; ++bi) {
+        if (bl[bi].empty()) continue;
+        // evenly spaced sample of the bucket's ligands
+        std::vector<int> sub;
+        const int m = std::min<int>(sample_ligands, int(bl[bi].size()));
+        for (int k = 0; k < m; ++k) sub.push_back(bl[bi][size_t(k) * bl[bi].size() / m]);
+        const LaunchCap cap = caps_of(c, &sub);
+        int best = 0;
+        double best_rate = -1.0;
+        for (int lanes : {8, 16, 32, 64}) {
+            if (fit_lanes(lanes, cap) != lanes) continue;
+            double ms_min = 1e300;
+            long long tot = 0;
+            for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and code
+                rc = dock_run(c, p, &sub, lanes);
+                if (rc) return rc;
+                UD_HIP(c, hipStreamSynchronize(c->stream));
+                dock_times(c);
+                ms_min = std::min(ms_min, c->last_ms);
+            }
+            ev.resize(c->out_n);
+            UD_HIP(c, hipMemcpy(ev.data(), c->d_out_evals, ev.size() * sizeof(long long), hipMemcpyDeviceToHost));
+            for (int l : sub)
+                for (int k = 0; k < exh; ++k) tot += ev[size_t(l) * exh + k];
+            const double rate = double(tot) / std::max(ms_min, 1e-6);  // evaluations per ms
+            if (rate > best_rate) {
+                best_rate = rate;
+                best = lanes;
+            }
+        }
+        c->tuned_lanes[bi] = best;
+    }
+    if (lanes_out)
+        for (int b = 0; b < UD_N_BUCKETS; ++b) lanes_out[b] = c->tuned_lanes[b];
+    c->out_n = 0;  // the tuning runs' results are not docking results
     return UD_OK;
 }
 
@@ -869,16 +990,16 @@ int ud_score_terms(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const flo
     if (rc) return rc;
     rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * 2);
     if (rc) return rc;
-    int na, nn;
-    caps_all(c, na, nn);
-    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, na, nn);
-    if (lds_bytes(lanes, na, nn) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
+    const LaunchCap cap = caps_all(c);
+    const int na = cap.n_atoms;
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, cap);
+    if (lds_bytes(lanes, cap) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
     std::vector<int32_t> perm;
     std::vector<WorkItem> work;
     make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
     rc = work_upload(c, perm, work);
     if (rc) return rc;
-    UD_HIP(c, launch_terms(lanes, c->precision, lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm,
+    UD_HIP(c, launch_terms(lanes, c->precision, lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
                            c->d_poses, 1000.f, c->d_f1, c->d_f1 + n_poses, c->stream));
     UD_HIP(c, hipMemcpyAsync(inter_out, c->d_f1, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
     UD_HIP(c, hipMemcpyAsync(intra_out, c->d_f1 + n_poses, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
@@ -896,16 +1017,16 @@ int ud_pose_coords(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const flo
     if (rc) return rc;
     rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * UD_MAX_ATOMS * 3);
     if (rc) return rc;
-    int na, nn;
-    caps_all(c, na, nn);
-    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, na, nn);
-    if (lds_bytes(lanes, na, nn) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
+    const LaunchCap cap = caps_all(c);
+    const int na = cap.n_atoms;
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, cap);
+    if (lds_bytes(lanes, cap) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
     std::vector<int32_t> perm;
     std::vector<WorkItem> work;
     make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
     rc = work_upload(c, perm, work);
     if (rc) return rc;
-    UD_HIP(c, launch_coords(lanes, lig_dev(c), grid_dev(c), na, nn, c->d_work, int(work.size()), c->d_perm, c->d_poses, c->d_f1,
+    UD_HIP(c, launch_coords(lanes, lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm, c->d_poses, c->d_f1,
                             c->stream));
     UD_HIP(c, hipMemcpyAsync(coords_out, c->d_f1, sizeof(float) * size_t(n_poses) * UD_MAX_ATOMS * 3, hipMemcpyDeviceToHost, c->stream));
     UD_HIP(c, hipStreamSynchronize(c->stream));

@@ -98,6 +98,7 @@ struct SearchDev {
 struct LaunchCap {
     int32_t n_atoms;  // max atoms
     int32_t n_nodes;  // max nodes
+    int32_t n_pairs;  // max intramolecular pairs
 };
 
 }  // namespace ud

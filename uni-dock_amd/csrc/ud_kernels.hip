@@ -4,8 +4,8 @@
 //   grid_build_kernel   receptor affinity maps (cache::populate_no_bias)
 //   grid_absmax_kernel / grid_to_half_kernel   FP16 map storage
 //   score_kernel        energy + gradient of poses (model::eval_deriv)
-//   optimize_kernel     BFGS from given starts (quasi_newton / bfgs)
-//   dock_kernel         Monte Carlo chains (kernel<32, Config>)
+//   search_kernel       BFGS from given starts (quasi_newton / bfgs) and,
+//                       with MC, Monte Carlo chains (kernel<32, Config>)
 //   terms_kernel        final-score components (Vina::score_gpu)
 //   coords_kernel       atom coordinates of poses (model::set)
 #include <hip/hip_fp16.h>
@@ -177,22 +177,22 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 // per lane group (coordinates, minus-forces, node frames, node force/torque
 // accumulators, six pose buffers, three broadcast vectors, inverse Hessian).
 struct Layout {
-    int na, nn, nc, nh;
-    int o_coords, o_mf, o_frames, o_nacc, o_poses, o_bc, o_H, group_floats, shared_floats;
-    __host__ __device__ void init(int na_, int nn_) {
+    int na, nn, np, nc, nh;
+    int o_coords, o_mf, o_frames, o_poses, o_bc, o_H, group_floats, shared_floats;
+    __host__ __device__ void init(int na_, int nn_, int np_) {
         na = na_;
         nn = nn_;
+        np = (np_ + 3) & ~3;
         nc = 6 + nn - 1;
-        nh = (nc * (nc + 1) / 2 + 3) & ~3;
+        nh = (nc * (nc | 1) + 3) & ~3;  // full symmetric inverse Hessian, odd row stride
         o_coords = 0;
         o_mf = 4 * na;
         o_frames = 8 * na;
-        o_nacc = o_frames + 20 * nn;
-        o_poses = o_nacc + 8 * nn;
+        o_poses = o_frames + 20 * nn;
         o_bc = o_poses + 6 * kPoseStride;
         o_H = o_bc + 3 * kPoseStride;
         group_floats = (o_H + nh + 3) & ~3;
-        shared_floats = 4 * na + 8 * nn;
+        shared_floats = 4 * na + 8 * nn + np;  // atoms, nodes, pair records
     }
     __host__ __device__ size_t bytes(int groups) const { return size_t(shared_floats + groups * group_floats) * 4; }
 };
@@ -201,7 +201,7 @@ enum { PX = 0, PXN = 1, PXO = 2, PTMP = 3, PBEST = 4, PCAND = 5 };
 struct KArgs {
     LigandSetDev ls;
     GridDev grid;
-    int na_cap, nn_cap;
+    int na_cap, nn_cap, np_cap;
 };
 
 template <int L> struct Grp {
@@ -209,6 +209,7 @@ template <int L> struct Grp {
     static constexpr int KV = (kNMax + L - 1) / L;
     const AtomRec* atoms;  // LDS (shared)
     const NodeRec* nodes;  // LDS (shared)
+    const PairRec* pairs;  // LDS (shared)
     float* gb;             // LDS group block
     Layout lay;
     LigHdr h;
@@ -216,7 +217,6 @@ template <int L> struct Grp {
     __device__ float4* coords() const { return reinterpret_cast<float4*>(gb + lay.o_coords); }
     __device__ float4* mf() const { return reinterpret_cast<float4*>(gb + lay.o_mf); }
     __device__ float* frames() const { return gb + lay.o_frames; }
-    __device__ float* nacc() const { return gb + lay.o_nacc; }
     __device__ float* pose(int k) const { return gb + lay.o_poses + k * kPoseStride; }
     __device__ float* bc(int k) const { return gb + lay.o_bc + k * kPoseStride; }
     __device__ float* H() const { return gb + lay.o_H; }
@@ -228,14 +228,17 @@ template <int L> struct Grp {
 template <int L>
 __device__ Grp<L> setup_group(const KArgs& ka, int lig, float* lds) {
     Grp<L> s;
-    s.lay.init(ka.na_cap, ka.nn_cap);
+    s.lay.init(ka.na_cap, ka.nn_cap, ka.np_cap);
     s.h = ka.ls.hdr[lig];
     AtomRec* atoms = reinterpret_cast<AtomRec*>(lds);
     NodeRec* nodes = reinterpret_cast<NodeRec*>(lds + 4 * s.lay.na);
+    PairRec* pairs = reinterpret_cast<PairRec*>(lds + 4 * s.lay.na + 8 * s.lay.nn);
     for (int i = threadIdx.x; i < s.h.n_atoms; i += kWG) atoms[i] = ka.ls.atoms[s.h.atom_begin + i];
     for (int i = threadIdx.x; i < s.h.n_nodes; i += kWG) nodes[i] = ka.ls.nodes[s.h.node_begin + i];
+    for (int i = threadIdx.x; i < s.h.n_pairs; i += kWG) pairs[i] = ka.ls.pairs[s.h.pair_begin + i];
     s.atoms = atoms;
     s.nodes = nodes;
+    s.pairs = pairs;
     s.g = threadIdx.x / L;
     s.gl = threadIdx.x % L;
     s.gb = lds + s.lay.shared_floats + s.g * s.lay.group_floats;
@@ -288,116 +291,207 @@ __device__ __forceinline__ float3 frame_apply(const float* f, const AtomRec& a) 
                        f[2] + (R[6] * a.x + R[7] * a.y + R[8] * a.z));
 }
 
-// grid::evaluate_aux (grid.cpp:39-129) + curl (curl.h:28-36); energy-only
-// when d == nullptr.  FP16 maps interpolate the stored halves and apply the
-// power-of-two map scale to value and gradient before the curl.
-template <int GF>
-__device__ __forceinline__ float grid_sample(const GridDev& g, int slot, float3 p, float v, float3* d) {
+// grid::evaluate_aux (grid.cpp:39-129) + curl (curl.h:28-36), split in three
+// steps so that the corner gathers of several atoms are in flight together:
+// locate (cell, fractions, out-of-box penalty), gather (8 loads), finish.
+struct GridPt {
+    size_t base;
+    float x, y, z, penalty;
+    int reg[3];
+};
+
+__device__ __forceinline__ GridPt grid_locate(const GridDev& g, int slot, float3 p) {
     float s[3] = {(p.x - g.init[0]) * g.factor[0], (p.y - g.init[1]) * g.factor[1], (p.z - g.init[2]) * g.factor[2]};
     float miss[3];
-    int reg[3], a[3];
+    int a[3];
+    GridPt q;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         if (s[i] < 0.f) {
-            miss[i] = -s[i]; reg[i] = -1; a[i] = 0; s[i] = 0.f;
+            miss[i] = -s[i]; q.reg[i] = -1; a[i] = 0; s[i] = 0.f;
         } else if (s[i] >= g.dimm1[i]) {
-            miss[i] = s[i] - g.dimm1[i]; reg[i] = 1; a[i] = g.n[i] - 2; s[i] = 1.f;
+            miss[i] = s[i] - g.dimm1[i]; q.reg[i] = 1; a[i] = g.n[i] - 2; s[i] = 1.f;
         } else {
-            miss[i] = 0.f; reg[i] = 0; a[i] = int(s[i]); s[i] -= float(a[i]);
+            miss[i] = 0.f; q.reg[i] = 0; a[i] = int(s[i]); s[i] -= float(a[i]);
         }
         a[i] = min(max(a[i], 0), g.n[i] - 2);  // a NaN coordinate must not index outside the map
     }
-    const float penalty = g.slope * (miss[0] * g.factor_inv[0] + miss[1] * g.factor_inv[1] + miss[2] * g.factor_inv[2]);
+    q.penalty = g.slope * (miss[0] * g.factor_inv[0] + miss[1] * g.factor_inv[1] + miss[2] * g.factor_inv[2]);
+    q.base = size_t(slot) * g.n_points + a[0] + size_t(g.n[0]) * a[1] + size_t(g.n[0]) * g.n[1] * a[2];
+    q.x = s[0]; q.y = s[1]; q.z = s[2];
+    return q;
+}
+
+template <int GF>
+__device__ __forceinline__ void grid_gather(const GridDev& g, size_t base, float (&f)[8]) {
     const size_t sy = size_t(g.n[0]), sz = size_t(g.n[0]) * g.n[1];
-    const size_t base = size_t(slot) * g.n_points + a[0] + sy * a[1] + sz * a[2];
-    float f000, f100, f010, f110, f001, f101, f011, f111;
     if (GF == UD_GRID_F32) {
         const float* M = reinterpret_cast<const float*>(g.maps);
-        f000 = M[base]; f100 = M[base + 1]; f010 = M[base + sy]; f110 = M[base + sy + 1];
-        f001 = M[base + sz]; f101 = M[base + sz + 1]; f011 = M[base + sy + sz]; f111 = M[base + sy + sz + 1];
+        f[0] = M[base]; f[1] = M[base + 1]; f[2] = M[base + sy]; f[3] = M[base + sy + 1];
+        f[4] = M[base + sz]; f[5] = M[base + sz + 1]; f[6] = M[base + sy + sz]; f[7] = M[base + sy + sz + 1];
     } else {
         const __half* M = reinterpret_cast<const __half*>(g.maps);
-        f000 = __half2float(M[base]); f100 = __half2float(M[base + 1]);
-        f010 = __half2float(M[base + sy]); f110 = __half2float(M[base + sy + 1]);
-        f001 = __half2float(M[base + sz]); f101 = __half2float(M[base + sz + 1]);
-        f011 = __half2float(M[base + sy + sz]); f111 = __half2float(M[base + sy + sz + 1]);
+        f[0] = __half2float(M[base]); f[1] = __half2float(M[base + 1]);
+        f[2] = __half2float(M[base + sy]); f[3] = __half2float(M[base + sy + 1]);
+        f[4] = __half2float(M[base + sz]); f[5] = __half2float(M[base + sz + 1]);
+        f[6] = __half2float(M[base + sy + sz]); f[7] = __half2float(M[base + sy + sz + 1]);
     }
-    const float x = s[0], y = s[1], z = s[2];
+}
+
+// corners f = (000, 100, 010, 110, 001, 101, 011, 111); FP16 maps apply the
+// power-of-two map scale to value and gradient before the curl
+template <int GF>
+__device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const GridPt& q, const float (&f)[8], float v,
+                                             float3* d) {
+    const float x = q.x, y = q.y, z = q.z;
     const float mx = 1.f - x, my = 1.f - y, mz = 1.f - z;
-    float f = f000 * mx * my * mz + f100 * x * my * mz + f010 * mx * y * mz + f110 * x * y * mz +
-              f001 * mx * my * z + f101 * x * my * z + f011 * mx * y * z + f111 * x * y * z;
-    if (GF == UD_GRID_F16) f *= g.scale[slot];
+    float e = f[0] * mx * my * mz + f[1] * x * my * mz + f[2] * mx * y * mz + f[3] * x * y * mz +
+              f[4] * mx * my * z + f[5] * x * my * z + f[6] * mx * y * z + f[7] * x * y * z;
+    if (GF == UD_GRID_F16) e *= g.scale[slot];
     if (d) {
-        float gx = -f000 * my * mz + f100 * my * mz - f010 * y * mz + f110 * y * mz - f001 * my * z + f101 * my * z -
-                   f011 * y * z + f111 * y * z;
-        float gy = -f000 * mx * mz - f100 * x * mz + f010 * mx * mz + f110 * x * mz - f001 * mx * z - f101 * x * z +
-                   f011 * mx * z + f111 * x * z;
-        float gz = -f000 * mx * my - f100 * x * my - f010 * mx * y - f110 * x * y + f001 * mx * my + f101 * x * my +
-                   f011 * mx * y + f111 * x * y;
+        float gx = -f[0] * my * mz + f[1] * my * mz - f[2] * y * mz + f[3] * y * mz - f[4] * my * z + f[5] * my * z -
+                   f[6] * y * z + f[7] * y * z;
+        float gy = -f[0] * mx * mz - f[1] * x * mz + f[2] * mx * mz + f[3] * x * mz - f[4] * mx * z - f[5] * x * z +
+                   f[6] * mx * z + f[7] * x * z;
+        float gz = -f[0] * mx * my - f[1] * x * my - f[2] * mx * y - f[3] * x * y + f[4] * mx * my + f[5] * x * my +
+                   f[6] * mx * y + f[7] * x * y;
         if (GF == UD_GRID_F16) {
             const float sc = g.scale[slot];
             gx *= sc; gy *= sc; gz *= sc;
         }
-        if (f > 0.f && v < 0.1f * kMaxFl) {
-            const float tmp = (v < kEpsFl) ? 0.f : (v / (v + f));
-            f *= tmp;
+        if (e > 0.f && v < 0.1f * kMaxFl) {
+            const float tmp = (v < kEpsFl) ? 0.f : (v / (v + e));
+            e *= tmp;
             const float t2 = tmp * tmp;
             gx *= t2; gy *= t2; gz *= t2;
         }
-        d->x = g.factor[0] * (reg[0] == 0 ? gx : 0.f) + g.slope * float(reg[0]);
-        d->y = g.factor[1] * (reg[1] == 0 ? gy : 0.f) + g.slope * float(reg[1]);
-        d->z = g.factor[2] * (reg[2] == 0 ? gz : 0.f) + g.slope * float(reg[2]);
-    } else {
-        if (f > 0.f && v < 0.1f * kMaxFl) {
-            const float tmp = (v < kEpsFl) ? 0.f : (v / (v + f));
-            f *= tmp;
-        }
+        d->x = g.factor[0] * (q.reg[0] == 0 ? gx : 0.f) + g.slope * float(q.reg[0]);
+        d->y = g.factor[1] * (q.reg[1] == 0 ? gy : 0.f) + g.slope * float(q.reg[1]);
+        d->z = g.factor[2] * (q.reg[2] == 0 ? gz : 0.f) + g.slope * float(q.reg[2]);
+    } else if (e > 0.f && v < 0.1f * kMaxFl) {
+        e *= (v < kEpsFl) ? 0.f : (v / (v + e));
     }
-    return f + penalty;
+    return e + q.penalty;
 }
 
-// Energy (+ gradient) of the group's pose: model::eval_deriv (model.cpp:954-981)
-// = grid term per atom, intramolecular pairs (model.cpp:907-934, pair forces
-// accumulated with LDS atomics), then the torsion-tree derivative
-// (tree.h:75-84, 152-193): per-node force/torque over its own atoms, then
-// children pushed into parents level by level, deepest first.
-// gout[j] receives gradient element gl + j * L.
+// Compile-time tuning of the search kernel: load batching and the occupancy
+// target handed to the register allocator (waves per SIMD; 1 = no target).
+#ifndef UD_ATOM_BATCH
+#define UD_ATOM_BATCH 1
+#endif
+#ifndef UD_PAIR_BATCH
+#define UD_PAIR_BATCH 2
+#endif
+#ifndef UD_SEARCH_WAVES_PER_EU
+#define UD_SEARCH_WAVES_PER_EU 1
+#endif
+static constexpr int kAtomBatch = UD_ATOM_BATCH;  // atoms per lane whose corner gathers are issued together
+static constexpr int kPairBatch = UD_PAIR_BATCH;  // pairs per lane whose table loads are issued together
+
+// Grid energy (+ minus-forces into mf) of the group's atoms: each lane owns a
+// contiguous block of atoms, computes their coordinates from the node frames
+// and gathers the 8 map corners of kAtomBatch atoms at once.
 template <int L, int GF>
-__device__ float eval_group(const KArgs& ka, const Grp<L>& s, const float* pose, float v_intra, float v_inter,
-                            float (&gout)[Grp<L>::KV], bool want_grad) {
-    fk_frames(s, pose);
+__device__ __forceinline__ float atoms_grid(const KArgs& ka, const Grp<L>& s, float v_inter, bool want_grad,
+                                            float& rg2) {
     float4* coords = s.coords();
     float4* mf = s.mf();
-    float* fr = s.frames();
+    const float* fr = s.frames();
     const int N = s.h.n_atoms;
     const int c = (N + L - 1) / L;
     const int i0 = s.gl * c, i1 = min(N, i0 + c);
     float e = 0.f;
-    for (int i = i0; i < i1; ++i) {
-        const AtomRec a = s.atoms[i];
-        const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
-        coords[i] = make_float4(p.x, p.y, p.z, 0.f);
-        const int slot = (a.meta >> 8) & 0xff;
-        float3 d = make_float3(0.f, 0.f, 0.f);
-        if (slot != 0xff) e += grid_sample<GF>(ka.grid, slot, p, v_inter, &d);
-        mf[i] = make_float4(d.x, d.y, d.z, 0.f);
+    for (int ib = i0; ib < i1; ib += kAtomBatch) {
+        GridPt q[kAtomBatch];
+        int slot[kAtomBatch];
+        float f[kAtomBatch][8];
+#pragma unroll
+        for (int u = 0; u < kAtomBatch; ++u) {
+            const int i = ib + u;
+            slot[u] = 0xff;
+            if (i < i1) {
+                const AtomRec a = s.atoms[i];
+                const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
+                coords[i] = make_float4(p.x, p.y, p.z, 0.f);
+                if ((a.meta >> 16) & 1) {  // heavy atom: gyration radius about the root origin
+                    const float dx = p.x - fr[0], dy = p.y - fr[1], dz = p.z - fr[2];
+                    rg2 += dx * dx + dy * dy + dz * dz;
+                }
+                slot[u] = (a.meta >> 8) & 0xff;
+                if (slot[u] != 0xff) q[u] = grid_locate(ka.grid, slot[u], p);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kAtomBatch; ++u)
+            if (slot[u] != 0xff) grid_gather<GF>(ka.grid, q[u].base, f[u]);
+#pragma unroll
+        for (int u = 0; u < kAtomBatch; ++u) {
+            const int i = ib + u;
+            if (i >= i1) continue;
+            float3 d = make_float3(0.f, 0.f, 0.f);
+            if (slot[u] != 0xff) e += grid_finish<GF>(ka.grid, slot[u], q[u], f[u], v_inter, want_grad ? &d : nullptr);
+            if (want_grad) mf[i] = make_float4(d.x, d.y, d.z, 0.f);
+        }
     }
+    return e;
+}
+
+// Energy (+ gradient) of the group's pose: model::eval_deriv (model.cpp:954-981)
+// = grid term per atom, intramolecular pairs (model.cpp:907-934; pair forces
+// accumulated with LDS atomics), then the torsion-tree derivative
+// (tree.h:75-84, 152-193).  Each node's force and torque are summed directly
+// over the atoms of its subtree (a contiguous range in pre-order), which is
+// the reference's bottom-up accumulation written out:
+//   F_k = sum f_i,  T_k = sum (c_i - o_k) x f_i  over atoms i below node k,
+//   dE/dtorsion_k = T_k . axis_k,  root: (F_0, T_0).
+// gout[j] receives gradient element gl + j * L.
+template <int L, int GF>
+__device__ float eval_group(const KArgs& ka, const Grp<L>& s, const float* pose, float v_intra, float v_inter,
+                            float (&gout)[Grp<L>::KV], bool want_grad, float* rgyr = nullptr) {
+    fk_frames(s, pose);
+    float4* coords = s.coords();
+    float4* mf = s.mf();
+    const float* fr = s.frames();
+    float rg2 = 0.f;
+    float e = atoms_grid<L, GF>(ka, s, v_inter, true, rg2);
     __syncthreads();
-    const PairRec* prs = ka.ls.pairs + s.h.pair_begin;
-    for (int q = s.gl; q < s.h.n_pairs; q += L) {
-        const PairRec rec = prs[q];
-        const int a = rec & 0xff, b = (rec >> 8) & 0xff;
-        const float4 ca = coords[a], cb = coords[b];
-        const float rx = cb.x - ca.x, ry = cb.y - ca.y, rz = cb.z - ca.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < 64.f) {
-            const float rf = 32.f * r2;
-            const int i1t = int(rf);
-            const float rem = rf - float(i1t);
-            const float4 t = ka.ls.table_deriv[size_t(rec >> 16) * kTableStride + i1t];
-            float pe = t.x + rem * t.z;
-            const float dor = t.y + rem * t.w;
-            float fx = dor * rx, fy = dor * ry, fz = dor * rz;
+    const int P = s.h.n_pairs;
+    for (int q0 = s.gl; q0 < P; q0 += kPairBatch * L) {
+        int a[kPairBatch], b[kPairBatch];
+        float rx[kPairBatch], ry[kPairBatch], rz[kPairBatch], rem[kPairBatch];
+        int idx[kPairBatch];
+        bool in[kPairBatch];
+        float4 t[kPairBatch];
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u) {
+            const int q = q0 + u * L;
+            in[u] = false;
+            a[u] = 0; b[u] = 0; idx[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
+            if (q < P) {
+                const PairRec rec = s.pairs[q];
+                a[u] = rec & 0xff;
+                b[u] = (rec >> 8) & 0xff;
+                const float4 ca = coords[a[u]], cb = coords[b[u]];
+                rx[u] = cb.x - ca.x; ry[u] = cb.y - ca.y; rz[u] = cb.z - ca.z;
+                const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+                if (r2 < 64.f) {
+                    const float rf = 32.f * r2;
+                    const int i1t = int(rf);
+                    rem[u] = rf - float(i1t);
+                    idx[u] = int(rec >> 16) * kTableStride + i1t;
+                    in[u] = true;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u)
+            if (in[u]) t[u] = ka.ls.table_deriv[idx[u]];
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u) {
+            if (!in[u]) continue;
+            float pe = t[u].x + rem[u] * t[u].z;
+            const float dor = t[u].y + rem[u] * t[u].w;
+            float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
             if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
                 const float tmp = (v_intra < kEpsFl) ? 0.f : (v_intra / (v_intra + pe));
                 pe *= tmp;
@@ -405,50 +499,55 @@ __device__ float eval_group(const KArgs& ka, const Grp<L>& s, const float* pose,
                 fx *= t2; fy *= t2; fz *= t2;
             }
             e += pe;
-            atomicAdd(&mf[a].x, -fx); atomicAdd(&mf[a].y, -fy); atomicAdd(&mf[a].z, -fz);
-            atomicAdd(&mf[b].x, fx); atomicAdd(&mf[b].y, fy); atomicAdd(&mf[b].z, fz);
+            atomicAdd(&mf[a[u]].x, -fx); atomicAdd(&mf[a[u]].y, -fy); atomicAdd(&mf[a[u]].z, -fz);
+            atomicAdd(&mf[b[u]].x, fx); atomicAdd(&mf[b[u]].y, fy); atomicAdd(&mf[b[u]].z, fz);
         }
     }
     e = gsum<L>(e);
+    if (rgyr) {
+        // model::gyration_radius (model.cpp:871-883): heavy atoms about the root origin
+        rg2 = gsum<L>(rg2);
+        *rgyr = s.h.n_heavy > 0 ? sqrtf(rg2 / float(s.h.n_heavy)) : 0.f;
+    }
     __syncthreads();
     if (!want_grad) return e;
-    float* nacc = s.nacc();
     float* gbuf = s.bc(2);
-    for (int k = s.gl; k < s.h.n_nodes; k += L) {
-        const NodeRec nd = s.nodes[k];
-        const int ab = nd.packed & 0xff, ae = (nd.packed >> 8) & 0xff;
-        const float* f = fr + 20 * k;
+    // S lanes share one node's subtree sum (strided over its atoms, then a
+    // shuffle reduction); L / S nodes are handled per pass
+    const int nn = s.h.n_nodes;
+    int S = L;
+    while (S > 1 && S * nn > L) S >>= 1;
+    const int per = L / S;
+    const int kk = s.gl / S, jj = s.gl % S;
+    for (int k0 = 0; k0 < nn; k0 += per) {
+        const int k = k0 + kk;
+        const float* f = fr + 20 * min(k, nn - 1);
         float F0 = 0.f, F1 = 0.f, F2 = 0.f, T0 = 0.f, T1 = 0.f, T2 = 0.f;
-        for (int i = ab; i < ae; ++i) {
-            const float4 fi = mf[i], ci = coords[i];
-            const float dx = ci.x - f[0], dy = ci.y - f[1], dz = ci.z - f[2];
-            F0 += fi.x; F1 += fi.y; F2 += fi.z;
-            T0 += dy * fi.z - dz * fi.y;
-            T1 += dz * fi.x - dx * fi.z;
-            T2 += dx * fi.y - dy * fi.x;
-        }
-        float* na = nacc + 8 * k;
-        na[0] = F0; na[1] = F1; na[2] = F2; na[3] = T0; na[4] = T1; na[5] = T2;
-    }
-    __syncthreads();
-    for (int d = s.h.max_depth; d >= 1; --d) {
-        for (int k = 1 + s.gl; k < s.h.n_nodes; k += L) {
+        if (k < nn) {
             const NodeRec nd = s.nodes[k];
-            if (int(nd.packed >> 24) != d) continue;
-            const float* na = nacc + 8 * k;
-            const float* f = fr + 20 * k;
-            const float* pf = fr + 20 * nd.parent;
-            gbuf[5 + k] = na[3] * f[3] + na[4] * f[4] + na[5] * f[5];
-            const float rx = f[0] - pf[0], ry = f[1] - pf[1], rz = f[2] - pf[2];
-            float* pa = nacc + 8 * nd.parent;
-            atomicAdd(pa + 0, na[0]); atomicAdd(pa + 1, na[1]); atomicAdd(pa + 2, na[2]);
-            atomicAdd(pa + 3, (ry * na[2] - rz * na[1]) + na[3]);
-            atomicAdd(pa + 4, (rz * na[0] - rx * na[2]) + na[4]);
-            atomicAdd(pa + 5, (rx * na[1] - ry * na[0]) + na[5]);
+            const int ab = nd.packed & 0xff, se = (nd.packed >> 16) & 0xff;
+            for (int i = ab + jj; i < se; i += S) {
+                const float4 fi = mf[i], ci = coords[i];
+                const float dx = ci.x - f[0], dy = ci.y - f[1], dz = ci.z - f[2];
+                F0 += fi.x; F1 += fi.y; F2 += fi.z;
+                T0 += dy * fi.z - dz * fi.y;
+                T1 += dz * fi.x - dx * fi.z;
+                T2 += dx * fi.y - dy * fi.x;
+            }
         }
-        __syncthreads();
+        for (int o = S >> 1; o > 0; o >>= 1) {
+            F0 += __shfl_xor(F0, o, kWG); F1 += __shfl_xor(F1, o, kWG); F2 += __shfl_xor(F2, o, kWG);
+            T0 += __shfl_xor(T0, o, kWG); T1 += __shfl_xor(T1, o, kWG); T2 += __shfl_xor(T2, o, kWG);
+        }
+        if (k < nn && jj == 0) {
+            if (k == 0) {
+                gbuf[0] = F0; gbuf[1] = F1; gbuf[2] = F2;
+                gbuf[3] = T0; gbuf[4] = T1; gbuf[5] = T2;
+            } else {
+                gbuf[5 + k] = T0 * f[3] + T1 * f[4] + T2 * f[5];
+            }
+        }
     }
-    if (s.gl < 6) gbuf[s.gl] = nacc[s.gl];
     __syncthreads();
     const int n = s.ndof();
 #pragma unroll
@@ -477,7 +576,7 @@ __device__ __forceinline__ double gdot_d(const float (&a)[Grp<L>::KV], const flo
 
 // out = -H * in over the packed upper triangle (minus_mat_vec_product, bfgs.h:30-38)
 template <int L>
-__device__ void minus_hmatvec(const Grp<L>& s, const float (&in)[Grp<L>::KV], float (&out)[Grp<L>::KV], int n) {
+__device__ __forceinline__ void minus_hmatvec(const Grp<L>& s, const float (&in)[Grp<L>::KV], float (&out)[Grp<L>::KV], int n) {
     float* bc = s.bc(0);
     const float* H = s.H();
 #pragma unroll
@@ -491,7 +590,8 @@ __device__ void minus_hmatvec(const Grp<L>& s, const float (&in)[Grp<L>::KV], fl
         const int i = s.gl + j * L;
         float sum = 0.f;
         if (i < n) {
-            for (int c2 = 0; c2 < n; ++c2) sum += H[tri_index(i, c2)] * bc[c2];
+            const float* row = H + i * (n | 1);
+            for (int c2 = 0; c2 < n; ++c2) sum += row[c2] * bc[c2];
         }
         out[j] = (i < n) ? -sum : 0.f;
     }
@@ -501,7 +601,7 @@ __device__ void minus_hmatvec(const Grp<L>& s, const float (&in)[Grp<L>::KV], fl
 // conf::increment (conf.h:98-104, 50-56, 169-172): position += a p,
 // orientation <- exp(a rot) * q (normalised), torsions wrapped.
 template <int L>
-__device__ void pose_increment(const Grp<L>& s, float* dst, const float* src, const float* p, float alpha, bool pred) {
+__device__ __forceinline__ void pose_increment(const Grp<L>& s, float* dst, const float* src, const float* p, float alpha, bool pred) {
     if (!pred) return;
     const int T = s.ntors();
     for (int idx = s.gl; idx < 4 + T; idx += L) {
@@ -523,114 +623,6 @@ __device__ __forceinline__ void copy_pose(const Grp<L>& s, float* dst, const flo
     for (int idx = s.gl; idx < 7 + s.ntors(); idx += L) dst[idx] = src[idx];
 }
 
-// bfgs() (bfgs.h:97-147) with line_search (bfgs.h:65-85) and bfgs_update
-// (bfgs.h:46-63).  x is the group's pose buffer (in/out).  The curvature
-// scalar y.p is accumulated in FP64.  Groups with active == false run along
-// (uniform control flow) without committing anything.
-template <int L, int GF>
-__device__ float bfgs_group(const KArgs& ka, const Grp<L>& s, float* x, float v_intra, float v_inter, int max_steps,
-                            bool active, int& evals) {
-    constexpr int KV = Grp<L>::KV;
-    float* xn = s.pose(PXN);
-    float* xo = s.pose(PXO);
-    float* H = s.H();
-    const int n = s.ndof();
-    const int nh = n * (n + 1) / 2;
-    for (int idx = s.gl; idx < nh; idx += L) H[idx] = 0.f;
-    for (int i = s.gl; i < n; i += L) H[i + i * (i + 1) / 2] = 1.f;
-    copy_pose(s, xn, x, true);
-    copy_pose(s, xo, x, true);
-    __syncthreads();
-    float g[KV], gn[KV], p[KV], y[KV], go[KV], t[KV];
-    float f0 = eval_group<L, GF>(ka, s, x, v_intra, v_inter, g, true);
-    if (active) ++evals;
-    const float f_orig = f0;
-#pragma unroll
-    for (int j = 0; j < KV; ++j) { go[j] = g[j]; gn[j] = g[j]; y[j] = 0.f; }
-    bool done = !active;
-    for (int step = 0; step < max_steps; ++step) {
-        if (!wave_any(!done)) break;
-        minus_hmatvec<L>(s, g, p, n);
-        const float pg = gdot<L>(p, g);
-        float* pb = s.bc(1);
-#pragma unroll
-        for (int j = 0; j < KV; ++j) {
-            const int idx = s.gl + j * L;
-            if (idx < n) pb[idx] = p[j];
-        }
-        __syncthreads();
-        float alpha = 1.f, f1 = 0.f;
-        bool srch = !done;
-        for (int trial = 0; trial < 10; ++trial) {
-            if (!wave_any(srch)) break;
-            pose_increment(s, xn, x, pb, alpha, srch);
-            __syncthreads();
-            const float ft = eval_group<L, GF>(ka, s, xn, v_intra, v_inter, t, true);
-            if (srch) {
-                ++evals;
-                f1 = ft;
-#pragma unroll
-                for (int j = 0; j < KV; ++j) gn[j] = t[j];
-                if (f1 - f0 < 0.0001f * alpha * pg) srch = false;
-                else alpha *= 0.5f;
-            }
-        }
-        if (!done) {
-#pragma unroll
-            for (int j = 0; j < KV; ++j) y[j] = gn[j] - g[j];
-            f0 = f1;
-        }
-        copy_pose(s, x, xn, !done);
-        __syncthreads();
-        const float gg = gdot<L>(g, g);
-        if (!done && !(sqrtf(gg) >= 1e-5f)) done = true;  // breaks before g = g_new (bfgs.h:131-132)
-        const bool upd = !done;
-        if (upd) {
-#pragma unroll
-            for (int j = 0; j < KV; ++j) g[j] = gn[j];
-        }
-        if (step == 0) {
-            const float yy = gdot<L>(y, y);
-            const float yp0 = gdot<L>(y, p);
-            if (upd && fabsf(yy) > kEpsFl) {
-                const float dv = alpha * yp0 / yy;
-                for (int i = s.gl; i < n; i += L) H[i + i * (i + 1) / 2] = dv;
-            }
-            __syncthreads();
-        }
-        const float yp = float(gdot_d<L>(y, p));
-        const bool do_upd = upd && !(alpha * yp < kEpsFl);
-        minus_hmatvec<L>(s, y, t, n);
-        const float yhy = -gdot<L>(y, t);
-        const float r = 1.f / (alpha * yp);
-        float* ma = s.bc(0);
-        float* pp = s.bc(1);
-#pragma unroll
-        for (int j = 0; j < KV; ++j) {
-            const int idx = s.gl + j * L;
-            if (idx < n) { ma[idx] = t[j]; pp[idx] = p[j]; }
-        }
-        __syncthreads();
-        if (do_upd) {
-            const float c1 = alpha * r, c2 = alpha * alpha * (r * r * yhy + r);
-            for (int idx = s.gl; idx < nh; idx += L) {
-                int jj = int((sqrtf(8.f * float(idx) + 1.f) - 1.f) * 0.5f);
-                while (jj * (jj + 1) / 2 > idx) --jj;
-                while ((jj + 1) * (jj + 2) / 2 <= idx) ++jj;
-                const int ii = idx - jj * (jj + 1) / 2;
-                H[idx] += c1 * (ma[ii] * pp[jj] + ma[jj] * pp[ii]) + c2 * pp[ii] * pp[jj];
-            }
-        }
-        __syncthreads();
-    }
-    const bool revert = active && !(f0 <= f_orig);
-    if (revert) f0 = f_orig;
-    copy_pose(s, x, xo, revert);
-    __syncthreads();
-    return f0;
-}
-
-// ====================================================================== kernels
 template <int L, int GF>
 __global__ __launch_bounds__(64) void score_kernel(KArgs ka, const WorkItem* work, const int32_t* perm,
                                                    const float* poses, float v_intra, float v_inter, float* e_out,
@@ -653,32 +645,6 @@ __global__ __launch_bounds__(64) void score_kernel(KArgs ka, const WorkItem* wor
         for (int j = 0; j < Grp<L>::KV; ++j) {
             const int idx = s.gl + j * L;
             if (idx < UD_GRAD_FLOATS) g_out[size_t(pi) * UD_GRAD_FLOATS + idx] = gout[j];
-        }
-    }
-}
-
-template <int L, int GF>
-__global__ __launch_bounds__(64) void optimize_kernel(KArgs ka, const WorkItem* work, const int32_t* perm, float* poses,
-                                                      float v_intra, float v_inter, int max_steps, float* e_out,
-                                                      int32_t* evals_out) {
-    extern __shared__ float4 lds_raw[];
-    float* lds = reinterpret_cast<float*>(lds_raw);
-    const WorkItem w = work[blockIdx.x];
-    Grp<L> s = setup_group<L>(ka, w.lig, lds);
-    const bool active = s.g < w.count;
-    const int pi = active ? perm[w.first + s.g] : 0;
-    float* x = s.pose(PX);
-    const int np = 7 + s.ntors();
-    for (int i = s.gl; i < np; i += L) x[i] = active ? poses[size_t(pi) * UD_POSE_FLOATS + i] : (i == 3 ? 1.f : 0.f);
-    __syncthreads();
-    const int steps = max_steps > 0 ? max_steps : s.h.local_steps;
-    int evals = 0;
-    const float e = bfgs_group<L, GF>(ka, s, x, v_intra, v_inter, steps, active, evals);
-    if (active) {
-        for (int i = s.gl; i < np; i += L) poses[size_t(pi) * UD_POSE_FLOATS + i] = x[i];
-        if (s.gl == 0) {
-            if (e_out) e_out[pi] = e;
-            if (evals_out) evals_out[pi] = evals;
         }
     }
 }
@@ -740,80 +706,258 @@ __device__ void mutate(float* pose, int T, float rgyr, const SearchDev& sp, uint
     }
 }
 
-// gyration radius of the group's pose (model::gyration_radius, model.cpp:871-883)
-template <int L>
-__device__ float gyration_group(const Grp<L>& s, const float* pose) {
-    fk_frames(s, pose);
-    const float* fr = s.frames();
-    float acc = 0.f;
-    for (int i = s.gl; i < s.h.n_atoms; i += L) {
-        const AtomRec a = s.atoms[i];
-        if (!((a.meta >> 16) & 1)) continue;
-        const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
-        const float dx = p.x - pose[0], dy = p.y - pose[1], dz = p.z - pose[2];
-        acc += dx * dx + dy * dy + dz * dz;
-    }
-    acc = gsum<L>(acc);
-    __syncthreads();
-    return s.h.n_heavy > 0 ? sqrtf(acc / float(s.h.n_heavy)) : 0.f;
-}
+// States of a lane group's search.  Every loop iteration evaluates exactly
+// one pose per group (X in ST_INIT / ST_RGYR, the line-search trial XN in
+// ST_TRIAL), then advances each group's own state, so groups never wait for
+// one another's line searches and the energy+gradient code exists once.
+enum { ST_RGYR = 0, ST_INIT = 1, ST_TRIAL = 2, ST_IDLE = 3 };
 
-// Monte Carlo chains (kernel<32, Config>, monte_carlo.cu:236-356): one chain
-// per lane group, all groups of the workgroup on one ligand.
-template <int L, int GF>
-__global__ __launch_bounds__(64) void dock_kernel(KArgs ka, SearchDev sp, const WorkItem* work, float* out_pose,
-                                                  float* out_e, long long* out_evals) {
+// BFGS local search (bfgs() with line_search and bfgs_update, bfgs.h:30-147;
+// device twin bfgs_warp, warp_ops.cuh:1051-1119) and, when MC is set, the
+// Monte Carlo chain around it (kernel<32, Config>, monte_carlo.cu:236-356):
+// mutate -> BFGS with hunt_cap -> Metropolis at T -> if the accepted pose
+// beats the chain's best, BFGS with authentic_v and keep it.
+// MC == false runs one BFGS per group on the given poses (quasi_newton).
+// The curvature scalar y.p is accumulated in FP64.
+template <int L, int GF, bool MC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(UD_SEARCH_WAVES_PER_EU)))
+void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* perm,
+                                                    float* poses_io, float v_intra_opt, float v_inter_opt,
+                                                    int max_steps_opt, float* out_pose, float* out_e,
+                                                    long long* out_evals, int32_t* out_evals32) {
+    constexpr int KV = Grp<L>::KV;
     extern __shared__ float4 lds_raw[];
     float* lds = reinterpret_cast<float*>(lds_raw);
     const WorkItem w = work[blockIdx.x];
     Grp<L> s = setup_group<L>(ka, w.lig, lds);
     const bool active = s.g < w.count;
-    const uint32_t chain = uint32_t(w.first + (active ? s.g : 0));
     const int T = s.ntors();
     const int np = 7 + T;
-    float* tmp = s.pose(PTMP);
-    float* best = s.pose(PBEST);
-    float* cand = s.pose(PCAND);
-    if (s.gl == 0) random_start(tmp, T, sp, chain);
-    __syncthreads();
-    float rgyr = gyration_group<L>(s, tmp);
-    copy_pose(s, best, tmp, true);
-    float tmp_e = 0.f, best_e = __builtin_huge_valf();
-    int evals = 0;
-    const int lsteps = sp.local_steps_override > 0 ? sp.local_steps_override : s.h.local_steps;
+    const int n = s.ndof();
+    const int hs = n | 1;  // row stride of the full inverse Hessian
+    const int nh = n * hs;
+    float* X = s.pose(PX);
+    float* XN = s.pose(PXN);
+    float* XO = s.pose(PXO);
+    float* TMP = s.pose(PTMP);
+    float* BEST = s.pose(PBEST);
+    float* H = s.H();
+    float* pb = s.bc(1);  // search direction, read by the pose increment
+    const uint32_t chain = uint32_t(w.first + (active ? s.g : 0));
+    const int pi = (!MC && active) ? perm[w.first + s.g] : 0;
+    int lsteps;
+    if (MC) lsteps = sp.local_steps_override > 0 ? sp.local_steps_override : s.h.local_steps;
+    else lsteps = max_steps_opt > 0 ? max_steps_opt : s.h.local_steps;
     const uint2 key = make_uint2(sp.seed_lo, sp.seed_hi);
-    for (int step = 0; step < sp.global_steps; ++step) {
-        copy_pose(s, cand, tmp, true);
+
+    // ---- per-group state (group-uniform registers)
+    int state = ST_INIT, phase = 0, mc_step = 0, step = 0, trial = 0, evals = 0;
+    float vI = v_intra_opt, vE = v_inter_opt;
+    float f0 = 0.f, f_orig = 0.f, alpha = 1.f, pg = 0.f;
+    float tmp_e = 0.f, best_e = __builtin_huge_valf(), rg_tmp = 0.f, rg_x = 0.f, rg_o = 0.f;
+    float g[KV], p[KV], t[KV], y[KV], mhy[KV];
+#pragma unroll
+    for (int j = 0; j < KV; ++j) { g[j] = 0.f; p[j] = 0.f; y[j] = 0.f; }
+
+    if (MC) {
+        if (s.gl == 0) random_start(TMP, T, sp, chain);
         __syncthreads();
-        if (s.gl == 0) mutate(cand, T, rgyr, sp, chain, uint32_t(step));
-        __syncthreads();
-        const float cand_e = bfgs_group<L, GF>(ka, s, cand, sp.hunt[0], sp.hunt[1], lsteps, active, evals);
-        const uint4 o = philox4x32_10(make_uint4(uint32_t(step) + 1u, chain, 2u, 0u), key);
-        bool acc = cand_e < tmp_e;
-        if (!acc) acc = u01(o.x) < expf((tmp_e - cand_e) / sp.temperature);  // metropolis_accept (monte_carlo.cu:92-97)
-        const bool take = active && (step == 0 || acc);
-        copy_pose(s, tmp, cand, take);
-        if (take) tmp_e = cand_e;
-        __syncthreads();
-        const bool need = take && tmp_e < best_e;
-        if (wave_any(need)) {
-            const float e2 = bfgs_group<L, GF>(ka, s, tmp, sp.authentic[0], sp.authentic[1], lsteps, need, evals);
-            if (need) tmp_e = e2;
-        }
-        const bool better = need && tmp_e < best_e;
-        copy_pose(s, best, tmp, better);
-        if (better) best_e = tmp_e;
-        __syncthreads();
-        if (wave_any(take)) {
-            const float rg = gyration_group<L>(s, tmp);
-            if (take) rgyr = rg;
-        }
+        copy_pose(s, X, TMP, true);
+        copy_pose(s, BEST, TMP, true);
+        state = ST_RGYR;  // the first evaluation only measures the start's gyration radius
+        if (sp.global_steps == 0) state = ST_IDLE;
+    } else {
+        for (int i = s.gl; i < np; i += L) X[i] = active ? poses_io[size_t(pi) * UD_POSE_FLOATS + i] : (i == 3 ? 1.f : 0.f);
     }
-    if (active) {
-        for (int i = s.gl; i < UD_POSE_FLOATS; i += L) out_pose[size_t(chain) * UD_POSE_FLOATS + i] = (i < np) ? best[i] : 0.f;
+    if (!active) state = ST_IDLE;
+    __syncthreads();
+
+    for (;;) {
+        if (!wave_any(state != ST_IDLE)) break;
+        // ---------------------------------------------------------------- evaluate
+        const bool trialing = state == ST_TRIAL;
+        float rg = 0.f;
+        const float f = eval_group<L, GF>(ka, s, trialing ? XN : X, vI, vE, t, true, &rg);
+        if (state != ST_IDLE) ++evals;
+        // ---------------------------------------------------------------- line search / BFGS bookkeeping
+        const float gg = gdot<L>(g, g);
+        const bool is_init = state == ST_INIT;
+        const bool ok = trialing && (f - f0 < 0.0001f * alpha * pg);
+        const bool ls_end = trialing && (ok || trial == 9);  // 10 trials (bfgs.h:70-83)
+        if (trialing && !ok) alpha *= 0.5f;
+        if (trialing && !ls_end) ++trial;
+        const int step_now = step;
+        bool finished = false;
+        if (ls_end) {
+#pragma unroll
+            for (int j = 0; j < KV; ++j) y[j] = t[j] - g[j];
+            f0 = f;
+            rg_x = rg;
+        }
+        const bool conv = ls_end && !(sqrtf(gg) >= 1e-5f);  // uses g before g = g_new (bfgs.h:131-132)
+        const bool cont = ls_end && !conv;
+        if (cont) {
+#pragma unroll
+            for (int j = 0; j < KV; ++j) g[j] = t[j];
+            ++step;
+        }
+        if (conv || (cont && step >= lsteps)) finished = true;
+        if (is_init) {
+            f0 = f;
+            f_orig = f;
+            rg_x = rg;
+            rg_o = rg;
+            step = 0;
+#pragma unroll
+            for (int j = 0; j < KV; ++j) g[j] = t[j];
+            if (lsteps <= 0) finished = true;
+        }
+        // X <- XN after a finished line search; XO <- X and H <- I at BFGS start
+        copy_pose(s, X, XN, ls_end);
+        copy_pose(s, XO, X, is_init);
+        if (is_init) {
+            for (int idx = s.gl; idx < nh; idx += L) H[idx] = 0.f;
+            for (int i = s.gl; i < n; i += L) H[i * hs + i] = 1.f;
+        }
+        __syncthreads();
+        // initial scaling H = (y.p / y.y) I after the first step (bfgs.h:135-138)
+        const float yy = gdot<L>(y, y);
+        const float yp0 = gdot<L>(y, p);
+        if (cont && step_now == 0 && fabsf(yy) > kEpsFl) {
+            const float dv = alpha * yp0 / yy;
+            for (int i = s.gl; i < n; i += L) H[i * hs + i] = dv;
+        }
+        __syncthreads();
+        // rank-2 update (bfgs_update, bfgs.h:46-63); skipped on the last step,
+        // where the reference's update is never used
+        const bool upd_pre = cont && !finished;
+        const float yp = float(gdot_d<L>(y, p));
+        const bool do_upd = upd_pre && !(alpha * yp < kEpsFl);
+        minus_hmatvec<L>(s, y, mhy, n);
+        const float yhy = -gdot<L>(y, mhy);
+        {
+            float* ma = s.bc(0);
+#pragma unroll
+            for (int j = 0; j < KV; ++j) {
+                const int idx = s.gl + j * L;
+                if (idx < n) { ma[idx] = mhy[j]; pb[idx] = p[j]; }
+            }
+            __syncthreads();
+            if (do_upd) {
+                const float r = 1.f / (alpha * yp);
+                const float c1 = alpha * r, c2 = alpha * alpha * (r * r * yhy + r);
+                // one row per lane; each entry uses the (min, max) index order of
+                // the reference's upper-triangle update, so H stays exactly symmetric
+                for (int i = s.gl; i < n; i += L) {
+                    float* row = H + i * hs;
+                    for (int j = 0; j < n; ++j) {
+                        const int a = min(i, j), b = max(i, j);
+                        row[j] += c1 * (ma[a] * pb[b] + ma[b] * pb[a]) + c2 * pb[a] * pb[b];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // new search direction p = -H g (minus_mat_vec_product, bfgs.h:30-38)
+        const bool new_dir = !finished && (is_init || cont);
+        minus_hmatvec<L>(s, g, t, n);
+        if (new_dir) {
+#pragma unroll
+            for (int j = 0; j < KV; ++j) p[j] = t[j];
+        }
+        const float pg_new = gdot<L>(p, g);
+        if (new_dir) {
+            pg = pg_new;
+            alpha = 1.f;
+            trial = 0;
+            state = ST_TRIAL;
+        }
+        // ---------------------------------------------------------------- BFGS end
+        bool revert = false;
+        float e_res = 0.f;
+        if (finished) {
+            revert = !(f0 <= f_orig);  // bfgs.h:140-144
+            if (revert) { f0 = f_orig; rg_x = rg_o; }
+            e_res = f0;
+        }
+        copy_pose(s, X, XO, revert);
+        __syncthreads();
+        // ---------------------------------------------------------------- Monte Carlo transitions
+        bool start_bfgs = false, next_step = false;
+        if (!MC) {
+            if (finished) state = ST_IDLE;
+        } else {
+            if (state == ST_RGYR) {
+                rg_tmp = rg;
+                next_step = true;
+                mc_step = -1;
+            }
+            if (finished && phase == 0) {
+                const uint4 o = philox4x32_10(make_uint4(uint32_t(mc_step) + 1u, chain, 2u, 0u), key);
+                bool acc = e_res < tmp_e;
+                if (!acc) acc = u01(o.x) < expf((tmp_e - e_res) / sp.temperature);  // metropolis_accept (monte_carlo.cu:92-97)
+                const bool take = mc_step == 0 || acc;
+                if (take) { tmp_e = e_res; rg_tmp = rg_x; }
+                if (take && tmp_e < best_e) {
+                    phase = 1;
+                    start_bfgs = true;
+                } else {
+                    next_step = true;
+                }
+                copy_pose(s, TMP, X, take);
+            } else if (finished && phase == 1) {
+                tmp_e = e_res;
+                rg_tmp = rg_x;
+                copy_pose(s, TMP, X, true);
+                const bool better = tmp_e < best_e;
+                if (better) best_e = tmp_e;
+                copy_pose(s, BEST, X, better);
+                phase = 0;
+                next_step = true;
+            }
+        }
+        __syncthreads();
+        if (MC && next_step) {
+            ++mc_step;
+            if (mc_step >= sp.global_steps) {
+                state = ST_IDLE;
+                next_step = false;
+            } else {
+                copy_pose(s, X, TMP, true);
+            }
+        }
+        __syncthreads();
+        if (MC && next_step && s.gl == 0) mutate(X, T, rg_tmp, sp, chain, uint32_t(mc_step));
+        if (MC && (next_step || start_bfgs)) {
+            state = ST_INIT;
+            vI = phase == 0 ? sp.hunt[0] : sp.authentic[0];
+            vE = phase == 0 ? sp.hunt[1] : sp.authentic[1];
+        }
+        // ---------------------------------------------------------------- next trial pose
+        if (state == ST_TRIAL) {
+#pragma unroll
+            for (int j = 0; j < KV; ++j) {
+                const int idx = s.gl + j * L;
+                if (idx < n) pb[idx] = p[j];
+            }
+        }
+        __syncthreads();
+        pose_increment(s, XN, X, pb, alpha, state == ST_TRIAL);
+        __syncthreads();
+    }
+
+    if (!active) return;
+    if (MC) {
+        for (int i = s.gl; i < UD_POSE_FLOATS; i += L) out_pose[size_t(chain) * UD_POSE_FLOATS + i] = (i < np) ? BEST[i] : 0.f;
         if (s.gl == 0) {
             out_e[chain] = best_e;
             out_evals[chain] = evals;
+        }
+    } else {
+        for (int i = s.gl; i < np; i += L) poses_io[size_t(pi) * UD_POSE_FLOATS + i] = X[i];
+        if (s.gl == 0) {
+            if (out_e) out_e[pi] = f0;
+            if (out_evals32) out_evals32[pi] = evals;
         }
     }
 }
@@ -836,17 +980,10 @@ __global__ __launch_bounds__(64) void terms_kernel(KArgs ka, const WorkItem* wor
     __syncthreads();
     fk_frames(s, x);
     float4* coords = s.coords();
-    const float* fr = s.frames();
-    float ei = 0.f, ea = 0.f;
-    for (int i = s.gl; i < s.h.n_atoms; i += L) {
-        const AtomRec a = s.atoms[i];
-        const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
-        coords[i] = make_float4(p.x, p.y, p.z, 0.f);
-        const int slot = (a.meta >> 8) & 0xff;
-        if (slot != 0xff) ei += grid_sample<GF>(ka.grid, slot, p, v, nullptr);
-    }
+    float rg2 = 0.f;
+    float ei = atoms_grid<L, GF>(ka, s, v, false, rg2), ea = 0.f;
     __syncthreads();
-    const PairRec* prs = ka.ls.pairs + s.h.pair_begin;
+    const PairRec* prs = s.pairs;
     for (int q = s.gl; q < s.h.n_pairs; q += L) {
         const PairRec rec = prs[q];
         const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
@@ -895,18 +1032,19 @@ __global__ __launch_bounds__(64) void coords_kernel(KArgs ka, const WorkItem* wo
 }
 
 // ====================================================================== launchers
-size_t lds_bytes(int lanes, int na_cap, int nn_cap) {
+size_t lds_bytes(int lanes, const LaunchCap& cap) {
     Layout l;
-    l.init(na_cap, nn_cap);
+    l.init(cap.n_atoms, cap.n_nodes, cap.n_pairs);
     return l.bytes(kWG / lanes);
 }
 
-static KArgs make_kargs(const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap) {
+static KArgs make_kargs(const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap) {
     KArgs k;
     k.ls = ls;
     k.grid = g;
-    k.na_cap = na_cap;
-    k.nn_cap = nn_cap;
+    k.na_cap = cap.n_atoms;
+    k.nn_cap = cap.n_nodes;
+    k.np_cap = cap.n_pairs;
     return k;
 }
 
@@ -944,11 +1082,11 @@ hipError_t launch_grid_to_half(const float* maps, int n_points, int n_maps, cons
     return hipGetLastError();
 }
 
-hipError_t launch_score(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+hipError_t launch_score(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                         const WorkItem* work, int n_work, const int32_t* perm, const float* poses, float v_intra,
                         float v_inter, float* e_out, float* g_out, hipStream_t st) {
-    const KArgs ka = make_kargs(ls, g, na_cap, nn_cap);
-    const size_t shm = lds_bytes(lanes, na_cap, nn_cap);
+    const KArgs ka = make_kargs(ls, g, cap);
+    const size_t shm = lds_bytes(lanes, cap);
     UD_DISPATCH_L(lanes, {
         if (gf == UD_GRID_F16)
             hipLaunchKernelGGL((score_kernel<L, UD_GRID_F16>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
@@ -960,43 +1098,44 @@ hipError_t launch_score(int lanes, int gf, const LigandSetDev& ls, const GridDev
     return hipGetLastError();
 }
 
-hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                            const WorkItem* work, int n_work, const int32_t* perm, float* poses, float v_intra,
                            float v_inter, int max_steps, float* e_out, int32_t* evals_out, hipStream_t st) {
-    const KArgs ka = make_kargs(ls, g, na_cap, nn_cap);
-    const size_t shm = lds_bytes(lanes, na_cap, nn_cap);
+    const KArgs ka = make_kargs(ls, g, cap);
+    const size_t shm = lds_bytes(lanes, cap);
+    SearchDev sp = {};
     UD_DISPATCH_L(lanes, {
         if (gf == UD_GRID_F16)
-            hipLaunchKernelGGL((optimize_kernel<L, UD_GRID_F16>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm,
-                               poses, v_intra, v_inter, max_steps, e_out, evals_out);
+            hipLaunchKernelGGL((search_kernel<L, UD_GRID_F16, false>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work,
+                               perm, poses, v_intra, v_inter, max_steps, nullptr, e_out, nullptr, evals_out);
         else
-            hipLaunchKernelGGL((optimize_kernel<L, UD_GRID_F32>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm,
-                               poses, v_intra, v_inter, max_steps, e_out, evals_out);
+            hipLaunchKernelGGL((search_kernel<L, UD_GRID_F32, false>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work,
+                               perm, poses, v_intra, v_inter, max_steps, nullptr, e_out, nullptr, evals_out);
     });
     return hipGetLastError();
 }
 
-hipError_t launch_dock(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+hipError_t launch_dock(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                        const SearchDev& sp, const WorkItem* work, int n_work, float* out_pose, float* out_e,
                        long long* out_evals, hipStream_t st) {
-    const KArgs ka = make_kargs(ls, g, na_cap, nn_cap);
-    const size_t shm = lds_bytes(lanes, na_cap, nn_cap);
+    const KArgs ka = make_kargs(ls, g, cap);
+    const size_t shm = lds_bytes(lanes, cap);
     UD_DISPATCH_L(lanes, {
         if (gf == UD_GRID_F16)
-            hipLaunchKernelGGL((dock_kernel<L, UD_GRID_F16>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work, out_pose,
-                               out_e, out_evals);
+            hipLaunchKernelGGL((search_kernel<L, UD_GRID_F16, true>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work,
+                               nullptr, nullptr, sp.hunt[0], sp.hunt[1], 0, out_pose, out_e, out_evals, nullptr);
         else
-            hipLaunchKernelGGL((dock_kernel<L, UD_GRID_F32>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work, out_pose,
-                               out_e, out_evals);
+            hipLaunchKernelGGL((search_kernel<L, UD_GRID_F32, true>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work,
+                               nullptr, nullptr, sp.hunt[0], sp.hunt[1], 0, out_pose, out_e, out_evals, nullptr);
     });
     return hipGetLastError();
 }
 
-hipError_t launch_terms(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+hipError_t launch_terms(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                         const WorkItem* work, int n_work, const int32_t* perm, const float* poses, float v,
                         float* inter_out, float* intra_out, hipStream_t st) {
-    const KArgs ka = make_kargs(ls, g, na_cap, nn_cap);
-    const size_t shm = lds_bytes(lanes, na_cap, nn_cap);
+    const KArgs ka = make_kargs(ls, g, cap);
+    const size_t shm = lds_bytes(lanes, cap);
     UD_DISPATCH_L(lanes, {
         if (gf == UD_GRID_F16)
             hipLaunchKernelGGL((terms_kernel<L, UD_GRID_F16>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
@@ -1008,11 +1147,11 @@ hipError_t launch_terms(int lanes, int gf, const LigandSetDev& ls, const GridDev
     return hipGetLastError();
 }
 
-hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                          const WorkItem* work, int n_work, const int32_t* perm, const float* poses, float* coords_out,
                          hipStream_t st) {
-    const KArgs ka = make_kargs(ls, g, na_cap, nn_cap);
-    const size_t shm = lds_bytes(lanes, na_cap, nn_cap);
+    const KArgs ka = make_kargs(ls, g, cap);
+    const size_t shm = lds_bytes(lanes, cap);
     UD_DISPATCH_L(lanes, {
         hipLaunchKernelGGL((coords_kernel<L>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses, coords_out);
     });

@@ -18,7 +18,7 @@ struct GridBuildArgs {
     int32_t types[UD_MAX_MAPS];
 };
 
-size_t lds_bytes(int lanes, int na_cap, int nn_cap);
+size_t lds_bytes(int lanes, const LaunchCap& cap);
 
 hipError_t launch_tables(int sf, const float* w, float4* deriv, float* fast, hipStream_t st);
 hipError_t launch_grid_build(const float4* rec, int n_rec, const GridBuildArgs& a, const float* table_fast,
@@ -26,19 +26,19 @@ hipError_t launch_grid_build(const float4* rec, int n_rec, const GridBuildArgs& 
 hipError_t launch_grid_absmax(const float* maps, int n_points, int n_maps, unsigned int* out, hipStream_t st);
 hipError_t launch_grid_to_half(const float* maps, int n_points, int n_maps, const float* inv_scale, void* out,
                                hipStream_t st);
-hipError_t launch_score(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+hipError_t launch_score(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                         const WorkItem* work, int n_work, const int32_t* perm, const float* poses, float v_intra,
                         float v_inter, float* e_out, float* g_out, hipStream_t st);
-hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                            const WorkItem* work, int n_work, const int32_t* perm, float* poses, float v_intra,
                            float v_inter, int max_steps, float* e_out, int32_t* evals_out, hipStream_t st);
-hipError_t launch_dock(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap,
+hipError_t launch_dock(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                        const SearchDev& sp, const WorkItem* work, int n_work, float* out_pose, float* out_e,
                        long long* out_evals, hipStream_t st);
-hipError_t launch_terms(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap, const WorkItem* work,
+hipError_t launch_terms(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap, const WorkItem* work,
                         int n_work, const int32_t* perm, const float* poses, float v, float* inter_out,
                         float* intra_out, hipStream_t st);
-hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, int na_cap, int nn_cap, const WorkItem* work,
+hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap, const WorkItem* work,
                          int n_work, const int32_t* perm, const float* poses, float* coords_out, hipStream_t st);
 
 }  // namespace ud

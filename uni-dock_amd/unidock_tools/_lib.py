@@ -19,6 +19,8 @@ UD_MAX_ATOMS = 128
 UD_MAX_TORSIONS = 32
 UD_POSE_FLOATS = 7 + UD_MAX_TORSIONS
 UD_GRAD_FLOATS = 6 + UD_MAX_TORSIONS
+UD_N_BUCKETS = 6
+BUCKET_ATOMS = (16, 32, 48, 64, 96, 128)
 
 VINA_WEIGHTS = (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 50.0, 0.05846)  # vina.h:129-132
 VINARDO_WEIGHTS = (-0.045, 0.8, -0.035, -0.600, 50.0, 0.05846)                      # vina.h:133-135
@@ -34,6 +36,11 @@ class UdLigandBatch(ctypes.Structure):
                 ("node_parent", _c_int_p), ("node_abegin", _c_int_p), ("node_aend", _c_int_p),
                 ("node_origin", _c_float_p), ("node_axis", _c_float_p),
                 ("pairs", _c_int_p), ("num_tors", _c_float_p)]
+
+
+class UdLaunchInfo(ctypes.Structure):
+    _fields_ = [("atom_cap", ctypes.c_int32), ("lanes", ctypes.c_int32), ("n_ligands", ctypes.c_int32),
+                ("n_workgroups", ctypes.c_int32), ("ms", ctypes.c_double)]
 
 
 class UdSearchParams(ctypes.Structure):
@@ -82,6 +89,9 @@ def load() -> ctypes.CDLL:
         "ud_pose_coords": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp]),
         "ud_last_kernel_time": (ctypes.c_int, [vp, dp, ip]),
         "ud_set_lanes": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ud_last_launches": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(UdLaunchInfo), ip]),
+        "ud_set_bucket_lanes": (ctypes.c_int, [vp, ip]),
+        "ud_autotune": (ctypes.c_int, [vp, ctypes.POINTER(UdSearchParams), ctypes.c_int, ip]),
         "ud_synth_receptor": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                              ctypes.c_longlong, dp]),
     }
@@ -219,7 +229,7 @@ class Context:
                                     _p(vv, ctypes.c_float), max_steps, _p(e, ctypes.c_float), _p(ev, ctypes.c_int32)))
         return P, e, ev
 
-    def dock(self, exhaustiveness=8, global_steps=20, seed=181129, local_steps=0, lanes=0, fetch=True):
+    def _params(self, exhaustiveness, global_steps, seed, local_steps, lanes):
         p = UdSearchParams()
         self.L.ud_default_search_params(ctypes.byref(p))
         p.exhaustiveness = exhaustiveness
@@ -227,6 +237,10 @@ class Context:
         p.seed = seed
         p.local_steps = local_steps
         p.lanes_per_chain = lanes
+        return p
+
+    def dock(self, exhaustiveness=8, global_steps=20, seed=181129, local_steps=0, lanes=0, fetch=True):
+        p = self._params(exhaustiveness, global_steps, seed, local_steps, lanes)
         self._ck(self.L.ud_dock(self.h, ctypes.byref(p), None, None, None))
         self._last_dock = (self.n_ligands, exhaustiveness)
         if fetch:
@@ -266,3 +280,23 @@ class Context:
         n = ctypes.c_int32()
         self._ck(self.L.ud_last_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def last_launches(self):
+        """Per-launch (bucket) timing of the last dock: list of dicts."""
+        info = (UdLaunchInfo * UD_N_BUCKETS)()
+        n = ctypes.c_int32()
+        self._ck(self.L.ud_last_launches(self.h, UD_N_BUCKETS, info, ctypes.byref(n)))
+        return [dict(atom_cap=i.atom_cap, lanes=i.lanes, n_ligands=i.n_ligands, n_workgroups=i.n_workgroups,
+                     ms=i.ms) for i in info[:n.value]]
+
+    def set_bucket_lanes(self, lanes):
+        a = np.zeros(UD_N_BUCKETS, dtype=np.int32)
+        a[:len(lanes)] = lanes
+        self._ck(self.L.ud_set_bucket_lanes(self.h, _p(a, ctypes.c_int32)))
+
+    def autotune(self, exhaustiveness=8, global_steps=2, seed=181129, local_steps=0, sample_ligands=64):
+        """Pick lanes per chain per size bucket by timing short docks; returns the chosen lanes."""
+        p = self._params(exhaustiveness, global_steps, seed, local_steps, 0)
+        out = np.zeros(UD_N_BUCKETS, dtype=np.int32)
+        self._ck(self.L.ud_autotune(self.h, ctypes.byref(p), sample_ligands, _p(out, ctypes.c_int32)))
+        return out
