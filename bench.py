@@ -29,6 +29,10 @@ sys.path.insert(0, ROOT)
 from unidock_tools import _lib, synth  # noqa: E402
 
 HBM_PEAK = 8.0e12
+# HBM bytes per search-kernel launch measured by FETCH_SIZE / WRITE_SIZE
+# counter runs of this benchmark (copy-kernel calibrated), one table row per
+# kernel and workgroup count
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "search_kernel_hbm_traffic_batched_docking.md")
 LIGAND_TYPES = [0, 1, 2, 3, 8, 9, 10]
 
 
@@ -138,6 +142,19 @@ def fp64_error(rec, ligs, ctx16, res, exh, n_sample):
     return float(np.max(errs)), float(np.mean(errs))
 
 
+def measured_traffic(kernel, workgroups):
+    """HBM bytes of one launch from the committed counter table, or None."""
+    try:
+        lines = open(TRAFFIC_TABLE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].strip("`") == kernel and cells[1] == str(workgroups):
+            return float(cells[4].split()[0]) * 1e9
+    return None
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of `mib` MiB in one elementwise kernel: the
     reference point for the FETCH_SIZE / WRITE_SIZE counters (it reads and
@@ -189,7 +206,9 @@ def main():
     tuned = None
     if args.lanes == 0:
         t1 = time.perf_counter()
-        tuned = [int(x) for x in ctx16.autotune(exhaustiveness=exh, global_steps=2, sample_ligands=64)]
+        # whole buckets, 2 MC steps: a sample must fill the GPU as the real
+        # launch does, or lane counts with more workgroups win by occupancy
+        tuned = [int(x) for x in ctx16.autotune(exhaustiveness=exh, global_steps=2, sample_ligands=len(ligs))]
         log(f"[bench] autotuned lanes per bucket {dict(zip(_lib.BUCKET_ATOMS, tuned))} "
             f"in {time.perf_counter() - t1:.1f}s (untimed set-up)")
     log("[bench] batched FP16 form")
@@ -210,8 +229,9 @@ def main():
     sel = bucket_mask(ligs, dom["atom_cap"])
     bytes_dom = algorithmic_bytes(ligs, exh, b["evals"], 2, sel)
     achieved = bytes_dom / (dom["ms"] * 1e-3)
+    dom_kernel = f"void ud::search_kernel<{dom['lanes']}, 1, true, 16>"
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=None)
+                traffic=measured_traffic(dom_kernel, dom["n_workgroups"]))
     log(f"[bench] dominant launch: bucket <= {dom['atom_cap']} atoms, {dom['lanes']} lanes, "
         f"{dom['n_ligands']} ligands, {dom['ms']:.2f} ms, {bytes_dom / 1e6:.1f} MB algorithmic")
     err_max, err_mean = fp64_error(rec, ligs, ctx16, b, exh, 64)

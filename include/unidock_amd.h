@@ -73,7 +73,7 @@ typedef struct ud_launch_info {
     int32_t lanes;         /* lanes per chain */
     int32_t n_ligands;
     int32_t n_workgroups;  /* one wavefront each */
-    double ms;             /* kernel time, HIP events on the context's stream */
+    double ms;             /* kernel time, HIP events around the launch on its stream */
 } ud_launch_info;
 
 typedef struct ud_ctx ud_ctx;
@@ -186,7 +186,9 @@ int ud_optimize(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, float* pos
 
 /* Monte Carlo docking of every ligand of the batch, exhaustiveness chains
  * each, in one launch per size bucket (kernel<32, Config>,
- * monte_carlo.cu:236-356, launched at monte_carlo.cu:2877-2882).
+ * monte_carlo.cu:236-356, launched at monte_carlo.cu:2877-2882).  The
+ * bucket launches run concurrently on streams of the context, ordered after
+ * earlier work on the context's stream and before later work on it.
  * Outputs per chain (ligand-major, chain-minor): best_poses
  * [n_ligands * exh * UD_POSE_FLOATS], best_e [n_ligands * exh], evals
  * [n_ligands * exh] (energy+gradient evaluations).  Any output may be NULL:

@@ -360,7 +360,9 @@ def test_autotune_and_bucket_lanes(scene):
     assert [a["atom_cap"] for a in li] == sorted(occupied, reverse=True)
     for a in li:
         assert a["lanes"] == tuned[_lib.BUCKET_ATOMS.index(a["atom_cap"])] and a["ms"] > 0
-    assert sum(a["ms"] for a in li) == pytest.approx(total_ms, rel=1e-3, abs=1e-2)
+    # bucket launches overlap on their own streams: each lies inside the dock's span
+    assert max(a["ms"] for a in li) <= total_ms * (1 + 1e-3) + 1e-2
+    assert total_ms <= sum(a["ms"] for a in li) * (1 + 1e-3) + 0.5
     ctx.set_bucket_lanes([0] * _lib.UD_N_BUCKETS)
     ctx.set_bucket_lanes(list(tuned))
     P2, e2, ev2 = ctx.dock(exhaustiveness=8, global_steps=3, seed=4)

@@ -23,7 +23,7 @@ struct ud_ctx {
     int sf = -1;
     float w[7] = {0, 0, 0, 0, 0, 0, 0};  // term weights (glue at [5] for Vina, [4] for Vinardo) + internal rot
     float rot_internal = 0.f;
-    float4* d_table_deriv = nullptr;
+    float2* d_table_deriv = nullptr;
     float* d_table_fast = nullptr;
     // grid
     bool box_set = false;
@@ -45,6 +45,8 @@ struct ud_ctx {
     AtomRec* d_atoms = nullptr;
     NodeRec* d_nodes = nullptr;
     PairRec* d_pairs = nullptr;
+    PartnerRec* d_partners = nullptr;
+    AtomPartners* d_apar = nullptr;
     // scratch / results
     WorkItem* d_work = nullptr;
     size_t work_cap = 0;
@@ -67,7 +69,11 @@ struct ud_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
     int last_launches = 0;
-    // per-launch timing of the last ud_dock: event k marks the end of launch k
+    // ud_dock runs its size-bucket launches concurrently, launch k on
+    // bstream[k] (forked from and joined back to `stream` with events);
+    // lev0[k] / lev[k] bracket launch k
+    hipStream_t bstream[UD_N_BUCKETS] = {};
+    hipEvent_t lev0[UD_N_BUCKETS] = {};
     hipEvent_t lev[UD_N_BUCKETS] = {};
     ud_launch_info linfo[UD_N_BUCKETS] = {};
     // lanes per chain per size bucket (ud_autotune / ud_set_bucket_lanes; 0 = default)
@@ -134,6 +140,8 @@ LigandSetDev lig_dev(const ud_ctx* c) {
     l.atoms = c->d_atoms;
     l.nodes = c->d_nodes;
     l.pairs = c->d_pairs;
+    l.partners = c->d_partners;
+    l.atom_partners = c->d_apar;
     l.table_deriv = c->d_table_deriv;
     l.table_fast = c->d_table_fast;
     return l;
@@ -241,7 +249,9 @@ int ud_create(int device_id, void* stream, ud_ctx** out) {
     for (int i = 0; i < 36; ++i) c->slot_of_type[i] = -1;
     for (int i = 0; i < UD_MAX_MAPS; ++i) c->scale[i] = 1.f;
     bool ev_ok = hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
-    for (int k = 0; k < UD_N_BUCKETS && ev_ok; ++k) ev_ok = hipEventCreate(&c->lev[k]) == hipSuccess;
+    for (int k = 0; k < UD_N_BUCKETS && ev_ok; ++k)
+        ev_ok = hipEventCreate(&c->lev[k]) == hipSuccess && hipEventCreate(&c->lev0[k]) == hipSuccess &&
+                hipStreamCreateWithFlags(&c->bstream[k], hipStreamNonBlocking) == hipSuccess;
     if (!ev_ok) {
         ud_destroy(c);
         return UD_ERR_HIP;
@@ -255,14 +265,18 @@ int ud_destroy(ud_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_table_deriv, c->d_table_fast, c->d_maps, c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs,
+                    c->d_partners, c->d_apar,
                     c->d_work, c->d_perm, c->d_poses, c->d_f1, c->d_f2, c->d_i1, c->d_out_pose, c->d_out_e,
                     c->d_out_evals};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    for (hipEvent_t e : c->lev)
-        if (e) (void)hipEventDestroy(e);
+    for (int k = 0; k < UD_N_BUCKETS; ++k) {
+        if (c->lev[k]) (void)hipEventDestroy(c->lev[k]);
+        if (c->lev0[k]) (void)hipEventDestroy(c->lev0[k]);
+        if (c->bstream[k]) (void)hipStreamDestroy(c->bstream[k]);
+    }
     delete c;
     return UD_OK;
 }
@@ -489,6 +503,8 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     std::vector<AtomRec> atoms;
     std::vector<NodeRec> nodes;
     std::vector<PairRec> pairs;
+    std::vector<PartnerRec> partners;
+    std::vector<AtomPartners> apar;
     std::vector<int> nat(L), nnd(L);
     std::vector<std::vector<int>> xs(L);
     for (int l = 0; l < L; ++l) {
@@ -498,6 +514,8 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
         const int na = a1 - a0, nn = k1 - k0, np = p1 - p0;
         const std::string tag = "ligand " + std::to_string(l) + ": ";
         if (na <= 0 || nn <= 0 || np < 0) return fail(c, UD_ERR_INVALID, tag + "empty or negative ranges");
+        if (a0 != int(atoms.size()) || k0 != int(nodes.size()) || p0 != int(pairs.size()))
+            return fail(c, UD_ERR_INVALID, tag + "offsets must start at 0 and be contiguous");
         if (na > UD_MAX_ATOMS) return fail(c, UD_ERR_CAPACITY, tag + "more than UD_MAX_ATOMS atoms");
         if (nn - 1 > UD_MAX_TORSIONS) return fail(c, UD_ERR_CAPACITY, tag + "more than UD_MAX_TORSIONS torsions");
         if (np > UD_MAX_PAIRS) return fail(c, UD_ERR_CAPACITY, tag + "more than UD_MAX_PAIRS pairs");
@@ -570,6 +588,19 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
             if (ta < 0 || tb < 0) return fail(c, UD_ERR_INVALID, tag + "pair with an untyped atom");
             pairs.push_back(uint32_t(pa) | (uint32_t(pb) << 8) | (uint32_t(tri_index(ta, tb)) << 16));
         }
+        // every pair under both of its atoms, in pair-list order
+        std::vector<std::vector<PartnerRec>> part(na);
+        for (int q = 0; q < np; ++q) {
+            const int pa = b->pairs[2 * (p0 + q)], pb = b->pairs[2 * (p0 + q) + 1];
+            const uint32_t slot = uint32_t(tri_index(b->atom_xs[a0 + pa], b->atom_xs[a0 + pb])) << 16;
+            part[pa].push_back(uint32_t(pb) | slot);
+            part[pb].push_back(uint32_t(pa) | slot);
+        }
+        const size_t pbeg = partners.size();
+        for (int i = 0; i < na; ++i) {
+            apar.push_back(uint32_t(partners.size() - pbeg) | (uint32_t(part[i].size()) << 20));
+            partners.insert(partners.end(), part[i].begin(), part[i].end());
+        }
         LigHdr& h = hdr[l];
         std::memset(&h, 0, sizeof(h));
         h.atom_begin = a0;
@@ -582,15 +613,19 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
         h.local_steps = (25 + na) / 3;  // vina.h:197
         h.num_tors = b->num_tors[l];
         h.n_heavy = n_heavy;
+        h.partner_begin = int32_t(pbeg);
         nat[l] = na;
         nnd[l] = nn;
     }
     // commit
-    for (void* p : {static_cast<void*>(c->d_hdr), static_cast<void*>(c->d_nodes), static_cast<void*>(c->d_pairs)})
+    for (void* p : {static_cast<void*>(c->d_hdr), static_cast<void*>(c->d_nodes), static_cast<void*>(c->d_pairs),
+                    static_cast<void*>(c->d_partners), static_cast<void*>(c->d_apar)})
         if (p) (void)hipFree(p);
     c->d_hdr = nullptr;
     c->d_nodes = nullptr;
     c->d_pairs = nullptr;
+    c->d_partners = nullptr;
+    c->d_apar = nullptr;
     size_t cap = 0;
     int rc = grow(c, c->d_hdr, cap, hdr.size());
     if (rc) return rc;
@@ -600,6 +635,15 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     cap = 0;
     rc = grow(c, c->d_pairs, cap, std::max<size_t>(pairs.size(), 1));
     if (rc) return rc;
+    cap = 0;
+    rc = grow(c, c->d_partners, cap, std::max<size_t>(partners.size(), 1));
+    if (rc) return rc;
+    cap = 0;
+    rc = grow(c, c->d_apar, cap, apar.size());
+    if (rc) return rc;
+    if (!partners.empty())
+        UD_HIP(c, hipMemcpy(c->d_partners, partners.data(), partners.size() * sizeof(PartnerRec), hipMemcpyHostToDevice));
+    UD_HIP(c, hipMemcpy(c->d_apar, apar.data(), apar.size() * sizeof(AtomPartners), hipMemcpyHostToDevice));
     UD_HIP(c, hipMemcpy(c->d_hdr, hdr.data(), hdr.size() * sizeof(LigHdr), hipMemcpyHostToDevice));
     UD_HIP(c, hipMemcpy(c->d_nodes, nodes.data(), nodes.size() * sizeof(NodeRec), hipMemcpyHostToDevice));
     if (!pairs.empty()) UD_HIP(c, hipMemcpy(c->d_pairs, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
@@ -757,6 +801,7 @@ int ud_optimize(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses
 namespace {
 
 const int kBucketCaps[UD_N_BUCKETS] = {16, 32, 48, 64, 96, 128};
+const int kChainsPerGroup = 4;
 
 int bucket_of(int n_atoms) {
     int bi = 0;
@@ -842,9 +887,13 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
         std::stable_sort(order.begin(), order.end(), [&](int a, int b2) {
             return c->lig_nat[a] * c->lig_nn[a] > c->lig_nat[b2] * c->lig_nn[b2];
         });
+        // a work item holds up to kChainsPerGroup chains per lane group; a
+        // group that finishes a chain takes the item's next one
+        const int per_item = G * kChainsPerGroup;
         std::vector<WorkItem> w;
         for (int l : order)
-            for (int c0 = 0; c0 < exh; c0 += G) w.push_back(WorkItem{l, l * exh + c0, std::min(G, exh - c0), 0});
+            for (int c0 = 0; c0 < exh; c0 += per_item)
+                w.push_back(WorkItem{l, l * exh + c0, std::min(per_item, exh - c0), 0});
         total_work += w.size();
         ud_launch_info& li = c->linfo[nl++];
         li.atom_cap = kBucketCaps[bi];
@@ -864,11 +913,15 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
     UD_HIP(c, hipEventRecord(c->ev0, c->stream));
     size_t off = 0;
     for (size_t k = 0; k < works.size(); ++k) {
+        hipStream_t bs = c->bstream[k];
+        UD_HIP(c, hipStreamWaitEvent(bs, c->ev0, 0));
+        UD_HIP(c, hipEventRecord(c->lev0[k], bs));
         UD_HIP(c, launch_dock(lanes_v[k], c->precision, lig_dev(c), grid_dev(c), cap_v[k], sp, c->d_work + off,
-                              int(works[k].size()), c->d_out_pose, c->d_out_e, c->d_out_evals, c->stream));
-        UD_HIP(c, hipEventRecord(c->lev[k], c->stream));
+                              int(works[k].size()), c->d_out_pose, c->d_out_e, c->d_out_evals, bs));
+        UD_HIP(c, hipEventRecord(c->lev[k], bs));
         off += works[k].size();
     }
+    for (size_t k = 0; k < works.size(); ++k) UD_HIP(c, hipStreamWaitEvent(c->stream, c->lev[k], 0));
     UD_HIP(c, hipEventRecord(c->ev1, c->stream));
     c->last_launches = nl;
     return UD_OK;
@@ -881,7 +934,7 @@ void dock_times(ud_ctx* c) {
     c->last_ms = ms;
     for (int k = 0; k < c->last_launches; ++k) {
         float m = 0;
-        (void)hipEventElapsedTime(&m, k == 0 ? c->ev0 : c->lev[k - 1], c->lev[k]);
+        (void)hipEventElapsedTime(&m, c->lev0[k], c->lev[k]);
         c->linfo[k].ms = m;
     }
 }

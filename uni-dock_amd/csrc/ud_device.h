@@ -13,28 +13,91 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ud_internal.h"
 
 namespace ud {
 
 static constexpr int kWG = 64;
 static constexpr int kNMax = 6 + UD_MAX_TORSIONS;   // gradient length capacity
-static constexpr int kPoseStride = 40;              // >= UD_POSE_FLOATS, LDS pose buffers
 static constexpr float kPi = 3.14159265358979f;     // common.h:204 (fl pi)
 static constexpr float kEpsFl = 1.1920928955078125e-07f;  // epsilon_fl (common.h:227-230)
 static constexpr float kMaxFl = 3.402823466e+38f;
 
 // ------------------------------------------------------------ lane groups
-template <int L> __device__ __forceinline__ float gsum(float v) {
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWG);
+// Sums over aligned groups of lanes with DPP moves, which run in the vector
+// ALU (a lane shuffle would go through the LDS pipe, the busiest unit of the
+// search loop).  Within a 16-lane row: quad_perm [1,0,3,2] and [2,3,0,1],
+// then half-row and row mirrors (after the quad step every lane of a quad
+// holds the quad's sum, so mirrored partners always sit in the other half).
+// Across rows: row_bcast:15 / row_bcast:31 accumulate row totals into the
+// last lane of the group, which readlane broadcasts.  Every lane of the
+// wavefront must be active.
+enum : int {
+    kDppQuadXor1 = 0xB1,     // quad_perm [1,0,3,2]
+    kDppQuadXor2 = 0x4E,     // quad_perm [2,3,0,1]
+    kDppRowHalfMirror = 0x141,
+    kDppRowMirror = 0x140,
+    kDppRowBcast15 = 0x142,
+    kDppRowBcast31 = 0x143,
+};
+template <int CTRL, int ROWS = 0xF> __device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS = 0xF> __device__ __forceinline__ double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROWS, 0xF, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(int(b), lane);
+    const int hi = __builtin_amdgcn_readlane(int(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// sum over the S-lane subgroup (S in 1, 2, 4, ..., 64; runtime, wave-uniform)
+template <class T> __device__ __forceinline__ T sub_sum(T v, int S) {
+    auto mv = [](T x, auto ctrl_tag, auto rows_tag) -> T {
+        constexpr int C = decltype(ctrl_tag)::value, R = decltype(rows_tag)::value;
+        if constexpr (sizeof(T) == 8) return dpp_d<C, R>(x);
+        else return dpp_f<C, R>(x);
+    };
+    using I1 = std::integral_constant<int, kDppQuadXor1>;
+    using I2 = std::integral_constant<int, kDppQuadXor2>;
+    using I4 = std::integral_constant<int, kDppRowHalfMirror>;
+    using I8 = std::integral_constant<int, kDppRowMirror>;
+    using B15 = std::integral_constant<int, kDppRowBcast15>;
+    using B31 = std::integral_constant<int, kDppRowBcast31>;
+    using RA = std::integral_constant<int, 0xF>;
+    using R13 = std::integral_constant<int, 0xA>;
+    using R23 = std::integral_constant<int, 0xC>;
+    if (S >= 2) v += mv(v, I1{}, RA{});
+    if (S >= 4) v += mv(v, I2{}, RA{});
+    if (S >= 8) v += mv(v, I4{}, RA{});
+    if (S >= 16) v += mv(v, I8{}, RA{});
+    if (S >= 32) {
+        v += mv(v, B15{}, R13{});  // rows 1 and 3 add the totals of rows 0 and 2
+        if (S == 64) {
+            v += mv(v, B31{}, R23{});  // rows 2 and 3 add lane 31
+            if constexpr (sizeof(T) == 8) v = readlane_d(v, 63);
+            else v = readlane_f(v, 63);
+        } else {
+            T a, b;
+            if constexpr (sizeof(T) == 8) { a = readlane_d(v, 31); b = readlane_d(v, 63); }
+            else { a = readlane_f(v, 31); b = readlane_f(v, 63); }
+            v = (threadIdx.x < 32) ? a : b;
+        }
+    }
     return v;
 }
-template <int L> __device__ __forceinline__ double gsum_d(double v) {
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWG);
-    return v;
-}
+template <int L> __device__ __forceinline__ float gsum(float v) { return sub_sum<float>(v, L); }
+template <int L> __device__ __forceinline__ double gsum_d(double v) { return sub_sum<double>(v, L); }
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
 
 // ------------------------------------------------------------ angles / quaternions

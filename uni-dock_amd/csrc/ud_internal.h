@@ -20,7 +20,8 @@ struct LigHdr {
     int32_t local_steps;  // BFGS iterations, (25 + n_atoms) / 3 (vina.h:197)
     float num_tors;       // conf_independent torsion count
     int32_t n_heavy;
-    int32_t pad[2];
+    int32_t partner_begin;  // into the partner array (2 * n_pairs records)
+    int32_t pad;
 };
 static_assert(sizeof(LigHdr) == 48, "LigHdr layout");
 
@@ -47,6 +48,11 @@ struct NodeRec {
 
 // Pair record: a | b << 8 | table slot << 16
 typedef uint32_t PairRec;
+// Partner record (each pair listed once under each of its atoms): other
+// atom | table slot << 16.  Per atom, AtomPartners = first partner (relative
+// to the ligand's partner_begin) | count << 20.
+typedef uint32_t PartnerRec;
+typedef uint32_t AtomPartners;
 
 // Pair-table entry for the derivative path: e_k, dor_k, e_{k+1} - e_k,
 // dor_{k+1} - dor_k (precalculate_element::eval_deriv, precalculate.h:48-65)
@@ -70,7 +76,9 @@ struct LigandSetDev {
     const AtomRec* atoms;
     const NodeRec* nodes;
     const PairRec* pairs;
-    const float4* table_deriv;  // slot * kTableUsed
+    const PartnerRec* partners;
+    const AtomPartners* atom_partners;  // per atom, indexed like atoms
+    const float2* table_deriv;  // (e, dor) per sample, slot * kTableStride
     const float* table_fast;    // slot * kTableUsed
 };
 

@@ -80,7 +80,9 @@ __device__ float sf_eval(int sf, int t1, int t2, float r, const SfWeights& W) {
 
 // One workgroup per type pair: smooth e at rs[k] = sqrt(k / 32), dor by
 // central difference, fast = midpoint (precalculate.h:66-85, 160-171).
-__global__ __launch_bounds__(256) void table_kernel(int sf, SfWeights W, float4* deriv_out, float* fast_out) {
+// The derivative table holds (e, dor) per sample (8 B); the interpolation
+// forms the differences to the next sample itself, as eval_deriv does.
+__global__ __launch_bounds__(256) void table_kernel(int sf, SfWeights W, float2* deriv_out, float* fast_out) {
     __shared__ float e[kTableN];
     __shared__ float rs[kTableN];
     const int slot = blockIdx.x;
@@ -99,8 +101,7 @@ __global__ __launch_bounds__(256) void table_kernel(int sf, SfWeights W, float4*
         return (e[k + 1] - e[k - 1]) / (delta * rs[k]);
     };
     for (int k = threadIdx.x; k < kTableStride; k += blockDim.x) {
-        const float d0 = dor(k), d1 = dor(k + 1);
-        deriv_out[size_t(slot) * kTableStride + k] = make_float4(e[k], d0, e[k + 1] - e[k], d1 - d0);
+        deriv_out[size_t(slot) * kTableStride + k] = make_float2(e[k], dor(k));
         fast_out[size_t(slot) * kTableStride + k] = (e[k + 1] + e[k]) / 2.f;
     }
 }
@@ -174,29 +175,33 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 
 // ====================================================================== per-group machinery
 // LDS layout of one workgroup: the ligand's atoms and nodes, then one block
-// per lane group (coordinates, minus-forces, node frames, node force/torque
-// accumulators, six pose buffers, three broadcast vectors, inverse Hessian).
+// per lane group (coordinates, minus-forces, node frames, five pose buffers,
+// three broadcast vectors, inverse Hessian).  Pose and vector strides follow
+// the launch's node capacity, so small ligands take little LDS per chain and
+// many chains fit a compute unit.  Pair records stay in global memory.
 struct Layout {
-    int na, nn, np, nc, nh;
+    int na, nn, nc, nh, ps;
     int o_coords, o_mf, o_frames, o_poses, o_bc, o_H, group_floats, shared_floats;
-    __host__ __device__ void init(int na_, int nn_, int np_) {
+    __host__ __device__ void init(int na_, int nn_) {
         na = na_;
         nn = nn_;
-        np = (np_ + 3) & ~3;
         nc = 6 + nn - 1;
+        ps = (7 + nn - 1 + 3) & ~3;  // pose floats (7 + torsions), >= nc
         nh = (nc * (nc | 1) + 3) & ~3;  // full symmetric inverse Hessian, odd row stride
         o_coords = 0;
         o_mf = 4 * na;
         o_frames = 8 * na;
         o_poses = o_frames + 20 * nn;
-        o_bc = o_poses + 6 * kPoseStride;
-        o_H = o_bc + 3 * kPoseStride;
-        group_floats = (o_H + nh + 3) & ~3;
-        shared_floats = 4 * na + 8 * nn + np;  // atoms, nodes, pair records
+        o_bc = o_poses + 5 * ps;
+        o_H = o_bc + 3 * ps;
+        // group blocks start 4 floats apart modulo 32 banks, so that the
+        // same offset in up to 8 groups falls into different LDS banks
+        group_floats = ((o_H + nh + 31) & ~31) + 4;
+        shared_floats = 4 * na + 8 * nn;  // atoms, nodes
     }
     __host__ __device__ size_t bytes(int groups) const { return size_t(shared_floats + groups * group_floats) * 4; }
 };
-enum { PX = 0, PXN = 1, PXO = 2, PTMP = 3, PBEST = 4, PCAND = 5 };
+enum { PX = 0, PXN = 1, PXO = 2, PTMP = 3, PBEST = 4 };
 
 struct KArgs {
     LigandSetDev ls;
@@ -204,12 +209,14 @@ struct KArgs {
     int na_cap, nn_cap, np_cap;
 };
 
-template <int L> struct Grp {
+// NM: compile-time capacity of the gradient / search vectors (6 + torsions);
+// each lane keeps KV = ceil(NM / L) of their entries in registers.
+template <int L, int NM = kNMax> struct Grp {
     static constexpr int G = kWG / L;
-    static constexpr int KV = (kNMax + L - 1) / L;
+    static constexpr int KV = (NM + L - 1) / L;
     const AtomRec* atoms;  // LDS (shared)
     const NodeRec* nodes;  // LDS (shared)
-    const PairRec* pairs;  // LDS (shared)
+    const PairRec* pairs;  // global memory
     float* gb;             // LDS group block
     Layout lay;
     LigHdr h;
@@ -217,28 +224,26 @@ template <int L> struct Grp {
     __device__ float4* coords() const { return reinterpret_cast<float4*>(gb + lay.o_coords); }
     __device__ float4* mf() const { return reinterpret_cast<float4*>(gb + lay.o_mf); }
     __device__ float* frames() const { return gb + lay.o_frames; }
-    __device__ float* pose(int k) const { return gb + lay.o_poses + k * kPoseStride; }
-    __device__ float* bc(int k) const { return gb + lay.o_bc + k * kPoseStride; }
+    __device__ float* pose(int k) const { return gb + lay.o_poses + k * lay.ps; }
+    __device__ float* bc(int k) const { return gb + lay.o_bc + k * lay.ps; }
     __device__ float* H() const { return gb + lay.o_H; }
     __device__ int ntors() const { return h.n_nodes - 1; }
     __device__ int ndof() const { return 6 + h.n_nodes - 1; }
 };
 
 // Stage the ligand of `lig` into LDS and set up the lane's group.
-template <int L>
-__device__ Grp<L> setup_group(const KArgs& ka, int lig, float* lds) {
-    Grp<L> s;
-    s.lay.init(ka.na_cap, ka.nn_cap, ka.np_cap);
+template <int L, int NM>
+__device__ Grp<L, NM> setup_group(const KArgs& ka, int lig, float* lds) {
+    Grp<L, NM> s;
+    s.lay.init(ka.na_cap, ka.nn_cap);
     s.h = ka.ls.hdr[lig];
     AtomRec* atoms = reinterpret_cast<AtomRec*>(lds);
     NodeRec* nodes = reinterpret_cast<NodeRec*>(lds + 4 * s.lay.na);
-    PairRec* pairs = reinterpret_cast<PairRec*>(lds + 4 * s.lay.na + 8 * s.lay.nn);
     for (int i = threadIdx.x; i < s.h.n_atoms; i += kWG) atoms[i] = ka.ls.atoms[s.h.atom_begin + i];
     for (int i = threadIdx.x; i < s.h.n_nodes; i += kWG) nodes[i] = ka.ls.nodes[s.h.node_begin + i];
-    for (int i = threadIdx.x; i < s.h.n_pairs; i += kWG) pairs[i] = ka.ls.pairs[s.h.pair_begin + i];
     s.atoms = atoms;
     s.nodes = nodes;
-    s.pairs = pairs;
+    s.pairs = ka.ls.pairs + s.h.pair_begin;
     s.g = threadIdx.x / L;
     s.gl = threadIdx.x % L;
     s.gb = lds + s.lay.shared_floats + s.g * s.lay.group_floats;
@@ -249,8 +254,8 @@ __device__ Grp<L> setup_group(const KArgs& ka, int lig, float* lds) {
 // Forward kinematics (model::set, tree.h:121-144, 176-214): root frame from
 // the pose, then tree levels parent-first, then atom coordinates.  Coordinates
 // land in LDS; node frames (origin, axis, quaternion, rotation matrix) too.
-template <int L>
-__device__ void fk_frames(const Grp<L>& s, const float* pose) {
+template <int L, int NM>
+__device__ void fk_frames(const Grp<L, NM>& s, const float* pose) {
     float* fr = s.frames();
     if (s.gl == 0) {
         const float4 q = make_float4(pose[3], pose[4], pose[5], pose[6]);
@@ -376,6 +381,8 @@ __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const G
 
 // Compile-time tuning of the search kernel: load batching and the occupancy
 // target handed to the register allocator (waves per SIMD; 1 = no target).
+// Full-wavefront groups (64 lanes) always target 4 waves per SIMD: with one
+// chain per wavefront they are register-bound, and measured faster so.
 #ifndef UD_ATOM_BATCH
 #define UD_ATOM_BATCH 1
 #endif
@@ -391,8 +398,8 @@ static constexpr int kPairBatch = UD_PAIR_BATCH;  // pairs per lane whose table 
 // Grid energy (+ minus-forces into mf) of the group's atoms: each lane owns a
 // contiguous block of atoms, computes their coordinates from the node frames
 // and gathers the 8 map corners of kAtomBatch atoms at once.
-template <int L, int GF>
-__device__ __forceinline__ float atoms_grid(const KArgs& ka, const Grp<L>& s, float v_inter, bool want_grad,
+template <int L, int NM, int GF>
+__device__ __forceinline__ float atoms_grid(const KArgs& ka, const Grp<L, NM>& s, float v_inter, bool want_grad,
                                             float& rg2) {
     float4* coords = s.coords();
     float4* mf = s.mf();
@@ -436,73 +443,127 @@ __device__ __forceinline__ float atoms_grid(const KArgs& ka, const Grp<L>& s, fl
     return e;
 }
 
+// Atom coordinates from the node frames (model::set, tree.h:176-214), lane
+// gl handling atoms gl, gl + L, ...; also the heavy-atom sum of squared
+// distances to the root origin (model::gyration_radius, model.cpp:871-883).
+template <int L, int NM>
+__device__ __forceinline__ void atoms_coords(const Grp<L, NM>& s, float& rg2) {
+    float4* coords = s.coords();
+    const float* fr = s.frames();
+    for (int i = s.gl; i < s.h.n_atoms; i += L) {
+        const AtomRec a = s.atoms[i];
+        const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
+        coords[i] = make_float4(p.x, p.y, p.z, 0.f);
+        if ((a.meta >> 16) & 1) {
+            const float dx = p.x - fr[0], dy = p.y - fr[1], dz = p.z - fr[2];
+            rg2 += dx * dx + dy * dy + dz * dz;
+        }
+    }
+}
+
+// Energy and gradient of each of the lane's atoms (atoms gl, gl + L, ...):
+// the grid term (cache::eval_deriv, cache.cpp:160-199) plus every
+// intramolecular pair the atom belongs to (eval_interacting_pairs_deriv,
+// model.cpp:907-934, with precalculate_element::eval_deriv,
+// precalculate.h:48-65).  Each pair is evaluated by the owners of both of
+// its atoms; each keeps the force on its own atom (the two results are exact
+// negatives of each other), so every gradient is written once by one lane
+// and no atomics are needed.  A pair's energy is counted on the side of its
+// lower atom index.
+template <int L, int NM, int GF>
+__device__ __forceinline__ float atoms_forces(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter) {
+    const float4* coords = s.coords();
+    float4* mf = s.mf();
+    const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
+    const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
+    float e = 0.f;
+    for (int i = s.gl; i < s.h.n_atoms; i += L) {
+        const AtomRec a = s.atoms[i];
+        const float4 ci = coords[i];
+        const int slot = (a.meta >> 8) & 0xff;
+        GridPt gq;
+        float gf[8];
+        if (slot != 0xff) {
+            gq = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
+            grid_gather<GF>(ka.grid, gq.base, gf);
+        }
+        const uint32_t ap = apl[i];
+        const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        for (int k = 0; k < kn; k += kPairBatch) {
+            int other[kPairBatch], idx[kPairBatch];
+            float rx[kPairBatch], ry[kPairBatch], rz[kPairBatch], rem[kPairBatch];
+            bool in[kPairBatch];
+            float2 t0[kPairBatch], t1[kPairBatch];
+#pragma unroll
+            for (int u = 0; u < kPairBatch; ++u) {
+                in[u] = false;
+                other[u] = 0; idx[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
+                if (k + u < kn) {
+                    const PartnerRec rec = part[k0 + k + u];
+                    other[u] = int(rec & 0xff);
+                    const float4 cb = coords[other[u]];
+                    rx[u] = cb.x - ci.x; ry[u] = cb.y - ci.y; rz[u] = cb.z - ci.z;
+                    const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+                    if (r2 < 64.f) {
+                        const float rf = 32.f * r2;
+                        const int i1t = int(rf);
+                        rem[u] = rf - float(i1t);
+                        idx[u] = int(rec >> 16) * kTableStride + i1t;
+                        in[u] = true;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kPairBatch; ++u)
+                if (in[u]) {
+                    t0[u] = ka.ls.table_deriv[idx[u]];
+                    t1[u] = ka.ls.table_deriv[idx[u] + 1];
+                }
+#pragma unroll
+            for (int u = 0; u < kPairBatch; ++u) {
+                if (!in[u]) continue;
+                float pe = t0[u].x + rem[u] * (t1[u].x - t0[u].x);
+                const float dor = t0[u].y + rem[u] * (t1[u].y - t0[u].y);
+                float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
+                if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
+                    const float tmp = (v_intra < kEpsFl) ? 0.f : (v_intra / (v_intra + pe));
+                    pe *= tmp;
+                    const float t2 = tmp * tmp;
+                    fx *= t2; fy *= t2; fz *= t2;
+                }
+                if (other[u] > i) e += pe;
+                gx -= fx; gy -= fy; gz -= fz;
+            }
+        }
+        float3 d = make_float3(0.f, 0.f, 0.f);
+        if (slot != 0xff) e += grid_finish<GF>(ka.grid, slot, gq, gf, v_inter, &d);
+        mf[i] = make_float4(d.x + gx, d.y + gy, d.z + gz, 0.f);
+    }
+    return e;
+}
+
 // Energy (+ gradient) of the group's pose: model::eval_deriv (model.cpp:954-981)
-// = grid term per atom, intramolecular pairs (model.cpp:907-934; pair forces
-// accumulated with LDS atomics), then the torsion-tree derivative
+// = grid term and intramolecular pairs per atom (atoms_forces), then the
+// torsion-tree derivative
 // (tree.h:75-84, 152-193).  Each node's force and torque are summed directly
 // over the atoms of its subtree (a contiguous range in pre-order), which is
 // the reference's bottom-up accumulation written out:
 //   F_k = sum f_i,  T_k = sum (c_i - o_k) x f_i  over atoms i below node k,
 //   dE/dtorsion_k = T_k . axis_k,  root: (F_0, T_0).
 // gout[j] receives gradient element gl + j * L.
-template <int L, int GF>
-__device__ float eval_group(const KArgs& ka, const Grp<L>& s, const float* pose, float v_intra, float v_inter,
-                            float (&gout)[Grp<L>::KV], bool want_grad, float* rgyr = nullptr) {
+template <int L, int NM, int GF>
+__device__ float eval_group(const KArgs& ka, const Grp<L, NM>& s, const float* pose, float v_intra, float v_inter,
+                            float (&gout)[Grp<L, NM>::KV], bool want_grad, float* rgyr = nullptr) {
     fk_frames(s, pose);
     float4* coords = s.coords();
     float4* mf = s.mf();
     const float* fr = s.frames();
     float rg2 = 0.f;
-    float e = atoms_grid<L, GF>(ka, s, v_inter, true, rg2);
+    atoms_coords(s, rg2);
     __syncthreads();
-    const int P = s.h.n_pairs;
-    for (int q0 = s.gl; q0 < P; q0 += kPairBatch * L) {
-        int a[kPairBatch], b[kPairBatch];
-        float rx[kPairBatch], ry[kPairBatch], rz[kPairBatch], rem[kPairBatch];
-        int idx[kPairBatch];
-        bool in[kPairBatch];
-        float4 t[kPairBatch];
-#pragma unroll
-        for (int u = 0; u < kPairBatch; ++u) {
-            const int q = q0 + u * L;
-            in[u] = false;
-            a[u] = 0; b[u] = 0; idx[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
-            if (q < P) {
-                const PairRec rec = s.pairs[q];
-                a[u] = rec & 0xff;
-                b[u] = (rec >> 8) & 0xff;
-                const float4 ca = coords[a[u]], cb = coords[b[u]];
-                rx[u] = cb.x - ca.x; ry[u] = cb.y - ca.y; rz[u] = cb.z - ca.z;
-                const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
-                if (r2 < 64.f) {
-                    const float rf = 32.f * r2;
-                    const int i1t = int(rf);
-                    rem[u] = rf - float(i1t);
-                    idx[u] = int(rec >> 16) * kTableStride + i1t;
-                    in[u] = true;
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kPairBatch; ++u)
-            if (in[u]) t[u] = ka.ls.table_deriv[idx[u]];
-#pragma unroll
-        for (int u = 0; u < kPairBatch; ++u) {
-            if (!in[u]) continue;
-            float pe = t[u].x + rem[u] * t[u].z;
-            const float dor = t[u].y + rem[u] * t[u].w;
-            float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
-            if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
-                const float tmp = (v_intra < kEpsFl) ? 0.f : (v_intra / (v_intra + pe));
-                pe *= tmp;
-                const float t2 = tmp * tmp;
-                fx *= t2; fy *= t2; fz *= t2;
-            }
-            e += pe;
-            atomicAdd(&mf[a[u]].x, -fx); atomicAdd(&mf[a[u]].y, -fy); atomicAdd(&mf[a[u]].z, -fz);
-            atomicAdd(&mf[b[u]].x, fx); atomicAdd(&mf[b[u]].y, fy); atomicAdd(&mf[b[u]].z, fz);
-        }
-    }
+    float e = atoms_forces<L, NM, GF>(ka, s, v_intra, v_inter);
+    __syncthreads();
     e = gsum<L>(e);
     if (rgyr) {
         // model::gyration_radius (model.cpp:871-883): heavy atoms about the root origin
@@ -535,10 +596,8 @@ __device__ float eval_group(const KArgs& ka, const Grp<L>& s, const float* pose,
                 T2 += dx * fi.y - dy * fi.x;
             }
         }
-        for (int o = S >> 1; o > 0; o >>= 1) {
-            F0 += __shfl_xor(F0, o, kWG); F1 += __shfl_xor(F1, o, kWG); F2 += __shfl_xor(F2, o, kWG);
-            T0 += __shfl_xor(T0, o, kWG); T1 += __shfl_xor(T1, o, kWG); T2 += __shfl_xor(T2, o, kWG);
-        }
+        F0 = sub_sum(F0, S); F1 = sub_sum(F1, S); F2 = sub_sum(F2, S);
+        T0 = sub_sum(T0, S); T1 = sub_sum(T1, S); T2 = sub_sum(T2, S);
         if (k < nn && jj == 0) {
             if (k == 0) {
                 gbuf[0] = F0; gbuf[1] = F1; gbuf[2] = F2;
@@ -551,7 +610,7 @@ __device__ float eval_group(const KArgs& ka, const Grp<L>& s, const float* pose,
     __syncthreads();
     const int n = s.ndof();
 #pragma unroll
-    for (int j = 0; j < Grp<L>::KV; ++j) {
+    for (int j = 0; j < Grp<L, NM>::KV; ++j) {
         const int idx = s.gl + j * L;
         gout[j] = (idx < n) ? gbuf[idx] : 0.f;
     }
@@ -559,34 +618,35 @@ __device__ float eval_group(const KArgs& ka, const Grp<L>& s, const float* pose,
     return e;
 }
 
-template <int L>
-__device__ __forceinline__ float gdot(const float (&a)[Grp<L>::KV], const float (&b)[Grp<L>::KV]) {
+template <int L, int KV>
+__device__ __forceinline__ float gdot(const float (&a)[KV], const float (&b)[KV]) {
     float s = 0.f;
 #pragma unroll
-    for (int j = 0; j < Grp<L>::KV; ++j) s += a[j] * b[j];  // padded entries are zero
+    for (int j = 0; j < KV; ++j) s += a[j] * b[j];  // padded entries are zero
     return gsum<L>(s);
 }
-template <int L>
-__device__ __forceinline__ double gdot_d(const float (&a)[Grp<L>::KV], const float (&b)[Grp<L>::KV]) {
+template <int L, int KV>
+__device__ __forceinline__ double gdot_d(const float (&a)[KV], const float (&b)[KV]) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < Grp<L>::KV; ++j) s += double(a[j]) * double(b[j]);
+    for (int j = 0; j < KV; ++j) s += double(a[j]) * double(b[j]);
     return gsum_d<L>(s);
 }
 
 // out = -H * in over the packed upper triangle (minus_mat_vec_product, bfgs.h:30-38)
-template <int L>
-__device__ __forceinline__ void minus_hmatvec(const Grp<L>& s, const float (&in)[Grp<L>::KV], float (&out)[Grp<L>::KV], int n) {
+template <int L, int NM>
+__device__ __forceinline__ void minus_hmatvec(const Grp<L, NM>& s, const float (&in)[Grp<L, NM>::KV],
+                                              float (&out)[Grp<L, NM>::KV], int n) {
     float* bc = s.bc(0);
     const float* H = s.H();
 #pragma unroll
-    for (int j = 0; j < Grp<L>::KV; ++j) {
+    for (int j = 0; j < Grp<L, NM>::KV; ++j) {
         const int idx = s.gl + j * L;
         if (idx < n) bc[idx] = in[j];
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < Grp<L>::KV; ++j) {
+    for (int j = 0; j < Grp<L, NM>::KV; ++j) {
         const int i = s.gl + j * L;
         float sum = 0.f;
         if (i < n) {
@@ -600,8 +660,8 @@ __device__ __forceinline__ void minus_hmatvec(const Grp<L>& s, const float (&in)
 
 // conf::increment (conf.h:98-104, 50-56, 169-172): position += a p,
 // orientation <- exp(a rot) * q (normalised), torsions wrapped.
-template <int L>
-__device__ __forceinline__ void pose_increment(const Grp<L>& s, float* dst, const float* src, const float* p, float alpha, bool pred) {
+template <int L, int NM>
+__device__ __forceinline__ void pose_increment(const Grp<L, NM>& s, float* dst, const float* src, const float* p, float alpha, bool pred) {
     if (!pred) return;
     const int T = s.ntors();
     for (int idx = s.gl; idx < 4 + T; idx += L) {
@@ -617,8 +677,8 @@ __device__ __forceinline__ void pose_increment(const Grp<L>& s, float* dst, cons
     }
 }
 
-template <int L>
-__device__ __forceinline__ void copy_pose(const Grp<L>& s, float* dst, const float* src, bool pred) {
+template <int L, int NM>
+__device__ __forceinline__ void copy_pose(const Grp<L, NM>& s, float* dst, const float* src, bool pred) {
     if (!pred) return;
     for (int idx = s.gl; idx < 7 + s.ntors(); idx += L) dst[idx] = src[idx];
 }
@@ -630,7 +690,7 @@ __global__ __launch_bounds__(64) void score_kernel(KArgs ka, const WorkItem* wor
     extern __shared__ float4 lds_raw[];
     float* lds = reinterpret_cast<float*>(lds_raw);
     const WorkItem w = work[blockIdx.x];
-    Grp<L> s = setup_group<L>(ka, w.lig, lds);
+    Grp<L> s = setup_group<L, kNMax>(ka, w.lig, lds);
     const bool active = s.g < w.count;
     const int pi = active ? perm[w.first + s.g] : 0;
     float* x = s.pose(PX);
@@ -638,7 +698,7 @@ __global__ __launch_bounds__(64) void score_kernel(KArgs ka, const WorkItem* wor
     for (int i = s.gl; i < np; i += L) x[i] = active ? poses[size_t(pi) * UD_POSE_FLOATS + i] : (i == 3 ? 1.f : 0.f);
     __syncthreads();
     float gout[Grp<L>::KV];
-    const float e = eval_group<L, GF>(ka, s, x, v_intra, v_inter, gout, true);
+    const float e = eval_group<L, kNMax, GF>(ka, s, x, v_intra, v_inter, gout, true);
     if (active) {
         if (s.gl == 0) e_out[pi] = e;
 #pragma unroll
@@ -719,17 +779,17 @@ enum { ST_RGYR = 0, ST_INIT = 1, ST_TRIAL = 2, ST_IDLE = 3 };
 // beats the chain's best, BFGS with authentic_v and keep it.
 // MC == false runs one BFGS per group on the given poses (quasi_newton).
 // The curvature scalar y.p is accumulated in FP64.
-template <int L, int GF, bool MC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(UD_SEARCH_WAVES_PER_EU)))
+template <int L, int GF, bool MC, int NM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L == 64 ? 4 : UD_SEARCH_WAVES_PER_EU)))
 void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* perm,
                                                     float* poses_io, float v_intra_opt, float v_inter_opt,
                                                     int max_steps_opt, float* out_pose, float* out_e,
                                                     long long* out_evals, int32_t* out_evals32) {
-    constexpr int KV = Grp<L>::KV;
+    constexpr int KV = Grp<L, NM>::KV;
     extern __shared__ float4 lds_raw[];
     float* lds = reinterpret_cast<float*>(lds_raw);
     const WorkItem w = work[blockIdx.x];
-    Grp<L> s = setup_group<L>(ka, w.lig, lds);
+    Grp<L, NM> s = setup_group<L, NM>(ka, w.lig, lds);
     const bool active = s.g < w.count;
     const int T = s.ntors();
     const int np = 7 + T;
@@ -743,7 +803,10 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
     float* BEST = s.pose(PBEST);
     float* H = s.H();
     float* pb = s.bc(1);  // search direction, read by the pose increment
-    const uint32_t chain = uint32_t(w.first + (active ? s.g : 0));
+    // MC: the work item's `count` chains are taken by the groups in turn; a
+    // group whose chain ends writes its result and starts the next one
+    uint32_t chain = uint32_t(w.first + (active ? s.g : 0));
+    int next_chain = Grp<L, NM>::G;  // wave-uniform
     const int pi = (!MC && active) ? perm[w.first + s.g] : 0;
     int lsteps;
     if (MC) lsteps = sp.local_steps_override > 0 ? sp.local_steps_override : s.h.local_steps;
@@ -759,13 +822,27 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
 #pragma unroll
     for (int j = 0; j < KV; ++j) { g[j] = 0.f; p[j] = 0.f; y[j] = 0.f; }
 
+    if (MC && sp.global_steps == 0) {
+        // no Monte Carlo steps: each chain reports its random start
+        for (int c = s.g; c < w.count; c += Grp<L, NM>::G) {
+            const uint32_t ch = uint32_t(w.first + c);
+            if (s.gl == 0) random_start(TMP, T, sp, ch);
+            __syncthreads();
+            for (int i = s.gl; i < UD_POSE_FLOATS; i += L) out_pose[size_t(ch) * UD_POSE_FLOATS + i] = (i < np) ? TMP[i] : 0.f;
+            if (s.gl == 0) {
+                out_e[ch] = __builtin_huge_valf();
+                out_evals[ch] = 0;
+            }
+            __syncthreads();
+        }
+        return;
+    }
     if (MC) {
-        if (s.gl == 0) random_start(TMP, T, sp, chain);
+        if (s.gl == 0 && active) random_start(TMP, T, sp, chain);
         __syncthreads();
         copy_pose(s, X, TMP, true);
         copy_pose(s, BEST, TMP, true);
         state = ST_RGYR;  // the first evaluation only measures the start's gyration radius
-        if (sp.global_steps == 0) state = ST_IDLE;
     } else {
         for (int i = s.gl; i < np; i += L) X[i] = active ? poses_io[size_t(pi) * UD_POSE_FLOATS + i] : (i == 3 ? 1.f : 0.f);
     }
@@ -777,7 +854,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         // ---------------------------------------------------------------- evaluate
         const bool trialing = state == ST_TRIAL;
         float rg = 0.f;
-        const float f = eval_group<L, GF>(ka, s, trialing ? XN : X, vI, vE, t, true, &rg);
+        const float f = eval_group<L, NM, GF>(ka, s, trialing ? XN : X, vI, vE, t, true, &rg);
         if (state != ST_IDLE) ++evals;
         // ---------------------------------------------------------------- line search / BFGS bookkeeping
         const float gg = gdot<L>(g, g);
@@ -833,7 +910,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         const bool upd_pre = cont && !finished;
         const float yp = float(gdot_d<L>(y, p));
         const bool do_upd = upd_pre && !(alpha * yp < kEpsFl);
-        minus_hmatvec<L>(s, y, mhy, n);
+        minus_hmatvec<L, NM>(s, y, mhy, n);
         const float yhy = -gdot<L>(y, mhy);
         {
             float* ma = s.bc(0);
@@ -860,7 +937,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         }
         // new search direction p = -H g (minus_mat_vec_product, bfgs.h:30-38)
         const bool new_dir = !finished && (is_init || cont);
-        minus_hmatvec<L>(s, g, t, n);
+        minus_hmatvec<L, NM>(s, g, t, n);
         if (new_dir) {
 #pragma unroll
             for (int j = 0; j < KV; ++j) p[j] = t[j];
@@ -917,14 +994,44 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
             }
         }
         __syncthreads();
+        bool chain_done = false;
         if (MC && next_step) {
             ++mc_step;
             if (mc_step >= sp.global_steps) {
                 state = ST_IDLE;
                 next_step = false;
+                chain_done = true;
             } else {
                 copy_pose(s, X, TMP, true);
             }
+        }
+        if (MC && wave_any(chain_done)) {
+            // report finished chains, then hand out the item's next chains in
+            // group order (one claim per finished group)
+            if (chain_done) {
+                for (int i = s.gl; i < UD_POSE_FLOATS; i += L)
+                    out_pose[size_t(chain) * UD_POSE_FLOATS + i] = (i < np) ? BEST[i] : 0.f;
+                if (s.gl == 0) {
+                    out_e[chain] = best_e;
+                    out_evals[chain] = evals;
+                }
+            }
+            const uint64_t fin = __ballot(chain_done && s.gl == 0);
+            const int rank = __popcll(fin & ((uint64_t(1) << (s.g * L)) - 1));
+            const int c = next_chain + rank;
+            next_chain += __popcll(fin);
+            const bool refill = chain_done && c < w.count;
+            if (refill) {
+                chain = uint32_t(w.first + c);
+                if (s.gl == 0) random_start(TMP, T, sp, chain);
+                state = ST_RGYR;
+                phase = 0;
+                evals = 0;
+                best_e = __builtin_huge_valf();
+            }
+            __syncthreads();
+            copy_pose(s, X, TMP, refill);
+            copy_pose(s, BEST, TMP, refill);
         }
         __syncthreads();
         if (MC && next_step && s.gl == 0) mutate(X, T, rg_tmp, sp, chain, uint32_t(mc_step));
@@ -947,13 +1054,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
     }
 
     if (!active) return;
-    if (MC) {
-        for (int i = s.gl; i < UD_POSE_FLOATS; i += L) out_pose[size_t(chain) * UD_POSE_FLOATS + i] = (i < np) ? BEST[i] : 0.f;
-        if (s.gl == 0) {
-            out_e[chain] = best_e;
-            out_evals[chain] = evals;
-        }
-    } else {
+    if (!MC) {
         for (int i = s.gl; i < np; i += L) poses_io[size_t(pi) * UD_POSE_FLOATS + i] = X[i];
         if (s.gl == 0) {
             if (out_e) out_e[pi] = f0;
@@ -971,7 +1072,7 @@ __global__ __launch_bounds__(64) void terms_kernel(KArgs ka, const WorkItem* wor
     extern __shared__ float4 lds_raw[];
     float* lds = reinterpret_cast<float*>(lds_raw);
     const WorkItem w = work[blockIdx.x];
-    Grp<L> s = setup_group<L>(ka, w.lig, lds);
+    Grp<L> s = setup_group<L, kNMax>(ka, w.lig, lds);
     const bool active = s.g < w.count;
     const int pi = active ? perm[w.first + s.g] : 0;
     float* x = s.pose(PX);
@@ -981,7 +1082,7 @@ __global__ __launch_bounds__(64) void terms_kernel(KArgs ka, const WorkItem* wor
     fk_frames(s, x);
     float4* coords = s.coords();
     float rg2 = 0.f;
-    float ei = atoms_grid<L, GF>(ka, s, v, false, rg2), ea = 0.f;
+    float ei = atoms_grid<L, kNMax, GF>(ka, s, v, false, rg2), ea = 0.f;
     __syncthreads();
     const PairRec* prs = s.pairs;
     for (int q = s.gl; q < s.h.n_pairs; q += L) {
@@ -1009,7 +1110,7 @@ __global__ __launch_bounds__(64) void coords_kernel(KArgs ka, const WorkItem* wo
     extern __shared__ float4 lds_raw[];
     float* lds = reinterpret_cast<float*>(lds_raw);
     const WorkItem w = work[blockIdx.x];
-    Grp<L> s = setup_group<L>(ka, w.lig, lds);
+    Grp<L> s = setup_group<L, kNMax>(ka, w.lig, lds);
     const bool active = s.g < w.count;
     const int pi = active ? perm[w.first + s.g] : 0;
     float* x = s.pose(PX);
@@ -1034,7 +1135,7 @@ __global__ __launch_bounds__(64) void coords_kernel(KArgs ka, const WorkItem* wo
 // ====================================================================== launchers
 size_t lds_bytes(int lanes, const LaunchCap& cap) {
     Layout l;
-    l.init(cap.n_atoms, cap.n_nodes, cap.n_pairs);
+    l.init(cap.n_atoms, cap.n_nodes);
     return l.bytes(kWG / lanes);
 }
 
@@ -1048,15 +1149,15 @@ static KArgs make_kargs(const LigandSetDev& ls, const GridDev& g, const LaunchCa
     return k;
 }
 
-#define UD_DISPATCH_L(lanes, BODY)                 \
-    switch (lanes) {                               \
-        case 8: { constexpr int L = 8; BODY; } break;   \
-        case 16: { constexpr int L = 16; BODY; } break; \
-        case 32: { constexpr int L = 32; BODY; } break; \
-        default: { constexpr int L = 64; BODY; } break; \
+#define UD_DISPATCH_L(lanes, ...)                          \
+    switch (lanes) {                                       \
+        case 8: { constexpr int L = 8; __VA_ARGS__; } break;   \
+        case 16: { constexpr int L = 16; __VA_ARGS__; } break; \
+        case 32: { constexpr int L = 32; __VA_ARGS__; } break; \
+        default: { constexpr int L = 64; __VA_ARGS__; } break; \
     }
 
-hipError_t launch_tables(int sf, const float* w, float4* deriv, float* fast, hipStream_t st) {
+hipError_t launch_tables(int sf, const float* w, float2* deriv, float* fast, hipStream_t st) {
     SfWeights W;
     for (int i = 0; i < 7; ++i) W.w[i] = w[i];
     hipLaunchKernelGGL(table_kernel, dim3(kTypePairs), dim3(256), 0, st, sf, W, deriv, fast);
@@ -1098,6 +1199,40 @@ hipError_t launch_score(int lanes, int gf, const LigandSetDev& ls, const GridDev
     return hipGetLastError();
 }
 
+// Search vectors sized 16 (up to 10 torsions) or the full capacity.
+static constexpr int kNSmall = 16;
+
+template <int L, int GF, bool MC, int NM>
+static void launch_search(const KArgs& ka, const SearchDev& sp, const WorkItem* work, int n_work, size_t shm,
+                          hipStream_t st, const int32_t* perm, float* poses, float v_intra, float v_inter,
+                          int max_steps, float* out_pose, float* out_e, long long* out_evals, int32_t* evals32) {
+    hipLaunchKernelGGL((search_kernel<L, GF, MC, NM>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work, perm, poses,
+                       v_intra, v_inter, max_steps, out_pose, out_e, out_evals, evals32);
+}
+
+template <int L, bool MC>
+static void launch_search_nm(int gf, const LaunchCap& cap, const KArgs& ka, const SearchDev& sp,
+                             const WorkItem* work, int n_work, size_t shm, hipStream_t st, const int32_t* perm,
+                             float* poses, float v_intra, float v_inter, int max_steps, float* out_pose, float* out_e,
+                             long long* out_evals, int32_t* evals32) {
+    const bool small = 6 + cap.n_nodes - 1 <= kNSmall;
+    if (gf == UD_GRID_F16) {
+        if (small)
+            launch_search<L, UD_GRID_F16, MC, kNSmall>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter,
+                                                        max_steps, out_pose, out_e, out_evals, evals32);
+        else
+            launch_search<L, UD_GRID_F16, MC, kNMax>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter,
+                                                      max_steps, out_pose, out_e, out_evals, evals32);
+    } else {
+        if (small)
+            launch_search<L, UD_GRID_F32, MC, kNSmall>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter,
+                                                        max_steps, out_pose, out_e, out_evals, evals32);
+        else
+            launch_search<L, UD_GRID_F32, MC, kNMax>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter,
+                                                      max_steps, out_pose, out_e, out_evals, evals32);
+    }
+}
+
 hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                            const WorkItem* work, int n_work, const int32_t* perm, float* poses, float v_intra,
                            float v_inter, int max_steps, float* e_out, int32_t* evals_out, hipStream_t st) {
@@ -1105,12 +1240,8 @@ hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const Grid
     const size_t shm = lds_bytes(lanes, cap);
     SearchDev sp = {};
     UD_DISPATCH_L(lanes, {
-        if (gf == UD_GRID_F16)
-            hipLaunchKernelGGL((search_kernel<L, UD_GRID_F16, false>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work,
-                               perm, poses, v_intra, v_inter, max_steps, nullptr, e_out, nullptr, evals_out);
-        else
-            hipLaunchKernelGGL((search_kernel<L, UD_GRID_F32, false>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work,
-                               perm, poses, v_intra, v_inter, max_steps, nullptr, e_out, nullptr, evals_out);
+        launch_search_nm<L, false>(gf, cap, ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter, max_steps,
+                                   nullptr, e_out, nullptr, evals_out);
     });
     return hipGetLastError();
 }
@@ -1121,12 +1252,8 @@ hipError_t launch_dock(int lanes, int gf, const LigandSetDev& ls, const GridDev&
     const KArgs ka = make_kargs(ls, g, cap);
     const size_t shm = lds_bytes(lanes, cap);
     UD_DISPATCH_L(lanes, {
-        if (gf == UD_GRID_F16)
-            hipLaunchKernelGGL((search_kernel<L, UD_GRID_F16, true>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work,
-                               nullptr, nullptr, sp.hunt[0], sp.hunt[1], 0, out_pose, out_e, out_evals, nullptr);
-        else
-            hipLaunchKernelGGL((search_kernel<L, UD_GRID_F32, true>), dim3(n_work), dim3(kWG), shm, st, ka, sp, work,
-                               nullptr, nullptr, sp.hunt[0], sp.hunt[1], 0, out_pose, out_e, out_evals, nullptr);
+        launch_search_nm<L, true>(gf, cap, ka, sp, work, n_work, shm, st, nullptr, nullptr, sp.hunt[0], sp.hunt[1], 0,
+                                  out_pose, out_e, out_evals, nullptr);
     });
     return hipGetLastError();
 }

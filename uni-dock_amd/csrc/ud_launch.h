@@ -20,7 +20,7 @@ struct GridBuildArgs {
 
 size_t lds_bytes(int lanes, const LaunchCap& cap);
 
-hipError_t launch_tables(int sf, const float* w, float4* deriv, float* fast, hipStream_t st);
+hipError_t launch_tables(int sf, const float* w, float2* deriv, float* fast, hipStream_t st);
 hipError_t launch_grid_build(const float4* rec, int n_rec, const GridBuildArgs& a, const float* table_fast,
                              float* maps_out, hipStream_t st);
 hipError_t launch_grid_absmax(const float* maps, int n_points, int n_maps, unsigned int* out, hipStream_t st);

@@ -294,7 +294,7 @@ class Context:
         a[:len(lanes)] = lanes
         self._ck(self.L.ud_set_bucket_lanes(self.h, _p(a, ctypes.c_int32)))
 
-    def autotune(self, exhaustiveness=8, global_steps=2, seed=181129, local_steps=0, sample_ligands=64):
+    def autotune(self, exhaustiveness=8, global_steps=2, seed=181129, local_steps=0, sample_ligands=1 << 20):
         """Pick lanes per chain per size bucket by timing short docks; returns the chosen lanes."""
         p = self._params(exhaustiveness, global_steps, seed, local_steps, 0)
         out = np.zeros(UD_N_BUCKETS, dtype=np.int32)
