@@ -371,3 +371,48 @@ def test_autotune_and_bucket_lanes(scene):
         ctx.set_bucket_lanes([12, 0, 0, 0, 0, 0])
     assert ei.value.code == _lib.UD_ERR_INVALID
     ctx.set_bucket_lanes([0] * _lib.UD_N_BUCKETS)
+
+
+def test_large_ligand_without_pair_buffer(scene):
+    """A 120-atom ligand has too many pairs for the per-pair force buffer to
+    fit a workgroup's LDS, so its forces take the two-sided partner path;
+    a 30-atom one takes the pair-buffer path.  Both match the FP32 oracle."""
+    from helpers import box_of as _box
+    from oracle.oracle import Oracle
+    _say("[gpu] large ligand (two-sided pair path)")
+    big = synth.make_ligand(120, 12, seed=77)
+    mid = synth.make_ligand(30, 6, seed=78)
+    assert len(big.pairs) * 16 > 65536  # the buffer alone would exceed 64 KiB
+    ligs = [big, mid]
+    ctx = _lib.Context(0)
+    ctx.set_scoring(_lib.UD_SF_VINA)
+    c, s = _box(scene["cav"])
+    ctx.set_box(c, s, 0.375)
+    ctx.compute_maps(scene["xyz"], scene["xs"], LIGAND_TYPES, _lib.UD_GRID_F32)
+    ctx.set_ligands(synth.LigandBatch(ligs).arrays())
+    o32 = Oracle(prec=32, sf=0)
+    o32.set_grid(c, s, 0.375)
+    for t in LIGAND_TYPES:
+        o32.set_map(t, ctx.get_map(t).astype(np.float64))
+    for l in ligs:
+        o32.add_ligand(l)
+    rng = np.random.default_rng(3)
+    pl, P = [], []
+    for li, lig in enumerate(ligs):
+        for p in random_poses(rng, lig, scene["cav"], 2.0, 6):
+            pl.append(li)
+            P.append(p)
+    pl = np.array(pl, dtype=np.int32)
+    P = np.array(P)
+    for lanes in (64, 16):
+        ctx.set_lanes(lanes)
+        e, g = ctx.score_grad(pl, P, (10.0, 10.0, 10.0))
+        for i in range(len(pl)):
+            li = int(pl[i])
+            nt = ligs[li].n_tors
+            e32, g32, _, _ = o32.eval_deriv(li, P[i], (10.0, 10.0, 10.0))
+            assert e[i] == pytest.approx(e32, rel=1e-5, abs=1e-4), (lanes, i, e[i], e32)
+            sc = max(1.0, np.abs(g32).max())
+            np.testing.assert_allclose(g[i, :6 + nt], g32, rtol=0, atol=1e-4 * sc)
+    ctx.set_lanes(0)
+    ctx.close()

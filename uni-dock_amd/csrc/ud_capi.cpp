@@ -148,8 +148,13 @@ LigandSetDev lig_dev(const ud_ctx* c) {
 }
 
 // largest lanes-per-group whose LDS footprint fits one workgroup (64 KiB)
-int fit_lanes(int lanes, const LaunchCap& cap) {
+int fit_lanes(int lanes, LaunchCap& cap) {
+    // the per-pair force buffer is used when the launch still fits with it
+    cap.pair_forces = 0;
     while (lanes < 64 && lds_bytes(lanes, cap) > 65536) lanes *= 2;
+    LaunchCap with = cap;
+    with.pair_forces = 1;
+    if (lds_bytes(lanes, with) <= 65536) cap.pair_forces = 1;
     return lanes;
 }
 
@@ -198,7 +203,7 @@ void make_pose_work(int n_poses, const int32_t* pose_ligand, int groups, std::ve
 }
 
 LaunchCap caps_of(const ud_ctx* c, const std::vector<int>* ligs) {
-    LaunchCap cap{1, 1, 0};
+    LaunchCap cap{1, 1, 0, 0};
     const int n = ligs ? int(ligs->size()) : c->n_lig;
     for (int k = 0; k < n; ++k) {
         const int l = ligs ? (*ligs)[k] : k;
@@ -592,9 +597,9 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
         std::vector<std::vector<PartnerRec>> part(na);
         for (int q = 0; q < np; ++q) {
             const int pa = b->pairs[2 * (p0 + q)], pb = b->pairs[2 * (p0 + q) + 1];
-            const uint32_t slot = uint32_t(tri_index(b->atom_xs[a0 + pa], b->atom_xs[a0 + pb])) << 16;
-            part[pa].push_back(uint32_t(pb) | slot);
-            part[pb].push_back(uint32_t(pa) | slot);
+            const uint32_t slot21 = uint32_t(tri_index(b->atom_xs[a0 + pa], b->atom_xs[a0 + pb])) << 21;
+            part[pa].push_back(uint32_t(pb) | (uint32_t(q) << 8) | slot21 | 0x80000000u);
+            part[pb].push_back(uint32_t(pa) | (uint32_t(q) << 8) | slot21);
         }
         const size_t pbeg = partners.size();
         for (int i = 0; i < na; ++i) {
@@ -717,7 +722,7 @@ int ud_score_grad_device(ud_ctx* c, int n_poses, const int32_t* pose_ligand, con
     UD_HIP(c, hipSetDevice(c->device));
     rc = refresh_slots(c);
     if (rc) return rc;
-    const LaunchCap cap = caps_all(c);
+    LaunchCap cap = caps_all(c);
     const int na = cap.n_atoms;
     const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, cap);
     std::vector<int32_t> perm;
@@ -773,7 +778,7 @@ int ud_optimize(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses
     if (rc) return rc;
     rc = grow(c, c->d_i1, c->i_cap, size_t(n_poses));
     if (rc) return rc;
-    const LaunchCap cap = caps_all(c);
+    LaunchCap cap = caps_all(c);
     const int na = cap.n_atoms;
     const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : default_lanes(na), cap);
     std::vector<int32_t> perm;
@@ -875,7 +880,7 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
     size_t total_work = 0;
     for (int bi = UD_N_BUCKETS - 1; bi >= 0; --bi) {
         if (bl[bi].empty()) continue;
-        const LaunchCap cap = caps_of(c, &bl[bi]);
+        LaunchCap cap = caps_of(c, &bl[bi]);
         int lanes = lanes_all ? lanes_all
                               : (p->lanes_per_chain ? p->lanes_per_chain
                                                     : (c->tuned_lanes[bi] ? c->tuned_lanes[bi] : default_lanes(cap.n_atoms)));
@@ -999,7 +1004,7 @@ int ud_autotune(ud_ctx* c, const ud_search_params* p, int sample_ligands, int32_
         std::vector<int> sub;
         const int m = std::min<int>(sample_ligands, int(bl[bi].size()));
         for (int k = 0; k < m; ++k) sub.push_back(bl[bi][size_t(k) * bl[bi].size() / m]);
-        const LaunchCap cap = caps_of(c, &sub);
+        LaunchCap cap = caps_of(c, &sub);
         int best = 0;
         double best_rate = -1.0;
         for (int lanes : {8, 16, 32, 64}) {
@@ -1043,7 +1048,7 @@ int ud_score_terms(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const flo
     if (rc) return rc;
     rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * 2);
     if (rc) return rc;
-    const LaunchCap cap = caps_all(c);
+    LaunchCap cap = caps_all(c);
     const int na = cap.n_atoms;
     const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, cap);
     if (lds_bytes(lanes, cap) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
@@ -1070,7 +1075,7 @@ int ud_pose_coords(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const flo
     if (rc) return rc;
     rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * UD_MAX_ATOMS * 3);
     if (rc) return rc;
-    const LaunchCap cap = caps_all(c);
+    LaunchCap cap = caps_all(c);
     const int na = cap.n_atoms;
     const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, cap);
     if (lds_bytes(lanes, cap) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");

@@ -48,9 +48,10 @@ struct NodeRec {
 
 // Pair record: a | b << 8 | table slot << 16
 typedef uint32_t PairRec;
-// Partner record (each pair listed once under each of its atoms): other
-// atom | table slot << 16.  Per atom, AtomPartners = first partner (relative
-// to the ligand's partner_begin) | count << 20.
+// Partner record (each pair listed once under each of its atoms, in pair
+// order): bits 0..7 other atom, 8..20 pair index, 21..30 table slot, bit 31
+// set when the atom is the pair's first atom.  Per atom, AtomPartners =
+// first partner (relative to the ligand's partner_begin) | count << 20.
 typedef uint32_t PartnerRec;
 typedef uint32_t AtomPartners;
 
@@ -107,6 +108,7 @@ struct LaunchCap {
     int32_t n_atoms;  // max atoms
     int32_t n_nodes;  // max nodes
     int32_t n_pairs;  // max intramolecular pairs
+    int32_t pair_forces;  // 1: per-pair force buffer in LDS (pair-parallel forces)
 };
 
 }  // namespace ud

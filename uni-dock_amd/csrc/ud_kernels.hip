@@ -181,8 +181,9 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 // many chains fit a compute unit.  Pair records stay in global memory.
 struct Layout {
     int na, nn, nc, nh, ps;
-    int o_coords, o_mf, o_frames, o_poses, o_bc, o_H, group_floats, shared_floats;
-    __host__ __device__ void init(int na_, int nn_) {
+    int o_coords, o_mf, o_frames, o_poses, o_bc, o_H, o_pf, group_floats, shared_floats;
+    // pf: per-pair buffer (one float per pair) for pair-parallel forces
+    __host__ __device__ void init(int na_, int nn_, int np_, bool pf) {
         na = na_;
         nn = nn_;
         nc = 6 + nn - 1;
@@ -194,9 +195,11 @@ struct Layout {
         o_poses = o_frames + 20 * nn;
         o_bc = o_poses + 5 * ps;
         o_H = o_bc + 3 * ps;
+        o_pf = pf ? o_H + nh : -1;
+        const int end = pf ? o_pf + ((np_ + 3) & ~3) : o_H + nh;
         // group blocks start 4 floats apart modulo 32 banks, so that the
         // same offset in up to 8 groups falls into different LDS banks
-        group_floats = ((o_H + nh + 31) & ~31) + 4;
+        group_floats = ((end + 31) & ~31) + 4;
         shared_floats = 4 * na + 8 * nn;  // atoms, nodes
     }
     __host__ __device__ size_t bytes(int groups) const { return size_t(shared_floats + groups * group_floats) * 4; }
@@ -207,6 +210,7 @@ struct KArgs {
     LigandSetDev ls;
     GridDev grid;
     int na_cap, nn_cap, np_cap;
+    int pair_forces;
 };
 
 // NM: compile-time capacity of the gradient / search vectors (6 + torsions);
@@ -227,6 +231,7 @@ template <int L, int NM = kNMax> struct Grp {
     __device__ float* pose(int k) const { return gb + lay.o_poses + k * lay.ps; }
     __device__ float* bc(int k) const { return gb + lay.o_bc + k * lay.ps; }
     __device__ float* H() const { return gb + lay.o_H; }
+    __device__ float* pf() const { return gb + lay.o_pf; }
     __device__ int ntors() const { return h.n_nodes - 1; }
     __device__ int ndof() const { return 6 + h.n_nodes - 1; }
 };
@@ -235,7 +240,7 @@ template <int L, int NM = kNMax> struct Grp {
 template <int L, int NM>
 __device__ Grp<L, NM> setup_group(const KArgs& ka, int lig, float* lds) {
     Grp<L, NM> s;
-    s.lay.init(ka.na_cap, ka.nn_cap);
+    s.lay.init(ka.na_cap, ka.nn_cap, ka.np_cap, ka.pair_forces != 0);
     s.h = ka.ls.hdr[lig];
     AtomRec* atoms = reinterpret_cast<AtomRec*>(lds);
     NodeRec* nodes = reinterpret_cast<NodeRec*>(lds + 4 * s.lay.na);
@@ -471,7 +476,7 @@ __device__ __forceinline__ void atoms_coords(const Grp<L, NM>& s, float& rg2) {
 // and no atomics are needed.  A pair's energy is counted on the side of its
 // lower atom index.
 template <int L, int NM, int GF>
-__device__ __forceinline__ float atoms_forces(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter) {
+__device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter) {
     const float4* coords = s.coords();
     float4* mf = s.mf();
     const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
@@ -509,7 +514,7 @@ __device__ __forceinline__ float atoms_forces(const KArgs& ka, const Grp<L, NM>&
                         const float rf = 32.f * r2;
                         const int i1t = int(rf);
                         rem[u] = rf - float(i1t);
-                        idx[u] = int(rec >> 16) * kTableStride + i1t;
+                        idx[u] = int((rec >> 21) & 0x3FFu) * kTableStride + i1t;
                         in[u] = true;
                     }
                 }
@@ -543,6 +548,112 @@ __device__ __forceinline__ float atoms_forces(const KArgs& ka, const Grp<L, NM>&
     return e;
 }
 
+// Same result as atoms_forces_2s with every pair evaluated once: pass A
+// evaluates the grid term of the lane's atoms and the pairs gl, gl + L, ...,
+// storing each pair's force scale d (dor with the curl factor; the force on
+// the pair's second atom is d (c_b - c_a)) in the group's per-pair buffer;
+// pass B adds to each atom's grid gradient d (c_i - c_other) for its pairs
+// in pair order (forces[a] -= f, forces[b] += f, model.cpp:923-931; the
+// difference is formed from the same coordinates, so both ends see exact
+// negatives).  Lanes stay balanced whatever the atoms' pair counts, and the
+// per-atom accumulation order is the reference's.
+template <int L, int NM, int GF>
+__device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter) {
+    const float4* coords = s.coords();
+    float4* mf = s.mf();
+    float* pf = s.pf();
+    const int N = s.h.n_atoms;
+    float e = 0.f;
+    for (int i = s.gl; i < N; i += L) {
+        const AtomRec a = s.atoms[i];
+        const int slot = (a.meta >> 8) & 0xff;
+        float3 d = make_float3(0.f, 0.f, 0.f);
+        if (slot != 0xff) {
+            const float4 ci = coords[i];
+            const GridPt gq = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
+            float gf[8];
+            grid_gather<GF>(ka.grid, gq.base, gf);
+            e += grid_finish<GF>(ka.grid, slot, gq, gf, v_inter, &d);
+        }
+        mf[i] = make_float4(d.x, d.y, d.z, 0.f);
+    }
+    const int P = s.h.n_pairs;
+    for (int q0 = s.gl; q0 < P; q0 += kPairBatch * L) {
+        float rx[kPairBatch], ry[kPairBatch], rz[kPairBatch], rem[kPairBatch];
+        int idx[kPairBatch];
+        bool in[kPairBatch];
+        float2 t0[kPairBatch], t1[kPairBatch];
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u) {
+            const int q = q0 + u * L;
+            in[u] = false;
+            idx[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
+            if (q < P) {
+                const PairRec rec = s.pairs[q];
+                const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
+                rx[u] = cb.x - ca.x; ry[u] = cb.y - ca.y; rz[u] = cb.z - ca.z;
+                const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+                if (r2 < 64.f) {
+                    const float rf = 32.f * r2;
+                    const int i1t = int(rf);
+                    rem[u] = rf - float(i1t);
+                    idx[u] = int(rec >> 16) * kTableStride + i1t;
+                    in[u] = true;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u)
+            if (in[u]) {
+                t0[u] = ka.ls.table_deriv[idx[u]];
+                t1[u] = ka.ls.table_deriv[idx[u] + 1];
+            }
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u) {
+            const int q = q0 + u * L;
+            if (q >= P) continue;
+            float d = 0.f;
+            if (in[u]) {
+                float pe = t0[u].x + rem[u] * (t1[u].x - t0[u].x);
+                d = t0[u].y + rem[u] * (t1[u].y - t0[u].y);
+                if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
+                    const float tmp = (v_intra < kEpsFl) ? 0.f : (v_intra / (v_intra + pe));
+                    pe *= tmp;
+                    d *= tmp * tmp;
+                }
+                e += pe;
+            }
+            pf[q] = d;
+        }
+    }
+    __syncthreads();
+    const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
+    const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
+    constexpr int kRec = 4;  // partner records fetched together
+    for (int i = s.gl; i < N; i += L) {
+        float4 g = mf[i];
+        const float4 ci = coords[i];
+        const uint32_t ap = apl[i];
+        const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
+        for (int k = 0; k < kn; k += kRec) {
+            uint32_t rec[kRec];
+#pragma unroll
+            for (int u = 0; u < kRec; ++u) rec[u] = (k + u < kn) ? part[k0 + k + u] : 0u;
+#pragma unroll
+            for (int u = 0; u < kRec; ++u) {
+                if (k + u >= kn) continue;
+                const float d = pf[(rec[u] >> 8) & 0x1FFFu];
+                const float4 co = coords[rec[u] & 0xFFu];
+                g.x += d * (ci.x - co.x);
+                g.y += d * (ci.y - co.y);
+                g.z += d * (ci.z - co.z);
+            }
+        }
+        mf[i] = g;
+    }
+    return e;
+}
+
 // Energy (+ gradient) of the group's pose: model::eval_deriv (model.cpp:954-981)
 // = grid term and intramolecular pairs per atom (atoms_forces), then the
 // torsion-tree derivative
@@ -562,7 +673,8 @@ __device__ float eval_group(const KArgs& ka, const Grp<L, NM>& s, const float* p
     float rg2 = 0.f;
     atoms_coords(s, rg2);
     __syncthreads();
-    float e = atoms_forces<L, NM, GF>(ka, s, v_intra, v_inter);
+    float e = ka.pair_forces ? atoms_forces_pp<L, NM, GF>(ka, s, v_intra, v_inter)
+                             : atoms_forces_2s<L, NM, GF>(ka, s, v_intra, v_inter);
     __syncthreads();
     e = gsum<L>(e);
     if (rgyr) {
@@ -1135,7 +1247,7 @@ __global__ __launch_bounds__(64) void coords_kernel(KArgs ka, const WorkItem* wo
 // ====================================================================== launchers
 size_t lds_bytes(int lanes, const LaunchCap& cap) {
     Layout l;
-    l.init(cap.n_atoms, cap.n_nodes);
+    l.init(cap.n_atoms, cap.n_nodes, cap.n_pairs, cap.pair_forces != 0);
     return l.bytes(kWG / lanes);
 }
 
@@ -1146,6 +1258,7 @@ static KArgs make_kargs(const LigandSetDev& ls, const GridDev& g, const LaunchCa
     k.na_cap = cap.n_atoms;
     k.nn_cap = cap.n_nodes;
     k.np_cap = cap.n_pairs;
+    k.pair_forces = cap.pair_forces;
     return k;
 }
 
