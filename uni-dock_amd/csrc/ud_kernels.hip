@@ -174,6 +174,12 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 }
 
 // ====================================================================== per-group machinery
+// Launches whose ligands have at most 10 torsions size the search vectors 16
+// (kNSmall); others use the full capacity.  With kNSmall vectors and groups
+// of at most 32 lanes the inverse Hessian lives in registers, otherwise in LDS
+// (full-wavefront groups are register-bound already).
+static constexpr int kNSmall = 16;
+
 // LDS layout of one workgroup: the ligand's atoms and nodes, then one block
 // per lane group (coordinates, minus-forces, node frames, five pose buffers,
 // three broadcast vectors, inverse Hessian).  Pose and vector strides follow
@@ -183,12 +189,14 @@ struct Layout {
     int na, nn, nc, nh, ps;
     int o_coords, o_mf, o_frames, o_poses, o_bc, o_H, o_pf, group_floats, shared_floats;
     // pf: per-pair buffer (one float per pair) for pair-parallel forces
-    __host__ __device__ void init(int na_, int nn_, int np_, bool pf) {
+    __host__ __device__ void init(int na_, int nn_, int np_, bool pf, int lanes) {
         na = na_;
         nn = nn_;
         nc = 6 + nn - 1;
+        const bool hreg = nc <= kNSmall && lanes <= 32;  // inverse Hessian in registers
         ps = (7 + nn - 1 + 3) & ~3;  // pose floats (7 + torsions), >= nc
-        nh = (nc * (nc | 1) + 3) & ~3;  // full symmetric inverse Hessian, odd row stride
+        if (hreg && ps < kNSmall) ps = kNSmall;  // broadcast vectors hold all kNSmall entries
+        nh = hreg ? 0 : (nc * (nc | 1) + 3) & ~3;  // full symmetric inverse Hessian, odd row stride
         o_coords = 0;
         o_mf = 4 * na;
         o_frames = 8 * na;
@@ -240,7 +248,7 @@ template <int L, int NM = kNMax> struct Grp {
 template <int L, int NM>
 __device__ Grp<L, NM> setup_group(const KArgs& ka, int lig, float* lds) {
     Grp<L, NM> s;
-    s.lay.init(ka.na_cap, ka.nn_cap, ka.np_cap, ka.pair_forces != 0);
+    s.lay.init(ka.na_cap, ka.nn_cap, ka.np_cap, ka.pair_forces != 0, L);
     s.h = ka.ls.hdr[lig];
     AtomRec* atoms = reinterpret_cast<AtomRec*>(lds);
     NodeRec* nodes = reinterpret_cast<NodeRec*>(lds + 4 * s.lay.na);
@@ -395,7 +403,7 @@ __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const G
 #define UD_PAIR_BATCH 2
 #endif
 #ifndef UD_SEARCH_WAVES_PER_EU
-#define UD_SEARCH_WAVES_PER_EU 1
+#define UD_SEARCH_WAVES_PER_EU 3
 #endif
 static constexpr int kAtomBatch = UD_ATOM_BATCH;  // atoms per lane whose corner gathers are issued together
 static constexpr int kPairBatch = UD_PAIR_BATCH;  // pairs per lane whose table loads are issued together
@@ -770,6 +778,56 @@ __device__ __forceinline__ void minus_hmatvec(const Grp<L, NM>& s, const float (
     __syncthreads();
 }
 
+// Register form of the inverse Hessian (NM <= kNSmall): lane gl keeps rows
+// gl + j L, all NM columns, so every index is a compile-time constant.
+// Vectors are broadcast through the group's bc(0) with entries n..NM-1
+// zero, which keeps the columns beyond n at zero.
+template <int L, int NM, int KV>
+__device__ __forceinline__ void minus_hmatvec_reg(const Grp<L, NM>& s, const float (&Hr)[KV][NM],
+                                                  const float (&in)[KV], float (&out)[KV], int n) {
+    float* bc = s.bc(0);
+#pragma unroll
+    for (int j = 0; j < KV; ++j) {
+        const int idx = s.gl + j * L;
+        if (idx < NM) bc[idx] = (idx < n) ? in[j] : 0.f;
+    }
+    __syncthreads();
+    float v[NM];
+#pragma unroll
+    for (int c = 0; c < NM; ++c) v[c] = bc[c];
+#pragma unroll
+    for (int j = 0; j < KV; ++j) {
+        const int i = s.gl + j * L;
+        float sum = 0.f;
+#pragma unroll
+        for (int c = 0; c < NM; ++c) sum += Hr[j][c] * v[c];
+        out[j] = (i < n) ? -sum : 0.f;
+    }
+    __syncthreads();
+}
+
+// rank-2 update of the register rows (bfgs_update, bfgs.h:46-63).  The mixed
+// term ma_i p_c + ma_c p_i is formed without contraction, so entries (i, c)
+// and (c, i) get identical values and H stays exactly symmetric.
+template <int L, int NM, int KV>
+__device__ __forceinline__ void hess_update_reg(const Grp<L, NM>& s, float (&Hr)[KV][NM], const float (&mhy)[KV],
+                                                const float (&p)[KV], bool pred, float c1, float c2,
+                                                const float* ma, const float* pb) {
+    float mav[NM], pbv[NM];
+#pragma unroll
+    for (int c = 0; c < NM; ++c) { mav[c] = ma[c]; pbv[c] = pb[c]; }
+#pragma unroll
+    for (int j = 0; j < KV; ++j) {
+        const float mi = mhy[j], pi = p[j];
+#pragma unroll
+        for (int c = 0; c < NM; ++c) {
+            const float t = __fadd_rn(__fmul_rn(mi, pbv[c]), __fmul_rn(mav[c], pi));
+            const float nv = Hr[j][c] + (c1 * t + c2 * (pi * pbv[c]));
+            Hr[j][c] = pred ? nv : Hr[j][c];
+        }
+    }
+}
+
 // conf::increment (conf.h:98-104, 50-56, 169-172): position += a p,
 // orientation <- exp(a rot) * q (normalised), torsions wrapped.
 template <int L, int NM>
@@ -915,6 +973,8 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
     float* BEST = s.pose(PBEST);
     float* H = s.H();
     float* pb = s.bc(1);  // search direction, read by the pose increment
+    constexpr bool HREG = NM <= kNSmall && L <= 32;
+    float Hr[HREG ? KV : 1][HREG ? NM : 1];
     // MC: the work item's `count` chains are taken by the groups in turn; a
     // group whose chain ends writes its result and starts the next one
     uint32_t chain = uint32_t(w.first + (active ? s.g : 0));
@@ -1004,7 +1064,12 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         // X <- XN after a finished line search; XO <- X and H <- I at BFGS start
         copy_pose(s, X, XN, ls_end);
         copy_pose(s, XO, X, is_init);
-        if (is_init) {
+        if constexpr (HREG) {
+#pragma unroll
+            for (int j = 0; j < KV; ++j)
+#pragma unroll
+                for (int c = 0; c < NM; ++c) Hr[j][c] = is_init ? ((c == s.gl + j * L) ? 1.f : 0.f) : Hr[j][c];
+        } else if (is_init) {
             for (int idx = s.gl; idx < nh; idx += L) H[idx] = 0.f;
             for (int i = s.gl; i < n; i += L) H[i * hs + i] = 1.f;
         }
@@ -1012,7 +1077,14 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         // initial scaling H = (y.p / y.y) I after the first step (bfgs.h:135-138)
         const float yy = gdot<L>(y, y);
         const float yp0 = gdot<L>(y, p);
-        if (cont && step_now == 0 && fabsf(yy) > kEpsFl) {
+        const bool scale0 = cont && step_now == 0 && fabsf(yy) > kEpsFl;
+        if constexpr (HREG) {
+            const float dv = alpha * yp0 / yy;
+#pragma unroll
+            for (int j = 0; j < KV; ++j)
+#pragma unroll
+                for (int c = 0; c < NM; ++c) Hr[j][c] = (scale0 && c == s.gl + j * L) ? dv : Hr[j][c];
+        } else if (scale0) {
             const float dv = alpha * yp0 / yy;
             for (int i = s.gl; i < n; i += L) H[i * hs + i] = dv;
         }
@@ -1022,17 +1094,25 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         const bool upd_pre = cont && !finished;
         const float yp = float(gdot_d<L>(y, p));
         const bool do_upd = upd_pre && !(alpha * yp < kEpsFl);
-        minus_hmatvec<L, NM>(s, y, mhy, n);
+        if constexpr (HREG) minus_hmatvec_reg<L, NM, KV>(s, Hr, y, mhy, n);
+        else minus_hmatvec<L, NM>(s, y, mhy, n);
         const float yhy = -gdot<L>(y, mhy);
         {
             float* ma = s.bc(0);
 #pragma unroll
             for (int j = 0; j < KV; ++j) {
                 const int idx = s.gl + j * L;
-                if (idx < n) { ma[idx] = mhy[j]; pb[idx] = p[j]; }
+                // entries n..NM-1 of mhy and p are zero
+                if (idx < (HREG ? NM : n)) { ma[idx] = mhy[j]; pb[idx] = p[j]; }
             }
             __syncthreads();
-            if (do_upd) {
+            if constexpr (HREG) {
+                if (wave_any(do_upd)) {
+                    const float r = 1.f / (alpha * yp);
+                    const float c1 = alpha * r, c2 = alpha * alpha * (r * r * yhy + r);
+                    hess_update_reg<L, NM, KV>(s, Hr, mhy, p, do_upd, c1, c2, ma, pb);
+                }
+            } else if (do_upd) {
                 const float r = 1.f / (alpha * yp);
                 const float c1 = alpha * r, c2 = alpha * alpha * (r * r * yhy + r);
                 // one row per lane; each entry uses the (min, max) index order of
@@ -1049,7 +1129,8 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         }
         // new search direction p = -H g (minus_mat_vec_product, bfgs.h:30-38)
         const bool new_dir = !finished && (is_init || cont);
-        minus_hmatvec<L, NM>(s, g, t, n);
+        if constexpr (HREG) minus_hmatvec_reg<L, NM, KV>(s, Hr, g, t, n);
+        else minus_hmatvec<L, NM>(s, g, t, n);
         if (new_dir) {
 #pragma unroll
             for (int j = 0; j < KV; ++j) p[j] = t[j];
@@ -1247,7 +1328,7 @@ __global__ __launch_bounds__(64) void coords_kernel(KArgs ka, const WorkItem* wo
 // ====================================================================== launchers
 size_t lds_bytes(int lanes, const LaunchCap& cap) {
     Layout l;
-    l.init(cap.n_atoms, cap.n_nodes, cap.n_pairs, cap.pair_forces != 0);
+    l.init(cap.n_atoms, cap.n_nodes, cap.n_pairs, cap.pair_forces != 0, lanes);
     return l.bytes(kWG / lanes);
 }
 
@@ -1312,8 +1393,6 @@ hipError_t launch_score(int lanes, int gf, const LigandSetDev& ls, const GridDev
     return hipGetLastError();
 }
 
-// Search vectors sized 16 (up to 10 torsions) or the full capacity.
-static constexpr int kNSmall = 16;
 
 template <int L, int GF, bool MC, int NM>
 static void launch_search(const KArgs& ka, const SearchDev& sp, const WorkItem* work, int n_work, size_t shm,
