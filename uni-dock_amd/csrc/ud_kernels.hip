@@ -571,6 +571,15 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
     float4* mf = s.mf();
     float* pf = s.pf();
     const int N = s.h.n_atoms;
+    const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
+    const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
+    constexpr int kRec = 8;  // partner records fetched together
+    // partner range and first records of the lane's first atom, requested
+    // before pass A so that their latency overlaps it
+    const uint32_t ap0 = (s.gl < N) ? apl[s.gl] : 0u;
+    uint32_t pre[kRec];
+#pragma unroll
+    for (int u = 0; u < kRec; ++u) pre[u] = (u < int(ap0 >> 20)) ? part[int(ap0 & 0xFFFFFu) + u] : 0u;
     float e = 0.f;
     for (int i = s.gl; i < N; i += L) {
         const AtomRec a = s.atoms[i];
@@ -635,18 +644,17 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         }
     }
     __syncthreads();
-    const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
-    const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
-    constexpr int kRec = 4;  // partner records fetched together
     for (int i = s.gl; i < N; i += L) {
         float4 g = mf[i];
         const float4 ci = coords[i];
-        const uint32_t ap = apl[i];
+        const bool first = i == s.gl;
+        const uint32_t ap = first ? ap0 : apl[i];
         const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
         for (int k = 0; k < kn; k += kRec) {
             uint32_t rec[kRec];
 #pragma unroll
-            for (int u = 0; u < kRec; ++u) rec[u] = (k + u < kn) ? part[k0 + k + u] : 0u;
+            for (int u = 0; u < kRec; ++u)
+                rec[u] = (first && k == 0) ? pre[u] : ((k + u < kn) ? part[k0 + k + u] : 0u);
 #pragma unroll
             for (int u = 0; u < kRec; ++u) {
                 if (k + u >= kn) continue;
