@@ -240,6 +240,38 @@ int ud_set_bucket_lanes(ud_ctx* ctx, const int32_t* lanes /* [UD_N_BUCKETS] */);
  * 32 threads per chain). */
 int ud_autotune(ud_ctx* ctx, const ud_search_params* params, int sample_ligands, int32_t* lanes_out);
 
+/* One output mode of a ligand after ud_postprocess. */
+typedef struct ud_mode {
+    float pose[UD_POSE_FLOATS];
+    float energy;            /* final affinity: conf-independent form of inter + intra - intra_best */
+    float inter;             /* ligand-grid energy (caps 1000) */
+    float intra;             /* ligand intramolecular energy (caps 1000) */
+    float conf_independent;  /* energy - (inter + intra - intra_best), the torsion term */
+    float search_energy;     /* the chain's best energy from the search */
+    float rmsd_lb, rmsd_ub;  /* from the best mode (model::rmsd_lower/upper_bound) */
+    int32_t chain;           /* chain the mode comes from (ligand * exhaustiveness + k) */
+} ud_mode;
+
+/* Host post-processing of the last ud_dock: the part of
+ * Vina::global_search_gpu after do_docking (vina.h:232-345) without the
+ * explicit-receptor refinement (the reference's no_refine path).  Per
+ * ligand: the chains' best poses in chain order go through
+ * add_to_output_container (coords.cpp:42-68; heavy-atom RMSD upper bound <
+ * min_rmsd keeps the better of two, at most max_modes kept) as
+ * monte_carlo.cu:2940-2943 does, then remove_redundant (vina.cpp:1465-1470);
+ * the intramolecular energy of the best pose is subtracted and every mode is
+ * rescored (score_gpu, vina.cpp:1245-1327), sorted by the final energy and
+ * given RMSD bounds from the best mode.  modes_out[l * max_modes + k],
+ * n_modes_out[l]. */
+int ud_postprocess(ud_ctx* ctx, int max_modes, double min_rmsd, ud_mode* modes_out, int32_t* n_modes_out);
+
+/* add_to_output_container over n_cand candidates in order (coords.cpp:42-68):
+ * heavy_xyz[n_cand][n_heavy][3], energy[n_cand]; writes the kept candidate
+ * indices sorted by energy to kept_out[max_size] and their number to
+ * n_kept_out.  Host only. */
+int ud_cluster_modes(int n_cand, int n_heavy, const float* heavy_xyz, const float* energy, double min_rmsd,
+                     int max_size, int32_t* kept_out, int32_t* n_kept_out);
+
 /* Seeded random sequential addition of n_atoms points in [0, edge)^3 with a
  * minimum spacing (synthetic receptors); host-only. */
 int ud_synth_receptor(int n_atoms, double edge, double min_dist, uint64_t seed, long long max_attempts,

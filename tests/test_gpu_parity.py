@@ -416,3 +416,57 @@ def test_large_ligand_without_pair_buffer(scene):
             np.testing.assert_allclose(g[i, :6 + nt], g32, rtol=0, atol=1e-4 * sc)
     ctx.set_lanes(0)
     ctx.close()
+
+
+def _xs_element(t):
+    if t in (0, 1, 19, 20, 22, 23, 25, 26, 28, 29):
+        return 6
+    if t in (2, 3, 4, 5, 33, 35):
+        return 7
+    if t in (6, 7, 8, 9, 32, 34):
+        return 8
+    return 100 + t
+
+
+def test_postprocess_matches_oracle(scene):
+    """ud_postprocess (clustering, rescoring, RMSD bounds) against the
+    restatement in oracle/post.py fed with the same chains' coordinates and
+    energy terms."""
+    from oracle import post
+    _say("[gpu] postprocess")
+    ctx, ligs = scene["ctx"], scene["ligs"]
+    exh = 24
+    P, E, _ = ctx.dock(exhaustiveness=exh, global_steps=4, seed=12)
+    max_modes, min_rmsd = 9, 1.0
+    modes = ctx.postprocess(max_modes=max_modes, min_rmsd=min_rmsd)
+    arr = synth.LigandBatch(list(ligs)).arrays()
+    rot_internal = 5 * _lib.VINA_WEIGHTS[6] / 0.1 - 1
+    n_checked = 0
+    for li, lig in enumerate(ligs):
+        a0, a1 = arr["atom_offset"][li], arr["atom_offset"][li + 1]
+        heavy = np.nonzero(arr["atom_heavy"][a0:a1])[0]
+        element = np.array([_xs_element(int(t)) for t in arr["atom_xs"][a0:a1][heavy]])
+        pl = np.full(exh, li, dtype=np.int32)
+        Pl = P[li * exh:(li + 1) * exh]
+        C = ctx.pose_coords(pl, Pl)[:, heavy, :]
+        inter, intra = ctx.score_terms(pl, Pl)
+        ref = post.postprocess(list(C), list(E[li * exh:(li + 1) * exh]), lambda j: float(inter[j]),
+                               lambda j: float(intra[j]), float(arr["num_tors"][li]), rot_internal, element,
+                               min_rmsd, max_modes)
+        got = modes[li]
+        assert [m["chain"] - li * exh for m in got] == [m["chain"] for m in ref], li
+        for g, r in zip(got, ref):
+            assert g["energy"] == pytest.approx(r["energy"], rel=1e-5, abs=1e-5)
+            assert g["inter"] == pytest.approx(r["inter"], rel=1e-6, abs=1e-6)
+            assert g["intra"] == pytest.approx(r["intra"], rel=1e-6, abs=1e-6)
+            assert g["rmsd_ub"] == pytest.approx(r["rmsd_ub"], rel=1e-5, abs=1e-5)
+            assert g["rmsd_lb"] == pytest.approx(r["rmsd_lb"], rel=1e-5, abs=1e-5)
+            np.testing.assert_array_equal(g["pose"], P[g["chain"]])
+        energies = [m["energy"] for m in got]
+        assert energies == sorted(energies) and got[0]["rmsd_ub"] == 0.0
+        for i in range(len(got)):
+            for j in range(i + 1, len(got)):
+                ci, cj = C[got[i]["chain"] - li * exh], C[got[j]["chain"] - li * exh]
+                assert post.rmsd_upper_bound(ci, cj) >= min_rmsd
+        n_checked += len(got)
+    assert n_checked >= 2 * len(ligs)

@@ -7,11 +7,13 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/unidock_amd.h"
 #include "ud_internal.h"
 #include "ud_launch.h"
+#include "ud_post.h"
 
 using namespace ud;
 
@@ -66,6 +68,7 @@ struct ud_ctx {
     long long* d_out_evals = nullptr;
     size_t out_cap = 0;
     size_t out_n = 0;
+    int last_exh = 0;  // exhaustiveness of the last dock
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
     int last_launches = 0;
@@ -850,6 +853,7 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
         c->out_cap = nch;
     }
     c->out_n = nch;
+    c->last_exh = exh;
     SearchDev sp;
     std::memset(&sp, 0, sizeof(sp));
     sp.exh = exh;
@@ -1096,6 +1100,133 @@ int ud_set_lanes(ud_ctx* c, int lanes) {
     if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
         return fail(c, UD_ERR_INVALID, "lanes must be 0, 8, 16, 32 or 64");
     c->forced_lanes = lanes;
+    return UD_OK;
+}
+
+int ud_cluster_modes(int n_cand, int n_heavy, const float* heavy_xyz, const float* energy, double min_rmsd,
+                     int max_size, int32_t* kept_out, int32_t* n_kept_out) {
+    if (n_cand < 0 || n_heavy < 0 || max_size < 1 || !(min_rmsd >= 0.0) || !kept_out || !n_kept_out ||
+        (n_cand > 0 && (!energy || (n_heavy > 0 && !heavy_xyz))))
+        return UD_ERR_INVALID;
+    std::vector<int> kept;
+    cluster_modes(n_cand, n_heavy, heavy_xyz, energy, float(min_rmsd), max_size, kept);
+    for (size_t k = 0; k < kept.size(); ++k) kept_out[k] = kept[k];
+    *n_kept_out = int32_t(kept.size());
+    return UD_OK;
+}
+
+int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, ud_mode* modes_out, int32_t* n_modes_out) {
+    if (!c || max_modes < 1 || !(min_rmsd >= 0.0) || !modes_out || !n_modes_out) return UD_ERR_INVALID;
+    if (c->out_n == 0 || c->last_exh <= 0) return fail(c, UD_ERR_STATE, "no docking results");
+    const int exh = c->last_exh, L = c->n_lig;
+    const size_t nch = size_t(L) * exh;
+    std::vector<float> P(nch * UD_POSE_FLOATS), E(nch);
+    int rc = ud_fetch_dock_results(c, P.data(), E.data(), nullptr);
+    if (rc) return rc;
+    // heavy atoms of each ligand and their element classes
+    std::vector<std::vector<int>> heavy(L), elem(L);
+    std::vector<size_t> hoff(L + 1, 0);
+    for (int l = 0; l < L; ++l) {
+        const LigHdr& h = c->hdr[l];
+        for (int i = 0; i < h.n_atoms; ++i) {
+            const uint32_t meta = c->h_atoms[size_t(h.atom_begin) + i].meta;
+            if ((meta >> 16) & 1) {
+                heavy[l].push_back(i);
+                elem[l].push_back(xs_element(int(meta >> 24)));
+            }
+        }
+        hoff[l + 1] = hoff[l] + size_t(exh) * heavy[l].size() * 3;
+    }
+    // heavy-atom coordinates of every chain's best pose, in chunks through the coords kernel
+    std::vector<float> hx(hoff[L]);
+    const int chunk = 4096;
+    std::vector<float> tmp(size_t(chunk) * UD_MAX_ATOMS * 3);
+    std::vector<int32_t> pl(chunk);
+    for (size_t c0 = 0; c0 < nch; c0 += chunk) {
+        const int n = int(std::min<size_t>(chunk, nch - c0));
+        for (int k = 0; k < n; ++k) pl[k] = int32_t((c0 + k) / exh);
+        rc = ud_pose_coords(c, n, pl.data(), P.data() + c0 * UD_POSE_FLOATS, tmp.data());
+        if (rc) return rc;
+        for (int k = 0; k < n; ++k) {
+            const int l = pl[k], j = int((c0 + k) % exh);
+            const std::vector<int>& hv = heavy[l];
+            float* dst = hx.data() + hoff[l] + size_t(j) * hv.size() * 3;
+            for (size_t a = 0; a < hv.size(); ++a)
+                for (int d = 0; d < 3; ++d) dst[3 * a + d] = tmp[(size_t(k) * UD_MAX_ATOMS + hv[a]) * 3 + d];
+        }
+    }
+    // per ligand: merge the chains in order, then remove_redundant over the result
+    std::vector<std::vector<int>> kept(L);
+    auto cluster = [&](int l) {
+        const int nh = int(heavy[l].size());
+        const float* xyz = hx.data() + hoff[l];
+        std::vector<int> first;
+        cluster_modes(exh, nh, xyz, E.data() + size_t(l) * exh, float(min_rmsd), max_modes, first);
+        std::vector<float> sx(first.size() * nh * 3), se(first.size());
+        for (size_t k = 0; k < first.size(); ++k) {
+            std::copy(xyz + size_t(first[k]) * nh * 3, xyz + size_t(first[k] + 1) * nh * 3, sx.begin() + k * nh * 3);
+            se[k] = E[size_t(l) * exh + first[k]];
+        }
+        std::vector<int> second;
+        cluster_modes(int(first.size()), nh, sx.data(), se.data(), float(min_rmsd), int(first.size()), second);
+        for (int k : second) kept[l].push_back(first[k]);
+    };
+    const int nt = std::max(1, std::min<int>({16, L, int(std::thread::hardware_concurrency())}));
+    {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nt; ++t)
+            pool.emplace_back([&, t] {
+                for (int l = t; l < L; l += nt) cluster(l);
+            });
+        for (auto& th : pool) th.join();
+    }
+    // rescoring of every kept mode with authentic caps
+    std::vector<int32_t> kl;
+    std::vector<float> kp;
+    for (int l = 0; l < L; ++l)
+        for (int k : kept[l]) {
+            kl.push_back(l);
+            const float* src = P.data() + (size_t(l) * exh + k) * UD_POSE_FLOATS;
+            kp.insert(kp.end(), src, src + UD_POSE_FLOATS);
+        }
+    std::vector<float> inter(kl.size()), intra(kl.size());
+    if (!kl.empty()) {
+        rc = ud_score_terms(c, int(kl.size()), kl.data(), kp.data(), inter.data(), intra.data());
+        if (rc) return rc;
+    }
+    size_t q = 0;
+    for (int l = 0; l < L; ++l) {
+        const int nk = int(kept[l].size());
+        ud_mode* out = modes_out + size_t(l) * max_modes;
+        n_modes_out[l] = nk;
+        if (nk == 0) continue;
+        // the intramolecular energy of the best pose of the search is subtracted from every mode
+        const float intra_best = intra[q];
+        for (int k = 0; k < nk; ++k) {
+            ud_mode& m = out[k];
+            const int j = kept[l][k];
+            std::copy(P.data() + (size_t(l) * exh + j) * UD_POSE_FLOATS,
+                      P.data() + (size_t(l) * exh + j + 1) * UD_POSE_FLOATS, m.pose);
+            m.inter = inter[q + k];
+            m.intra = intra[q + k];
+            const float x = m.inter + m.intra - intra_best;
+            m.energy = conf_independent(x, c->hdr[l].num_tors, c->rot_internal);
+            m.conf_independent = m.energy - x;
+            m.search_energy = E[size_t(l) * exh + j];
+            m.chain = int32_t(size_t(l) * exh + j);
+        }
+        q += nk;
+        std::stable_sort(out, out + nk, [](const ud_mode& a, const ud_mode& b) { return a.energy < b.energy; });
+        // RMSD bounds from the best mode
+        const int nh = int(heavy[l].size());
+        const float* xyz = hx.data() + hoff[l];
+        const float* best = xyz + size_t(out[0].chain - size_t(l) * exh) * nh * 3;
+        for (int k = 0; k < nk; ++k) {
+            const float* cur = xyz + size_t(out[k].chain - size_t(l) * exh) * nh * 3;
+            out[k].rmsd_ub = rmsd_upper_bound(cur, best, nh);
+            out[k].rmsd_lb = rmsd_lower_bound(cur, best, elem[l].data(), nh);
+        }
+    }
     return UD_OK;
 }
 

@@ -43,6 +43,12 @@ class UdLaunchInfo(ctypes.Structure):
                 ("n_workgroups", ctypes.c_int32), ("ms", ctypes.c_double)]
 
 
+class UdMode(ctypes.Structure):
+    _fields_ = [("pose", ctypes.c_float * UD_POSE_FLOATS), ("energy", ctypes.c_float), ("inter", ctypes.c_float),
+                ("intra", ctypes.c_float), ("conf_independent", ctypes.c_float), ("search_energy", ctypes.c_float),
+                ("rmsd_lb", ctypes.c_float), ("rmsd_ub", ctypes.c_float), ("chain", ctypes.c_int32)]
+
+
 class UdSearchParams(ctypes.Structure):
     _fields_ = [("exhaustiveness", ctypes.c_int32), ("global_steps", ctypes.c_int32),
                 ("local_steps", ctypes.c_int32), ("lanes_per_chain", ctypes.c_int32),
@@ -92,6 +98,8 @@ def load() -> ctypes.CDLL:
         "ud_last_launches": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(UdLaunchInfo), ip]),
         "ud_set_bucket_lanes": (ctypes.c_int, [vp, ip]),
         "ud_autotune": (ctypes.c_int, [vp, ctypes.POINTER(UdSearchParams), ctypes.c_int, ip]),
+        "ud_postprocess": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(UdMode), ip]),
+        "ud_cluster_modes": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, fp, fp, ctypes.c_double, ctypes.c_int, ip, ip]),
         "ud_synth_receptor": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                              ctypes.c_longlong, dp]),
     }
@@ -111,6 +119,22 @@ class UdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"unidock_amd error {code}: {msg}")
         self.code = code
+
+
+def cluster_modes(heavy_xyz, energy, min_rmsd=1.0, max_size=9):
+    """add_to_output_container over candidates in order (host only); kept indices."""
+    L = load()
+    x = np.ascontiguousarray(heavy_xyz, dtype=np.float32)
+    e = np.ascontiguousarray(energy, dtype=np.float32)
+    n = x.shape[0]
+    nh = x.shape[1] if x.ndim == 3 else (x.shape[1] // 3 if x.ndim == 2 else 0)
+    kept = np.zeros(max_size, dtype=np.int32)
+    nk = ctypes.c_int32()
+    rc = L.ud_cluster_modes(n, nh, _p(x, ctypes.c_float), _p(e, ctypes.c_float), min_rmsd, max_size,
+                            _p(kept, ctypes.c_int32), ctypes.byref(nk))
+    if rc != UD_OK:
+        raise UdError(rc, "ud_cluster_modes failed")
+    return kept[:nk.value].tolist()
 
 
 def synth_receptor_xyz(n_atoms: int, edge: float, min_dist: float, seed: int) -> np.ndarray:
@@ -270,6 +294,24 @@ class Context:
         out = np.zeros((len(pl), UD_MAX_ATOMS, 3), dtype=np.float32)
         self._ck(self.L.ud_pose_coords(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float),
                                        _p(out, ctypes.c_float)))
+        return out
+
+    def postprocess(self, max_modes=9, min_rmsd=1.0):
+        """Cluster and rescore the last dock's chains; one list of mode dicts per ligand."""
+        n = self.n_ligands
+        modes = (UdMode * (n * max_modes))()
+        cnt = np.zeros(n, dtype=np.int32)
+        self._ck(self.L.ud_postprocess(self.h, max_modes, min_rmsd, modes, _p(cnt, ctypes.c_int32)))
+        out = []
+        for l in range(n):
+            lst = []
+            for k in range(int(cnt[l])):
+                m = modes[l * max_modes + k]
+                lst.append(dict(pose=np.array(m.pose[:], dtype=np.float32), energy=m.energy, inter=m.inter,
+                                intra=m.intra, conf_independent=m.conf_independent,
+                                search_energy=m.search_energy, rmsd_lb=m.rmsd_lb, rmsd_ub=m.rmsd_ub,
+                                chain=int(m.chain)))
+            out.append(lst)
         return out
 
     def set_lanes(self, lanes: int):
