@@ -470,3 +470,47 @@ def test_postprocess_matches_oracle(scene):
                 assert post.rmsd_upper_bound(ci, cj) >= min_rmsd
         n_checked += len(got)
     assert n_checked >= 2 * len(ligs)
+
+
+def test_vinardo_maps_energy_and_gradient():
+    """Vinardo scoring (scoring_function.h:60-70): maps vs the FP32
+    restatement, energies and gradients vs FP32 and FP64 oracles."""
+    _say("[gpu] vinardo")
+    xyz, xs, cav = small_receptor()
+    ligs = small_ligands()
+    ctx = _lib.Context(0)
+    ctx.set_scoring(_lib.UD_SF_VINARDO)
+    c, s = box_of(cav)
+    ctx.set_box(c, s, 0.375)
+    ctx.compute_maps(xyz, xs, LIGAND_TYPES, _lib.UD_GRID_F32)
+    ctx.set_ligands(synth.LigandBatch(list(ligs)).arrays())
+    o32, _, _ = oracle_scene(32, sf=1)
+    o64m, _, _ = oracle_scene(64, sf=1, with_maps=False)
+    for t in LIGAND_TYPES:
+        m = ctx.get_map(t)
+        r = o32.get_map(t)
+        bad = ~np.isclose(m, r, rtol=1e-5, atol=1e-5)
+        assert bad.mean() < 2e-3, (t, bad.mean())
+        o64m.set_map(t, m.astype(np.float64))
+    rng = np.random.default_rng(21)
+    pl, P = [], []
+    for li, lig in enumerate(ligs):
+        for p in random_poses(rng, lig, cav, 2.5, 5):
+            pl.append(li)
+            P.append(p)
+    pl = np.array(pl, dtype=np.int32)
+    P = np.array(P)
+    e, g = ctx.score_grad(pl, P, (10.0, 10.0, 10.0))
+    lo = o64m.geo[:3]
+    hi = lo + o64m.geo[6:9] * (np.array(o64m.n) - 1)
+    for i in range(len(pl)):
+        li = int(pl[i])
+        nt = ligs[li].n_tors
+        e32, g32, _, _ = o32.eval_deriv(li, P[i], (10.0, 10.0, 10.0))
+        e64, g64, c64, _ = o64m.eval_deriv(li, P[i], (10.0, 10.0, 10.0))
+        assert e[i] == pytest.approx(e32, rel=1e-5, abs=1e-4), (i, e[i], e32)
+        if bool(np.all(c64 >= lo) and np.all(c64 <= hi)):
+            assert e[i] == pytest.approx(e64, rel=1e-4, abs=1e-4), (i, e[i], e64)
+        sc = max(1.0, np.abs(g32).max())
+        np.testing.assert_allclose(g[i, :6 + nt], g32, rtol=0, atol=1e-4 * sc)
+    ctx.close()
