@@ -272,6 +272,33 @@ int ud_postprocess(ud_ctx* ctx, int max_modes, double min_rmsd, ud_mode* modes_o
 int ud_cluster_modes(int n_cand, int n_heavy, const float* heavy_xyz, const float* energy, double min_rmsd,
                      int max_size, int32_t* kept_out, int32_t* n_kept_out);
 
+/* ---- ingestion (host only) ----------------------------------------------
+ * A ligand set built from PDBQT text: the torsion tree of the ROOT / BRANCH
+ * blocks in the reference's pre-order (parse_pdbqt.cpp:440-540), bonds,
+ * X-Score types and the intramolecular pair list (model::assign_bonds /
+ * assign_types / initialize_pairs, model.cpp:343-640) and the
+ * conf-independent torsion count (conf_independent.cpp:36-103). */
+typedef struct ud_ligset ud_ligset;
+int ud_ligset_create(ud_ligset** out);
+int ud_ligset_destroy(ud_ligset* set);
+const char* ud_ligset_last_error(const ud_ligset* set);
+/* Parse one ligand (a single MODEL); index_out receives its index. */
+int ud_ligset_add_pdbqt(ud_ligset* set, const char* text, int32_t* index_out);
+int ud_ligset_count(const ud_ligset* set);
+/* Fill *batch with arrays owned by the set (valid until the next add or
+ * destroy), ready for ud_set_ligands. */
+int ud_ligset_batch(ud_ligset* set, ud_ligand_batch* batch);
+/* Input coordinates and atom numbers of ligand `lig` in the batch's atom
+ * order (for writing poses back out); n_atoms_out first, arrays may be NULL. */
+int ud_ligset_atoms(const ud_ligset* set, int lig, int32_t* n_atoms_out, float* xyz_out, int32_t* serial_out);
+
+/* Receptor heavy atoms with X-Score types from PDBQT text (the rigid
+ * receptor of Vina::set_receptor, typed like the ligand: bonds decide C_P /
+ * C_H and donors).  n_out receives the atom count; with capacity too small
+ * (or NULL arrays) nothing is written beyond it and UD_ERR_CAPACITY is
+ * returned when arrays were given. */
+int ud_receptor_from_pdbqt(const char* text, int32_t capacity, int32_t* n_out, float* xyz_out, int32_t* xs_out);
+
 /* Seeded random sequential addition of n_atoms points in [0, edge)^3 with a
  * minimum spacing (synthetic receptors); host-only. */
 int ud_synth_receptor(int n_atoms, double edge, double min_dist, uint64_t seed, long long max_attempts,

@@ -514,3 +514,47 @@ def test_vinardo_maps_energy_and_gradient():
         sc = max(1.0, np.abs(g32).max())
         np.testing.assert_allclose(g[i, :6 + nt], g32, rtol=0, atol=1e-4 * sc)
     ctx.close()
+
+
+def test_redock_1iep_from_reference_fixture():
+    """The reference's own test (unidock/test/ut/test_unidock.py): the 1iep
+    ligand docked into its receptor, box centre (15.19, 53.903, 16.917), size
+    20 A, fast search (exhaustiveness 128, max_step 20), seed 181129, Vina
+    and Vinardo; the best mode's score lies in [-20, -1] kcal/mol.  The best
+    mode is also compared with the crystal pose the ligand file holds."""
+    import os
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    rec_xyz, rec_xs = _lib.receptor_from_pdbqt(open(os.path.join(golden, "1iep_receptor.pdbqt")).read())
+    ls = _lib.LigandSet()
+    ls.add_pdbqt(open(os.path.join(golden, "1iep_ligand.pdbqt")).read())
+    lig = ls.ligands()[0]
+    heavy = np.nonzero(lig.heavy)[0]
+    for sf in (_lib.UD_SF_VINA, _lib.UD_SF_VINARDO):
+        _say(f"[gpu] 1iep redock sf={sf}")
+        ctx = _lib.Context(0)
+        ctx.set_scoring(sf)
+        ctx.set_box((15.19, 53.903, 16.917), (20.0, 20.0, 20.0), 0.375)
+        types = sorted({int(t) for t in lig.xs if t >= 0})
+        ctx.compute_maps(rec_xyz, rec_xs, types, _lib.UD_GRID_F32)
+        ctx.set_ligands(ls.arrays())
+        ctx.dock(exhaustiveness=128, global_steps=20, seed=181129)
+        modes = ctx.postprocess(max_modes=9, min_rmsd=1.0)[0]
+        best = modes[0]
+        c = ctx.pose_coords(np.array([0], dtype=np.int32), best["pose"][None])[0, :lig.n_atoms]
+        rmsd = float(np.sqrt(np.mean(np.sum((c[heavy] - lig.coords[heavy]) ** 2, axis=1))))
+        _say(f"[gpu] 1iep sf={sf}: best {best['energy']:.3f} kcal/mol, {len(modes)} modes, "
+             f"RMSD to crystal {rmsd:.2f} A")
+        assert -20.0 <= best["energy"] <= -1.0
+        # the crystal pose is a minimum of the scoring function: a local
+        # optimisation from it stays within 1 A and lowers the energy
+        P0 = np.zeros((1, _lib.UD_POSE_FLOATS), dtype=np.float32)
+        P0[0, :3] = lig.origin[0]
+        P0[0, 3] = 1.0
+        pl = np.zeros(1, dtype=np.int32)
+        e0, _ = ctx.score_grad(pl, P0)
+        Po, eo, _ = ctx.optimize(pl, P0, max_steps=200)
+        co = ctx.pose_coords(pl, Po)[0, :lig.n_atoms]
+        r0 = float(np.sqrt(np.mean(np.sum((co[heavy] - lig.coords[heavy]) ** 2, axis=1))))
+        _say(f"[gpu] 1iep sf={sf}: crystal pose {e0[0]:.3f} -> optimised {eo[0]:.3f} kcal/mol, moved {r0:.2f} A")
+        assert r0 < 1.0 and eo[0] < e0[0]
+        ctx.close()

@@ -1,0 +1,568 @@
+// Ligand and receptor ingestion from PDBQT text: the torsion tree of
+// ROOT / BRANCH blocks, bonds, X-Score atom types, the intramolecular pair
+// list and the conf-independent torsion count, as the reference builds them
+// when it parses a ligand (parse_pdbqt.cpp:107-126, 440-540) and initialises
+// its model (model::assign_bonds / assign_types / initialize_pairs,
+// model.cpp:343-640; conf_independent_inputs, conf_independent.cpp:36-103).
+// Host code only; the result feeds ud_set_ligands / ud_compute_maps.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/unidock_amd.h"
+
+namespace {
+
+// AutoDock atom kinds (atom_constants.h:156-192): name, element class,
+// covalent radius.  Element classes: 0 H, 1 C, 2 N, 3 O, 4 S, 5 P, 6 F, 7 Cl,
+// 8 Br, 9 I, 10 Si, 11 At, 12 metal, 13 dummy.
+struct AdKind {
+    const char* name;
+    int el;
+    float cov;
+};
+enum { EL_H = 0, EL_C, EL_N, EL_O, EL_S, EL_P, EL_F, EL_Cl, EL_Br, EL_I, EL_Si, EL_At, EL_Met, EL_Dummy };
+enum {
+    AD_C = 0, AD_A, AD_N, AD_O, AD_P, AD_S, AD_H, AD_F, AD_I, AD_NA, AD_OA, AD_SA, AD_HD, AD_Mg, AD_Mn, AD_Zn,
+    AD_Ca, AD_Fe, AD_Cl, AD_Br, AD_Si, AD_At, AD_G0, AD_G1, AD_G2, AD_G3, AD_CG0, AD_CG1, AD_CG2, AD_CG3, AD_W,
+    AD_OXA, AD_NXA, AD_OXD, AD_NXD, AD_SIZE
+};
+const AdKind kAd[AD_SIZE] = {
+    {"C", EL_C, 0.77f},   {"A", EL_C, 0.77f},   {"N", EL_N, 0.75f},    {"O", EL_O, 0.73f},   {"P", EL_P, 1.06f},
+    {"S", EL_S, 1.02f},   {"H", EL_H, 0.37f},   {"F", EL_F, 0.71f},    {"I", EL_I, 1.33f},   {"NA", EL_N, 0.75f},
+    {"OA", EL_O, 0.73f},  {"SA", EL_S, 1.02f},  {"HD", EL_H, 0.37f},   {"Mg", EL_Met, 1.30f}, {"Mn", EL_Met, 1.39f},
+    {"Zn", EL_Met, 1.31f}, {"Ca", EL_Met, 1.74f}, {"Fe", EL_Met, 1.25f}, {"Cl", EL_Cl, 0.99f}, {"Br", EL_Br, 1.14f},
+    {"Si", EL_Si, 1.11f}, {"At", EL_At, 1.44f}, {"G0", EL_Dummy, 0.77f}, {"G1", EL_Dummy, 0.77f},
+    {"G2", EL_Dummy, 0.77f}, {"G3", EL_Dummy, 0.77f}, {"CG0", EL_C, 0.77f}, {"CG1", EL_C, 0.77f},
+    {"CG2", EL_C, 0.77f}, {"CG3", EL_C, 0.77f}, {"W", EL_Dummy, 0.0f}, {"OXA", EL_O, 0.73f},
+    {"NXA", EL_N, 0.75f}, {"OXD", EL_O, 0.73f}, {"NXD", EL_N, 0.75f}};
+const float kMaxCov = 1.74f;  // largest covalent radius of the table
+const float kBondAllowance = 1.1f;
+
+// X-Score types (atom_constants.h:84-120)
+enum {
+    XS_C_H = 0, XS_C_P, XS_N_P, XS_N_D, XS_N_A, XS_N_DA, XS_O_P, XS_O_D, XS_O_A, XS_O_DA, XS_S_P, XS_P_P, XS_F_H,
+    XS_Cl_H, XS_Br_H, XS_I_H, XS_Si, XS_At, XS_Met_D, XS_C_H_CG0, XS_C_P_CG0, XS_G0, XS_C_H_CG1, XS_C_P_CG1, XS_G1,
+    XS_C_H_CG2, XS_C_P_CG2, XS_G2, XS_C_H_CG3, XS_C_P_CG3, XS_G3, XS_W, XS_O_XA, XS_N_XA, XS_O_XD, XS_N_XD
+};
+
+int ad_of(const std::string& name) {
+    std::string n = name;
+    if (n == "Se") n = "S";  // atom_equivalence_data (atom_constants.h:208)
+    if (n == "CL") n = "Cl";
+    for (int i = 0; i < AD_SIZE; ++i)
+        if (n == kAd[i].name) return i;
+    return -1;
+}
+// ad_is_heteroatom (atom_constants.h:227-230): everything but C, A, H, HD
+bool ad_is_heteroatom(int ad) { return ad != AD_A && ad != AD_C && ad != AD_H && ad != AD_HD; }
+
+struct Atom {
+    int serial = 0;
+    int ad = 0;
+    int el = 0;
+    float x = 0, y = 0, z = 0;
+    int node = 0;
+    int xs = -1;
+};
+
+std::string trim(const std::string& s) {
+    size_t a = s.find_first_not_of(" \t\r"), b = s.find_last_not_of(" \t\r");
+    return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+// 1-based inclusive columns, as checked_convert_substring (parse_pdbqt.cpp:107-126)
+std::string cols(const std::string& s, size_t i, size_t j) {
+    if (s.size() < i) return std::string();
+    return s.substr(i - 1, std::min(j, s.size()) - (i - 1));
+}
+
+bool parse_atom_line(const std::string& ln, Atom& a, std::string& err) {
+    char* end = nullptr;
+    const std::string sn = trim(cols(ln, 7, 11));
+    a.serial = int(std::strtol(sn.c_str(), &end, 10));
+    if (sn.empty() || *end) { err = "bad atom number: " + ln; return false; }
+    float v[3];
+    for (int d = 0; d < 3; ++d) {
+        const std::string c = trim(cols(ln, 31 + 8 * d, 38 + 8 * d));
+        v[d] = std::strtof(c.c_str(), &end);
+        if (c.empty() || *end) { err = "bad coordinate: " + ln; return false; }
+    }
+    a.x = v[0]; a.y = v[1]; a.z = v[2];
+    const std::string t = trim(cols(ln, 78, 79));
+    a.ad = ad_of(t);
+    if (a.ad < 0) { err = "atom type " + t + " is not a valid AutoDock type: " + ln; return false; }
+    a.el = kAd[a.ad].el;
+    return true;
+}
+
+float dist2(const Atom& a, const Atom& b) {
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// model::assign_bonds (model.cpp:343-409): i-j is a bond when their mobility
+// is not variable, r < 1.1 (R_i + R_j), and no atom immobile relative to both
+// lies closer to both of them than they are to each other.  `fixed(i, j)`
+// tells whether two atoms cannot move relative to each other.
+template <class Fixed>
+void assign_bonds(const std::vector<Atom>& at, Fixed fixed, std::vector<std::vector<int>>& bonds) {
+    const int n = int(at.size());
+    bonds.assign(n, {});
+    // cell list of edge 2 * 1.1 * kMaxCov, enough for every bond candidate
+    const float cell = 2.f * kBondAllowance * kMaxCov;
+    std::map<long long, std::vector<int>> grid;
+    auto key = [&](int kx, int ky, int kz) { return (static_cast<long long>(kx) * 1000003LL + ky) * 1000003LL + kz; };
+    std::vector<int> cx(n), cy(n), cz(n);
+    for (int i = 0; i < n; ++i) {
+        cx[i] = int(std::floor(at[i].x / cell));
+        cy[i] = int(std::floor(at[i].y / cell));
+        cz[i] = int(std::floor(at[i].z / cell));
+        grid[key(cx[i], cy[i], cz[i])].push_back(i);
+    }
+    for (int i = 0; i < n; ++i) {
+        const float ri = kAd[at[i].ad].cov;
+        std::vector<int> rel;  // relevant atoms: immobile relative to i and near it
+        const float cut2 = (kBondAllowance * (ri + kMaxCov)) * (kBondAllowance * (ri + kMaxCov));
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    auto it = grid.find(key(cx[i] + dx, cy[i] + dy, cz[i] + dz));
+                    if (it == grid.end()) continue;
+                    for (int j : it->second)
+                        if (j != i && fixed(i, j) && dist2(at[i], at[j]) < cut2) rel.push_back(j);
+                }
+        std::sort(rel.begin(), rel.end());
+        for (int j : rel) {
+            if (j <= i) continue;
+            const float bl = ri + kAd[at[j].ad].cov;
+            const float r2 = dist2(at[i], at[j]);
+            if (!(r2 < (kBondAllowance * bl) * (kBondAllowance * bl))) continue;
+            bool between = false;  // atom_exists_between (model.cpp:307-321)
+            for (int c : rel) {
+                if (c == j) continue;
+                if (fixed(j, c) && dist2(at[i], at[c]) < r2 && dist2(at[j], at[c]) < r2) {
+                    between = true;
+                    break;
+                }
+            }
+            if (between) continue;
+            bonds[i].push_back(j);
+            bonds[j].push_back(i);
+        }
+    }
+}
+
+// model::assign_types (model.cpp:427-530)
+void assign_types(std::vector<Atom>& at, const std::vector<std::vector<int>>& bonds) {
+    for (size_t i = 0; i < at.size(); ++i) {
+        Atom& a = at[i];
+        bool hd = false, hetero = false;
+        for (int j : bonds[i]) {
+            if (at[j].ad == AD_HD) hd = true;
+            if (ad_is_heteroatom(at[j].ad)) hetero = true;
+        }
+        const bool acceptor = a.ad == AD_OA || a.ad == AD_NA;
+        const bool donor = a.el == EL_Met || hd;
+        switch (a.el) {
+            case EL_H: a.xs = -1; break;
+            case EL_C:
+                if (a.ad == AD_CG0) a.xs = hetero ? XS_C_P_CG0 : XS_C_H_CG0;
+                else if (a.ad == AD_CG1) a.xs = hetero ? XS_C_P_CG1 : XS_C_H_CG1;
+                else if (a.ad == AD_CG2) a.xs = hetero ? XS_C_P_CG2 : XS_C_H_CG2;
+                else if (a.ad == AD_CG3) a.xs = hetero ? XS_C_P_CG3 : XS_C_H_CG3;
+                else a.xs = hetero ? XS_C_P : XS_C_H;
+                break;
+            case EL_N:
+                if (a.ad == AD_NXA) a.xs = XS_N_XA;
+                else if (a.ad == AD_NXD) a.xs = XS_N_XD;
+                else a.xs = (acceptor && donor) ? XS_N_DA : (acceptor ? XS_N_A : (donor ? XS_N_D : XS_N_P));
+                break;
+            case EL_O:
+                // the reference maps OXA / OXD to the N_X types (model.cpp:465-470)
+                if (a.ad == AD_OXA) a.xs = XS_N_XA;
+                else if (a.ad == AD_OXD) a.xs = XS_N_XD;
+                else a.xs = (acceptor && donor) ? XS_O_DA : (acceptor ? XS_O_A : (donor ? XS_O_D : XS_O_P));
+                break;
+            case EL_S: a.xs = XS_S_P; break;
+            case EL_P: a.xs = XS_P_P; break;
+            case EL_F: a.xs = XS_F_H; break;
+            case EL_Cl: a.xs = XS_Cl_H; break;
+            case EL_Br: a.xs = XS_Br_H; break;
+            case EL_I: a.xs = XS_I_H; break;
+            case EL_Si: a.xs = XS_Si; break;
+            case EL_At: a.xs = XS_At; break;
+            case EL_Met: a.xs = XS_Met_D; break;
+            default:  // dummies: closure atoms G0..G3 keep their types, W has none
+                if (a.ad == AD_G0) a.xs = XS_G0;
+                else if (a.ad == AD_G1) a.xs = XS_G1;
+                else if (a.ad == AD_G2) a.xs = XS_G2;
+                else if (a.ad == AD_G3) a.xs = XS_G3;
+                else a.xs = -1;
+        }
+    }
+}
+
+struct Node {
+    int parent = -1;
+    int axis_begin = -1;  // atom (in the parent) the rotatable bond starts from
+    int axis_end = -1;    // first atom of the branch: the node's origin
+    std::vector<int> atoms;  // file order
+    int file_order = 0;
+};
+
+struct Ligand {
+    std::vector<Atom> atoms;  // reference order (nodes in pre-order, file order inside)
+    std::vector<int> node_parent, node_abegin, node_aend;
+    std::vector<float> node_origin, node_axis;
+    std::vector<int> pairs;
+    float num_tors = 0;
+    int torsdof = 0;
+};
+
+bool build_ligand(const std::string& text, Ligand& L, std::string& err) {
+    std::vector<Atom> fa;  // file order
+    std::vector<Node> nodes;
+    std::vector<int> stack;
+    std::map<int, int> by_serial;
+    bool root_seen = false;
+    int torsdof = -1;
+    std::istringstream in(text);
+    std::string ln;
+    while (std::getline(in, ln)) {
+        if (!ln.empty() && ln.back() == '\r') ln.pop_back();
+        if (ln.rfind("ATOM", 0) == 0 || ln.rfind("HETATM", 0) == 0) {
+            if (stack.empty()) { err = "atom outside ROOT / BRANCH: " + ln; return false; }
+            Atom a;
+            if (!parse_atom_line(ln, a, err)) return false;
+            a.node = stack.back();
+            if (by_serial.count(a.serial)) { err = "duplicate atom number: " + ln; return false; }
+            by_serial[a.serial] = int(fa.size());
+            nodes[a.node].atoms.push_back(int(fa.size()));
+            fa.push_back(a);
+        } else if (ln.rfind("ROOT", 0) == 0) {
+            if (root_seen) { err = "second ROOT"; return false; }
+            root_seen = true;
+            nodes.push_back(Node());
+            stack.push_back(0);
+        } else if (ln.rfind("ENDROOT", 0) == 0) {
+            if (stack.size() != 1) { err = "misplaced ENDROOT"; return false; }
+            stack.pop_back();
+        } else if (ln.rfind("BRANCH", 0) == 0) {
+            std::istringstream ss(ln.substr(6));
+            int a = 0, b = 0;
+            if (!(ss >> a >> b)) { err = "bad BRANCH line: " + ln; return false; }
+            auto it = by_serial.find(a);
+            if (it == by_serial.end()) { err = "BRANCH from an unknown atom: " + ln; return false; }
+            Node nd;
+            nd.parent = fa[it->second].node;
+            nd.axis_begin = it->second;
+            nd.axis_end = b;  // serial for now, resolved after parsing
+            nd.file_order = int(nodes.size());
+            nodes.push_back(nd);
+            stack.push_back(int(nodes.size()) - 1);
+        } else if (ln.rfind("ENDBRANCH", 0) == 0) {
+            if (stack.empty() || stack.back() == 0) { err = "misplaced ENDBRANCH"; return false; }
+            stack.pop_back();
+        } else if (ln.rfind("TORSDOF", 0) == 0) {
+            torsdof = std::atoi(ln.c_str() + 7);
+        } else if (ln.rfind("MODEL", 0) == 0) {
+            err = "multi-MODEL ligand text: split it first";
+            return false;
+        }
+    }
+    if (!root_seen || nodes.empty() || nodes[0].atoms.empty()) { err = "no ROOT atoms"; return false; }
+    if (!stack.empty()) { err = "unterminated ROOT or BRANCH"; return false; }
+    for (size_t k = 1; k < nodes.size(); ++k) {
+        auto it = by_serial.find(nodes[k].axis_end);
+        if (it == by_serial.end() || fa[it->second].node != int(k) || nodes[k].atoms.empty() ||
+            nodes[k].atoms.front() != it->second) {
+            err = "BRANCH must start with its second axis atom";
+            return false;
+        }
+        nodes[k].axis_end = it->second;
+    }
+    // pre-order: a node's children ordered by the position of their first axis
+    // atom in the node, then file order (postprocess_branch, parse_pdbqt.cpp:498-533)
+    std::vector<std::vector<int>> kids(nodes.size());
+    for (size_t k = 1; k < nodes.size(); ++k) kids[nodes[k].parent].push_back(int(k));
+    for (size_t k = 0; k < nodes.size(); ++k) {
+        const std::vector<int>& na = nodes[k].atoms;
+        auto pos = [&](int child) { return int(std::find(na.begin(), na.end(), nodes[child].axis_begin) - na.begin()); };
+        std::stable_sort(kids[k].begin(), kids[k].end(), [&](int a, int b) { return pos(a) < pos(b); });
+    }
+    std::vector<int> order;
+    std::vector<int> st{0};
+    while (!st.empty()) {
+        const int k = st.back();
+        st.pop_back();
+        order.push_back(k);
+        for (auto it = kids[k].rbegin(); it != kids[k].rend(); ++it) st.push_back(*it);
+    }
+    std::vector<int> new_node(nodes.size()), new_atom(fa.size());
+    for (size_t k = 0; k < order.size(); ++k) new_node[order[k]] = int(k);
+    L.atoms.clear();
+    for (int k : order) {
+        const int nk = new_node[k];
+        L.node_abegin.push_back(int(L.atoms.size()));
+        for (int i : nodes[k].atoms) {
+            new_atom[i] = int(L.atoms.size());
+            Atom a = fa[i];
+            a.node = nk;
+            L.atoms.push_back(a);
+        }
+        L.node_aend.push_back(int(L.atoms.size()));
+        L.node_parent.push_back(k == 0 ? -1 : new_node[nodes[k].parent]);
+    }
+    const int n = int(L.atoms.size());
+    // node origins and axes: the root at its first atom, a branch at its first
+    // atom with the axis from the parent's bond atom (segment, tree.h:101-131)
+    for (int k : order) {
+        const Atom& o = (k == 0) ? fa[nodes[0].atoms.front()] : fa[nodes[k].axis_end];
+        L.node_origin.insert(L.node_origin.end(), {o.x, o.y, o.z});
+        if (k == 0) {
+            L.node_axis.insert(L.node_axis.end(), {1.f, 0.f, 0.f});
+        } else {
+            const Atom& b0 = fa[nodes[k].axis_begin];
+            float ax = o.x - b0.x, ay = o.y - b0.y, az = o.z - b0.z;
+            const float nr = std::sqrt(ax * ax + ay * ay + az * az);
+            if (!(nr > 1e-6f)) { err = "zero-length rotatable bond"; return false; }
+            L.node_axis.insert(L.node_axis.end(), {ax / nr, ay / nr, az / nr});
+        }
+    }
+    // mobility: same node fixed; a branch's first axis atom is fixed relative to
+    // the branch's atoms, the axis pair itself is the rotor (parse_pdbqt.cpp:455-496)
+    const int nn = int(order.size());
+    std::vector<int> axis_b(nn, -1), axis_e(nn, -1);
+    for (int k = 1; k < int(nodes.size()); ++k) {
+        axis_b[new_node[k]] = new_atom[nodes[k].axis_begin];
+        axis_e[new_node[k]] = new_atom[nodes[k].axis_end];
+    }
+    auto fixed = [&](int i, int j) {
+        const int ni = L.atoms[i].node, nj = L.atoms[j].node;
+        if (ni == nj) return true;
+        return axis_b[nj] == i || axis_b[ni] == j;
+    };
+    std::vector<std::vector<int>> bonds;
+    assign_bonds(L.atoms, fixed, bonds);
+    assign_types(L.atoms, bonds);
+    auto rotatable = [&](int i, int j) {
+        const int ni = L.atoms[i].node, nj = L.atoms[j].node;
+        return (axis_b[nj] == i && axis_e[nj] == j) || (axis_b[ni] == j && axis_e[ni] == i);
+    };
+    // pairs (model::initialize_pairs, model.cpp:592-630): variable mobility,
+    // more than 3 bonds apart, both atoms X-Score typed, not a closure clash
+    // or a glue pair
+    auto within3 = [&](int i) {
+        std::vector<int> seen{i}, frontier{i};
+        for (int d = 0; d < 3; ++d) {
+            std::vector<int> next;
+            for (int u : frontier)
+                for (int v : bonds[u])
+                    if (std::find(seen.begin(), seen.end(), v) == seen.end()) {
+                        seen.push_back(v);
+                        next.push_back(v);
+                    }
+            frontier.swap(next);
+        }
+        return seen;
+    };
+    auto is_cg = [](int ad, int k) { return ad == AD_CG0 + k; };
+    auto is_g = [](int ad, int k) { return ad == AD_G0 + k; };
+    auto glue = [&](int i, int j) {
+        for (int k = 0; k < 4; ++k)
+            if ((is_cg(L.atoms[i].ad, k) && is_g(L.atoms[j].ad, k)) || (is_cg(L.atoms[j].ad, k) && is_g(L.atoms[i].ad, k)))
+                return true;
+        return false;
+    };
+    auto closure_clash = [&](int i, int j) {  // model::is_closure_clash (model.cpp:552-582)
+        if (glue(i, j)) return false;
+        bool has[4] = {false, false, false, false};
+        std::vector<int> ni{i};
+        ni.insert(ni.end(), bonds[i].begin(), bonds[i].end());
+        for (int u : ni)
+            for (int k = 0; k < 4; ++k)
+                if (is_cg(L.atoms[u].ad, k)) has[k] = true;
+        std::vector<int> nj{j};
+        nj.insert(nj.end(), bonds[j].begin(), bonds[j].end());
+        for (int u : nj)
+            for (int k = 0; k < 4; ++k)
+                if (is_cg(L.atoms[u].ad, k) && has[k]) return true;
+        return false;
+    };
+    for (int i = 0; i < n; ++i) {
+        const std::vector<int> near = within3(i);
+        for (int j = i + 1; j < n; ++j) {
+            if (fixed(i, j)) continue;
+            if (std::find(near.begin(), near.end(), j) != near.end()) continue;
+            if (closure_clash(i, j)) continue;
+            if (L.atoms[i].xs < 0 || L.atoms[j].xs < 0) continue;
+            if (glue(i, j)) continue;  // glue pairs are not ligand pairs
+            L.pairs.push_back(i);
+            L.pairs.push_back(j);
+        }
+    }
+    // conf_independent_inputs::num_tors (conf_independent.cpp:36-103): half the
+    // rotatable bonds of heavy atoms to heavy atoms that have more than one
+    // heavy neighbour
+    auto heavy_deg = [&](int i) {
+        int d = 0;
+        for (int j : bonds[i]) d += L.atoms[j].el != EL_H;
+        return d;
+    };
+    float nt = 0.f;
+    for (int i = 0; i < n; ++i) {
+        if (L.atoms[i].el == EL_H) continue;
+        int ar = 0;
+        for (int j : bonds[i])
+            if (rotatable(i, j) && L.atoms[j].el != EL_H && heavy_deg(j) > 1) ++ar;
+        nt += 0.5f * float(ar);
+    }
+    L.num_tors = nt;
+    L.torsdof = torsdof < 0 ? int(nodes.size()) - 1 : torsdof;
+    return true;
+}
+
+}  // namespace
+
+struct ud_ligset {
+    std::vector<Ligand> ligs;
+    std::string err;
+    // flattened batch arrays (rebuilt by ud_ligset_batch)
+    std::vector<int32_t> atom_offset, node_offset, pair_offset, atom_xs, atom_heavy, node_parent, node_abegin,
+        node_aend, pairs;
+    std::vector<float> atom_local, node_origin, node_axis, num_tors;
+};
+
+extern "C" {
+
+int ud_ligset_create(ud_ligset** out) {
+    if (!out) return UD_ERR_INVALID;
+    *out = new ud_ligset();
+    return UD_OK;
+}
+
+int ud_ligset_destroy(ud_ligset* s) {
+    delete s;
+    return UD_OK;
+}
+
+const char* ud_ligset_last_error(const ud_ligset* s) { return s ? s->err.c_str() : "null ligand set"; }
+
+int ud_ligset_add_pdbqt(ud_ligset* s, const char* text, int32_t* index_out) {
+    if (!s || !text) return UD_ERR_INVALID;
+    Ligand L;
+    std::string err;
+    if (!build_ligand(text, L, err)) {
+        s->err = err;
+        return UD_ERR_INVALID;
+    }
+    if (int(L.atoms.size()) > UD_MAX_ATOMS || int(L.node_parent.size()) - 1 > UD_MAX_TORSIONS ||
+        int(L.pairs.size() / 2) > UD_MAX_PAIRS) {
+        s->err = "ligand exceeds the library's capacities";
+        return UD_ERR_CAPACITY;
+    }
+    s->ligs.push_back(std::move(L));
+    if (index_out) *index_out = int32_t(s->ligs.size() - 1);
+    return UD_OK;
+}
+
+int ud_ligset_count(const ud_ligset* s) { return s ? int(s->ligs.size()) : 0; }
+
+int ud_ligset_batch(ud_ligset* s, ud_ligand_batch* b) {
+    if (!s || !b) return UD_ERR_INVALID;
+    s->atom_offset.assign(1, 0);
+    s->node_offset.assign(1, 0);
+    s->pair_offset.assign(1, 0);
+    s->atom_xs.clear(); s->atom_heavy.clear(); s->node_parent.clear(); s->node_abegin.clear();
+    s->node_aend.clear(); s->pairs.clear(); s->atom_local.clear(); s->node_origin.clear();
+    s->node_axis.clear(); s->num_tors.clear();
+    for (const Ligand& L : s->ligs) {
+        for (const Atom& a : L.atoms) {
+            const float* o = &L.node_origin[3 * a.node];
+            s->atom_local.insert(s->atom_local.end(), {a.x - o[0], a.y - o[1], a.z - o[2]});
+            s->atom_xs.push_back(a.xs);
+            s->atom_heavy.push_back(a.el != EL_H ? 1 : 0);
+        }
+        for (size_t k = 0; k < L.node_parent.size(); ++k) {
+            s->node_parent.push_back(L.node_parent[k]);
+            s->node_abegin.push_back(L.node_abegin[k]);
+            s->node_aend.push_back(L.node_aend[k]);
+        }
+        s->node_origin.insert(s->node_origin.end(), L.node_origin.begin(), L.node_origin.end());
+        s->node_axis.insert(s->node_axis.end(), L.node_axis.begin(), L.node_axis.end());
+        s->pairs.insert(s->pairs.end(), L.pairs.begin(), L.pairs.end());
+        s->num_tors.push_back(L.num_tors);
+        s->atom_offset.push_back(s->atom_offset.back() + int32_t(L.atoms.size()));
+        s->node_offset.push_back(s->node_offset.back() + int32_t(L.node_parent.size()));
+        s->pair_offset.push_back(s->pair_offset.back() + int32_t(L.pairs.size() / 2));
+    }
+    b->n_ligands = int32_t(s->ligs.size());
+    b->atom_offset = s->atom_offset.data();
+    b->node_offset = s->node_offset.data();
+    b->pair_offset = s->pair_offset.data();
+    b->atom_local = s->atom_local.data();
+    b->atom_xs = s->atom_xs.data();
+    b->atom_heavy = s->atom_heavy.data();
+    b->node_parent = s->node_parent.data();
+    b->node_abegin = s->node_abegin.data();
+    b->node_aend = s->node_aend.data();
+    b->node_origin = s->node_origin.data();
+    b->node_axis = s->node_axis.data();
+    b->pairs = s->pairs.data();
+    b->num_tors = s->num_tors.data();
+    return UD_OK;
+}
+
+int ud_ligset_atoms(const ud_ligset* s, int lig, int32_t* n_atoms_out, float* xyz_out, int32_t* serial_out) {
+    if (!s || lig < 0 || lig >= int(s->ligs.size()) || !n_atoms_out) return UD_ERR_INVALID;
+    const Ligand& L = s->ligs[lig];
+    *n_atoms_out = int32_t(L.atoms.size());
+    for (size_t i = 0; i < L.atoms.size(); ++i) {
+        if (xyz_out) {
+            xyz_out[3 * i] = L.atoms[i].x;
+            xyz_out[3 * i + 1] = L.atoms[i].y;
+            xyz_out[3 * i + 2] = L.atoms[i].z;
+        }
+        if (serial_out) serial_out[i] = L.atoms[i].serial;
+    }
+    return UD_OK;
+}
+
+int ud_receptor_from_pdbqt(const char* text, int32_t capacity, int32_t* n_out, float* xyz_out, int32_t* xs_out) {
+    if (!text || !n_out) return UD_ERR_INVALID;
+    std::vector<Atom> at;
+    std::istringstream in(text);
+    std::string ln, err;
+    while (std::getline(in, ln)) {
+        if (!ln.empty() && ln.back() == '\r') ln.pop_back();
+        if (ln.rfind("ATOM", 0) == 0 || ln.rfind("HETATM", 0) == 0) {
+            Atom a;
+            if (!parse_atom_line(ln, a, err)) return UD_ERR_INVALID;
+            at.push_back(a);
+        }
+    }
+    std::vector<std::vector<int>> bonds;
+    assign_bonds(at, [](int, int) { return true; }, bonds);  // a rigid receptor: all atoms fixed
+    assign_types(at, bonds);
+    int n = 0;
+    for (const Atom& a : at) {
+        if (a.xs < 0) continue;  // hydrogens and untyped atoms have no grid term
+        if (xyz_out && xs_out && n < capacity) {
+            xyz_out[3 * n] = a.x;
+            xyz_out[3 * n + 1] = a.y;
+            xyz_out[3 * n + 2] = a.z;
+            xs_out[n] = a.xs;
+        }
+        ++n;
+    }
+    *n_out = n;
+    if ((xyz_out || xs_out) && n > capacity) return UD_ERR_CAPACITY;
+    return UD_OK;
+}
+
+}  // extern "C"

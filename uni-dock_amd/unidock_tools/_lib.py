@@ -99,6 +99,14 @@ def load() -> ctypes.CDLL:
         "ud_set_bucket_lanes": (ctypes.c_int, [vp, ip]),
         "ud_autotune": (ctypes.c_int, [vp, ctypes.POINTER(UdSearchParams), ctypes.c_int, ip]),
         "ud_postprocess": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(UdMode), ip]),
+        "ud_ligset_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
+        "ud_ligset_destroy": (ctypes.c_int, [vp]),
+        "ud_ligset_last_error": (ctypes.c_char_p, [vp]),
+        "ud_ligset_add_pdbqt": (ctypes.c_int, [vp, ctypes.c_char_p, ip]),
+        "ud_ligset_count": (ctypes.c_int, [vp]),
+        "ud_ligset_batch": (ctypes.c_int, [vp, ctypes.POINTER(UdLigandBatch)]),
+        "ud_ligset_atoms": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, ip]),
+        "ud_receptor_from_pdbqt": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ip, fp, ip]),
         "ud_cluster_modes": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, fp, fp, ctypes.c_double, ctypes.c_int, ip, ip]),
         "ud_synth_receptor": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                              ctypes.c_longlong, dp]),
@@ -119,6 +127,103 @@ class UdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"unidock_amd error {code}: {msg}")
         self.code = code
+
+
+class ParsedLigand:
+    """One ligand of a LigandSet, with the fields of synth.Ligand (batch order)."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+class LigandSet:
+    """Ligands parsed from PDBQT text by the library (ud_ligset_*)."""
+
+    def __init__(self):
+        self.L = load()
+        h = ctypes.c_void_p()
+        rc = self.L.ud_ligset_create(ctypes.byref(h))
+        if rc != UD_OK:
+            raise UdError(rc, "ud_ligset_create failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.ud_ligset_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_pdbqt(self, text: str) -> int:
+        idx = ctypes.c_int32()
+        rc = self.L.ud_ligset_add_pdbqt(self.h, text.encode(), ctypes.byref(idx))
+        if rc != UD_OK:
+            raise UdError(rc, self.L.ud_ligset_last_error(self.h).decode())
+        return idx.value
+
+    def __len__(self):
+        return self.L.ud_ligset_count(self.h)
+
+    def arrays(self) -> dict:
+        """The batch in the dict layout Context.set_ligands takes (copies)."""
+        b = UdLigandBatch()
+        rc = self.L.ud_ligset_batch(self.h, ctypes.byref(b))
+        if rc != UD_OK:
+            raise UdError(rc, "ud_ligset_batch failed")
+        n = b.n_ligands
+        ao = np.ctypeslib.as_array(b.atom_offset, (n + 1,)).copy()
+        no = np.ctypeslib.as_array(b.node_offset, (n + 1,)).copy()
+        po = np.ctypeslib.as_array(b.pair_offset, (n + 1,)).copy()
+        na, nn, npr = int(ao[-1]), int(no[-1]), int(po[-1])
+        get = lambda ptr, m: np.ctypeslib.as_array(ptr, (m,)).copy() if m > 0 else np.zeros(0, np.int32)
+        return dict(
+            n_ligands=n, atom_offset=ao, node_offset=no, pair_offset=po,
+            atom_local=get(b.atom_local, 3 * na).reshape(-1, 3), atom_xs=get(b.atom_xs, na),
+            atom_heavy=get(b.atom_heavy, na), node_parent=get(b.node_parent, nn),
+            node_abegin=get(b.node_abegin, nn), node_aend=get(b.node_aend, nn),
+            node_origin=get(b.node_origin, 3 * nn).reshape(-1, 3), node_axis=get(b.node_axis, 3 * nn).reshape(-1, 3),
+            pairs=get(b.pairs, 2 * npr).reshape(-1, 2), num_tors=get(b.num_tors, n),
+        )
+
+    def ligands(self) -> list:
+        a = self.arrays()
+        out = []
+        for l in range(a["n_ligands"]):
+            a0, a1 = a["atom_offset"][l], a["atom_offset"][l + 1]
+            k0, k1 = a["node_offset"][l], a["node_offset"][l + 1]
+            p0, p1 = a["pair_offset"][l], a["pair_offset"][l + 1]
+            n_at = ctypes.c_int32()
+            self.L.ud_ligset_atoms(self.h, l, ctypes.byref(n_at), None, None)
+            xyz = np.zeros((n_at.value, 3), dtype=np.float32)
+            serial = np.zeros(n_at.value, dtype=np.int32)
+            self.L.ud_ligset_atoms(self.h, l, ctypes.byref(n_at), _p(xyz, ctypes.c_float), _p(serial, ctypes.c_int32))
+            out.append(ParsedLigand(
+                n_atoms=int(a1 - a0), n_tors=int(k1 - k0 - 1), local=a["atom_local"][a0:a1].astype(np.float64),
+                xs=a["atom_xs"][a0:a1], heavy=a["atom_heavy"][a0:a1], parent=a["node_parent"][k0:k1],
+                abegin=a["node_abegin"][k0:k1], aend=a["node_aend"][k0:k1],
+                origin=a["node_origin"][k0:k1].astype(np.float64), axis=a["node_axis"][k0:k1].astype(np.float64),
+                pairs=a["pairs"][p0:p1], num_tors=float(a["num_tors"][l]), coords=xyz.astype(np.float64),
+                serial=serial))
+        return out
+
+
+def receptor_from_pdbqt(text: str):
+    """Heavy receptor atoms with X-Score types: (xyz float32 [n, 3], xs int32 [n])."""
+    L = load()
+    n = ctypes.c_int32()
+    rc = L.ud_receptor_from_pdbqt(text.encode(), 0, ctypes.byref(n), None, None)
+    if rc != UD_OK:
+        raise UdError(rc, "receptor PDBQT could not be parsed")
+    xyz = np.zeros((n.value, 3), dtype=np.float32)
+    xs = np.zeros(n.value, dtype=np.int32)
+    rc = L.ud_receptor_from_pdbqt(text.encode(), n.value, ctypes.byref(n), _p(xyz, ctypes.c_float), _p(xs, ctypes.c_int32))
+    if rc != UD_OK:
+        raise UdError(rc, "receptor PDBQT could not be parsed")
+    return xyz, xs
 
 
 def cluster_modes(heavy_xyz, energy, min_rmsd=1.0, max_size=9):
