@@ -249,6 +249,7 @@ typedef struct ud_mode {
     float conf_independent;  /* energy - (inter + intra - intra_best), the torsion term */
     float search_energy;     /* the chain's best energy from the search */
     float rmsd_lb, rmsd_ub;  /* from the best mode (model::rmsd_lower/upper_bound) */
+    float unbound;           /* intramolecular energy of the best pose (subtracted above) */
     int32_t chain;           /* chain the mode comes from (ligand * exhaustiveness + k) */
 } ud_mode;
 
@@ -291,6 +292,18 @@ int ud_ligset_batch(ud_ligset* set, ud_ligand_batch* batch);
 /* Input coordinates and atom numbers of ligand `lig` in the batch's atom
  * order (for writing poses back out); n_atoms_out first, arrays may be NULL. */
 int ud_ligset_atoms(const ud_ligset* set, int lig, int32_t* n_atoms_out, float* xyz_out, int32_t* serial_out);
+
+/* PDBQT text of ligand `lig` with its modes (Vina::get_poses and
+ * vina_remarks, vina.cpp:725-830; model::write_model, model.cpp:771-847):
+ * per mode MODEL k, the REMARK VINA RESULT / INTER + INTRA / INTER / INTRA /
+ * UNBOUND lines, the ligand's input lines with atom coordinates from
+ * coords[mode][atom][3] (batch atom order) written into columns 31-54, and
+ * ENDMDL.  Writing stops after how_many modes or at the first mode whose
+ * energy exceeds the first one's by more than energy_range.  out may be NULL:
+ * *len_out receives the text length (without the terminating 0) either way;
+ * UD_ERR_CAPACITY if cap is too small. */
+int ud_ligset_write_pdbqt(const ud_ligset* set, int lig, int n_modes, const ud_mode* modes, const float* coords,
+                          int how_many, double energy_range, char* out, size_t cap, size_t* len_out);
 
 /* Receptor heavy atoms with X-Score types from PDBQT text (the rigid
  * receptor of Vina::set_receptor, typed like the ligand: bonds decide C_P /

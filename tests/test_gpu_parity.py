@@ -545,6 +545,16 @@ def test_redock_1iep_from_reference_fixture():
         _say(f"[gpu] 1iep sf={sf}: best {best['energy']:.3f} kcal/mol, {len(modes)} modes, "
              f"RMSD to crystal {rmsd:.2f} A")
         assert -20.0 <= best["energy"] <= -1.0
+        # modes written as PDBQT read back with the docked coordinates
+        pl_m = np.zeros(len(modes), dtype=np.int32)
+        cm = ctx.pose_coords(pl_m, np.array([m["pose"] for m in modes]))[:, :lig.n_atoms]
+        text = ls.write_pdbqt(0, modes, cm, how_many=9, energy_range=3.0)
+        assert text.startswith("MODEL 1\nREMARK VINA RESULT:")
+        body = "\n".join(ln for ln in text.split("ENDMDL\n")[0].splitlines()
+                         if not ln.startswith(("MODEL", "REMARK VINA", "REMARK I", "REMARK U")))
+        back = _lib.LigandSet()
+        back.add_pdbqt(body)
+        np.testing.assert_allclose(back.ligands()[0].coords, cm[0], atol=6e-4)
         # the crystal pose is a minimum of the scoring function: a local
         # optimisation from it stays within 1 A and lowers the energy
         P0 = np.zeros((1, _lib.UD_POSE_FLOATS), dtype=np.float32)

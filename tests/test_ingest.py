@@ -121,3 +121,38 @@ def test_errors_are_reported():
     with pytest.raises(_lib.UdError):
         s.add_pdbqt("BRANCH 1 2\n")
     assert len(s) == 0
+
+
+def test_write_pdbqt_round_trip(lig):
+    """Modes written as PDBQT (MODEL / REMARK VINA RESULT / input lines with
+    new coordinates / ENDMDL): with the input coordinates the atom lines come
+    back unchanged, the count follows how_many and energy_range, and a written
+    model parses back to the same ligand."""
+    s = _lib.LigandSet()
+    s.add_pdbqt(LIGAND)
+    modes = [dict(energy=-9.5, inter=-12.0, intra=-0.5, rmsd_lb=0.0, rmsd_ub=0.0, unbound=-0.4),
+             dict(energy=-8.0, inter=-10.0, intra=-0.6, rmsd_lb=1.5, rmsd_ub=2.5, unbound=-0.4),
+             dict(energy=-5.0, inter=-7.0, intra=-0.6, rmsd_lb=3.0, rmsd_ub=4.0, unbound=-0.4)]
+    coords = np.stack([lig.coords] * 3).astype(np.float32)
+    shifted = coords.copy()
+    shifted[1] += np.float32([1.0, -2.0, 0.5])
+    text = s.write_pdbqt(0, modes, shifted, how_many=9, energy_range=3.0)
+    blocks = text.split("ENDMDL\n")
+    assert len(blocks) == 3 and blocks[-1] == ""  # the third mode is 4.5 above the best
+    first = blocks[0].splitlines()
+    assert first[0] == "MODEL 1"
+    assert first[1] == "REMARK VINA RESULT:    -9.500      0.000      0.000"
+    assert first[2] == "REMARK INTER + INTRA:         -12.500"
+    body = [ln for ln in first if not ln.startswith(("MODEL", "REMARK VINA", "REMARK INTER", "REMARK INTRA",
+                                                     "REMARK UNBOUND"))]
+    src = [ln.rstrip("\n") for ln in LIGAND.splitlines()]
+    assert body == src
+    assert len(s.write_pdbqt(0, modes, shifted, how_many=1).split("ENDMDL\n")) == 2
+    # the second model parses back with its shifted coordinates
+    second = "\n".join(ln for ln in blocks[1].splitlines() if not ln.startswith(("MODEL", "REMARK VINA", "REMARK I",
+                                                                                  "REMARK U")))
+    t = _lib.LigandSet()
+    t.add_pdbqt(second)
+    back = t.ligands()[0]
+    np.testing.assert_allclose(back.coords, shifted[1], atol=6e-4)
+    assert back.pairs.tolist() == lig.pairs.tolist() and back.xs.tolist() == lig.xs.tolist()

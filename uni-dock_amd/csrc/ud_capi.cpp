@@ -1213,6 +1213,7 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, ud_mode* modes_out
             m.energy = conf_independent(x, c->hdr[l].num_tors, c->rot_internal);
             m.conf_independent = m.energy - x;
             m.search_energy = E[size_t(l) * exh + j];
+            m.unbound = intra_best;
             m.chain = int32_t(size_t(l) * exh + j);
         }
         q += nk;

@@ -7,6 +7,7 @@
 // Host code only; the result feeds ud_set_ligands / ud_compute_maps.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -216,6 +217,8 @@ struct Node {
 };
 
 struct Ligand {
+    std::vector<std::string> lines;  // input lines (trailing blank lines dropped)
+    std::vector<int> line_atom;      // batch atom index of an ATOM / HETATM line, else -1
     std::vector<Atom> atoms;  // reference order (nodes in pre-order, file order inside)
     std::vector<int> node_parent, node_abegin, node_aend;
     std::vector<float> node_origin, node_axis;
@@ -424,6 +427,27 @@ bool build_ligand(const std::string& text, Ligand& L, std::string& err) {
     }
     L.num_tors = nt;
     L.torsdof = torsdof < 0 ? int(nodes.size()) - 1 : torsdof;
+    // the input lines, for writing poses back out
+    {
+        std::istringstream in2(text);
+        std::string l2;
+        std::map<int, int> batch_of_serial;
+        for (int i = 0; i < n; ++i) batch_of_serial[L.atoms[i].serial] = i;
+        while (std::getline(in2, l2)) {
+            if (!l2.empty() && l2.back() == '\r') l2.pop_back();
+            int idx = -1;
+            if (l2.rfind("ATOM", 0) == 0 || l2.rfind("HETATM", 0) == 0) {
+                const std::string sn = trim(cols(l2, 7, 11));
+                idx = batch_of_serial[std::atoi(sn.c_str())];
+            }
+            L.lines.push_back(l2);
+            L.line_atom.push_back(idx);
+        }
+        while (!L.lines.empty() && trim(L.lines.back()).empty()) {
+            L.lines.pop_back();
+            L.line_atom.pop_back();
+        }
+    }
     return true;
 }
 
@@ -530,6 +554,55 @@ int ud_ligset_atoms(const ud_ligset* s, int lig, int32_t* n_atoms_out, float* xy
         }
         if (serial_out) serial_out[i] = L.atoms[i].serial;
     }
+    return UD_OK;
+}
+
+int ud_ligset_write_pdbqt(const ud_ligset* s, int lig, int n_modes, const ud_mode* modes, const float* coords,
+                          int how_many, double energy_range, char* out, size_t cap, size_t* len_out) {
+    if (!s || lig < 0 || lig >= int(s->ligs.size()) || n_modes < 0 || (n_modes > 0 && (!modes || !coords)) ||
+        how_many < 0 || !(energy_range >= 0.0) || !len_out)
+        return UD_ERR_INVALID;
+    const Ligand& L = s->ligs[lig];
+    const size_t na = L.atoms.size();
+    std::string txt;
+    char buf[160];
+    int written = 0;
+    for (int k = 0; k < n_modes && written < how_many; ++k) {
+        const ud_mode& m = modes[k];
+        if (!(m.energy < 3.0e38f) || m.energy > modes[0].energy + energy_range) break;
+        ++written;
+        std::snprintf(buf, sizeof(buf), "MODEL %d\n", written);
+        txt += buf;
+        std::snprintf(buf, sizeof(buf), "REMARK VINA RESULT: %9.3f  %9.3f  %9.3f\n", m.energy, m.rmsd_lb, m.rmsd_ub);
+        txt += buf;
+        std::snprintf(buf, sizeof(buf), "REMARK INTER + INTRA:    %12.3f\n", m.inter + m.intra);
+        txt += buf;
+        std::snprintf(buf, sizeof(buf), "REMARK INTER:            %12.3f\n", m.inter);
+        txt += buf;
+        std::snprintf(buf, sizeof(buf), "REMARK INTRA:            %12.3f\n", m.intra);
+        txt += buf;
+        std::snprintf(buf, sizeof(buf), "REMARK UNBOUND:          %12.3f\n", m.unbound);
+        txt += buf;
+        const float* c = coords + size_t(k) * na * 3;
+        for (size_t i = 0; i < L.lines.size(); ++i) {
+            std::string ln = L.lines[i];
+            const int a = L.line_atom[i];
+            if (a >= 0) {  // string_write_coord at columns 31, 39, 47 (model.cpp:771-777)
+                if (ln.size() < 54) ln.resize(54, ' ');
+                for (int d = 0; d < 3; ++d) {
+                    std::snprintf(buf, sizeof(buf), "%8.3f", c[3 * a + d]);
+                    ln.replace(30 + 8 * d, 8, std::string(buf).substr(0, 8));
+                }
+            }
+            txt += ln;
+            txt += '\n';
+        }
+        txt += "ENDMDL\n";
+    }
+    *len_out = txt.size();
+    if (!out) return UD_OK;
+    if (cap < txt.size() + 1) return UD_ERR_CAPACITY;
+    std::memcpy(out, txt.c_str(), txt.size() + 1);
     return UD_OK;
 }
 

@@ -46,7 +46,8 @@ class UdLaunchInfo(ctypes.Structure):
 class UdMode(ctypes.Structure):
     _fields_ = [("pose", ctypes.c_float * UD_POSE_FLOATS), ("energy", ctypes.c_float), ("inter", ctypes.c_float),
                 ("intra", ctypes.c_float), ("conf_independent", ctypes.c_float), ("search_energy", ctypes.c_float),
-                ("rmsd_lb", ctypes.c_float), ("rmsd_ub", ctypes.c_float), ("chain", ctypes.c_int32)]
+                ("rmsd_lb", ctypes.c_float), ("rmsd_ub", ctypes.c_float), ("unbound", ctypes.c_float),
+                ("chain", ctypes.c_int32)]
 
 
 class UdSearchParams(ctypes.Structure):
@@ -107,6 +108,9 @@ def load() -> ctypes.CDLL:
         "ud_ligset_batch": (ctypes.c_int, [vp, ctypes.POINTER(UdLigandBatch)]),
         "ud_ligset_atoms": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, ip]),
         "ud_receptor_from_pdbqt": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ip, fp, ip]),
+        "ud_ligset_write_pdbqt": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(UdMode), fp, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_char_p, ctypes.c_size_t,
+                                                 ctypes.POINTER(ctypes.c_size_t)]),
         "ud_cluster_modes": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, fp, fp, ctypes.c_double, ctypes.c_int, ip, ip]),
         "ud_synth_receptor": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                              ctypes.c_longlong, dp]),
@@ -188,6 +192,28 @@ class LigandSet:
             node_origin=get(b.node_origin, 3 * nn).reshape(-1, 3), node_axis=get(b.node_axis, 3 * nn).reshape(-1, 3),
             pairs=get(b.pairs, 2 * npr).reshape(-1, 2), num_tors=get(b.num_tors, n),
         )
+
+    def write_pdbqt(self, lig: int, modes: list, coords, how_many=9, energy_range=3.0) -> str:
+        """PDBQT text of `lig` with the given modes (dicts as Context.postprocess
+        returns) and their atom coordinates coords[mode][atom][3] (batch order)."""
+        arr = (UdMode * max(1, len(modes)))()
+        for k, m in enumerate(modes):
+            for f in ("energy", "inter", "intra", "conf_independent", "search_energy", "rmsd_lb", "rmsd_ub",
+                      "unbound"):
+                setattr(arr[k], f, float(m.get(f, 0.0)))
+            arr[k].chain = int(m.get("chain", 0))
+        c = np.ascontiguousarray(coords, dtype=np.float32)
+        n = ctypes.c_size_t()
+        rc = self.L.ud_ligset_write_pdbqt(self.h, lig, len(modes), arr, _p(c, ctypes.c_float), how_many,
+                                          energy_range, None, 0, ctypes.byref(n))
+        if rc != UD_OK:
+            raise UdError(rc, "ud_ligset_write_pdbqt failed")
+        buf = ctypes.create_string_buffer(n.value + 1)
+        rc = self.L.ud_ligset_write_pdbqt(self.h, lig, len(modes), arr, _p(c, ctypes.c_float), how_many,
+                                          energy_range, buf, n.value + 1, ctypes.byref(n))
+        if rc != UD_OK:
+            raise UdError(rc, "ud_ligset_write_pdbqt failed")
+        return buf.value.decode()
 
     def ligands(self) -> list:
         a = self.arrays()
@@ -415,7 +441,7 @@ class Context:
                 lst.append(dict(pose=np.array(m.pose[:], dtype=np.float32), energy=m.energy, inter=m.inter,
                                 intra=m.intra, conf_independent=m.conf_independent,
                                 search_energy=m.search_energy, rmsd_lb=m.rmsd_lb, rmsd_ub=m.rmsd_ub,
-                                chain=int(m.chain)))
+                                unbound=m.unbound, chain=int(m.chain)))
             out.append(lst)
         return out
 
