@@ -60,6 +60,11 @@ enum { UD_GRID_F32 = 0, UD_GRID_F16 = 1 };
 /* A gradient is UD_GRAD_FLOATS floats: force[3], torque[3], d/dtorsion[T]
  * (the change vector of conf.h:182-227). */
 #define UD_GRAD_FLOATS (6 + UD_MAX_TORSIONS)
+/* Term-weight gradients: UD_WEIGHT_FLOATS floats per pose, in the order of
+ * the weights array of ud_set_scoring (Vina: gauss1, gauss2, repulsion,
+ * hydrophobic, hydrogen, glue, rot; Vinardo: gauss1, repulsion, hydrophobic,
+ * hydrogen, glue, rot, unused). */
+#define UD_WEIGHT_FLOATS 7
 
 /* ud_dock groups ligands into size buckets by heavy+hydrogen atom count
  * (<= 16, 32, 48, 64, 96, 128 atoms) and launches each bucket once; the
@@ -176,6 +181,25 @@ int ud_score_grad(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const fl
 /* Same with device pointers, asynchronous on the context's stream. */
 int ud_score_grad_device(ud_ctx* ctx, int n_poses, const int32_t* d_pose_ligand, const float* d_poses,
                          const float* v, float* d_energy, float* d_grad);
+
+/* Energy of poses with its gradient in the pose's own coordinates and in
+ * the scoring-term weights, for differentiable use (torch.autograd).  The
+ * energy is model::eval_deriv's (model.cpp:954-981) at the pose with its
+ * quaternion normalised, i.e. E(position, q / |q|, torsions).
+ * pose_grad_out[n_poses * UD_POSE_FLOATS]: dE/dposition[3], dE/dq[4]
+ * (w, x, y, z; the gradient of E(q / |q|)), dE/dtorsion[T], zeros after.
+ * weight_grad_out[n_poses * UD_WEIGHT_FLOATS]: dE/dw per term weight; the rot
+ * entry is 0 (the energy has no conf-independent part).  Needs maps built by
+ * ud_compute_maps (the per-term maps are built from its receptor on first
+ * use).  No counterpart in the reference, whose eval_deriv returns the
+ * change-space gradient only (ud_score_grad); its `ud_score_grad(...,
+ * dE_dw_out)` seam is this call.  Host pointers; pose_ligand is on the host. */
+int ud_energy_grad(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* poses, const float* v,
+                   float* energy_out, float* pose_grad_out, float* weight_grad_out);
+/* Same with device pointers for poses and outputs (pose_ligand stays on the
+ * host), asynchronous on the context's stream. */
+int ud_energy_grad_device(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* d_poses, const float* v,
+                          float* d_energy, float* d_pose_grad, float* d_weight_grad);
 
 /* BFGS local optimisation from given starts (poses in/out), quasi_newton +
  * bfgs (quasi_newton.cpp:43-53, bfgs.h:97-147; device: bfgs_warp,

@@ -38,6 +38,16 @@ struct ud_ctx {
     int slot_of_type[36];
     float scale[UD_MAX_MAPS];
     void* d_maps = nullptr;
+    // receptor atoms of the last ud_compute_maps (x, y, z, XS type bits)
+    float4* d_rec = nullptr;
+    size_t rec_cap = 0;
+    int n_rec = 0;
+    bool have_rec = false;
+    // per-term maps and tables for term-weight gradients (built on demand)
+    float2* d_term_tab = nullptr;
+    float* d_term_maps = nullptr;
+    size_t term_tab_cap = 0, term_maps_cap = 0;
+    bool terms_ok = false;
     // ligands
     int n_lig = 0;
     std::vector<LigHdr> hdr;
@@ -58,7 +68,8 @@ struct ud_ctx {
     size_t poses_cap = 0;
     float* d_f1 = nullptr;
     float* d_f2 = nullptr;
-    size_t f_cap = 0, f2_cap = 0;
+    float* d_f3 = nullptr;
+    size_t f_cap = 0, f2_cap = 0, f3_cap = 0;
     std::vector<AtomRec> h_atoms;  // host copy of the atom records
     bool slots_dirty = true;
     int32_t* d_i1 = nullptr;
@@ -272,9 +283,9 @@ int ud_destroy(ud_ctx* c) {
     if (!c) return UD_ERR_INVALID;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    void* ptrs[] = {c->d_table_deriv, c->d_table_fast, c->d_maps, c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs,
-                    c->d_partners, c->d_apar,
-                    c->d_work, c->d_perm, c->d_poses, c->d_f1, c->d_f2, c->d_i1, c->d_out_pose, c->d_out_e,
+    void* ptrs[] = {c->d_table_deriv, c->d_table_fast, c->d_maps, c->d_rec, c->d_term_tab, c->d_term_maps,
+                    c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs, c->d_partners, c->d_apar,
+                    c->d_work, c->d_perm, c->d_poses, c->d_f1, c->d_f2, c->d_f3, c->d_i1, c->d_out_pose, c->d_out_e,
                     c->d_out_evals};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -298,6 +309,7 @@ int ud_set_scoring(ud_ctx* c, int sf, const double* weights, int n_weights) {
     if (!weights || n_weights != need) return fail(c, UD_ERR_INVALID, "wrong number of weights");
     UD_HIP(c, hipSetDevice(c->device));
     c->sf = sf;
+    c->terms_ok = false;
     for (int i = 0; i < 7; ++i) c->w[i] = 0.f;
     for (int i = 0; i < need - 1; ++i) c->w[i] = float(weights[i]);
     // internal rotational weight 5 * w_rot / 0.1 - 1 (vina.cpp:404, 425)
@@ -337,6 +349,8 @@ int ud_set_box(ud_ctx* c, const double* center, const double* size, double granu
     if (pts > UD_MAX_GRID_POINTS) return fail(c, UD_ERR_CAPACITY, "grid has more points than UD_MAX_GRID_POINTS");
     c->n_points = int(pts);
     c->box_set = true;
+    c->have_rec = false;
+    c->terms_ok = false;
     if (c->d_maps) {
         (void)hipFree(c->d_maps);
         c->d_maps = nullptr;
@@ -361,6 +375,7 @@ static int store_maps(ud_ctx* c, float* d_f32, int n_types, const int32_t* types
     }
     c->precision = precision;
     c->slots_dirty = true;
+    c->terms_ok = false;
     if (precision == UD_GRID_F32) {
         c->d_maps = d_f32;
         return UD_OK;
@@ -399,6 +414,51 @@ static int store_maps(ud_ctx* c, float* d_f32, int n_types, const int32_t* types
     return UD_OK;
 }
 
+static GridBuildArgs build_args(const ud_ctx* c, int n_types, const int32_t* types) {
+    GridBuildArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (int i = 0; i < 3; ++i) {
+        a.init[i] = c->init[i];
+        a.factor_inv[i] = c->factor_inv[i];
+        a.n[i] = c->n[i];
+    }
+    a.n_points = c->n_points;
+    a.n_maps = n_types;
+    for (int m = 0; m < n_types; ++m) a.types[m] = types[m];
+    return a;
+}
+
+// Maps and tables of each scoring term alone (its weight 1, the others 0),
+// from the receptor of the last ud_compute_maps: the inputs of the
+// term-weight gradients.  Built once per scoring / box / map set.
+static int ensure_terms(ud_ctx* c) {
+    if (c->terms_ok) return UD_OK;
+    if (!c->have_rec || c->n_maps == 0)
+        return fail(c, UD_ERR_STATE, "term-weight gradients need maps built by ud_compute_maps");
+    const int nt = (c->sf == UD_SF_VINA) ? 6 : 5;
+    const size_t tab = size_t(kTypePairs) * kTableStride;
+    const size_t mp = size_t(c->n_maps) * c->n_points;
+    int rc = grow(c, c->d_term_tab, c->term_tab_cap, nt * tab);
+    if (rc) return rc;
+    rc = grow(c, c->d_term_maps, c->term_maps_cap, nt * mp);
+    if (rc) return rc;
+    float* d_fast = nullptr;
+    if (hipMalloc(&d_fast, sizeof(float) * tab) != hipSuccess) return fail(c, UD_ERR_NOMEM, "table allocation failed");
+    const GridBuildArgs a = build_args(c, c->n_maps, c->map_types);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < nt && e == hipSuccess; ++k) {
+        float w1[7] = {0, 0, 0, 0, 0, 0, 0};
+        w1[k] = 1.f;
+        e = launch_tables(c->sf, w1, c->d_term_tab + k * tab, d_fast, c->stream);
+        if (e == hipSuccess) e = launch_grid_build(c->d_rec, c->n_rec, a, d_fast, c->d_term_maps + k * mp, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_fast);
+    if (e != hipSuccess) return fail(c, UD_ERR_HIP, hipGetErrorString(e));
+    c->terms_ok = true;
+    return UD_OK;
+}
+
 static int check_types(ud_ctx* c, int n_types, const int32_t* types) {
     if (n_types <= 0 || n_types > UD_MAX_MAPS || !types) return fail(c, UD_ERR_INVALID, "bad map type list");
     for (int m = 0; m < n_types; ++m) {
@@ -429,33 +489,27 @@ int ud_compute_maps(ud_ctx* c, int n_rec, const float* rec_xyz, const int32_t* r
         std::memcpy(&tf, &t, 4);
         rec.push_back(make_float4(rec_xyz[3 * i], rec_xyz[3 * i + 1], rec_xyz[3 * i + 2], tf));
     }
-    float4* d_rec = nullptr;
-    UD_HIP(c, hipMalloc(&d_rec, sizeof(float4) * std::max<size_t>(rec.size(), 1)));
+    c->have_rec = false;
+    rc = grow(c, c->d_rec, c->rec_cap, std::max<size_t>(rec.size(), 1));
+    if (rc) return rc;
     if (!rec.empty())
-        UD_HIP(c, hipMemcpyAsync(d_rec, rec.data(), sizeof(float4) * rec.size(), hipMemcpyHostToDevice, c->stream));
+        UD_HIP(c, hipMemcpyAsync(c->d_rec, rec.data(), sizeof(float4) * rec.size(), hipMemcpyHostToDevice, c->stream));
+    c->n_rec = int(rec.size());
     float* d_f32 = nullptr;
     if (hipMalloc(&d_f32, sizeof(float) * size_t(n_types) * c->n_points) != hipSuccess)
         return fail(c, UD_ERR_NOMEM, "map allocation failed");
-    GridBuildArgs a;
-    std::memset(&a, 0, sizeof(a));
-    for (int i = 0; i < 3; ++i) {
-        a.init[i] = c->init[i];
-        a.factor_inv[i] = c->factor_inv[i];
-        a.n[i] = c->n[i];
-    }
-    a.n_points = c->n_points;
-    a.n_maps = n_types;
-    for (int m = 0; m < n_types; ++m) a.types[m] = types[m];
+    const GridBuildArgs a = build_args(c, n_types, types);
     UD_HIP(c, hipEventRecord(c->ev0, c->stream));
-    UD_HIP(c, launch_grid_build(d_rec, int(rec.size()), a, c->d_table_fast, d_f32, c->stream));
+    UD_HIP(c, launch_grid_build(c->d_rec, c->n_rec, a, c->d_table_fast, d_f32, c->stream));
     UD_HIP(c, hipEventRecord(c->ev1, c->stream));
     UD_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
     c->last_ms = ms;
     c->last_launches = 1;
-    (void)hipFree(d_rec);
-    return store_maps(c, d_f32, n_types, types, precision);
+    rc = store_maps(c, d_f32, n_types, types, precision);
+    if (rc == UD_OK) c->have_rec = true;
+    return rc;
 }
 
 int ud_set_maps(ud_ctx* c, int n_types, const int32_t* types, const float* values, int precision) {
@@ -470,6 +524,7 @@ int ud_set_maps(ud_ctx* c, int n_types, const int32_t* types, const float* value
     if (hipMalloc(&d_f32, sizeof(float) * size_t(n_types) * c->n_points) != hipSuccess)
         return fail(c, UD_ERR_NOMEM, "map allocation failed");
     UD_HIP(c, hipMemcpy(d_f32, values, sizeof(float) * size_t(n_types) * c->n_points, hipMemcpyHostToDevice));
+    c->have_rec = false;
     return store_maps(c, d_f32, n_types, types, precision);
 }
 
@@ -760,6 +815,67 @@ int ud_score_grad(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const floa
     UD_HIP(c, hipMemcpyAsync(energy_out, d_e, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
     if (grad_out)
         UD_HIP(c, hipMemcpyAsync(grad_out, d_g, sizeof(float) * size_t(n_poses) * UD_GRAD_FLOATS, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    return UD_OK;
+}
+
+int ud_energy_grad_device(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const float* d_poses, const float* v,
+                          float* d_energy, float* d_pose_grad, float* d_weight_grad) {
+    if (!c || !v) return UD_ERR_INVALID;
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc) return rc;
+    if (n_poses == 0) return UD_OK;
+    if (!d_poses || !d_energy || !d_pose_grad || !d_weight_grad) return fail(c, UD_ERR_INVALID, "null device pointer");
+    UD_HIP(c, hipSetDevice(c->device));
+    rc = refresh_slots(c);
+    if (rc) return rc;
+    rc = ensure_terms(c);
+    if (rc) return rc;
+    LaunchCap cap = caps_all(c);
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, cap);
+    std::vector<int32_t> perm;
+    std::vector<WorkItem> work;
+    make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
+    rc = work_upload(c, perm, work);
+    if (rc) return rc;
+    TermsDev td;
+    td.maps = c->d_term_maps;
+    td.tables = c->d_term_tab;
+    td.n_terms = (c->sf == UD_SF_VINA) ? 6 : 5;
+    td.n_maps = c->n_maps;
+    UD_HIP(c, hipEventRecord(c->ev0, c->stream));
+    UD_HIP(c, launch_energy_grad(lanes, c->precision, lig_dev(c), grid_dev(c), td, cap, c->d_work, int(work.size()),
+                                 c->d_perm, d_poses, v[0], v[1], d_energy, d_pose_grad, d_weight_grad, c->stream));
+    UD_HIP(c, hipEventRecord(c->ev1, c->stream));
+    c->last_launches = 1;
+    return UD_OK;
+}
+
+int ud_energy_grad(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const float* poses, const float* v,
+                   float* energy_out, float* pose_grad_out, float* weight_grad_out) {
+    if (!c || !v || (n_poses > 0 && (!poses || !energy_out || !pose_grad_out || !weight_grad_out)))
+        return UD_ERR_INVALID;
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc || n_poses == 0) return rc;
+    UD_HIP(c, hipSetDevice(c->device));
+    rc = pose_upload(c, n_poses, poses);
+    if (rc) return rc;
+    rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * UD_POSE_FLOATS);
+    if (rc) return rc;
+    rc = grow(c, c->d_f2, c->f2_cap, size_t(n_poses));
+    if (rc) return rc;
+    rc = grow(c, c->d_f3, c->f3_cap, size_t(n_poses) * UD_WEIGHT_FLOATS);
+    if (rc) return rc;
+    rc = ud_energy_grad_device(c, n_poses, pose_ligand, c->d_poses, v, c->d_f2, c->d_f1, c->d_f3);
+    if (rc) return rc;
+    UD_HIP(c, hipMemcpyAsync(energy_out, c->d_f2, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipMemcpyAsync(pose_grad_out, c->d_f1, sizeof(float) * size_t(n_poses) * UD_POSE_FLOATS,
+                             hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipMemcpyAsync(weight_grad_out, c->d_f3, sizeof(float) * size_t(n_poses) * UD_WEIGHT_FLOATS,
+                             hipMemcpyDeviceToHost, c->stream));
     UD_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);

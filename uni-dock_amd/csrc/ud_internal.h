@@ -83,6 +83,15 @@ struct LigandSetDev {
     const float* table_fast;    // slot * kTableUsed
 };
 
+// Maps and pair tables of each scoring term alone (weight 1, the others 0),
+// for term-weight gradients: the energy is linear in the weights before curl.
+struct TermsDev {
+    const float* maps;     // [term][slot][n_points], FP32, same slots as GridDev
+    const float2* tables;  // [term][type pair][kTableStride] (e, dor); e is read
+    int32_t n_terms;       // 6 (Vina) or 5 (Vinardo), glue included
+    int32_t n_maps;
+};
+
 // Work item: one ligand and a run of consecutive poses/chains of it.
 struct WorkItem {
     int32_t lig;

@@ -7,6 +7,7 @@
 //   search_kernel       BFGS from given starts (quasi_newton / bfgs) and,
 //                       with MC, Monte Carlo chains (kernel<32, Config>)
 //   terms_kernel        final-score components (Vina::score_gpu)
+//   energy_grad_kernel  energy with pose-coordinate and term-weight gradients
 //   coords_kernel       atom coordinates of poses (model::set)
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -1305,6 +1306,138 @@ __global__ __launch_bounds__(64) void terms_kernel(KArgs ka, const WorkItem* wor
     }
 }
 
+// d curl(e) / de (curl.h:28-36): (v / (v + e))^2 for e > 0, else 1
+__device__ __forceinline__ float curl_slope(float e, float v) {
+    if (!(e > 0.f) || !(v < 0.1f * kMaxFl)) return 1.f;
+    if (v < kEpsFl) return 0.f;
+    const float t = v / (v + e);
+    return t * t;
+}
+
+__device__ __forceinline__ float trilinear(const float (&f)[8], const GridPt& q) {
+    const float x = q.x, y = q.y, z = q.z;
+    const float mx = 1.f - x, my = 1.f - y, mz = 1.f - z;
+    return f[0] * mx * my * mz + f[1] * x * my * mz + f[2] * mx * y * mz + f[3] * x * y * mz + f[4] * mx * my * z +
+           f[5] * x * my * z + f[6] * mx * y * z + f[7] * x * y * z;
+}
+
+// Energy of poses with its gradient in the pose's own coordinates and in the
+// term weights (the differentiable-energy binding).
+//   * The quaternion is normalised before the evaluation, so the energy is
+//     E(q / |q|); its gradient is 2 (0, tau) q^ / |q| with tau the torque
+//     about the root origin (orientation part of model::eval_deriv's change,
+//     model.cpp:954-981), since a rotation w moves q^ to exp(w / 2) q^
+//     (quaternion_increment, quaternion.cpp:77-84) and changes E by tau . w.
+//   * Position and torsion entries are the change's force and torsion parts.
+//   * Map and pair energies are sums of w_k term_k before curl
+//     (precalculate.h:203-241, cache.cpp:518-548), so dE/dw_k is the sum over
+//     atoms of curl'(e) m_k(x) plus the sum over pairs of curl'(e) s_k(r^2),
+//     m_k / s_k being term k's own maps and tables interpolated exactly as the
+//     energy's.  Out-of-box penalties do not depend on the weights.
+// gw_out holds UD_WEIGHT_FLOATS entries per pose in the order of the weights
+// array; the conf-independent (rot) entry is 0: the energy does not contain it.
+template <int L, int GF>
+__global__ __launch_bounds__(64) void energy_grad_kernel(KArgs ka, TermsDev td, const WorkItem* work,
+                                                         const int32_t* perm, const float* poses, float v_intra,
+                                                         float v_inter, float* e_out, float* gpose_out, float* gw_out) {
+    extern __shared__ float4 lds_raw[];
+    float* lds = reinterpret_cast<float*>(lds_raw);
+    const WorkItem w = work[blockIdx.x];
+    Grp<L> s = setup_group<L, kNMax>(ka, w.lig, lds);
+    const bool active = s.g < w.count;
+    const int pi = active ? perm[w.first + s.g] : 0;
+    float* x = s.pose(PX);
+    const int np = 7 + s.ntors();
+    for (int i = s.gl; i < np; i += L) x[i] = active ? poses[size_t(pi) * UD_POSE_FLOATS + i] : (i == 3 ? 1.f : 0.f);
+    __syncthreads();
+    const float qn = sqrtf(x[3] * x[3] + x[4] * x[4] + x[5] * x[5] + x[6] * x[6]);
+    const bool qok = qn > kEpsFl;  // a zero quaternion is taken as the identity, with zero gradient
+    float qh[4] = {1.f, 0.f, 0.f, 0.f};
+    if (qok)
+        for (int j = 0; j < 4; ++j) qh[j] = x[3 + j] / qn;
+    __syncthreads();
+    if (s.gl == 0)
+        for (int j = 0; j < 4; ++j) x[3 + j] = qh[j];
+    __syncthreads();
+    float gout[Grp<L>::KV];
+    const float e = eval_group<L, kNMax, GF>(ka, s, x, v_intra, v_inter, gout, true);
+    const float* gb = s.bc(2);  // change-space gradient left by eval_group
+    if (active) {
+        const float tx = gb[3], ty = gb[4], tz = gb[5];
+        const float sc = qok ? 2.f / qn : 0.f;
+        for (int i = s.gl; i < UD_POSE_FLOATS; i += L) {
+            float v = 0.f;
+            if (i < 3) {
+                v = gb[i];
+            } else if (i == 3) {
+                v = -sc * (tx * qh[1] + ty * qh[2] + tz * qh[3]);
+            } else if (i < 7) {
+                // vector part of (0, tau) q^ = w tau + tau x v
+                const float vx = qh[1], vy = qh[2], vz = qh[3];
+                v = (i == 4) ? qh[0] * tx + (ty * vz - tz * vy)
+                  : (i == 5) ? qh[0] * ty + (tz * vx - tx * vz)
+                             : qh[0] * tz + (tx * vy - ty * vx);
+                v *= sc;
+            } else if (i < np) {
+                v = gb[6 + (i - 7)];
+            }
+            gpose_out[size_t(pi) * UD_POSE_FLOATS + i] = v;
+        }
+        if (s.gl == 0) e_out[pi] = e;
+    }
+    // term weights
+    float dw[UD_WEIGHT_FLOATS];
+#pragma unroll
+    for (int k = 0; k < UD_WEIGHT_FLOATS; ++k) dw[k] = 0.f;
+    const float4* coords = s.coords();
+    const size_t map_stride = size_t(td.n_maps) * ka.grid.n_points;
+    for (int i = s.gl; i < s.h.n_atoms; i += L) {
+        const AtomRec a = s.atoms[i];
+        const int slot = (a.meta >> 8) & 0xff;
+        if (slot == 0xff) continue;
+        const float4 ci = coords[i];
+        const GridPt q = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
+        float f[8];
+        grid_gather<GF>(ka.grid, q.base, f);
+        float er = trilinear(f, q);
+        if (GF == UD_GRID_F16) er *= ka.grid.scale[slot];
+        const float c = curl_slope(er, v_inter);
+        const size_t sy = size_t(ka.grid.n[0]), sz = sy * ka.grid.n[1];
+#pragma unroll
+        for (int k = 0; k < UD_WEIGHT_FLOATS; ++k) {
+            if (k >= td.n_terms) break;
+            const float* M = td.maps + size_t(k) * map_stride + q.base;
+            const float t[8] = {M[0], M[1], M[sy], M[sy + 1], M[sz], M[sz + 1], M[sy + sz], M[sy + sz + 1]};
+            dw[k] += c * trilinear(t, q);
+        }
+    }
+    const size_t tab_stride = size_t(kTypePairs) * kTableStride;
+    for (int q = s.gl; q < s.h.n_pairs; q += L) {
+        const PairRec rec = s.pairs[q];
+        const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
+        const float rx = cb.x - ca.x, ry = cb.y - ca.y, rz = cb.z - ca.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (!(r2 < 64.f)) continue;
+        const float rf = 32.f * r2;
+        const int it = int(rf);
+        const float rem = rf - float(it);
+        const size_t idx = size_t(rec >> 16) * kTableStride + it;
+        const float2 t0 = ka.ls.table_deriv[idx], t1 = ka.ls.table_deriv[idx + 1];
+        const float c = curl_slope(t0.x + rem * (t1.x - t0.x), v_intra);
+#pragma unroll
+        for (int k = 0; k < UD_WEIGHT_FLOATS; ++k) {
+            if (k >= td.n_terms) break;
+            const float2* T = td.tables + size_t(k) * tab_stride + idx;
+            const float e0 = T[0].x, e1 = T[1].x;
+            dw[k] += c * (e0 + rem * (e1 - e0));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < UD_WEIGHT_FLOATS; ++k) dw[k] = gsum<L>(dw[k]);
+    if (active && s.gl == 0)
+        for (int k = 0; k < UD_WEIGHT_FLOATS; ++k) gw_out[size_t(pi) * UD_WEIGHT_FLOATS + k] = dw[k];
+}
+
 template <int L>
 __global__ __launch_bounds__(64) void coords_kernel(KArgs ka, const WorkItem* work, const int32_t* perm,
                                                     const float* poses, float* coords_out) {
@@ -1470,6 +1603,23 @@ hipError_t launch_terms(int lanes, int gf, const LigandSetDev& ls, const GridDev
         else
             hipLaunchKernelGGL((terms_kernel<L, UD_GRID_F32>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
                                v, inter_out, intra_out);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_energy_grad(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const TermsDev& td,
+                              const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
+                              const float* poses, float v_intra, float v_inter, float* e_out, float* gpose_out,
+                              float* gw_out, hipStream_t st) {
+    const KArgs ka = make_kargs(ls, g, cap);
+    const size_t shm = lds_bytes(lanes, cap);
+    UD_DISPATCH_L(lanes, {
+        if (gf == UD_GRID_F16)
+            hipLaunchKernelGGL((energy_grad_kernel<L, UD_GRID_F16>), dim3(n_work), dim3(kWG), shm, st, ka, td, work,
+                               perm, poses, v_intra, v_inter, e_out, gpose_out, gw_out);
+        else
+            hipLaunchKernelGGL((energy_grad_kernel<L, UD_GRID_F32>), dim3(n_work), dim3(kWG), shm, st, ka, td, work,
+                               perm, poses, v_intra, v_inter, e_out, gpose_out, gw_out);
     });
     return hipGetLastError();
 }

@@ -38,6 +38,10 @@ hipError_t launch_dock(int lanes, int gf, const LigandSetDev& ls, const GridDev&
 hipError_t launch_terms(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap, const WorkItem* work,
                         int n_work, const int32_t* perm, const float* poses, float v, float* inter_out,
                         float* intra_out, hipStream_t st);
+hipError_t launch_energy_grad(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const TermsDev& td,
+                              const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
+                              const float* poses, float v_intra, float v_inter, float* e_out, float* gpose_out,
+                              float* gw_out, hipStream_t st);
 hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap, const WorkItem* work,
                          int n_work, const int32_t* perm, const float* poses, float* coords_out, hipStream_t st);
 

@@ -19,6 +19,7 @@ UD_MAX_ATOMS = 128
 UD_MAX_TORSIONS = 32
 UD_POSE_FLOATS = 7 + UD_MAX_TORSIONS
 UD_GRAD_FLOATS = 6 + UD_MAX_TORSIONS
+UD_WEIGHT_FLOATS = 7  # dE/dw per pose, in the order of the scoring weights
 UD_N_BUCKETS = 6
 BUCKET_ATOMS = (16, 32, 48, 64, 96, 128)
 
@@ -74,6 +75,14 @@ def load() -> ctypes.CDLL:
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() "
                            "(no CPU fallback exists)")
+    # One HIP runtime per process: with PyTorch installed, load it first, so
+    # that this library binds to the HIP runtime torch ships (libamdhip64 is
+    # resolved by soname).  The other order maps two runtimes into the
+    # process and torch can then not open the GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     vp, ip, fp, dp = ctypes.c_void_p, _c_int_p, _c_float_p, ctypes.POINTER(ctypes.c_double)
     sig = {
@@ -89,6 +98,8 @@ def load() -> ctypes.CDLL:
         "ud_set_ligands": (ctypes.c_int, [vp, ctypes.POINTER(UdLigandBatch)]),
         "ud_score_grad": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, fp, fp]),
         "ud_score_grad_device": (ctypes.c_int, [vp, ctypes.c_int, ip, vp, fp, vp, vp]),
+        "ud_energy_grad": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, fp, fp, fp]),
+        "ud_energy_grad_device": (ctypes.c_int, [vp, ctypes.c_int, ip, vp, fp, vp, vp, vp]),
         "ud_optimize": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, ctypes.c_int, fp, ip]),
         "ud_dock": (ctypes.c_int, [vp, ctypes.POINTER(UdSearchParams), fp, fp, ctypes.POINTER(ctypes.c_int64)]),
         "ud_fetch_dock_results": (ctypes.c_int, [vp, fp, fp, ctypes.POINTER(ctypes.c_int64)]),
@@ -277,6 +288,18 @@ def synth_receptor_xyz(n_atoms: int, edge: float, min_dist: float, seed: int) ->
     return out.reshape(-1, 3)
 
 
+def grid_type(t: int) -> int:
+    """XS type whose map an atom of type t reads: CG carbons read the plain
+    carbon maps, G atoms and untyped atoms none (cache.cpp:172-191)."""
+    if t < 0 or t >= 36 or t in (21, 24, 27, 30):
+        return -1
+    if t in (19, 22, 25, 28):
+        return 0
+    if t in (20, 23, 26, 29):
+        return 1
+    return t
+
+
 class Context:
     """One device context (ud_ctx) on a HIP stream."""
 
@@ -366,6 +389,29 @@ class Context:
         self._ck(self.L.ud_score_grad(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float),
                                       _p(vv, ctypes.c_float), _p(e, ctypes.c_float), _p(g, ctypes.c_float)))
         return e, g
+
+    def energy_grad(self, pose_ligand, poses, v=(1000.0, 1000.0, 1000.0)):
+        """Energy, dE/d(position, quaternion w-x-y-z, torsions) of E(q / |q|)
+        and dE/d(term weights) (ud_energy_grad)."""
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        P = np.ascontiguousarray(poses, dtype=np.float32).reshape(len(pl), UD_POSE_FLOATS)
+        vv = np.array(v, dtype=np.float32)
+        e = np.zeros(len(pl), dtype=np.float32)
+        gp = np.zeros((len(pl), UD_POSE_FLOATS), dtype=np.float32)
+        gw = np.zeros((len(pl), UD_WEIGHT_FLOATS), dtype=np.float32)
+        self._ck(self.L.ud_energy_grad(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float),
+                                       _p(vv, ctypes.c_float), _p(e, ctypes.c_float), _p(gp, ctypes.c_float),
+                                       _p(gw, ctypes.c_float)))
+        return e, gp, gw
+
+    def energy_grad_device(self, pose_ligand, d_poses: int, d_energy: int, d_pose_grad: int, d_weight_grad: int,
+                           v=(1000.0, 1000.0, 1000.0)):
+        """ud_energy_grad_device: device addresses, asynchronous on the context's stream."""
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        vv = np.array(v, dtype=np.float32)
+        self._ck(self.L.ud_energy_grad_device(self.h, len(pl), _p(pl, ctypes.c_int32), ctypes.c_void_p(d_poses),
+                                              _p(vv, ctypes.c_float), ctypes.c_void_p(d_energy),
+                                              ctypes.c_void_p(d_pose_grad), ctypes.c_void_p(d_weight_grad)))
 
     def score_grad_device(self, pose_ligand, d_poses: int, d_energy: int, d_grad: int, v=(1000.0, 1000.0, 1000.0)):
         pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
