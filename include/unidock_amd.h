@@ -268,27 +268,55 @@ int ud_autotune(ud_ctx* ctx, const ud_search_params* params, int sample_ligands,
 typedef struct ud_mode {
     float pose[UD_POSE_FLOATS];
     float energy;            /* final affinity: conf-independent form of inter + intra - intra_best */
-    float inter;             /* ligand-grid energy (caps 1000) */
+    float inter;             /* ligand-receptor energy (caps 1000; maps, or explicit atoms if refined) */
     float intra;             /* ligand intramolecular energy (caps 1000) */
     float conf_independent;  /* energy - (inter + intra - intra_best), the torsion term */
-    float search_energy;     /* the chain's best energy from the search */
+    float search_energy;     /* the chain's best energy from the search (before refinement) */
     float rmsd_lb, rmsd_ub;  /* from the best mode (model::rmsd_lower/upper_bound) */
     float unbound;           /* intramolecular energy of the best pose (subtracted above) */
     int32_t chain;           /* chain the mode comes from (ligand * exhaustiveness + k) */
 } ud_mode;
 
-/* Host post-processing of the last ud_dock: the part of
- * Vina::global_search_gpu after do_docking (vina.h:232-345) without the
- * explicit-receptor refinement (the reference's no_refine path).  Per
- * ligand: the chains' best poses in chain order go through
- * add_to_output_container (coords.cpp:42-68; heavy-atom RMSD upper bound <
- * min_rmsd keeps the better of two, at most max_modes kept) as
- * monte_carlo.cu:2940-2943 does, then remove_redundant (vina.cpp:1465-1470);
- * the intramolecular energy of the best pose is subtracted and every mode is
- * rescored (score_gpu, vina.cpp:1245-1327), sorted by the final energy and
- * given RMSD bounds from the best mode.  modes_out[l * max_modes + k],
- * n_modes_out[l]. */
-int ud_postprocess(ud_ctx* ctx, int max_modes, double min_rmsd, ud_mode* modes_out, int32_t* n_modes_out);
+/* Post-processing of the last ud_dock: the part of Vina::global_search_gpu
+ * after do_docking (vina.h:232-345).  Per ligand: the chains' best poses in
+ * chain order go through add_to_output_container (coords.cpp:42-68;
+ * heavy-atom RMSD upper bound < min_rmsd keeps the better of two, at most
+ * max_modes kept) as monte_carlo.cu:2940-2943 does, then remove_redundant
+ * (vina.cpp:1465-1470).  With refine_step > 0 and a receptor from
+ * ud_compute_maps, every kept mode is refined against the explicit receptor
+ * (ud_refine, vina.h:249-272) and the modes are ordered by refined energy;
+ * refine_step 0 (or maps from ud_set_maps) is the reference's no_refine path.
+ * The intramolecular energy of the first mode is subtracted and every mode is
+ * rescored (score_gpu, vina.cpp:1245-1327: maps, or the explicit receptor
+ * after refinement), sorted by the final energy and given RMSD bounds from
+ * the best mode.  modes_out[l * max_modes + k], n_modes_out[l].  The
+ * reference CLI's default refine_step is 3 (main.cpp:557-558). */
+int ud_postprocess(ud_ctx* ctx, int max_modes, double min_rmsd, int refine_step, ud_mode* modes_out,
+                   int32_t* n_modes_out);
+
+/* Refinement of poses against the explicit receptor atoms (non_cache,
+ * non_cache.cpp:420-860) as Vina::global_search_gpu does for the kept modes
+ * (vina.h:249-272): BFGS with caps 1000 and (25 + atoms) / 3 steps, for pass
+ * p = 0 .. refine_step - 1 with box slope 100 * 10^(2p), until every heavy
+ * atom is within the box (non_cache::within, margin 1e-4).  poses are updated
+ * in place; energy_out gets the last BFGS energy, or FLT_MAX for a pose that
+ * never got inside (vina.h:268); within_out (may be NULL) 1 / 0.  Needs the
+ * receptor of ud_compute_maps. */
+int ud_refine(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, float* poses, int refine_step, float* energy_out,
+              int32_t* within_out);
+
+/* Energy and gradient (as ud_score_grad) against the explicit receptor
+ * atoms with box slope `slope`: model::eval_deriv with non_cache as the
+ * receptor term (non_cache.cpp:420-860), what the refinement minimises. */
+int ud_score_grad_receptor(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* poses, const float* v,
+                           double slope, float* energy_out, float* grad_out);
+
+/* Final-score components against the explicit receptor (score_gpu's
+ * refined branch, vina.cpp:1262-1283): inter = non_cache::eval with caps
+ * 1000 and box slope 1e6 (non_cache.cpp:34-255, fast table), intra = the
+ * ligand pairs as in ud_score_terms. */
+int ud_score_terms_receptor(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* poses,
+                            float* inter_out, float* intra_out);
 
 /* add_to_output_container over n_cand candidates in order (coords.cpp:42-68):
  * heavy_xyz[n_cand][n_heavy][3], energy[n_cand]; writes the kept candidate

@@ -328,6 +328,7 @@ template <class R> void q_increment(Q<R>& q, const V3<R>& rot) {
 template <class R> struct Grid {
     int n[3] = {0, 0, 0};   // sample points per axis (= n_voxels + 1)
     R init[3], factor[3], factor_inv[3], dim_minus_1[3];
+    R hi[3];                // box end (grid_dims end, vina.cpp:536-538)
     std::vector<std::vector<R>> maps;  // one per XS type (index = type), empty if absent
     void set_geometry(const double center[3], const double size[3], double granularity) {
         for (int i = 0; i < 3; ++i) {
@@ -337,6 +338,7 @@ template <class R> struct Grid {
             R end = begin + real_span;
             n[i] = int(nv) + 1;
             init[i] = begin;
+            hi[i] = end;
             R range = end - begin;
             dim_minus_1[i] = R(n[i] - 1);
             factor[i] = dim_minus_1[i] / range;
@@ -448,6 +450,11 @@ template <class R> struct Ctx {
     Grid<R> grid;
     R slope = R(1e6);  // vina.cpp:515
     std::vector<Ligand<R>> ligs;
+    // typed receptor atoms of build_maps, for the explicit-receptor scoring
+    std::vector<V3<R>> rec;
+    std::vector<int> rec_t;
+    bool use_nc = false;   // eval_deriv scores against rec (non_cache) instead of the maps
+    R nc_slope = R(1e6);
     const PairTable<R>& table(int t1, int t2) {
         int k = tri_index(t1, t2);
         if (!have[k]) { tables[k].build(sf, t1, t2, w); have[k] = 1; }
@@ -510,6 +517,49 @@ template <class R> void forward_kinematics(const Ligand<R>& L, const Pose<R>& P,
     if (mats_out) *mats_out = ms;
 }
 
+// non_cache::eval_deriv / non_cache::eval (non_cache.cpp:420-860, 34-255) for
+// one ligand atom of XS type t at p: the coordinate clamped into the box
+// (penalty slope * |miss|, gradient slope * sign), the type-pair table value
+// of every typed receptor atom within the cutoff of the clamped point
+// (eval_deriv with a gradient, eval_fast without), curl, and the pair
+// gradient kept in clamped directions.  Brute force over the receptor atoms:
+// the reference's szv_grid (szv_grid.cpp:24-84) only lists, per cell, the
+// atoms that can pass the same r^2 test, in receptor order.
+template <class R> R nc_atom(Ctx<R>& C, int t, const V3<R>& p, R slope, R v, V3<R>* deriv) {
+    const Grid<R>& g = C.grid;
+    V3<R> a = p, oob;
+    R pen = 0;
+    for (int i = 0; i < 3; ++i) {
+        if (p[i] < g.init[i]) { a[i] = g.init[i]; oob[i] = R(-1); pen += std::abs(p[i] - g.init[i]); }
+        else if (p[i] > g.hi[i]) { a[i] = g.hi[i]; oob[i] = R(1); pen += std::abs(p[i] - g.hi[i]); }
+    }
+    pen *= slope;
+    R e = 0;
+    V3<R> d;
+    for (size_t j = 0; j < C.rec.size(); ++j) {
+        const V3<R> r = a - C.rec[j];
+        const R r2 = norm_sqr(r);
+        if (r2 < R(64)) {
+            const PairTable<R>& tab = C.table(t, C.rec_t[j]);
+            if (deriv) {
+                R pe, dor;
+                tab.eval_deriv(r2, pe, dor);
+                e += pe;
+                d = d + dor * r;
+            } else {
+                e += tab.eval_fast(r2);
+            }
+        }
+    }
+    if (deriv) {
+        curl(e, d, v);
+        *deriv = d + slope * oob;
+    } else {
+        curl(e, v);
+    }
+    return e + pen;
+}
+
 // model::eval_deriv (model.cpp:954-981) for a single ligand without flex
 // residues: grid term (cache::eval_deriv), intramolecular pairs
 // (eval_interacting_pairs_deriv, model.cpp:907-934), then the torsion-tree
@@ -523,6 +573,12 @@ template <class R> R eval_deriv(Ctx<R>& C, const Ligand<R>& L, const Pose<R>& P,
     R e = 0;
     for (int i = 0; i < L.n_atoms; ++i) {
         int t = grid_type(L.xs[i]);
+        if (C.use_nc) {  // the explicit receptor: G and untyped atoms skipped, table of the atom's own type
+            V3<R> d;
+            if (t >= 0) e += nc_atom(C, L.xs[i], coords[i], C.nc_slope, v[1], &d);
+            mf[i] = d;
+            continue;
+        }
         if (t < 0 || C.grid.maps[t].empty()) { mf[i] = V3<R>(); continue; }
         V3<R> d;
         e += grid_eval(C.grid, t, coords[i], C.slope, v[1], &d);
@@ -844,6 +900,55 @@ template <class R> R conf_independent(const Ctx<R>& C, R num_tors, R x) {
     return x / y;
 }
 
+// non_cache::within (non_cache.cpp:408-418, margin 1e-4): every heavy atom
+// inside the box.
+template <class R> bool nc_within(Ctx<R>& C, const Ligand<R>& L, const Pose<R>& P) {
+    std::vector<V3<R>> coords, origins, axes;
+    forward_kinematics(L, P, coords, origins, axes);
+    const R margin = R(0.0001);
+    for (int i = 0; i < L.n_atoms; ++i) {
+        if (!L.heavy[i]) continue;
+        for (int d = 0; d < 3; ++d)
+            if (coords[i][d] < C.grid.init[d] - margin || coords[i][d] > C.grid.hi[d] + margin) return false;
+    }
+    return true;
+}
+
+// Refinement of one kept pose (Vina::global_search_gpu, vina.h:249-272):
+// quasi_newton with authentic_v = 1000 and (25 + atoms) / 3 steps against the
+// explicit receptor, pass p with slope 100 * 10^(2p), stopping once the pose
+// is within the box; a pose never within gets max_fl.
+template <class R> R refine(Ctx<R>& C, const Ligand<R>& L, Pose<R>& P, int refine_step, bool* within) {
+    const R v[3] = {R(1000), R(1000), R(1000)};
+    const int max_steps = (25 + L.n_atoms) / 3;
+    C.use_nc = true;
+    R e = 0;
+    bool in = false;
+    for (int p = 0; p < refine_step; ++p) {
+        C.nc_slope = R(100.0 * std::pow(10.0, 2.0 * p));
+        int ec = 0;
+        e = bfgs(C, L, P, v, max_steps, &ec);
+        in = nc_within(C, L, P);
+        if (in) break;
+    }
+    C.use_nc = false;
+    *within = in;
+    return in ? e : R(kMaxFl);
+}
+
+// score_gpu's refined branch (vina.cpp:1262-1283): inter = non_cache::eval
+// with authentic_v and slope 1e6, intra as in score_terms.
+template <class R> void score_terms_nc(Ctx<R>& C, const Ligand<R>& L, const Pose<R>& P, R* inter, R* intra) {
+    std::vector<V3<R>> coords, origins, axes;
+    forward_kinematics(L, P, coords, origins, axes);
+    R e = 0;
+    for (int i = 0; i < L.n_atoms; ++i)
+        if (grid_type(L.xs[i]) >= 0) e += nc_atom(C, L.xs[i], coords[i], R(1e6), R(1000), (V3<R>*)nullptr);
+    R dummy;
+    score_terms(C, L, P, &dummy, intra);
+    *inter = e;
+}
+
 // Grid population, cache::populate_no_bias (cache.cpp:471-549): for each
 // sample point and needed type t2, the sum over typed receptor atoms with
 // r^2 <= cutoff^2 of eval_fast(t1, t2, r^2) of the TYPE-based table
@@ -858,7 +963,8 @@ template <class R> void build_maps(Ctx<R>& C, int n_rec, const double* rec_xyz, 
         ra.push_back(V3<R>(R(rec_xyz[3 * i]), R(rec_xyz[3 * i + 1]), R(rec_xyz[3 * i + 2])));
         rt.push_back(rec_xs[i]);
     }
-    std::vector<const PairTable<R>*> tabs;
+    C.rec = ra;
+    C.rec_t = rt;
     for (int j = 0; j < n_types; ++j) {
         g.maps[types[j]].assign(size_t(g.n[0]) * g.n[1] * g.n[2], R(0));
     }
@@ -1092,6 +1198,63 @@ extern "C" void orc_score_terms(OrcHandle* h, int lig, const double* pose, doubl
         score_terms(h->f, h->f.ligs[lig], P, &a, &b);
         *inter = a; *intra = b;
     }
+}
+template <class R> static double refine_impl(Ctx<R>& C, int lig, double* pose, int refine_step, int* within) {
+    const Ligand<R>& L = C.ligs[lig];
+    Pose<R> P;
+    load_pose(L, pose, P);
+    bool in = false;
+    R e = refine(C, L, P, refine_step, &in);
+    store_pose(P, pose);
+    *within = in ? 1 : 0;
+    return e;
+}
+// refinement of one pose against the explicit receptor (pose in/out)
+extern "C" double orc_refine(OrcHandle* h, int lig, double* pose, int refine_step, int* within) {
+    if (h->prec == 64) return refine_impl(h->d, lig, pose, refine_step, within);
+    return refine_impl(h->f, lig, pose, refine_step, within);
+}
+template <class R> static void terms_nc_impl(Ctx<R>& C, int lig, const double* pose, double* inter, double* intra) {
+    Pose<R> P;
+    load_pose(C.ligs[lig], pose, P);
+    R a, b;
+    score_terms_nc(C, C.ligs[lig], P, &a, &b);
+    *inter = a;
+    *intra = b;
+}
+extern "C" void orc_score_terms_nc(OrcHandle* h, int lig, const double* pose, double* inter, double* intra) {
+    if (h->prec == 64) terms_nc_impl(h->d, lig, pose, inter, intra);
+    else terms_nc_impl(h->f, lig, pose, inter, intra);
+}
+// energy + gradient against the explicit receptor with box slope `slope`
+extern "C" double orc_eval_deriv_nc(OrcHandle* h, int lig, const double* pose, const double* v, double slope,
+                                    double* grad) {
+    double r;
+    if (h->prec == 64) {
+        h->d.use_nc = true; h->d.nc_slope = slope;
+        r = eval_impl(h->d, lig, pose, v, grad, nullptr, nullptr);
+        h->d.use_nc = false;
+    } else {
+        h->f.use_nc = true; h->f.nc_slope = float(slope);
+        r = eval_impl(h->f, lig, pose, v, grad, nullptr, nullptr);
+        h->f.use_nc = false;
+    }
+    return r;
+}
+// one atom of XS type t at loc against the explicit receptor (deriv NULL: the
+// energy-only form with the fast table)
+extern "C" double orc_nc_atom(OrcHandle* h, int t, const double* loc, double slope, double v, double* deriv) {
+    if (h->prec == 64) {
+        V3<double> d;
+        double e = nc_atom(h->d, t, V3<double>(loc[0], loc[1], loc[2]), slope, v, deriv ? &d : nullptr);
+        if (deriv) for (int k = 0; k < 3; ++k) deriv[k] = d[k];
+        return e;
+    }
+    V3<float> d;
+    float e = nc_atom(h->f, t, V3<float>(float(loc[0]), float(loc[1]), float(loc[2])), float(slope), float(v),
+                      deriv ? &d : nullptr);
+    if (deriv) for (int k = 0; k < 3; ++k) deriv[k] = d[k];
+    return e;
 }
 extern "C" double orc_conf_independent(OrcHandle* h, double num_tors, double x) {
     if (h->prec == 64) return conf_independent(h->d, num_tors, x);

@@ -55,6 +55,10 @@ def load():
         "orc_mutate": (None, [vp, ctypes.c_int, dp, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
                               ctypes.c_uint32]),
         "orc_score_terms": (None, [vp, ctypes.c_int, dp, dp, dp]),
+        "orc_score_terms_nc": (None, [vp, ctypes.c_int, dp, dp, dp]),
+        "orc_refine": (ctypes.c_double, [vp, ctypes.c_int, dp, ctypes.c_int, ip]),
+        "orc_nc_atom": (ctypes.c_double, [vp, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, dp]),
+        "orc_eval_deriv_nc": (ctypes.c_double, [vp, ctypes.c_int, dp, dp, ctypes.c_double, dp]),
         "orc_conf_independent": (ctypes.c_double, [vp, ctypes.c_double, ctypes.c_double]),
         "orc_dock_cpu": (ctypes.c_double, [vp, ctypes.c_int, ip, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ip,
                                            ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)]),
@@ -204,6 +208,38 @@ class Oracle:
         b = ctypes.c_double()
         self.L.orc_score_terms(self.h, lig, _d(p), ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def refine(self, lig, pose, refine_step=3):
+        """Refinement against the explicit receptor; returns (energy, pose, within)."""
+        p = np.zeros(POSE)
+        p[:len(pose)] = pose
+        w = ctypes.c_int()
+        e = self.L.orc_refine(self.h, lig, p.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), refine_step,
+                              ctypes.byref(w))
+        return e, p, bool(w.value)
+
+    def nc_atom(self, t, loc, slope=1e6, v=1000.0, deriv=True):
+        """One atom of XS type t at loc against the explicit receptor atoms."""
+        d = np.zeros(3)
+        e = self.L.orc_nc_atom(self.h, t, _d(loc), slope, v,
+                               d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if deriv else None)
+        return e, d
+
+    def score_terms_nc(self, lig, pose):
+        p = np.zeros(POSE)
+        p[:len(pose)] = pose
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        self.L.orc_score_terms_nc(self.h, lig, _d(p), ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def eval_deriv_nc(self, lig, pose, v=(1000.0, 1000.0, 1000.0), slope=1e6):
+        n_t = self.ligs[lig].n_tors
+        p = np.zeros(POSE)
+        p[:len(pose)] = pose
+        g = np.zeros(6 + 32)
+        e = self.L.orc_eval_deriv_nc(self.h, lig, _d(p), _d(v), slope, g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return e, g[:6 + n_t]
 
     def conf_independent(self, num_tors, x):
         return self.L.orc_conf_independent(self.h, num_tors, x)

@@ -438,7 +438,7 @@ def test_postprocess_matches_oracle(scene):
     exh = 24
     P, E, _ = ctx.dock(exhaustiveness=exh, global_steps=4, seed=12)
     max_modes, min_rmsd = 9, 1.0
-    modes = ctx.postprocess(max_modes=max_modes, min_rmsd=min_rmsd)
+    modes = ctx.postprocess(max_modes=max_modes, min_rmsd=min_rmsd, refine_step=0)
     arr = synth.LigandBatch(list(ligs)).arrays()
     rot_internal = 5 * _lib.VINA_WEIGHTS[6] / 0.1 - 1
     n_checked = 0

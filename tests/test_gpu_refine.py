@@ -1,0 +1,190 @@
+"""Refinement against the explicit receptor (ud_refine, ud_score_terms_receptor,
+ud_postprocess with refine_step > 0) against the oracle's non_cache
+restatement.
+
+Tolerances
+  score terms   1e-5 relative vs the FP32 restatement (same atoms, same
+                order; the tables differ by the device's expf), 1e-4 vs FP64
+                for poses inside the box
+  refinement    from identical docked-like starts the within flag agrees and
+                every GPU energy is the explicit-receptor energy of its own
+                returned pose.  The refined energy is sensitive to rounding:
+                the explicit-receptor surface has the tables' kinks and
+                last-bin drops, and the FP32 and FP64 oracles themselves part
+                by more than 0.01 kcal/mol on about a fifth of these starts.
+                So the GPU must agree with the FP32 oracle (0.01 kcal/mol) on
+                as many starts as the two oracles agree with each other, less
+                10 %, and its mean distance to the FP64 oracle must be within
+                twice the FP32 oracle's plus 0.01
+  post-process  same clustered chains as without refinement; each mode's
+                components are those of its refined pose; the pipeline after
+                refinement (reorder by refined energy, intramolecular
+                reference, rescoring, final score) matches the FP32 oracle
+                fed with the GPU-refined poses to 1e-4 relative.  Started from
+                short-search poses, which are often strained, whole oracle
+                pipelines part from the GPU's more often than from each
+                other; at least half the modes must still agree within
+                0.02 kcal/mol
+"""
+import numpy as np
+import pytest
+
+from helpers import LIGAND_TYPES, box_of, oracle_scene, random_poses, small_ligands, small_receptor
+from unidock_tools import _lib, synth
+
+pytestmark = pytest.mark.gpu
+FLT_MAX = 3.4028234663852886e38
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def rs():
+    xyz, xs, cav = small_receptor()
+    ligs = small_ligands()
+    _say("[gpu] refine: creating context")
+    ctx = _lib.Context(0)
+    ctx.set_scoring(_lib.UD_SF_VINA)
+    c, s = box_of(cav)
+    ctx.set_box(c, s, 0.375)
+    ctx.compute_maps(xyz, xs, LIGAND_TYPES, _lib.UD_GRID_F32)
+    ctx.set_ligands(synth.LigandBatch(list(ligs)).arrays())
+    o64, _, _ = oracle_scene(64)
+    o32, _, _ = oracle_scene(32)
+    lo = o64.geo[:3]
+    hi = lo + o64.geo[6:9] * (np.array(o64.n) - 1)
+    yield dict(ctx=ctx, ligs=ligs, o64=o64, o32=o32, cav=cav, lo=lo, hi=hi)
+    ctx.close()
+
+
+def _starts(rs, n_per, seed, spread):
+    """Docked-like starts: grid-map minima of the FP64 oracle."""
+    rng = np.random.default_rng(seed)
+    pl, P = [], []
+    for li, lig in enumerate(rs["ligs"]):
+        for p in random_poses(rng, lig, rs["cav"], spread, n_per):
+            pl.append(li)
+            P.append(rs["o64"].bfgs(li, p, max_steps=40)[1])
+    return np.array(pl, dtype=np.int32), np.array(P)
+
+
+def test_score_terms_receptor_matches_oracle(rs):
+    _say("[gpu] score_terms_receptor vs oracle")
+    ctx, o32, o64 = rs["ctx"], rs["o32"], rs["o64"]
+    rng = np.random.default_rng(1)
+    pl, P = [], []
+    for li, lig in enumerate(rs["ligs"]):
+        for p in random_poses(rng, lig, rs["cav"], 4.0, 6):
+            pl.append(li)
+            P.append(p)
+    pl, P = np.array(pl, dtype=np.int32), np.array(P)
+    inter, intra = ctx.score_terms_receptor(pl, P)
+    n_in = 0
+    for i, li in enumerate(pl):
+        a32, b32 = o32.score_terms_nc(int(li), P[i])
+        a64, b64 = o64.score_terms_nc(int(li), P[i])
+        assert inter[i] == pytest.approx(a32, rel=1e-5, abs=1e-4), (i, inter[i], a32)
+        assert intra[i] == pytest.approx(b32, rel=1e-5, abs=1e-4)
+        c = o64.eval_deriv(int(li), P[i])[2]
+        if np.all(c >= rs["lo"]) and np.all(c <= rs["hi"]):
+            n_in += 1
+            assert inter[i] == pytest.approx(a64, rel=1e-4, abs=1e-4)
+    assert n_in >= 10
+
+
+def test_score_grad_receptor_matches_oracle(rs):
+    _say("[gpu] score_grad_receptor vs oracle")
+    ctx, o32, o64 = rs["ctx"], rs["o32"], rs["o64"]
+    pl, S = _starts(rs, 4, 3, 3.0)
+    for slope in (100.0, 1e6):
+        e, g = ctx.score_grad_receptor(pl, S, slope=slope)
+        for i, li in enumerate(pl):
+            li = int(li)
+            nt = rs["ligs"][li].n_tors
+            e32, g32 = o32.eval_deriv_nc(li, S[i], slope=slope)
+            e64, g64 = o64.eval_deriv_nc(li, S[i], slope=slope)
+            assert e[i] == pytest.approx(e32, rel=1e-5, abs=1e-4), (i, e[i], e32)
+            sc = max(1.0, np.abs(g32).max())
+            np.testing.assert_allclose(g[i, :6 + nt], g32, rtol=0, atol=1e-4 * sc, err_msg=f"pose {i}")
+            np.testing.assert_allclose(g[i, :6 + nt], g64, rtol=0, atol=3e-3 * sc, err_msg=f"pose {i}")
+
+
+def test_refine_matches_oracle(rs):
+    _say("[gpu] refine vs oracle")
+    ctx, o32, o64 = rs["ctx"], rs["o32"], rs["o64"]
+    pl, S = _starts(rs, 4, 2, 2.5)
+    Pg, eg, wg = ctx.refine(pl, S, refine_step=3)
+    agree32 = base = 0
+    d_gpu, d_32 = [], []
+    for i, li in enumerate(pl):
+        li = int(li)
+        e32, P32, w32 = o32.refine(li, S[i], 3)
+        e64, P64, w64 = o64.refine(li, S[i], 3)
+        assert bool(wg[i]) == w32, (i, wg[i], w32)
+        if not wg[i]:
+            assert eg[i] == pytest.approx(FLT_MAX)
+            continue
+        agree32 += abs(eg[i] - e32) <= 0.01
+        base += abs(e32 - e64) <= 0.01
+        d_gpu.append(abs(eg[i] - e64))
+        d_32.append(abs(e32 - e64))
+        _say(f"  pose {i}: gpu {eg[i]:.4f} fp32 {e32:.4f} fp64 {e64:.4f}")
+        # the energy is the explicit-receptor energy of the returned pose at one of the pass slopes
+        es = [o32.eval_deriv_nc(li, Pg[i].astype(np.float64), slope=s)[0] for s in (100.0, 1e4, 1e6)]
+        assert min(abs(eg[i] - x) for x in es) <= 1e-3 + 1e-4 * abs(eg[i]), (i, eg[i], es)
+    n = int(wg.sum())
+    assert n >= 0.5 * len(pl)
+    assert agree32 >= base - 0.1 * n, (agree32, base, n)
+    assert np.mean(d_gpu) <= 2 * np.mean(d_32) + 0.01, (np.mean(d_gpu), np.mean(d_32))
+
+
+def test_postprocess_with_refinement(rs):
+    _say("[gpu] postprocess with refinement")
+    ctx, o32 = rs["ctx"], rs["o32"]
+    ligs = rs["ligs"]
+    exh = 16
+    P, E, _ = ctx.dock(exhaustiveness=exh, global_steps=4, seed=21)
+    plain = ctx.postprocess(max_modes=9, min_rmsd=1.0, refine_step=0)
+    refined = ctx.postprocess(max_modes=9, min_rmsd=1.0, refine_step=3)
+    n_modes = n_close = 0
+    for li, lig in enumerate(ligs):
+        got = refined[li]
+        assert sorted(m["chain"] for m in got) == sorted(m["chain"] for m in plain[li])
+        if not got:
+            continue
+        energies = [m["energy"] for m in got]
+        assert energies == sorted(energies)
+        pl = np.full(len(got), li, dtype=np.int32)
+        Pm = np.array([m["pose"] for m in got])
+        inter, intra = ctx.score_terms_receptor(pl, Pm)
+        for k, m in enumerate(got):
+            assert m["inter"] == inter[k] and m["intra"] == intra[k]
+            assert m["search_energy"] == E[m["chain"]]
+        # the intramolecular reference is the mode with the lowest refined energy
+        _, er, _ = ctx.refine(pl, P[[m["chain"] for m in got]], refine_step=3)
+        assert got[0]["unbound"] == pytest.approx(intra[int(np.argmin(er))], abs=0)
+        # the pipeline after refinement, on the GPU-refined poses, by the oracle
+        Pr, _, _ = ctx.refine(pl, P[[m["chain"] for m in got]], refine_step=3)
+        ti = [o32.score_terms_nc(li, Pr[k].astype(np.float64)) for k in range(len(got))]
+        ref_best = ti[int(np.argmin(er))][1]
+        for k, m in enumerate(got):
+            assert m["inter"] == pytest.approx(ti[k][0], rel=1e-5, abs=1e-4)
+            assert m["intra"] == pytest.approx(ti[k][1], rel=1e-5, abs=1e-4)
+            final = o32.conf_independent(float(lig.num_tors), ti[k][0] + ti[k][1] - ref_best)
+            assert m["energy"] == pytest.approx(final, rel=1e-4, abs=1e-4)
+        # whole oracle pipeline (its own refinement) on the same kept chains
+        ref_e, ref_inter, ref_intra = {}, {}, {}
+        for m in got:
+            e, Pq, _ = o32.refine(li, P[m["chain"]].astype(np.float64), 3)
+            a, b = o32.score_terms_nc(li, Pq)
+            ref_e[m["chain"]], ref_inter[m["chain"]], ref_intra[m["chain"]] = e, a, b
+        best = min(ref_e, key=lambda c: ref_e[c])
+        for m in got:
+            final = o32.conf_independent(float(lig.num_tors),
+                                         ref_inter[m["chain"]] + ref_intra[m["chain"]] - ref_intra[best])
+            n_modes += 1
+            n_close += abs(m["energy"] - final) <= 0.02
+    assert n_modes >= 2 * len(ligs)
+    assert n_close >= 0.5 * n_modes, (n_close, n_modes)

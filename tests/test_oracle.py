@@ -283,3 +283,104 @@ def test_final_score_closed_form():
     w_int = 5 * 0.05846 / 0.1 - 1
     for n in (0, 1, 5, 10):
         assert o.conf_independent(n, -7.5) == pytest.approx(-7.5 / (1 + 0.1 * (w_int + 1) * n / 5))
+
+
+# ---------------------------------------------------------------- explicit receptor (refinement)
+def test_explicit_receptor_equals_maps_at_grid_points():
+    """non_cache::eval (fast table, energy only) at a grid point inside the box
+    is the map value there: both sum eval_fast over the receptor atoms within
+    the cutoff, then curl."""
+    o, ligs, (xyz, xs, cav) = oracle_scene(64)
+    rng = np.random.default_rng(4)
+    n = o.n
+    init, factor_inv = o.geo[:3], o.geo[6:9]
+    for _ in range(40):
+        idx = np.array([int(rng.integers(0, n[i])) for i in range(3)])
+        p = init + factor_inv * idx
+        t = int(rng.choice(LIGAND_TYPES))
+        for v in (1000.0, 10.0):
+            e_nc, _ = o.nc_atom(t, p, v=v, deriv=False)
+            e_map, _ = o.grid_eval(t, p, v=v, deriv=False)
+            assert e_nc == pytest.approx(e_map, rel=1e-9, abs=1e-9), (idx, t, v)
+
+
+def test_explicit_receptor_out_of_box():
+    """Outside the box the coordinate is clamped: penalty slope * |miss|, and
+    the gradient is the clamped point's pair gradient plus slope * sign (the
+    reference does not zero the pair part in clamped directions)."""
+    o, ligs, (xyz, xs, cav) = oracle_scene(64)
+    lo = o.geo[:3]
+    hi = lo + o.geo[6:9] * (np.array(o.n) - 1)
+    face = np.array([lo[0], 0.5 * (lo[1] + hi[1]), hi[2]])
+    out = face + np.array([-0.7, 0.0, 0.4])
+    for slope in (100.0, 1e6):
+        e_f, d_f = o.nc_atom(0, face, slope=slope)
+        e_o, d_o = o.nc_atom(0, out, slope=slope)
+        assert e_o - e_f == pytest.approx(slope * 1.1, rel=1e-9)
+        np.testing.assert_allclose(d_o, d_f + slope * np.array([-1.0, 0.0, 1.0]), rtol=1e-12, atol=1e-9)
+        e_fast_f, _ = o.nc_atom(0, face, slope=slope, deriv=False)
+        e_fast_o, _ = o.nc_atom(0, out, slope=slope, deriv=False)
+        assert e_fast_o - e_fast_f == pytest.approx(slope * 1.1, rel=1e-9)
+
+
+def test_explicit_receptor_gradient_central_differences():
+    """Inside the box the gradient is that of the interpolated pair energy, to
+    the table's own consistency: e and dor are interpolated from separate
+    samples (dor a central difference of e), so a sum of ~100 receptor pairs
+    differs from the finite difference of the piecewise-linear e by up to ~1 %
+    of the gradient; a wrong sign or factor would be off by 100 %.  Points
+    with a receptor atom in the table's last bin (2047/32 <= r^2 < 64) are
+    skipped: the terms vanish at r = 8 A, so e falls to 0 within that bin while
+    dor, a central difference across it, carries about half that slope.  The
+    same happens, more weakly, in the bin of each kink of the piecewise-linear
+    terms (hydrophobic and hydrogen-bond slope steps), so most points, not
+    all, must agree to 1 %."""
+    o, ligs, (xyz, xs, cav) = oracle_scene(64)
+    rng = np.random.default_rng(5)
+    h = 1e-5
+    n_checked = n_close = 0
+    for _ in range(60):
+        p = cav + rng.uniform(-5.0, 5.0, 3)
+        t = int(rng.choice(LIGAND_TYPES))
+        e, d = o.nc_atom(t, p)
+        r2 = ((xyz - p) ** 2).sum(1)
+        if e > 5.0 or np.any((r2 >= 2047 / 32) & (r2 < 64.0)):  # off the steep walls and the last bin
+            continue
+        fd = np.zeros(3)
+        for k in range(3):
+            a, b = p.copy(), p.copy()
+            a[k] += h
+            b[k] -= h
+            fd[k] = (o.nc_atom(t, a)[0] - o.nc_atom(t, b)[0]) / (2 * h)
+        sc = max(1.0, np.abs(d).max())
+        n_close += bool(np.all(np.abs(d - fd) <= 1e-2 * sc))
+        assert np.all(np.abs(d - fd) <= 0.5 * sc), (p, t, d, fd)
+        n_checked += 1
+    assert n_checked >= 20
+    assert n_close >= 0.8 * n_checked, (n_close, n_checked)
+
+
+def test_refine_descends_and_reports_within():
+    o, ligs, (xyz, xs, cav) = oracle_scene(64)
+    lo = o.geo[:3]
+    hi = lo + o.geo[6:9] * (np.array(o.n) - 1)
+    rng = np.random.default_rng(6)
+    n_in = 0
+    for li, lig in enumerate(ligs):
+        for pose in random_poses(rng, lig, cav, 1.5, 2):
+            _, start, _ = o.bfgs(li, pose, max_steps=30)  # a docked-like start: a grid-map minimum
+            e0 = o.eval_deriv_nc(li, start, slope=100.0)[0]
+            e, P, within = o.refine(li, start, 3)
+            c = o.eval_deriv(li, P)[2]
+            heavy_in = bool(np.all((c[lig.heavy == 1] >= lo - 1e-4) & (c[lig.heavy == 1] <= hi + 1e-4)))
+            assert within == heavy_in
+            if within:
+                n_in += 1
+                # the energy is that of the returned pose for one of the pass slopes
+                es = [o.eval_deriv_nc(li, P, slope=s)[0] for s in (100.0, 1e4, 1e6)]
+                assert min(abs(e - x) for x in es) <= 1e-9 * max(1.0, abs(e))
+                if abs(e - es[0]) <= 1e-9 * max(1.0, abs(e)):  # ended in the first pass
+                    assert e <= e0 + 1e-9
+            else:
+                assert e == pytest.approx(3.4028234663852886e38)
+    assert n_in >= 6

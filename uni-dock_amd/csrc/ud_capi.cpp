@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -30,6 +31,7 @@ struct ud_ctx {
     // grid
     bool box_set = false;
     float init[3], factor[3], factor_inv[3], dimm1[3];
+    float gend[3];  // box end (grid_dims end, vina.cpp:536-538)
     int n[3] = {0, 0, 0};
     int n_points = 0;
     int precision = UD_GRID_F32;
@@ -43,6 +45,11 @@ struct ud_ctx {
     size_t rec_cap = 0;
     int n_rec = 0;
     bool have_rec = false;
+    // receptor cell lists of the explicit-receptor scoring (refinement)
+    float4* d_cells = nullptr;
+    int32_t* d_cell_start = nullptr;
+    size_t cells_cap = 0, cstart_cap = 0;
+    int cell_dim[3] = {1, 1, 1};
     // per-term maps and tables for term-weight gradients (built on demand)
     float2* d_term_tab = nullptr;
     float* d_term_maps = nullptr;
@@ -283,7 +290,8 @@ int ud_destroy(ud_ctx* c) {
     if (!c) return UD_ERR_INVALID;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    void* ptrs[] = {c->d_table_deriv, c->d_table_fast, c->d_maps, c->d_rec, c->d_term_tab, c->d_term_maps,
+    void* ptrs[] = {c->d_table_deriv, c->d_table_fast, c->d_maps, c->d_rec, c->d_term_tab, c->d_term_maps, c->d_cells,
+                    c->d_cell_start,
                     c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs, c->d_partners, c->d_apar,
                     c->d_work, c->d_perm, c->d_poses, c->d_f1, c->d_f2, c->d_f3, c->d_i1, c->d_out_pose, c->d_out_e,
                     c->d_out_evals};
@@ -339,6 +347,7 @@ int ud_set_box(ud_ctx* c, const double* center, const double* size, double granu
         const float end = begin + real_span;
         c->n[i] = int(nv) + 1;
         c->init[i] = begin;
+        c->gend[i] = end;
         const float range = end - begin;
         c->dimm1[i] = float(c->n[i] - 1);
         c->factor[i] = c->dimm1[i] / range;
@@ -412,6 +421,69 @@ static int store_maps(ud_ctx* c, float* d_f32, int n_types, const int32_t* types
     (void)hipFree(d_f32);
     c->d_maps = d_h;
     return UD_OK;
+}
+
+// squared distance from v to the box [lo, hi] (brick_distance_sqr, brick.h:27-48)
+static float brick_distance_sqr(const float* lo, const float* hi, const float4& v) {
+    const float p[3] = {v.x, v.y, v.z};
+    float d2 = 0.f;
+    for (int i = 0; i < 3; ++i) {
+        const float cl = (p[i] <= lo[i]) ? lo[i] : (p[i] >= hi[i]) ? hi[i] : p[i];
+        d2 += (cl - p[i]) * (cl - p[i]);
+    }
+    return d2;
+}
+
+// Cell lists of the explicit-receptor scoring, szv_grid (szv_grid.cpp:24-55,
+// 76-96): about 3 A cells over the box; each lists, in receptor order, the
+// typed atoms closer than the cutoff to the cell (of those closer than the
+// cutoff to the box).
+static int build_cells(ud_ctx* c, const std::vector<float4>& rec) {
+    float range[3];
+    for (int i = 0; i < 3; ++i) {
+        range[i] = c->gend[i] - c->init[i];
+        const int n_int = int(range[i] / 3.f);
+        c->cell_dim[i] = n_int < 1 ? 1 : n_int;
+    }
+    const float cut2 = 64.f;
+    std::vector<int> relevant;
+    for (size_t i = 0; i < rec.size(); ++i)
+        if (brick_distance_sqr(c->init, c->gend, rec[i]) < cut2) relevant.push_back(int(i));
+    const int nx = c->cell_dim[0], ny = c->cell_dim[1], nz = c->cell_dim[2];
+    auto coord = [&](int d, int k) { return c->init[d] + range[d] * float(k) / float(c->cell_dim[d]); };
+    std::vector<int32_t> start(size_t(nx) * ny * nz + 1, 0);
+    std::vector<float4> cells;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const float lo[3] = {coord(0, x), coord(1, y), coord(2, z)};
+                const float hi[3] = {coord(0, x + 1), coord(1, y + 1), coord(2, z + 1)};
+                for (int i : relevant)
+                    if (brick_distance_sqr(lo, hi, rec[i]) < cut2) cells.push_back(rec[i]);
+                start[size_t(x) + size_t(nx) * (y + size_t(ny) * z) + 1] = int32_t(cells.size());
+            }
+    int rc = grow(c, c->d_cells, c->cells_cap, std::max<size_t>(cells.size(), 1));
+    if (rc) return rc;
+    rc = grow(c, c->d_cell_start, c->cstart_cap, start.size());
+    if (rc) return rc;
+    if (!cells.empty())
+        UD_HIP(c, hipMemcpyAsync(c->d_cells, cells.data(), sizeof(float4) * cells.size(), hipMemcpyHostToDevice, c->stream));
+    UD_HIP(c, hipMemcpyAsync(c->d_cell_start, start.data(), sizeof(int32_t) * start.size(), hipMemcpyHostToDevice, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    return UD_OK;
+}
+
+static RecDev rec_dev(const ud_ctx* c) {
+    RecDev r;
+    r.atoms = c->d_cells;
+    r.cell_start = c->d_cell_start;
+    for (int i = 0; i < 3; ++i) {
+        r.dim[i] = c->cell_dim[i];
+        r.init[i] = c->init[i];
+        r.end[i] = c->gend[i];
+        r.range[i] = c->gend[i] - c->init[i];
+    }
+    return r;
 }
 
 static GridBuildArgs build_args(const ud_ctx* c, int n_types, const int32_t* types) {
@@ -495,6 +567,8 @@ int ud_compute_maps(ud_ctx* c, int n_rec, const float* rec_xyz, const int32_t* r
     if (!rec.empty())
         UD_HIP(c, hipMemcpyAsync(c->d_rec, rec.data(), sizeof(float4) * rec.size(), hipMemcpyHostToDevice, c->stream));
     c->n_rec = int(rec.size());
+    rc = build_cells(c, rec);
+    if (rc) return rc;
     float* d_f32 = nullptr;
     if (hipMalloc(&d_f32, sizeof(float) * size_t(n_types) * c->n_points) != hipSuccess)
         return fail(c, UD_ERR_NOMEM, "map allocation failed");
@@ -767,6 +841,8 @@ int work_upload(ud_ctx* c, const std::vector<int32_t>& perm, const std::vector<W
 // lanes per chain without tuning, from MI355X timings of the search kernel
 // over ligand size buckets (32 lanes up to 32 atoms, a full wavefront above)
 int default_lanes(int na) { return na <= 32 ? 32 : 64; }
+
+constexpr float kMaxFloat = std::numeric_limits<float>::max();  // max_fl
 }  // namespace
 
 extern "C" {
@@ -1211,6 +1287,157 @@ int ud_pose_coords(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const flo
     return UD_OK;
 }
 
+int ud_score_terms_receptor(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const float* poses, float* inter_out,
+                            float* intra_out) {
+    if (!c || (n_poses > 0 && (!poses || !inter_out || !intra_out))) return UD_ERR_INVALID;
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc || n_poses == 0) return rc;
+    if (!c->have_rec) return fail(c, UD_ERR_STATE, "explicit-receptor scoring needs the receptor of ud_compute_maps");
+    UD_HIP(c, hipSetDevice(c->device));
+    rc = refresh_slots(c);
+    if (rc) return rc;
+    rc = pose_upload(c, n_poses, poses);
+    if (rc) return rc;
+    rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * 2);
+    if (rc) return rc;
+    LaunchCap cap = caps_all(c);
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, cap);
+    if (lds_bytes(lanes, cap) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
+    std::vector<int32_t> perm;
+    std::vector<WorkItem> work;
+    make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
+    rc = work_upload(c, perm, work);
+    if (rc) return rc;
+    UD_HIP(c, launch_terms_explicit(lanes, lig_dev(c), grid_dev(c), rec_dev(c), cap, c->d_work, int(work.size()),
+                                    c->d_perm, c->d_poses, 1000.f, c->d_f1, c->d_f1 + n_poses, c->stream));
+    UD_HIP(c, hipMemcpyAsync(inter_out, c->d_f1, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipMemcpyAsync(intra_out, c->d_f1 + n_poses, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    return UD_OK;
+}
+
+int ud_score_grad_receptor(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const float* poses, const float* v,
+                           double slope, float* energy_out, float* grad_out) {
+    if (!c || !v || (n_poses > 0 && (!poses || !energy_out))) return UD_ERR_INVALID;
+    if (!(slope >= 0.0)) return fail(c, UD_ERR_INVALID, "slope must be >= 0");
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc || n_poses == 0) return rc;
+    if (!c->have_rec) return fail(c, UD_ERR_STATE, "explicit-receptor scoring needs the receptor of ud_compute_maps");
+    UD_HIP(c, hipSetDevice(c->device));
+    rc = refresh_slots(c);
+    if (rc) return rc;
+    rc = pose_upload(c, n_poses, poses);
+    if (rc) return rc;
+    rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses) * UD_GRAD_FLOATS);
+    if (rc) return rc;
+    rc = grow(c, c->d_f2, c->f2_cap, size_t(n_poses));
+    if (rc) return rc;
+    LaunchCap cap = caps_all(c);
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, cap);
+    std::vector<int32_t> perm;
+    std::vector<WorkItem> work;
+    make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
+    rc = work_upload(c, perm, work);
+    if (rc) return rc;
+    GridDev g = grid_dev(c);
+    g.slope = float(slope);
+    UD_HIP(c, launch_score_explicit(lanes, lig_dev(c), g, rec_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
+                                    c->d_poses, v[0], v[1], c->d_f2, c->d_f1, c->stream));
+    UD_HIP(c, hipMemcpyAsync(energy_out, c->d_f2, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    if (grad_out)
+        UD_HIP(c, hipMemcpyAsync(grad_out, c->d_f1, sizeof(float) * size_t(n_poses) * UD_GRAD_FLOATS,
+                                 hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    return UD_OK;
+}
+
+int ud_refine(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses, int refine_step, float* energy_out,
+              int32_t* within_out) {
+    if (!c || (n_poses > 0 && (!poses || !energy_out))) return UD_ERR_INVALID;
+    if (refine_step < 1 || refine_step > 8) return fail(c, UD_ERR_INVALID, "refine_step must be in 1..8");
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc || n_poses == 0) return rc;
+    if (!c->have_rec) return fail(c, UD_ERR_STATE, "refinement needs the receptor of ud_compute_maps");
+    UD_HIP(c, hipSetDevice(c->device));
+    rc = refresh_slots(c);
+    if (rc) return rc;
+    LaunchCap cap = caps_all(c);
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : default_lanes(cap.n_atoms), cap);
+    // poses still refining (the reference stops a pose's passes once it is within the box)
+    std::vector<int> active(n_poses);
+    for (int i = 0; i < n_poses; ++i) active[i] = i;
+    std::vector<float> e(n_poses, kMaxFloat);
+    std::vector<char> inside(n_poses, 0);
+    std::vector<float> sub, se, xyz;
+    std::vector<int32_t> spl;
+    const float margin = 1e-4f;  // non_cache::within default (non_cache.h:41)
+    double ms_total = 0;
+    for (int pass = 0; pass < refine_step && !active.empty(); ++pass) {
+        const int na = int(active.size());
+        sub.resize(size_t(na) * UD_POSE_FLOATS);
+        spl.resize(na);
+        for (int k = 0; k < na; ++k) {
+            std::copy(poses + size_t(active[k]) * UD_POSE_FLOATS, poses + size_t(active[k] + 1) * UD_POSE_FLOATS,
+                      sub.begin() + size_t(k) * UD_POSE_FLOATS);
+            spl[k] = pose_ligand[active[k]];
+        }
+        rc = pose_upload(c, na, sub.data());
+        if (rc) return rc;
+        rc = grow(c, c->d_f1, c->f_cap, size_t(na));
+        if (rc) return rc;
+        rc = grow(c, c->d_i1, c->i_cap, size_t(na));
+        if (rc) return rc;
+        std::vector<int32_t> perm;
+        std::vector<WorkItem> work;
+        make_pose_work(na, spl.data(), 64 / lanes, perm, work);
+        rc = work_upload(c, perm, work);
+        if (rc) return rc;
+        GridDev g = grid_dev(c);
+        g.slope = float(100.0 * std::pow(10.0, 2.0 * pass));  // vina.h:261
+        UD_HIP(c, hipEventRecord(c->ev0, c->stream));
+        UD_HIP(c, launch_refine(lanes, lig_dev(c), g, rec_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
+                                c->d_poses, 1000.f, 1000.f, 0, c->d_f1, c->d_i1, c->stream));
+        UD_HIP(c, hipEventRecord(c->ev1, c->stream));
+        se.resize(na);
+        UD_HIP(c, hipMemcpyAsync(sub.data(), c->d_poses, sizeof(float) * size_t(na) * UD_POSE_FLOATS,
+                                 hipMemcpyDeviceToHost, c->stream));
+        UD_HIP(c, hipMemcpyAsync(se.data(), c->d_f1, sizeof(float) * na, hipMemcpyDeviceToHost, c->stream));
+        UD_HIP(c, hipStreamSynchronize(c->stream));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        ms_total += ms;
+        // non_cache::within (non_cache.cpp:408-418): heavy atoms inside the box with a margin
+        xyz.resize(size_t(na) * UD_MAX_ATOMS * 3);
+        rc = ud_pose_coords(c, na, spl.data(), sub.data(), xyz.data());
+        if (rc) return rc;
+        std::vector<int> still;
+        for (int k = 0; k < na; ++k) {
+            const int i = active[k];
+            std::copy(sub.begin() + size_t(k) * UD_POSE_FLOATS, sub.begin() + size_t(k + 1) * UD_POSE_FLOATS,
+                      poses + size_t(i) * UD_POSE_FLOATS);
+            e[i] = se[k];
+            const LigHdr& h = c->hdr[spl[k]];
+            bool in = true;
+            for (int a = 0; a < h.n_atoms && in; ++a) {
+                if (!((c->h_atoms[size_t(h.atom_begin) + a].meta >> 16) & 1)) continue;
+                for (int d = 0; d < 3; ++d) {
+                    const float x = xyz[(size_t(k) * UD_MAX_ATOMS + a) * 3 + d];
+                    if (x < c->init[d] - margin || x > c->gend[d] + margin) in = false;
+                }
+            }
+            if (in) inside[i] = 1;
+            else still.push_back(i);
+        }
+        active.swap(still);
+    }
+    for (int i = 0; i < n_poses; ++i) {
+        energy_out[i] = inside[i] ? e[i] : kMaxFloat;  // vina.h:268
+        if (within_out) within_out[i] = inside[i];
+    }
+    c->last_ms = ms_total;
+    return UD_OK;
+}
+
 int ud_set_lanes(ud_ctx* c, int lanes) {
     if (!c) return UD_ERR_INVALID;
     if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
@@ -1231,8 +1458,10 @@ int ud_cluster_modes(int n_cand, int n_heavy, const float* heavy_xyz, const floa
     return UD_OK;
 }
 
-int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, ud_mode* modes_out, int32_t* n_modes_out) {
+int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, ud_mode* modes_out,
+                   int32_t* n_modes_out) {
     if (!c || max_modes < 1 || !(min_rmsd >= 0.0) || !modes_out || !n_modes_out) return UD_ERR_INVALID;
+    if (refine_step < 0 || refine_step > 8) return fail(c, UD_ERR_INVALID, "refine_step must be in 0..8");
     if (c->out_n == 0 || c->last_exh <= 0) return fail(c, UD_ERR_STATE, "no docking results");
     const int exh = c->last_exh, L = c->n_lig;
     const size_t nch = size_t(L) * exh;
@@ -1296,19 +1525,46 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, ud_mode* modes_out
             });
         for (auto& th : pool) th.join();
     }
-    // rescoring of every kept mode with authentic caps
-    std::vector<int32_t> kl;
+    // kept modes of every ligand in cluster order
+    std::vector<int32_t> kl, kchain;
     std::vector<float> kp;
     for (int l = 0; l < L; ++l)
         for (int k : kept[l]) {
             kl.push_back(l);
+            kchain.push_back(k);
             const float* src = P.data() + (size_t(l) * exh + k) * UD_POSE_FLOATS;
             kp.insert(kp.end(), src, src + UD_POSE_FLOATS);
         }
-    std::vector<float> inter(kl.size()), intra(kl.size());
-    if (!kl.empty()) {
-        rc = ud_score_terms(c, int(kl.size()), kl.data(), kp.data(), inter.data(), intra.data());
+    const int nk_all = int(kl.size());
+    // refinement against the explicit receptor when it is known (vina.h:248-272),
+    // then rescoring with authentic caps against the same receptor model
+    const bool refine = refine_step > 0 && c->have_rec;
+    std::vector<float> rkey(nk_all, 0.f), inter(nk_all), intra(nk_all);
+    if (nk_all > 0) {
+        if (refine) {
+            rc = ud_refine(c, nk_all, kl.data(), kp.data(), refine_step, rkey.data(), nullptr);
+            if (rc) return rc;
+            rc = ud_score_terms_receptor(c, nk_all, kl.data(), kp.data(), inter.data(), intra.data());
+        } else {
+            rc = ud_score_terms(c, nk_all, kl.data(), kp.data(), inter.data(), intra.data());
+        }
         if (rc) return rc;
+    }
+    // heavy-atom coordinates of the kept modes (refined poses have moved)
+    std::vector<float> mx;
+    std::vector<size_t> moff(nk_all + 1, 0);
+    for (int q = 0; q < nk_all; ++q) moff[q + 1] = moff[q] + heavy[kl[q]].size() * 3;
+    mx.resize(moff[nk_all]);
+    for (int q0 = 0; q0 < nk_all; q0 += chunk) {
+        const int n = std::min(chunk, nk_all - q0);
+        rc = ud_pose_coords(c, n, kl.data() + q0, kp.data() + size_t(q0) * UD_POSE_FLOATS, tmp.data());
+        if (rc) return rc;
+        for (int k = 0; k < n; ++k) {
+            const std::vector<int>& hv = heavy[kl[q0 + k]];
+            float* dst = mx.data() + moff[q0 + k];
+            for (size_t a = 0; a < hv.size(); ++a)
+                for (int d = 0; d < 3; ++d) dst[3 * a + d] = tmp[(size_t(k) * UD_MAX_ATOMS + hv[a]) * 3 + d];
+        }
     }
     size_t q = 0;
     for (int l = 0; l < L; ++l) {
@@ -1316,33 +1572,40 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, ud_mode* modes_out
         ud_mode* out = modes_out + size_t(l) * max_modes;
         n_modes_out[l] = nk;
         if (nk == 0) continue;
-        // the intramolecular energy of the best pose of the search is subtracted from every mode
-        const float intra_best = intra[q];
+        // refined modes are ordered by their refined energy before the best one
+        // gives the intramolecular reference (vina.h:274-283)
+        std::vector<int> ord(nk);
+        for (int k = 0; k < nk; ++k) ord[k] = k;
+        if (refine)
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return rkey[q + a] < rkey[q + b]; });
+        const float intra_best = intra[q + ord[0]];
+        std::vector<int> slot_of(nk);  // mode k of `out` -> index among the ligand's kept modes
         for (int k = 0; k < nk; ++k) {
             ud_mode& m = out[k];
-            const int j = kept[l][k];
-            std::copy(P.data() + (size_t(l) * exh + j) * UD_POSE_FLOATS,
-                      P.data() + (size_t(l) * exh + j + 1) * UD_POSE_FLOATS, m.pose);
-            m.inter = inter[q + k];
-            m.intra = intra[q + k];
+            const size_t i = q + ord[k];
+            std::copy(kp.begin() + i * UD_POSE_FLOATS, kp.begin() + (i + 1) * UD_POSE_FLOATS, m.pose);
+            m.inter = inter[i];
+            m.intra = intra[i];
             const float x = m.inter + m.intra - intra_best;
             m.energy = conf_independent(x, c->hdr[l].num_tors, c->rot_internal);
             m.conf_independent = m.energy - x;
-            m.search_energy = E[size_t(l) * exh + j];
+            m.search_energy = E[size_t(l) * exh + kchain[i]];
             m.unbound = intra_best;
-            m.chain = int32_t(size_t(l) * exh + j);
+            m.chain = int32_t(size_t(l) * exh + kchain[i]);
         }
-        q += nk;
         std::stable_sort(out, out + nk, [](const ud_mode& a, const ud_mode& b) { return a.energy < b.energy; });
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < nk; ++j)
+                if (size_t(l) * exh + kchain[q + j] == size_t(out[k].chain)) slot_of[k] = j;
         // RMSD bounds from the best mode
         const int nh = int(heavy[l].size());
-        const float* xyz = hx.data() + hoff[l];
-        const float* best = xyz + size_t(out[0].chain - size_t(l) * exh) * nh * 3;
+        const float* best = mx.data() + moff[q + slot_of[0]];
         for (int k = 0; k < nk; ++k) {
-            const float* cur = xyz + size_t(out[k].chain - size_t(l) * exh) * nh * 3;
+            const float* cur = mx.data() + moff[q + slot_of[k]];
             out[k].rmsd_ub = rmsd_upper_bound(cur, best, nh);
             out[k].rmsd_lb = rmsd_lower_bound(cur, best, elem[l].data(), nh);
         }
+        q += nk;
     }
     return UD_OK;
 }

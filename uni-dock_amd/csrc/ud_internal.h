@@ -83,6 +83,17 @@ struct LigandSetDev {
     const float* table_fast;    // slot * kTableUsed
 };
 
+// Receptor atoms for scoring against the explicit receptor (non_cache,
+// non_cache.cpp) in the refinement: cells of about 3 A over the box
+// (szv_grid_dims, szv_grid.cpp:86-96), each listing in receptor order the
+// typed atoms within the cutoff of the cell (szv_grid.cpp:24-55).
+struct RecDev {
+    const float4* atoms;        // x, y, z, XS type bits; the cell lists concatenated
+    const int32_t* cell_start;  // n_cells + 1 offsets into atoms
+    int32_t dim[3];             // cells per axis
+    float init[3], range[3], end[3];  // box begin, end - begin, end
+};
+
 // Maps and pair tables of each scoring term alone (weight 1, the others 0),
 // for term-weight gradients: the energy is linear in the weights before curl.
 struct TermsDev {

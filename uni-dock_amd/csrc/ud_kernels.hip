@@ -10,6 +10,7 @@
 //   energy_grad_kernel  energy with pose-coordinate and term-weight gradients
 //   coords_kernel       atom coordinates of poses (model::set)
 #include <hip/hip_fp16.h>
+#include <cstring>
 #include <hip/hip_runtime.h>
 
 #include "ud_device.h"
@@ -218,6 +219,7 @@ enum { PX = 0, PXN = 1, PXO = 2, PTMP = 3, PBEST = 4 };
 struct KArgs {
     LigandSetDev ls;
     GridDev grid;
+    RecDev rec;  // explicit receptor (refinement launches only)
     int na_cap, nn_cap, np_cap;
     int pair_forces;
 };
@@ -393,6 +395,71 @@ __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const G
     return e + q.penalty;
 }
 
+// Grid-format value of the refinement launches: the inter term is scored
+// against the explicit receptor atoms instead of the maps.
+static constexpr int kGridExplicit = 2;
+
+// non_cache::eval_deriv / non_cache::eval (non_cache.cpp:420-860, 34-255)
+// for one ligand atom of XS type t at p: the coordinate is clamped into the
+// box (penalty slope * distance outside, gradient slope * sign there), every
+// receptor atom of the clamped point's cell within the cutoff contributes its
+// type-pair table value (interpolated e and dor with a gradient, the fast
+// table without), then curl; the pair gradient is kept in clamped directions
+// as in the reference.  The type pair uses t itself (get_type_pair_index).
+__device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p, float v, float3* d) {
+    const RecDev& r = ka.rec;
+    const float slope = ka.grid.slope;
+    float a[3] = {p.x, p.y, p.z}, oob[3] = {0.f, 0.f, 0.f};
+    float pen = 0.f;
+    int c[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (a[i] < r.init[i]) {
+            pen += r.init[i] - a[i];
+            a[i] = r.init[i];
+            oob[i] = -1.f;
+        } else if (a[i] > r.end[i]) {
+            pen += a[i] - r.end[i];
+            a[i] = r.end[i];
+            oob[i] = 1.f;
+        }
+        // szv_grid::possibilities (szv_grid.cpp:65-74) with fl_to_sz (common.h:205-211)
+        const float x = (a[i] - r.init[i]) * float(r.dim[i]) / r.range[i];
+        c[i] = (x <= 0.f) ? 0 : (x >= float(r.dim[i] - 1)) ? r.dim[i] - 1 : max(0, min(int(x), r.dim[i] - 1));
+    }
+    const int cell = c[0] + r.dim[0] * (c[1] + r.dim[1] * c[2]);
+    const int k0 = r.cell_start[cell], k1 = r.cell_start[cell + 1];
+    float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    for (int k = k0; k < k1; ++k) {
+        const float4 b = r.atoms[k];
+        const float rx = a[0] - b.x, ry = a[1] - b.y, rz = a[2] - b.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 < 64.f) {
+            const int t2 = __float_as_int(b.w);
+            const size_t base = size_t(t <= t2 ? t + t2 * (t2 + 1) / 2 : t2 + t * (t + 1) / 2) * kTableStride;
+            const float rf = 32.f * r2;
+            const int it = int(rf);
+            if (d) {
+                const float rem = rf - float(it);
+                const float2 t0 = ka.ls.table_deriv[base + it], t1 = ka.ls.table_deriv[base + it + 1];
+                e += t0.x + rem * (t1.x - t0.x);
+                const float dor = t0.y + rem * (t1.y - t0.y);
+                gx += dor * rx; gy += dor * ry; gz += dor * rz;
+            } else {
+                e += ka.ls.table_fast[base + it];
+            }
+        }
+    }
+    if (e > 0.f && v < 0.1f * kMaxFl) {
+        const float tmp = (v < kEpsFl) ? 0.f : (v / (v + e));
+        e *= tmp;
+        const float t2 = tmp * tmp;
+        gx *= t2; gy *= t2; gz *= t2;
+    }
+    if (d) *d = make_float3(gx + slope * oob[0], gy + slope * oob[1], gz + slope * oob[2]);
+    return e + slope * pen;
+}
+
 // Compile-time tuning of the search kernel: load batching and the occupancy
 // target handed to the register allocator (waves per SIMD; 1 = no target).
 // Full-wavefront groups (64 lanes) always target 4 waves per SIMD: with one
@@ -422,6 +489,21 @@ __device__ __forceinline__ float atoms_grid(const KArgs& ka, const Grp<L, NM>& s
     const int c = (N + L - 1) / L;
     const int i0 = s.gl * c, i1 = min(N, i0 + c);
     float e = 0.f;
+    if constexpr (GF == kGridExplicit) {
+        for (int i = i0; i < i1; ++i) {
+            const AtomRec a = s.atoms[i];
+            const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
+            coords[i] = make_float4(p.x, p.y, p.z, 0.f);
+            if ((a.meta >> 16) & 1) {
+                const float dx = p.x - fr[0], dy = p.y - fr[1], dz = p.z - fr[2];
+                rg2 += dx * dx + dy * dy + dz * dz;
+            }
+            float3 d = make_float3(0.f, 0.f, 0.f);
+            if (((a.meta >> 8) & 0xff) != 0xff) e += explicit_term(ka, int(a.meta >> 24), p, v_inter, want_grad ? &d : nullptr);
+            if (want_grad) mf[i] = make_float4(d.x, d.y, d.z, 0.f);
+        }
+        return e;
+    }
     for (int ib = i0; ib < i1; ib += kAtomBatch) {
         GridPt q[kAtomBatch];
         int slot[kAtomBatch];
@@ -497,9 +579,11 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
         const int slot = (a.meta >> 8) & 0xff;
         GridPt gq;
         float gf[8];
-        if (slot != 0xff) {
-            gq = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
-            grid_gather<GF>(ka.grid, gq.base, gf);
+        if constexpr (GF != kGridExplicit) {
+            if (slot != 0xff) {
+                gq = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
+                grid_gather<GF>(ka.grid, gq.base, gf);
+            }
         }
         const uint32_t ap = apl[i];
         const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
@@ -551,7 +635,11 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
             }
         }
         float3 d = make_float3(0.f, 0.f, 0.f);
-        if (slot != 0xff) e += grid_finish<GF>(ka.grid, slot, gq, gf, v_inter, &d);
+        if constexpr (GF == kGridExplicit) {
+            if (slot != 0xff) e += explicit_term(ka, int(a.meta >> 24), make_float3(ci.x, ci.y, ci.z), v_inter, &d);
+        } else {
+            if (slot != 0xff) e += grid_finish<GF>(ka.grid, slot, gq, gf, v_inter, &d);
+        }
         mf[i] = make_float4(d.x + gx, d.y + gy, d.z + gz, 0.f);
     }
     return e;
@@ -588,10 +676,14 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         float3 d = make_float3(0.f, 0.f, 0.f);
         if (slot != 0xff) {
             const float4 ci = coords[i];
-            const GridPt gq = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
-            float gf[8];
-            grid_gather<GF>(ka.grid, gq.base, gf);
-            e += grid_finish<GF>(ka.grid, slot, gq, gf, v_inter, &d);
+            if constexpr (GF == kGridExplicit) {
+                e += explicit_term(ka, int(a.meta >> 24), make_float3(ci.x, ci.y, ci.z), v_inter, &d);
+            } else {
+                const GridPt gq = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
+                float gf[8];
+                grid_gather<GF>(ka.grid, gq.base, gf);
+                e += grid_finish<GF>(ka.grid, slot, gq, gf, v_inter, &d);
+            }
         }
         mf[i] = make_float4(d.x, d.y, d.z, 0.f);
     }
@@ -1473,8 +1565,10 @@ size_t lds_bytes(int lanes, const LaunchCap& cap) {
     return l.bytes(kWG / lanes);
 }
 
-static KArgs make_kargs(const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap) {
+static KArgs make_kargs(const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap, const RecDev* rec = nullptr) {
     KArgs k;
+    std::memset(&k.rec, 0, sizeof(k.rec));
+    if (rec) k.rec = *rec;
     k.ls = ls;
     k.grid = g;
     k.na_cap = cap.n_atoms;
@@ -1535,6 +1629,9 @@ hipError_t launch_score(int lanes, int gf, const LigandSetDev& ls, const GridDev
 }
 
 
+// NM must be kNSmall exactly when the launch's ligands have at most
+// kNSmall - 6 torsions (5 + nodes <= kNSmall): Layout::init allocates no LDS
+// inverse Hessian for such launches at <= 32 lanes, which keep it in registers.
 template <int L, int GF, bool MC, int NM>
 static void launch_search(const KArgs& ka, const SearchDev& sp, const WorkItem* work, int n_work, size_t shm,
                           hipStream_t st, const int32_t* perm, float* poses, float v_intra, float v_inter,
@@ -1575,6 +1672,53 @@ hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const Grid
     UD_DISPATCH_L(lanes, {
         launch_search_nm<L, false>(gf, cap, ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter, max_steps,
                                    nullptr, e_out, nullptr, evals_out);
+    });
+    return hipGetLastError();
+}
+
+// BFGS of given poses against the explicit receptor (the refinement of
+// Vina::global_search_gpu, vina.h:249-272); ka.grid.slope is the pass's slope
+hipError_t launch_refine(int lanes, const LigandSetDev& ls, const GridDev& g, const RecDev& rec, const LaunchCap& cap,
+                         const WorkItem* work, int n_work, const int32_t* perm, float* poses, float v_intra,
+                         float v_inter, int max_steps, float* e_out, int32_t* evals_out, hipStream_t st) {
+    const KArgs ka = make_kargs(ls, g, cap, &rec);
+    const size_t shm = lds_bytes(lanes, cap);
+    SearchDev sp = {};
+    // the vector capacity must follow the LDS layout's choice (Layout::init):
+    // small launches keep the inverse Hessian in registers at <= 32 lanes
+    const bool small = 6 + cap.n_nodes - 1 <= kNSmall;
+    UD_DISPATCH_L(lanes, {
+        if (small)
+            launch_search<L, kGridExplicit, false, kNSmall>(ka, sp, work, n_work, shm, st, perm, poses, v_intra,
+                                                            v_inter, max_steps, nullptr, e_out, nullptr, evals_out);
+        else
+            launch_search<L, kGridExplicit, false, kNMax>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter,
+                                                          max_steps, nullptr, e_out, nullptr, evals_out);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_score_explicit(int lanes, const LigandSetDev& ls, const GridDev& g, const RecDev& rec,
+                                 const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
+                                 const float* poses, float v_intra, float v_inter, float* e_out, float* g_out,
+                                 hipStream_t st) {
+    const KArgs ka = make_kargs(ls, g, cap, &rec);
+    const size_t shm = lds_bytes(lanes, cap);
+    UD_DISPATCH_L(lanes, {
+        hipLaunchKernelGGL((score_kernel<L, kGridExplicit>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
+                           v_intra, v_inter, e_out, g_out);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_terms_explicit(int lanes, const LigandSetDev& ls, const GridDev& g, const RecDev& rec,
+                                 const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
+                                 const float* poses, float v, float* inter_out, float* intra_out, hipStream_t st) {
+    const KArgs ka = make_kargs(ls, g, cap, &rec);
+    const size_t shm = lds_bytes(lanes, cap);
+    UD_DISPATCH_L(lanes, {
+        hipLaunchKernelGGL((terms_kernel<L, kGridExplicit>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
+                           v, inter_out, intra_out);
     });
     return hipGetLastError();
 }

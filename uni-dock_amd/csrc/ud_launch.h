@@ -42,6 +42,16 @@ hipError_t launch_energy_grad(int lanes, int gf, const LigandSetDev& ls, const G
                               const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
                               const float* poses, float v_intra, float v_inter, float* e_out, float* gpose_out,
                               float* gw_out, hipStream_t st);
+hipError_t launch_refine(int lanes, const LigandSetDev& ls, const GridDev& g, const RecDev& rec, const LaunchCap& cap,
+                         const WorkItem* work, int n_work, const int32_t* perm, float* poses, float v_intra,
+                         float v_inter, int max_steps, float* e_out, int32_t* evals_out, hipStream_t st);
+hipError_t launch_score_explicit(int lanes, const LigandSetDev& ls, const GridDev& g, const RecDev& rec,
+                                 const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
+                                 const float* poses, float v_intra, float v_inter, float* e_out, float* g_out,
+                                 hipStream_t st);
+hipError_t launch_terms_explicit(int lanes, const LigandSetDev& ls, const GridDev& g, const RecDev& rec,
+                                 const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
+                                 const float* poses, float v, float* inter_out, float* intra_out, hipStream_t st);
 hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap, const WorkItem* work,
                          int n_work, const int32_t* perm, const float* poses, float* coords_out, hipStream_t st);
 

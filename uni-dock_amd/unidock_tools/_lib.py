@@ -110,7 +110,10 @@ def load() -> ctypes.CDLL:
         "ud_last_launches": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(UdLaunchInfo), ip]),
         "ud_set_bucket_lanes": (ctypes.c_int, [vp, ip]),
         "ud_autotune": (ctypes.c_int, [vp, ctypes.POINTER(UdSearchParams), ctypes.c_int, ip]),
-        "ud_postprocess": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(UdMode), ip]),
+        "ud_postprocess": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(UdMode), ip]),
+        "ud_refine": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, ctypes.c_int, fp, ip]),
+        "ud_score_terms_receptor": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, fp]),
+        "ud_score_grad_receptor": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, ctypes.c_double, fp, fp]),
         "ud_ligset_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
         "ud_ligset_destroy": (ctypes.c_int, [vp]),
         "ud_ligset_last_error": (ctypes.c_char_p, [vp]),
@@ -465,6 +468,39 @@ class Context:
                                        _p(a, ctypes.c_float), _p(b, ctypes.c_float)))
         return a, b
 
+    def score_terms_receptor(self, pose_ligand, poses):
+        """Final-score components against the explicit receptor (ud_score_terms_receptor)."""
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        P = np.ascontiguousarray(poses, dtype=np.float32).reshape(len(pl), UD_POSE_FLOATS)
+        a = np.zeros(len(pl), dtype=np.float32)
+        b = np.zeros(len(pl), dtype=np.float32)
+        self._ck(self.L.ud_score_terms_receptor(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float),
+                                                _p(a, ctypes.c_float), _p(b, ctypes.c_float)))
+        return a, b
+
+    def score_grad_receptor(self, pose_ligand, poses, v=(1000.0, 1000.0, 1000.0), slope=1e6):
+        """Energy and change-space gradient against the explicit receptor (ud_score_grad_receptor)."""
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        P = np.ascontiguousarray(poses, dtype=np.float32).reshape(len(pl), UD_POSE_FLOATS)
+        vv = np.array(v, dtype=np.float32)
+        e = np.zeros(len(pl), dtype=np.float32)
+        g = np.zeros((len(pl), UD_GRAD_FLOATS), dtype=np.float32)
+        self._ck(self.L.ud_score_grad_receptor(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float),
+                                               _p(vv, ctypes.c_float), slope, _p(e, ctypes.c_float),
+                                               _p(g, ctypes.c_float)))
+        return e, g
+
+    def refine(self, pose_ligand, poses, refine_step=3):
+        """Refine poses against the explicit receptor (ud_refine); returns
+        (refined poses, energies, within flags)."""
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        P = np.array(poses, dtype=np.float32).reshape(len(pl), UD_POSE_FLOATS)
+        e = np.zeros(len(pl), dtype=np.float32)
+        w = np.zeros(len(pl), dtype=np.int32)
+        self._ck(self.L.ud_refine(self.h, len(pl), _p(pl, ctypes.c_int32), _p(P, ctypes.c_float), refine_step,
+                                  _p(e, ctypes.c_float), _p(w, ctypes.c_int32)))
+        return P, e, w.astype(bool)
+
     def pose_coords(self, pose_ligand, poses):
         pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
         P = np.ascontiguousarray(poses, dtype=np.float32).reshape(len(pl), UD_POSE_FLOATS)
@@ -473,12 +509,13 @@ class Context:
                                        _p(out, ctypes.c_float)))
         return out
 
-    def postprocess(self, max_modes=9, min_rmsd=1.0):
-        """Cluster and rescore the last dock's chains; one list of mode dicts per ligand."""
+    def postprocess(self, max_modes=9, min_rmsd=1.0, refine_step=3):
+        """Cluster, refine (refine_step > 0, the reference CLI default 3) and
+        rescore the last dock's chains; one list of mode dicts per ligand."""
         n = self.n_ligands
         modes = (UdMode * (n * max_modes))()
         cnt = np.zeros(n, dtype=np.int32)
-        self._ck(self.L.ud_postprocess(self.h, max_modes, min_rmsd, modes, _p(cnt, ctypes.c_int32)))
+        self._ck(self.L.ud_postprocess(self.h, max_modes, min_rmsd, refine_step, modes, _p(cnt, ctypes.c_int32)))
         out = []
         for l in range(n):
             lst = []
