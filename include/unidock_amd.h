@@ -337,6 +337,17 @@ int ud_ligset_destroy(ud_ligset* set);
 const char* ud_ligset_last_error(const ud_ligset* set);
 /* Parse one ligand (a single MODEL); index_out receives its index. */
 int ud_ligset_add_pdbqt(ud_ligset* set, const char* text, int32_t* index_out);
+/* Add every record of SDF text (records end at "$$$$" lines) as a ligand:
+ * parse_sdf_aux / parse_sdf_branch (parse_pdbqt.cpp:376-420, 611-864,
+ * 962-1000).  Atoms come from the V2000 atom block; the torsion tree from the
+ * fragInfo / torsionInfo data fields (fragment 0 is the root; a torsion line
+ * "a b fa fb" joins atom a of fragment fa to atom b of fragment fb), AutoDock
+ * types from atomInfo where present, else from the element; a record without
+ * fragInfo is rigid.  Bonds, X-Score types, pairs and the torsion count are
+ * built as for PDBQT; hydrogens are kept.  Either every record is added or
+ * none: first_index_out / count_out (may be NULL) receive the index of the
+ * first added ligand and the number of records. */
+int ud_ligset_add_sdf(ud_ligset* set, const char* text, int32_t* first_index_out, int32_t* count_out);
 int ud_ligset_count(const ud_ligset* set);
 /* Fill *batch with arrays owned by the set (valid until the next add or
  * destroy), ready for ud_set_ligands. */
@@ -356,6 +367,17 @@ int ud_ligset_atoms(const ud_ligset* set, int lig, int32_t* n_atoms_out, float* 
  * UD_ERR_CAPACITY if cap is too small. */
 int ud_ligset_write_pdbqt(const ud_ligset* set, int lig, int n_modes, const ud_mode* modes, const float* coords,
                           int how_many, double energy_range, char* out, size_t cap, size_t* len_out);
+
+/* SDF text of ligand `lig` (read from SDF) with its modes
+ * (Vina::get_sdf_poses_gpu and sdf_remarks, vina.cpp:751-770, 925-970;
+ * model::write_sdf_model, model.cpp:758-862): per mode the record's input
+ * lines with atom coordinates from coords[mode][atom][3] (batch atom order)
+ * written into columns 1-30, the "> <Uni-Dock RESULT>" field (ENERGY /
+ * LOWER_BOUND / UPPER_BOUND, INTER + INTRA, INTER, INTRA, UNBOUND) and
+ * "$$$$".  Stops like ud_ligset_write_pdbqt.  UD_ERR_STATE for a ligand read
+ * from PDBQT (and ud_ligset_write_pdbqt returns it for one read from SDF). */
+int ud_ligset_write_sdf(const ud_ligset* set, int lig, int n_modes, const ud_mode* modes, const float* coords,
+                        int how_many, double energy_range, char* out, size_t cap, size_t* len_out);
 
 /* Receptor heavy atoms with X-Score types from PDBQT text (the rigid
  * receptor of Vina::set_receptor, typed like the ligand: bonds decide C_P /

@@ -568,3 +568,36 @@ def test_redock_1iep_from_reference_fixture():
         _say(f"[gpu] 1iep sf={sf}: crystal pose {e0[0]:.3f} -> optimised {eo[0]:.3f} kcal/mol, moved {r0:.2f} A")
         assert r0 < 1.0 and eo[0] < e0[0]
         ctx.close()
+
+
+def test_redock_1a30_sdf_from_reference_fixture():
+    """The reference's SDF test (unidock/test/ut/test_unidock_sdf.py): the 1a30
+    ligand (SDF with fragInfo / torsionInfo / atomInfo) docked into
+    1a30_protein.pdbqt, box centre (8.729, 25.62, 4.682), size (19.12, 16.56,
+    18.65), fast search, seed 181129, one mode written as SDF; the ENERGY of
+    the "> <Uni-Dock RESULT>" field lies in [-20, -1] kcal/mol, for Vina and
+    Vinardo."""
+    import os
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    rec_xyz, rec_xs = _lib.receptor_from_pdbqt(open(os.path.join(golden, "1a30_protein.pdbqt")).read())
+    ls = _lib.LigandSet()
+    ls.add_sdf(open(os.path.join(golden, "1a30_ligand.sdf")).read())
+    lig = ls.ligands()[0]
+    for sf in (_lib.UD_SF_VINA, _lib.UD_SF_VINARDO):
+        _say(f"[gpu] 1a30 SDF redock sf={sf}")
+        ctx = _lib.Context(0)
+        ctx.set_scoring(sf)
+        ctx.set_box((8.729, 25.62, 4.682), (19.12, 16.56, 18.65), 0.375)
+        types = sorted({_lib.grid_type(int(t)) for t in lig.xs} - {-1})
+        ctx.compute_maps(rec_xyz, rec_xs, types, _lib.UD_GRID_F32)
+        ctx.set_ligands(ls.arrays())
+        ctx.dock(exhaustiveness=128, global_steps=20, seed=181129)
+        modes = ctx.postprocess(max_modes=1, min_rmsd=1.0)[0]
+        xyz = ctx.pose_coords(np.zeros(len(modes), np.int32), np.array([m["pose"] for m in modes]))
+        text = ls.write_sdf(0, modes, xyz[:, :lig.n_atoms], how_many=1)
+        lines = text.splitlines()
+        score_line = lines[lines.index("> <Uni-Dock RESULT> ") + 1]
+        score = float(score_line[len("ENERGY="):].split()[0])
+        _say(f"[gpu] 1a30 sf={sf}: {score_line}")
+        assert -20.0 <= score <= -1.0
+        ctx.close()

@@ -57,21 +57,31 @@ def _bonds(coords, heavy, ad_cov):
 
 
 def test_pair_rule(lig):
-    """Pairs: atoms in different rigid bodies (a branch's first axis atom
-    belongs with the branch), more than 3 bonds apart, both typed."""
+    """Pairs (model::initialize_pairs, model.cpp:592-630): atoms of variable
+    mobility, more than 3 bonds apart, both typed.  Mobility as the reference
+    builds it from the BRANCH records (postprocess_branch,
+    parse_pdbqt.cpp:498-533): the second atom of "BRANCH a b" lies on the
+    rotation axis and is stored with the parent's atoms; a and b are fixed
+    relative to every atom stored with the branch."""
     node = np.zeros(lig.n_atoms, dtype=int)
     for k, (a, b) in enumerate(zip(lig.abegin, lig.aend)):
         node[a:b] = k
-    cov = np.where(lig.heavy == 1, 0.76, 0.37)  # C / N / O radii 0.73-0.77, H 0.37
-    bonds = _bonds(lig.coords, lig.heavy, cov)
-    axis_begin = {}
-    for k in range(1, len(lig.parent)):
-        first = lig.abegin[k]
-        axis_begin[k] = min((j for j in bonds[first] if node[j] == lig.parent[k]),
-                            key=lambda j: np.sum((lig.coords[j] - lig.coords[first]) ** 2))
+    idx = {int(s): i for i, s in enumerate(lig.serial)}
+    branches = [tuple(int(x) for x in ln.split()[1:3]) for ln in LIGAND.splitlines() if ln.startswith("BRANCH")]
+    body = node.copy()
+    axis = {}
+    for a, b in branches:
+        ia, ib = idx[a], idx[b]
+        k = node[ib]  # the branch's own rigid body
+        body[ib] = node[ia]
+        axis[k] = {ia, ib}
+    assert len(axis) == len(lig.parent) - 1
 
     def fixed(i, j):
-        return node[i] == node[j] or axis_begin.get(node[j]) == i or axis_begin.get(node[i]) == j
+        return body[i] == body[j] or i in axis.get(body[j], ()) or j in axis.get(body[i], ())
+
+    cov = np.where(lig.heavy == 1, 0.76, 0.37)  # C / N / O radii 0.73-0.77, H 0.37
+    bonds = _bonds(lig.coords, lig.heavy, cov)
 
     def within3(i):
         seen, front = {i}, {i}
@@ -156,3 +166,173 @@ def test_write_pdbqt_round_trip(lig):
     back = t.ligands()[0]
     np.testing.assert_allclose(back.coords, shifted[1], atol=6e-4)
     assert back.pairs.tolist() == lig.pairs.tolist() and back.xs.tolist() == lig.xs.tolist()
+
+
+# ---------------------------------------------------------------- SDF (unidock/test/ligands/1a30_ligand.sdf)
+SDF = open(os.path.join(GOLDEN, "1a30_ligand.sdf")).read()
+
+
+def _sdf_fields(text):
+    """fragInfo, torsionInfo and atomInfo of the fixture, read directly."""
+    out, cur = {}, None
+    for ln in text.splitlines():
+        if ln.startswith(">"):
+            cur = ln.split("<")[1].split(">")[0]
+            out[cur] = []
+        elif cur is not None:
+            if not ln.strip():
+                cur = None
+            else:
+                out[cur].append(ln)
+    frags = [[int(x) for x in ln.split()] for ln in out["fragInfo"]]
+    tors = [[int(x) for x in ln.split()] for ln in out["torsionInfo"]]
+    info = {int(ln[:3].split()[0]): ln[13:].strip() for ln in out["atomInfo"]}
+    return frags, tors, info
+
+
+@pytest.fixture(scope="module")
+def sdf_lig():
+    s = _lib.LigandSet()
+    assert s.add_sdf(SDF) == (0, 1)
+    return s.ligands()[0]
+
+
+def test_sdf_tree_from_fragments(sdf_lig):
+    """The tree follows fragInfo / torsionInfo: fragment 0 is the root, each
+    torsion opens the fragment on its far side; atoms in no fragment (here the
+    non-polar hydrogens) are not part of the ligand."""
+    frags, tors, info = _sdf_fields(SDF)
+    lig = sdf_lig
+    assert lig.n_atoms == sum(len(f) for f in frags) == 31
+    assert lig.n_tors == len(tors) == 13
+    nodes = [set(lig.serial[a:b].tolist()) for a, b in zip(lig.abegin, lig.aend)]
+    assert sorted(map(sorted, nodes)) == sorted(map(sorted, frags))
+    assert nodes[0] == set(frags[0])
+    # every torsion joins a node to its parent, the far atom being in the child
+    frag_of = {a: i for i, f in enumerate(frags) for a in f}
+    node_of_frag = {i: next(k for k, n in enumerate(nodes) if n == set(f)) for i, f in enumerate(frags)}
+    for a, b, fa, fb in tors:
+        assert frag_of[a] == fa and frag_of[b] == fb
+        ka, kb = node_of_frag[fa], node_of_frag[fb]
+        assert lig.parent[kb] == ka or lig.parent[ka] == kb
+    # pre-order: parents before children
+    assert all(lig.parent[k] < k for k in range(1, len(lig.parent)))
+    assert set(lig.serial.tolist()) == set(info)  # atomInfo lists exactly the kept atoms
+
+
+def test_sdf_types_from_atominfo(sdf_lig):
+    frags, tors, info = _sdf_fields(SDF)
+    got = dict(zip(sdf_lig.serial.tolist(), sdf_lig.xs.tolist()))
+    for serial, ad in info.items():
+        if ad in ("HD", "H"):
+            assert got[serial] == -1
+        elif ad == "OA":
+            assert got[serial] == O_A  # carboxylate / carbonyl oxygens: acceptors
+        elif ad == "N":
+            assert got[serial] == N_D  # each nitrogen carries a polar hydrogen
+        else:
+            assert ad == "C" and got[serial] in (C_H, C_P)
+    # carbons bonded to N / O are polar
+    assert got[11] == C_P and got[3] == C_P and got[22] == C_H and got[24] == C_H
+
+
+def test_sdf_identity_pose_reproduces_input(sdf_lig):
+    lig = sdf_lig
+    o = Oracle(64, 0)
+    o.set_grid(lig.origin[0], np.array([30.0, 30.0, 30.0]), 0.375)
+    o.add_ligand(lig)
+    pose = np.zeros(39)
+    pose[0:3] = lig.origin[0]
+    pose[3] = 1.0
+    _, _, coords, _ = o.eval_deriv(0, pose)
+    np.testing.assert_allclose(coords, lig.coords, atol=2e-4)
+    # the root is atom 11, the first (and only) atom of fragment 0
+    np.testing.assert_allclose(lig.origin[0], lig.coords[list(lig.serial).index(11)])
+
+
+def test_sdf_pair_rule(sdf_lig):
+    """As for PDBQT, with the branch axis atoms from torsionInfo: the far atom
+    of each torsion is stored with the near fragment's atoms."""
+    frags, tors, info = _sdf_fields(SDF)
+    lig = sdf_lig
+    idx = {int(s): i for i, s in enumerate(lig.serial)}
+    node = np.zeros(lig.n_atoms, dtype=int)
+    for k, (a, b) in enumerate(zip(lig.abegin, lig.aend)):
+        node[a:b] = k
+    body = node.copy()
+    axis = {}
+    for a, b, fa, fb in tors:
+        ia, ib = idx[a], idx[b]
+        if lig.parent[node[ia]] == node[ib]:  # oriented parent -> child
+            ia, ib = ib, ia
+        body[ib] = node[ia]
+        axis[node[ib]] = {ia, ib}
+    cov = np.where(lig.heavy == 1, 0.76, 0.37)
+    bonds = _bonds(lig.coords, lig.heavy, cov)
+
+    def fixed(i, j):
+        return body[i] == body[j] or i in axis.get(body[j], ()) or j in axis.get(body[i], ())
+
+    def within3(i):
+        seen, front = {i}, {i}
+        for _ in range(3):
+            front = {v for u in front for v in bonds[u]} - seen
+            seen |= front
+        return seen
+
+    expect = [(i, j) for i in range(lig.n_atoms) for j in range(i + 1, lig.n_atoms)
+              if not fixed(i, j) and j not in within3(i) and lig.xs[i] >= 0 and lig.xs[j] >= 0]
+    assert sorted(map(tuple, lig.pairs.tolist())) == expect
+
+
+def test_sdf_write_round_trip(sdf_lig):
+    s = _lib.LigandSet()
+    s.add_sdf(SDF)
+    lig = sdf_lig
+    modes = [dict(energy=-7.25, inter=-9.0, intra=-0.75, rmsd_lb=0.0, rmsd_ub=0.0, unbound=-0.5),
+             dict(energy=-6.5, inter=-8.0, intra=-0.8, rmsd_lb=1.25, rmsd_ub=2.5, unbound=-0.5)]
+    coords = np.stack([lig.coords, lig.coords + np.float32([0.5, -1.0, 2.0])]).astype(np.float32)
+    text = s.write_sdf(0, modes, coords, how_many=9, energy_range=3.0)
+    recs = text.split("$$$$\n")
+    assert len(recs) == 3 and recs[-1] == ""
+    first = recs[0].splitlines()
+    k = first.index("> <Uni-Dock RESULT> ")
+    assert first[k + 1] == "ENERGY=   -7.250  LOWER_BOUND=    0.000  UPPER_BOUND=    0.000"
+    assert first[k + 2] == "INTER + INTRA:          -9.750"
+    assert first[k + 5] == "UNBOUND:                -0.500"
+    # the input record comes back unchanged with the input coordinates
+    src = [ln for ln in SDF.splitlines() if "$$$$" not in ln]
+    assert first[:k] == src
+    # the second record parses back with its shifted coordinates
+    t = _lib.LigandSet()
+    t.add_sdf(recs[1])
+    back = t.ligands()[0]
+    np.testing.assert_allclose(back.coords, coords[1], atol=6e-5)
+    assert back.pairs.tolist() == lig.pairs.tolist() and back.xs.tolist() == lig.xs.tolist()
+    # atoms outside the ligand (non-polar hydrogens) keep their input coordinates
+    assert recs[1].splitlines()[4 + 29] == src[4 + 29]
+    with pytest.raises(_lib.UdError) as ei:
+        s.write_pdbqt(0, modes, coords)
+    assert ei.value.code == _lib.UD_ERR_STATE
+
+
+def test_sdf_without_fragments_is_rigid():
+    text = SDF.split(">  <fragInfo>")[0] + "$$$$\n"
+    s = _lib.LigandSet()
+    s.add_sdf(text)
+    lig = s.ligands()[0]
+    assert lig.n_atoms == 49 and lig.n_tors == 0 and lig.serial.tolist() == list(range(1, 50))
+    assert lig.pairs.size == 0
+
+
+def test_sdf_multiple_records_and_errors():
+    s = _lib.LigandSet()
+    assert s.add_sdf(SDF + SDF) == (0, 2)
+    assert len(s) == 2
+    bad = SDF.replace("11 10 0 6", "11 10 0 60")
+    with pytest.raises(_lib.UdError) as ei:
+        s.add_sdf(SDF + bad)
+    assert "record 2" in str(ei.value)
+    assert len(s) == 2  # all or nothing
+    with pytest.raises(_lib.UdError):
+        s.add_sdf(SDF.replace(" 49 48  0", " xx 48  0"))

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <sstream>
 #include <string>
@@ -211,15 +212,23 @@ void assign_types(std::vector<Atom>& at, const std::vector<std::vector<int>>& bo
 struct Node {
     int parent = -1;
     int axis_begin = -1;  // atom (in the parent) the rotatable bond starts from
-    int axis_end = -1;    // first atom of the branch: the node's origin
-    std::vector<int> atoms;  // file order
-    int file_order = 0;
+    int axis_end = -1;    // the branch's atom on the other end of that bond: the node's origin
+    std::vector<int> atoms;  // input order
+};
+
+// A parsed torsion tree: atoms in input order, nodes in creation order (root
+// first), each branch created after its parent.
+struct Tree {
+    std::vector<Atom> fa;
+    std::vector<Node> nodes;
+    int torsdof = -1;
 };
 
 struct Ligand {
-    std::vector<std::string> lines;  // input lines (trailing blank lines dropped)
-    std::vector<int> line_atom;      // batch atom index of an ATOM / HETATM line, else -1
-    std::vector<Atom> atoms;  // reference order (nodes in pre-order, file order inside)
+    int format = 0;                  // 0 PDBQT, 1 SDF
+    std::vector<std::string> lines;  // input lines kept for output (trailing blank lines dropped)
+    std::vector<int> line_atom;      // batch atom index of an atom line, else -1
+    std::vector<Atom> atoms;  // reference order (nodes in pre-order, input order inside)
     std::vector<int> node_parent, node_abegin, node_aend;
     std::vector<float> node_origin, node_axis;
     std::vector<int> pairs;
@@ -227,13 +236,14 @@ struct Ligand {
     int torsdof = 0;
 };
 
-bool build_ligand(const std::string& text, Ligand& L, std::string& err) {
-    std::vector<Atom> fa;  // file order
-    std::vector<Node> nodes;
+// ROOT / BRANCH blocks of a PDBQT ligand (parse_pdbqt_aux / parse_pdbqt_branch,
+// parse_pdbqt.cpp:440-540).
+bool parse_pdbqt_tree(const std::string& text, Tree& T, std::string& err) {
+    std::vector<Atom>& fa = T.fa;
+    std::vector<Node>& nodes = T.nodes;
     std::vector<int> stack;
     std::map<int, int> by_serial;
     bool root_seen = false;
-    int torsdof = -1;
     std::istringstream in(text);
     std::string ln;
     while (std::getline(in, ln)) {
@@ -265,14 +275,13 @@ bool build_ligand(const std::string& text, Ligand& L, std::string& err) {
             nd.parent = fa[it->second].node;
             nd.axis_begin = it->second;
             nd.axis_end = b;  // serial for now, resolved after parsing
-            nd.file_order = int(nodes.size());
             nodes.push_back(nd);
             stack.push_back(int(nodes.size()) - 1);
         } else if (ln.rfind("ENDBRANCH", 0) == 0) {
             if (stack.empty() || stack.back() == 0) { err = "misplaced ENDBRANCH"; return false; }
             stack.pop_back();
         } else if (ln.rfind("TORSDOF", 0) == 0) {
-            torsdof = std::atoi(ln.c_str() + 7);
+            T.torsdof = std::atoi(ln.c_str() + 7);
         } else if (ln.rfind("MODEL", 0) == 0) {
             err = "multi-MODEL ligand text: split it first";
             return false;
@@ -282,15 +291,23 @@ bool build_ligand(const std::string& text, Ligand& L, std::string& err) {
     if (!stack.empty()) { err = "unterminated ROOT or BRANCH"; return false; }
     for (size_t k = 1; k < nodes.size(); ++k) {
         auto it = by_serial.find(nodes[k].axis_end);
-        if (it == by_serial.end() || fa[it->second].node != int(k) || nodes[k].atoms.empty() ||
-            nodes[k].atoms.front() != it->second) {
-            err = "BRANCH must start with its second axis atom";
+        if (it == by_serial.end() || fa[it->second].node != int(k)) {
+            err = "atom " + std::to_string(nodes[k].axis_end) + " has not been found in this branch";
             return false;
         }
         nodes[k].axis_end = it->second;
     }
+    return true;
+}
+
+// The model the reference builds from a parsed tree (postprocess_ligand /
+// postprocess_branch, parse_pdbqt.cpp:498-566, then model::assign_bonds,
+// assign_types, initialize_pairs and the conf-independent torsion count).
+bool finish_ligand(const Tree& T, Ligand& L, std::string& err) {
+    const std::vector<Atom>& fa = T.fa;
+    const std::vector<Node>& nodes = T.nodes;
     // pre-order: a node's children ordered by the position of their first axis
-    // atom in the node, then file order (postprocess_branch, parse_pdbqt.cpp:498-533)
+    // atom in the node, then creation order (postprocess_branch, parse_pdbqt.cpp:498-533)
     std::vector<std::vector<int>> kids(nodes.size());
     for (size_t k = 1; k < nodes.size(); ++k) kids[nodes[k].parent].push_back(int(k));
     for (size_t k = 0; k < nodes.size(); ++k) {
@@ -306,7 +323,7 @@ bool build_ligand(const std::string& text, Ligand& L, std::string& err) {
         order.push_back(k);
         for (auto it = kids[k].rbegin(); it != kids[k].rend(); ++it) st.push_back(*it);
     }
-    std::vector<int> new_node(nodes.size()), new_atom(fa.size());
+    std::vector<int> new_node(nodes.size()), new_atom(fa.size(), -1);
     for (size_t k = 0; k < order.size(); ++k) new_node[order[k]] = int(k);
     L.atoms.clear();
     for (int k : order) {
@@ -322,8 +339,9 @@ bool build_ligand(const std::string& text, Ligand& L, std::string& err) {
         L.node_parent.push_back(k == 0 ? -1 : new_node[nodes[k].parent]);
     }
     const int n = int(L.atoms.size());
-    // node origins and axes: the root at its first atom, a branch at its first
-    // atom with the axis from the parent's bond atom (segment, tree.h:101-131)
+    // node origins and axes: the root at its first atom, a branch at its
+    // second axis atom with the axis from the parent's bond atom (segment,
+    // tree.h:101-131; postprocess_ligand / postprocess_branch)
     for (int k : order) {
         const Atom& o = (k == 0) ? fa[nodes[0].atoms.front()] : fa[nodes[k].axis_end];
         L.node_origin.insert(L.node_origin.end(), {o.x, o.y, o.z});
@@ -337,19 +355,21 @@ bool build_ligand(const std::string& text, Ligand& L, std::string& err) {
             L.node_axis.insert(L.node_axis.end(), {ax / nr, ay / nr, az / nr});
         }
     }
-    // mobility: same node fixed; a branch's first axis atom is fixed relative to
-    // the branch's atoms, the axis pair itself is the rotor (parse_pdbqt.cpp:455-496)
+    // mobility (parse_pdbqt.cpp:498-533): the reference stores a branch's
+    // second axis atom with its parent's atoms (it lies on the rotation axis,
+    // node::insert_immobiles), marks both axis atoms fixed relative to every
+    // atom stored in the branch (add_bonds) and the axis pair as the rotor.
     const int nn = int(order.size());
     std::vector<int> axis_b(nn, -1), axis_e(nn, -1);
     for (int k = 1; k < int(nodes.size()); ++k) {
         axis_b[new_node[k]] = new_atom[nodes[k].axis_begin];
         axis_e[new_node[k]] = new_atom[nodes[k].axis_end];
     }
-    auto fixed = [&](int i, int j) {
-        const int ni = L.atoms[i].node, nj = L.atoms[j].node;
-        if (ni == nj) return true;
-        return axis_b[nj] == i || axis_b[ni] == j;
-    };
+    std::vector<int> body(n);
+    for (int i = 0; i < n; ++i) body[i] = L.atoms[i].node;
+    for (int k = 1; k < nn; ++k) body[axis_e[k]] = L.node_parent[k];
+    auto axis_of = [&](int i, int k) { return k > 0 && (i == axis_b[k] || i == axis_e[k]); };
+    auto fixed = [&](int i, int j) { return body[i] == body[j] || axis_of(i, body[j]) || axis_of(j, body[i]); };
     std::vector<std::vector<int>> bonds;
     assign_bonds(L.atoms, fixed, bonds);
     assign_types(L.atoms, bonds);
@@ -426,27 +446,194 @@ bool build_ligand(const std::string& text, Ligand& L, std::string& err) {
         nt += 0.5f * float(ar);
     }
     L.num_tors = nt;
-    L.torsdof = torsdof < 0 ? int(nodes.size()) - 1 : torsdof;
+    L.torsdof = T.torsdof < 0 ? int(nodes.size()) - 1 : T.torsdof;
+    return true;
+}
+
+bool build_ligand(const std::string& text, Ligand& L, std::string& err) {
+    Tree T;
+    if (!parse_pdbqt_tree(text, T, err) || !finish_ligand(T, L, err)) return false;
+    L.format = 0;
     // the input lines, for writing poses back out
-    {
-        std::istringstream in2(text);
-        std::string l2;
-        std::map<int, int> batch_of_serial;
-        for (int i = 0; i < n; ++i) batch_of_serial[L.atoms[i].serial] = i;
-        while (std::getline(in2, l2)) {
-            if (!l2.empty() && l2.back() == '\r') l2.pop_back();
-            int idx = -1;
-            if (l2.rfind("ATOM", 0) == 0 || l2.rfind("HETATM", 0) == 0) {
-                const std::string sn = trim(cols(l2, 7, 11));
-                idx = batch_of_serial[std::atoi(sn.c_str())];
+    std::istringstream in2(text);
+    std::string l2;
+    std::map<int, int> batch_of_serial;
+    for (size_t i = 0; i < L.atoms.size(); ++i) batch_of_serial[L.atoms[i].serial] = int(i);
+    while (std::getline(in2, l2)) {
+        if (!l2.empty() && l2.back() == '\r') l2.pop_back();
+        int idx = -1;
+        if (l2.rfind("ATOM", 0) == 0 || l2.rfind("HETATM", 0) == 0) {
+            const std::string sn = trim(cols(l2, 7, 11));
+            idx = batch_of_serial[std::atoi(sn.c_str())];
+        }
+        L.lines.push_back(l2);
+        L.line_atom.push_back(idx);
+    }
+    while (!L.lines.empty() && trim(L.lines.back()).empty()) {
+        L.lines.pop_back();
+        L.line_atom.pop_back();
+    }
+    return true;
+}
+
+// whitespace-separated integers of one line of an SDF data field
+bool sdf_ints(const std::string& ln, std::vector<int>& out) {
+    out.clear();
+    std::istringstream ss(ln);
+    std::string tok;
+    while (ss >> tok) {
+        char* end = nullptr;
+        const long v = std::strtol(tok.c_str(), &end, 10);
+        if (*end) return false;
+        out.push_back(int(v));
+    }
+    return true;
+}
+
+// One SDF record (lines without the "$$$$" terminator) with the torsion tree
+// of its fragInfo / torsionInfo / atomInfo data fields (parse_sdf_aux,
+// parse_sdf_branch_aux / parse_sdf_branch, parse_pdbqt.cpp:129-145, 376-420,
+// 611-864, 962-1000): atoms from the V2000 atom block (coordinates in columns
+// 1-10, 11-20, 21-30, element at 32-33), AutoDock types from atomInfo where
+// given, the root = fragment 0 in its listed order, and each torsion
+// (a, b, fragment of a, fragment of b) that starts in an expanded fragment
+// opening the other fragment as a branch, depth first, fragments in their
+// listed order.  Without fragInfo the ligand is rigid (every atom in the
+// root, torsdof 0).  Bond lines are not read: bonds come from the geometry as
+// for PDBQT.  Hydrogens are kept (keep_nonpolar_H, main.cpp:412, 531).
+bool build_ligand_sdf(const std::vector<std::string>& lines, Ligand& L, std::string& err) {
+    if (lines.size() < 4) { err = "SDF record shorter than its header"; return false; }
+    const std::string& counts = lines[3];
+    char* end = nullptr;
+    const std::string sa = trim(cols(counts, 1, 3)), sb = trim(cols(counts, 4, 6));
+    const long n_atoms = std::strtol(sa.c_str(), &end, 10);
+    if (sa.empty() || *end || n_atoms <= 0) { err = "bad SDF counts line: " + counts; return false; }
+    const long n_bonds = std::strtol(sb.c_str(), &end, 10);
+    if (sb.empty() || *end || n_bonds < 0) { err = "bad SDF counts line: " + counts; return false; }
+    if (lines.size() < size_t(4 + n_atoms + n_bonds)) { err = "SDF atom / bond block truncated"; return false; }
+    Tree T;
+    std::vector<Atom> all(n_atoms);
+    for (long i = 0; i < n_atoms; ++i) {
+        const std::string& ln = lines[4 + i];
+        Atom& a = all[i];
+        a.serial = int(i + 1);
+        float v[3];
+        for (int d = 0; d < 3; ++d) {
+            const std::string c = trim(cols(ln, 1 + 10 * d, 10 + 10 * d));
+            v[d] = std::strtof(c.c_str(), &end);
+            if (c.empty() || *end) { err = "bad SDF coordinate: " + ln; return false; }
+        }
+        a.x = v[0]; a.y = v[1]; a.z = v[2];
+        std::string el = ln.size() > 32 ? ln.substr(31, 2) : (ln.size() > 31 ? ln.substr(31, 1) : "");
+        if (el.size() == 2 && el[1] == ' ') el = el.substr(0, 1);
+        a.ad = ad_of(el);
+        if (a.ad < 0) { err = "element " + el + " is not a valid AutoDock type: " + ln; return false; }
+    }
+    // data fields after "M  END"
+    size_t li = size_t(4 + n_atoms + n_bonds);
+    while (li < lines.size() && lines[li].find("M  END") == std::string::npos) ++li;
+    std::vector<std::vector<int>> frags, tors;
+    std::vector<int> ints;
+    for (++li; li < lines.size(); ++li) {
+        const std::string& hd = lines[li];
+        if (hd.empty() || hd[0] != '>') continue;
+        const bool info = hd.find("atomInfo") != std::string::npos || hd.find("atom_info") != std::string::npos;
+        const bool tor = !info && hd.find("torsion") != std::string::npos;
+        const bool frag = !info && !tor && hd.find("frag") != std::string::npos;
+        for (++li; li < lines.size() && !lines[li].empty(); ++li) {
+            const std::string& ln = lines[li];
+            if (info) {
+                const size_t sp = ln.find(' ');
+                const std::string sid = trim(cols(ln, 1, std::min<size_t>(sp == std::string::npos ? ln.size() : sp, 3)));
+                const long id = std::strtol(sid.c_str(), &end, 10);
+                if (sid.empty() || *end || id < 1 || id > n_atoms) { err = "bad atomInfo line: " + ln; return false; }
+                const std::string name = trim(ln.size() >= 14 ? ln.substr(13) : std::string());
+                const int ad = ad_of(name);
+                if (ad < 0) { err = "atom type " + name + " is not a valid AutoDock type: " + ln; return false; }
+                all[id - 1].ad = ad;
+            } else if (tor || frag) {
+                if (!sdf_ints(ln, ints) || ints.empty()) { err = "bad SDF data line: " + ln; return false; }
+                if (tor && ints.size() != 4) { err = "torsionInfo lines hold 4 numbers: " + ln; return false; }
+                (tor ? tors : frags).push_back(ints);
             }
-            L.lines.push_back(l2);
-            L.line_atom.push_back(idx);
         }
-        while (!L.lines.empty() && trim(L.lines.back()).empty()) {
-            L.lines.pop_back();
-            L.line_atom.pop_back();
+    }
+    for (Atom& a : all) a.el = kAd[a.ad].el;
+    if (frags.empty()) {
+        T.nodes.push_back(Node());
+        for (long i = 0; i < n_atoms; ++i) {
+            T.nodes[0].atoms.push_back(int(i));
+            T.fa.push_back(all[i]);
         }
+        T.torsdof = 0;
+    } else {
+        const int nf = int(frags.size());
+        for (const std::vector<int>& f : frags)
+            for (int id : f)
+                if (id < 1 || id > n_atoms) { err = "fragInfo names atom " + std::to_string(id) + " out of range"; return false; }
+        for (const std::vector<int>& t : tors)
+            if (t[0] < 1 || t[0] > n_atoms || t[1] < 1 || t[1] > n_atoms || t[2] < 0 || t[2] >= nf || t[3] < 0 ||
+                t[3] >= nf) {
+                err = "torsionInfo entry out of range";
+                return false;
+            }
+        std::vector<int> tree_of(n_atoms, -1);  // SDF atom -> tree atom
+        std::vector<char> been(nf, 0);
+        auto add_frag = [&](int f, int node, int to_id, int& to_idx) -> bool {
+            for (int id : frags[f]) {
+                if (tree_of[id - 1] >= 0) { err = "atom " + std::to_string(id) + " is in two fragments"; return false; }
+                tree_of[id - 1] = int(T.fa.size());
+                if (id == to_id) to_idx = int(T.fa.size());
+                Atom a = all[id - 1];
+                a.node = node;
+                T.nodes[node].atoms.push_back(int(T.fa.size()));
+                T.fa.push_back(a);
+            }
+            return true;
+        };
+        T.nodes.push_back(Node());
+        been[0] = 1;
+        int dummy = -1;
+        if (!add_frag(0, 0, -1, dummy)) return false;
+        // depth-first expansion, as parse_sdf_branch recurses inside its loops
+        std::function<bool(int, int)> expand = [&](int node, int f) -> bool {
+            for (int id : frags[f])
+                for (const std::vector<int>& t : tors) {
+                    int from, to, next;
+                    if (t[0] == id) { from = t[0]; to = t[1]; next = t[3]; }
+                    else if (t[1] == id) { from = t[1]; to = t[0]; next = t[2]; }
+                    else continue;
+                    if (been[next]) continue;
+                    been[next] = 1;
+                    Node nd;
+                    nd.parent = node;
+                    nd.axis_begin = tree_of[from - 1];
+                    T.nodes.push_back(nd);
+                    const int k = int(T.nodes.size()) - 1;
+                    int to_idx = -1;
+                    if (!add_frag(next, k, to, to_idx)) return false;
+                    if (to_idx < 0) { err = "atom " + std::to_string(to) + " has not been found in this branch"; return false; }
+                    T.nodes[k].axis_end = to_idx;
+                    if (!expand(k, next)) return false;
+                }
+            return true;
+        };
+        if (!expand(0, 0)) return false;
+        T.torsdof = int(tors.size());
+    }
+    if (!finish_ligand(T, L, err)) return false;
+    L.format = 1;
+    // SDF atom index -> batch atom index (the tree's atom order was changed by finish_ligand)
+    std::map<int, int> batch_of_serial;
+    for (size_t i = 0; i < L.atoms.size(); ++i) batch_of_serial[L.atoms[i].serial] = int(i);
+    for (size_t i = 0; i < lines.size(); ++i) {
+        int idx = -1;
+        if (i >= 4 && long(i) < 4 + n_atoms) {
+            auto it = batch_of_serial.find(int(i - 4 + 1));
+            if (it != batch_of_serial.end()) idx = it->second;
+        }
+        L.lines.push_back(lines[i]);
+        L.line_atom.push_back(idx);
     }
     return true;
 }
@@ -492,6 +679,50 @@ int ud_ligset_add_pdbqt(ud_ligset* s, const char* text, int32_t* index_out) {
     }
     s->ligs.push_back(std::move(L));
     if (index_out) *index_out = int32_t(s->ligs.size() - 1);
+    return UD_OK;
+}
+
+int ud_ligset_add_sdf(ud_ligset* s, const char* text, int32_t* first_index_out, int32_t* count_out) {
+    if (!s || !text) return UD_ERR_INVALID;
+    // records end at "$$$$" lines, which the reference does not keep
+    std::vector<std::vector<std::string>> recs(1);
+    std::istringstream in(text);
+    std::string ln;
+    while (std::getline(in, ln)) {
+        if (!ln.empty() && ln.back() == '\r') ln.pop_back();
+        if (ln.find("$$$$") != std::string::npos) {
+            recs.emplace_back();
+            continue;
+        }
+        recs.back().push_back(ln);
+    }
+    while (!recs.empty()) {  // drop a trailing empty record
+        bool blank = true;
+        for (const std::string& l : recs.back()) blank = blank && trim(l).empty();
+        if (!blank) break;
+        recs.pop_back();
+    }
+    if (recs.empty()) {
+        s->err = "no SDF record";
+        return UD_ERR_INVALID;
+    }
+    std::vector<Ligand> parsed(recs.size());
+    for (size_t r = 0; r < recs.size(); ++r) {
+        std::string err;
+        if (!build_ligand_sdf(recs[r], parsed[r], err)) {
+            s->err = "SDF record " + std::to_string(r + 1) + ": " + err;
+            return UD_ERR_INVALID;
+        }
+        const Ligand& L = parsed[r];
+        if (int(L.atoms.size()) > UD_MAX_ATOMS || int(L.node_parent.size()) - 1 > UD_MAX_TORSIONS ||
+            int(L.pairs.size() / 2) > UD_MAX_PAIRS) {
+            s->err = "SDF record " + std::to_string(r + 1) + ": ligand exceeds the library's capacities";
+            return UD_ERR_CAPACITY;
+        }
+    }
+    if (first_index_out) *first_index_out = int32_t(s->ligs.size());
+    if (count_out) *count_out = int32_t(parsed.size());
+    for (Ligand& L : parsed) s->ligs.push_back(std::move(L));
     return UD_OK;
 }
 
@@ -563,6 +794,7 @@ int ud_ligset_write_pdbqt(const ud_ligset* s, int lig, int n_modes, const ud_mod
         how_many < 0 || !(energy_range >= 0.0) || !len_out)
         return UD_ERR_INVALID;
     const Ligand& L = s->ligs[lig];
+    if (L.format != 0) return UD_ERR_STATE;  // the text written back is the PDBQT input's
     const size_t na = L.atoms.size();
     std::string txt;
     char buf[160];
@@ -598,6 +830,56 @@ int ud_ligset_write_pdbqt(const ud_ligset* s, int lig, int n_modes, const ud_mod
             txt += '\n';
         }
         txt += "ENDMDL\n";
+    }
+    *len_out = txt.size();
+    if (!out) return UD_OK;
+    if (cap < txt.size() + 1) return UD_ERR_CAPACITY;
+    std::memcpy(out, txt.c_str(), txt.size() + 1);
+    return UD_OK;
+}
+
+int ud_ligset_write_sdf(const ud_ligset* s, int lig, int n_modes, const ud_mode* modes, const float* coords,
+                        int how_many, double energy_range, char* out, size_t cap, size_t* len_out) {
+    if (!s || lig < 0 || lig >= int(s->ligs.size()) || n_modes < 0 || (n_modes > 0 && (!modes || !coords)) ||
+        how_many < 0 || !(energy_range >= 0.0) || !len_out)
+        return UD_ERR_INVALID;
+    const Ligand& L = s->ligs[lig];
+    if (L.format != 1) return UD_ERR_STATE;  // the text written back is the SDF input's
+    const size_t na = L.atoms.size();
+    std::string txt;
+    char buf[200];
+    int written = 0;
+    for (int k = 0; k < n_modes && written < how_many; ++k) {
+        const ud_mode& m = modes[k];
+        if (!(m.energy < 3.0e38f) || m.energy > modes[0].energy + energy_range) break;
+        ++written;
+        const float* c = coords + size_t(k) * na * 3;
+        for (size_t i = 0; i < L.lines.size(); ++i) {
+            std::string ln = L.lines[i];
+            const int a = L.line_atom[i];
+            if (a >= 0) {  // string_write_sdf_coord at columns 1, 11, 21 (model.cpp:758-785)
+                if (ln.size() < 30) ln.resize(30, ' ');
+                for (int d = 0; d < 3; ++d) {
+                    std::snprintf(buf, sizeof(buf), "%10.4f", c[3 * a + d]);
+                    ln.replace(10 * d, 10, std::string(buf).substr(0, 10));
+                }
+            }
+            txt += ln;
+            txt += '\n';
+        }
+        // sdf_remarks (vina.cpp:751-770)
+        std::snprintf(buf, sizeof(buf), "> <Uni-Dock RESULT> \nENERGY=%9.3f  LOWER_BOUND=%9.3f  UPPER_BOUND=%9.3f\n",
+                      m.energy, m.rmsd_lb, m.rmsd_ub);
+        txt += buf;
+        std::snprintf(buf, sizeof(buf), "INTER + INTRA:    %12.3f\n", m.inter + m.intra);
+        txt += buf;
+        std::snprintf(buf, sizeof(buf), "INTER:            %12.3f\n", m.inter);
+        txt += buf;
+        std::snprintf(buf, sizeof(buf), "INTRA:            %12.3f\n", m.intra);
+        txt += buf;
+        std::snprintf(buf, sizeof(buf), "UNBOUND:          %12.3f\n\n", m.unbound);
+        txt += buf;
+        txt += "$$$$\n";
     }
     *len_out = txt.size();
     if (!out) return UD_OK;

@@ -118,10 +118,14 @@ def load() -> ctypes.CDLL:
         "ud_ligset_destroy": (ctypes.c_int, [vp]),
         "ud_ligset_last_error": (ctypes.c_char_p, [vp]),
         "ud_ligset_add_pdbqt": (ctypes.c_int, [vp, ctypes.c_char_p, ip]),
+        "ud_ligset_add_sdf": (ctypes.c_int, [vp, ctypes.c_char_p, ip, ip]),
         "ud_ligset_count": (ctypes.c_int, [vp]),
         "ud_ligset_batch": (ctypes.c_int, [vp, ctypes.POINTER(UdLigandBatch)]),
         "ud_ligset_atoms": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, ip]),
         "ud_receptor_from_pdbqt": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ip, fp, ip]),
+        "ud_ligset_write_sdf": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(UdMode), fp, ctypes.c_int,
+                                               ctypes.c_double, ctypes.c_char_p, ctypes.c_size_t,
+                                               ctypes.POINTER(ctypes.c_size_t)]),
         "ud_ligset_write_pdbqt": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(UdMode), fp, ctypes.c_int,
                                                  ctypes.c_double, ctypes.c_char_p, ctypes.c_size_t,
                                                  ctypes.POINTER(ctypes.c_size_t)]),
@@ -155,7 +159,7 @@ class ParsedLigand:
 
 
 class LigandSet:
-    """Ligands parsed from PDBQT text by the library (ud_ligset_*)."""
+    """Ligands parsed from PDBQT or SDF text by the library (ud_ligset_*)."""
 
     def __init__(self):
         self.L = load()
@@ -183,6 +187,14 @@ class LigandSet:
             raise UdError(rc, self.L.ud_ligset_last_error(self.h).decode())
         return idx.value
 
+    def add_sdf(self, text: str) -> tuple:
+        """Add every record of SDF text; returns (index of the first, number added)."""
+        first, cnt = ctypes.c_int32(), ctypes.c_int32()
+        rc = self.L.ud_ligset_add_sdf(self.h, text.encode(), ctypes.byref(first), ctypes.byref(cnt))
+        if rc != UD_OK:
+            raise UdError(rc, self.L.ud_ligset_last_error(self.h).decode())
+        return first.value, cnt.value
+
     def __len__(self):
         return self.L.ud_ligset_count(self.h)
 
@@ -207,9 +219,7 @@ class LigandSet:
             pairs=get(b.pairs, 2 * npr).reshape(-1, 2), num_tors=get(b.num_tors, n),
         )
 
-    def write_pdbqt(self, lig: int, modes: list, coords, how_many=9, energy_range=3.0) -> str:
-        """PDBQT text of `lig` with the given modes (dicts as Context.postprocess
-        returns) and their atom coordinates coords[mode][atom][3] (batch order)."""
+    def _write(self, fn, lig, modes, coords, how_many, energy_range) -> str:
         arr = (UdMode * max(1, len(modes)))()
         for k, m in enumerate(modes):
             for f in ("energy", "inter", "intra", "conf_independent", "search_energy", "rmsd_lb", "rmsd_ub",
@@ -218,16 +228,27 @@ class LigandSet:
             arr[k].chain = int(m.get("chain", 0))
         c = np.ascontiguousarray(coords, dtype=np.float32)
         n = ctypes.c_size_t()
-        rc = self.L.ud_ligset_write_pdbqt(self.h, lig, len(modes), arr, _p(c, ctypes.c_float), how_many,
-                                          energy_range, None, 0, ctypes.byref(n))
+        f = getattr(self.L, fn)
+        rc = f(self.h, lig, len(modes), arr, _p(c, ctypes.c_float), how_many, energy_range, None, 0, ctypes.byref(n))
         if rc != UD_OK:
-            raise UdError(rc, "ud_ligset_write_pdbqt failed")
+            raise UdError(rc, f"{fn} failed")
         buf = ctypes.create_string_buffer(n.value + 1)
-        rc = self.L.ud_ligset_write_pdbqt(self.h, lig, len(modes), arr, _p(c, ctypes.c_float), how_many,
-                                          energy_range, buf, n.value + 1, ctypes.byref(n))
+        rc = f(self.h, lig, len(modes), arr, _p(c, ctypes.c_float), how_many, energy_range, buf, n.value + 1,
+               ctypes.byref(n))
         if rc != UD_OK:
-            raise UdError(rc, "ud_ligset_write_pdbqt failed")
+            raise UdError(rc, f"{fn} failed")
         return buf.value.decode()
+
+    def write_pdbqt(self, lig: int, modes: list, coords, how_many=9, energy_range=3.0) -> str:
+        """PDBQT text of `lig` (read from PDBQT) with the given modes (dicts as
+        Context.postprocess returns) and their atom coordinates
+        coords[mode][atom][3] (batch order)."""
+        return self._write("ud_ligset_write_pdbqt", lig, modes, coords, how_many, energy_range)
+
+    def write_sdf(self, lig: int, modes: list, coords, how_many=9, energy_range=3.0) -> str:
+        """SDF text of `lig` (read from SDF) with the given modes, one record per
+        mode with the "> <Uni-Dock RESULT>" field."""
+        return self._write("ud_ligset_write_sdf", lig, modes, coords, how_many, energy_range)
 
     def ligands(self) -> list:
         a = self.arrays()
