@@ -357,6 +357,7 @@ class Context:
         w = np.array(weights if weights is not None else (VINA_WEIGHTS if sf == UD_SF_VINA else VINARDO_WEIGHTS),
                      dtype=np.float64)
         self._ck(self.L.ud_set_scoring(self.h, sf, _p(w, ctypes.c_double), len(w)))
+        self.weights = tuple(float(x) for x in w)
 
     def set_box(self, center, size, granularity=0.375):
         c = np.array(center, dtype=np.float64)
