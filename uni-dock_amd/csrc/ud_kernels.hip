@@ -280,9 +280,23 @@ __device__ void fk_frames(const Grp<L, NM>& s, const float* pose) {
         fr[6] = q.x; fr[7] = q.y; fr[8] = q.z; fr[9] = q.w;
         q_to_mat(q, fr + 10);
     }
+    // half-angle sines and cosines of the lane's torsions, which do not
+    // depend on the parent frames: off the level-by-level critical path
+    constexpr int KN = (UD_MAX_TORSIONS + L - 1) / L;
+    float hc[KN], hs[KN];
+#pragma unroll
+    for (int m = 0; m < KN; ++m) {
+        const int k = 1 + s.gl + m * L;
+        hc[m] = 1.f;
+        hs[m] = 0.f;
+        if (k < s.h.n_nodes) sincosf(normalize_angle(pose[7 + k - 1]) * 0.5f, &hs[m], &hc[m]);
+    }
     __syncthreads();
     for (int d = 1; d <= s.h.max_depth; ++d) {
-        for (int k = 1 + s.gl; k < s.h.n_nodes; k += L) {
+#pragma unroll
+        for (int m = 0; m < KN; ++m) {
+            const int k = 1 + s.gl + m * L;
+            if (k >= s.h.n_nodes) continue;
             const NodeRec nd = s.nodes[k];
             if (int(nd.packed >> 24) != d) continue;
             const float* pf = fr + 20 * nd.parent;
@@ -295,7 +309,9 @@ __device__ void fk_frames(const Grp<L, NM>& s, const float* pose) {
             const float axy = R[3] * nd.ax[0] + R[4] * nd.ax[1] + R[5] * nd.ax[2];
             const float axz = R[6] * nd.ax[0] + R[7] * nd.ax[1] + R[8] * nd.ax[2];
             const float4 qp = make_float4(pf[6], pf[7], pf[8], pf[9]);
-            const float4 q = q_normalize_approx(qmul(axis_angle_q(axx, axy, axz, pose[7 + k - 1]), qp));
+            // angle_to_quaternion(axis, angle) (quaternion.cpp:37-47) times the parent's
+            const float4 qa = make_float4(hc[m], hs[m] * axx, hs[m] * axy, hs[m] * axz);
+            const float4 q = q_normalize_approx(qmul(qa, qp));
             f[0] = ox; f[1] = oy; f[2] = oz;
             f[3] = axx; f[4] = axy; f[5] = axz;
             f[6] = q.x; f[7] = q.y; f[8] = q.z; f[9] = q.w;
@@ -315,8 +331,14 @@ __device__ __forceinline__ float3 frame_apply(const float* f, const AtomRec& a) 
 // grid::evaluate_aux (grid.cpp:39-129) + curl (curl.h:28-36), split in three
 // steps so that the corner gathers of several atoms are in flight together:
 // locate (cell, fractions, out-of-box penalty), gather (8 loads), finish.
+// curl's factor v / (v + e) (curl.h:28-36) with the hardware reciprocal
+// (1 ulp), as the reference's GPU build divides (--use_fast_math)
+__device__ __forceinline__ float curl_factor(float v, float e) {
+    return (v < kEpsFl) ? 0.f : v * __builtin_amdgcn_rcpf(v + e);
+}
+
 struct GridPt {
-    size_t base;
+    uint32_t base;  // map element of corner 000 (maps hold fewer than 2^32 values)
     float x, y, z, penalty;
     int reg[3];
 };
@@ -338,14 +360,14 @@ __device__ __forceinline__ GridPt grid_locate(const GridDev& g, int slot, float3
         a[i] = min(max(a[i], 0), g.n[i] - 2);  // a NaN coordinate must not index outside the map
     }
     q.penalty = g.slope * (miss[0] * g.factor_inv[0] + miss[1] * g.factor_inv[1] + miss[2] * g.factor_inv[2]);
-    q.base = size_t(slot) * g.n_points + a[0] + size_t(g.n[0]) * a[1] + size_t(g.n[0]) * g.n[1] * a[2];
+    q.base = uint32_t(slot) * uint32_t(g.n_points) + uint32_t(a[0] + g.n[0] * (a[1] + g.n[1] * a[2]));
     q.x = s[0]; q.y = s[1]; q.z = s[2];
     return q;
 }
 
 template <int GF>
-__device__ __forceinline__ void grid_gather(const GridDev& g, size_t base, float (&f)[8]) {
-    const size_t sy = size_t(g.n[0]), sz = size_t(g.n[0]) * g.n[1];
+__device__ __forceinline__ void grid_gather(const GridDev& g, uint32_t base, float (&f)[8]) {
+    const uint32_t sy = uint32_t(g.n[0]), sz = uint32_t(g.n[0] * g.n[1]);
     if (GF == UD_GRID_F32) {
         const float* M = reinterpret_cast<const float*>(g.maps);
         f[0] = M[base]; f[1] = M[base + 1]; f[2] = M[base + sy]; f[3] = M[base + sy + 1];
@@ -381,7 +403,7 @@ __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const G
             gx *= sc; gy *= sc; gz *= sc;
         }
         if (e > 0.f && v < 0.1f * kMaxFl) {
-            const float tmp = (v < kEpsFl) ? 0.f : (v / (v + e));
+            const float tmp = curl_factor(v, e);
             e *= tmp;
             const float t2 = tmp * tmp;
             gx *= t2; gy *= t2; gz *= t2;
@@ -390,7 +412,7 @@ __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const G
         d->y = g.factor[1] * (q.reg[1] == 0 ? gy : 0.f) + g.slope * float(q.reg[1]);
         d->z = g.factor[2] * (q.reg[2] == 0 ? gz : 0.f) + g.slope * float(q.reg[2]);
     } else if (e > 0.f && v < 0.1f * kMaxFl) {
-        e *= (v < kEpsFl) ? 0.f : (v / (v + e));
+        e *= curl_factor(v, e);
     }
     return e + q.penalty;
 }
@@ -451,7 +473,7 @@ __device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p,
         }
     }
     if (e > 0.f && v < 0.1f * kMaxFl) {
-        const float tmp = (v < kEpsFl) ? 0.f : (v / (v + e));
+        const float tmp = curl_factor(v, e);
         e *= tmp;
         const float t2 = tmp * tmp;
         gx *= t2; gy *= t2; gz *= t2;
@@ -625,7 +647,7 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
                 const float dor = t0[u].y + rem[u] * (t1[u].y - t0[u].y);
                 float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
                 if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
-                    const float tmp = (v_intra < kEpsFl) ? 0.f : (v_intra / (v_intra + pe));
+                    const float tmp = curl_factor(v_intra, pe);
                     pe *= tmp;
                     const float t2 = tmp * tmp;
                     fx *= t2; fy *= t2; fz *= t2;
@@ -727,7 +749,7 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
                 float pe = t0[u].x + rem[u] * (t1[u].x - t0[u].x);
                 d = t0[u].y + rem[u] * (t1[u].y - t0[u].y);
                 if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
-                    const float tmp = (v_intra < kEpsFl) ? 0.f : (v_intra / (v_intra + pe));
+                    const float tmp = curl_factor(v_intra, pe);
                     pe *= tmp;
                     d *= tmp * tmp;
                 }
