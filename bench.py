@@ -13,6 +13,20 @@ resident on the device.  Two forms are timed:
 The best pose of sampled ligands is re-scored by the FP64 oracle to report
 the FP16 form's energy error.  The CPU baseline is the FP32 C++ restatement
 of the reference's CPU global search on a bounded sample of the same batch.
+
+--config 2 / 3 / 4 run the other benchmark configurations instead, each
+printing its own JSON line:
+  2  screening-scale batching: 5,000-atom receptor, 10,000 ligands (atoms
+     N(28, 7) in [8, 60], torsions Poisson(5) in [0, 15]), exhaustiveness 512,
+     max_step 40, autotuned lanes, FP16 maps, Vina then Vinardo
+  3  mixed-precision maps: 8,000-atom receptor, 30 A box (81^3), 4,096
+     ligands sized as in 2; FP16 vs FP32 maps docking at exhaustiveness 128,
+     max_step 20, and the energy error of 10^6 random poses against the FP64
+     oracle (Vina maps: AD4 maps are not built)
+  4  differentiable energy: 65,536 random poses over 2,048 ligands sized as
+     in 2 against a 5,000-atom receptor, forward and backward through the
+     torch.autograd binding, with central differences (h = 1e-3) of the FP64
+     oracle on a sample
 """
 import argparse
 import json
@@ -142,6 +156,184 @@ def fp64_error(rec, ligs, ctx16, res, exh, n_sample):
     return float(np.max(errs)), float(np.mean(errs))
 
 
+def random_poses(rng, ligs, pl, lo, hi):
+    """Translation uniform in the box, orientation uniform on SO(3) (normalised
+    4-D Gaussian), torsions U[-pi, pi)."""
+    P = np.zeros((len(pl), _lib.UD_POSE_FLOATS), dtype=np.float64)
+    P[:, 0:3] = lo + rng.random((len(pl), 3)) * (hi - lo)
+    q = rng.normal(size=(len(pl), 4))
+    P[:, 3:7] = q / np.linalg.norm(q, axis=1, keepdims=True)
+    for i, li in enumerate(pl):
+        nt = ligs[li].n_tors
+        P[i, 7:7 + nt] = rng.uniform(-np.pi, np.pi, nt)
+    return P
+
+
+def config2(args):
+    rec = synth.make_receptor(5000, seed=2000)
+    ligs = synth.make_ligand_set(10000, seed=2001, dist="screening")
+    types = sorted({int(t) for l in ligs for t in l.xs if t >= 0})
+    exh, gsteps = 512, global_steps_for(ligs, 40)
+    log(f"[config2] {len(ligs)} ligands, atoms {min(l.n_atoms for l in ligs)}-{max(l.n_atoms for l in ligs)}, "
+        f"torsions up to {max(l.n_tors for l in ligs)}, global_steps {gsteps}")
+    res = {}
+    for name, sf in (("vina", _lib.UD_SF_VINA), ("vinardo", _lib.UD_SF_VINARDO)):
+        ctx = _lib.Context(0)
+        ctx.set_scoring(sf)
+        ctx.set_box(rec.cavity, (22.5, 22.5, 22.5), 0.375)
+        ctx.compute_maps(rec.xyz, rec.xs, types, _lib.UD_GRID_F16)
+        ctx.set_ligands(synth.LigandBatch(ligs).arrays())
+        t0 = time.perf_counter()
+        tuned = ctx.autotune(exhaustiveness=exh, global_steps=2, sample_ligands=1024)
+        t_tune = time.perf_counter() - t0
+        r = time_form(ctx, len(ligs), exh, gsteps, 0, 1, 0, 181129)
+        ev = float(r["evals"].sum())
+        res[name] = dict(ligands_per_s=r["ligs_per_s"], wall_s=r["wall"], kernel_ms=r["kernel_ms"],
+                         evals_per_s=ev / (r["kernel_ms"] * 1e-3), autotune_s=t_tune,
+                         tuned_lanes=dict(zip([str(c) for c in _lib.BUCKET_ATOMS], [int(x) for x in tuned])),
+                         launches=r["launches"])
+        log(f"[config2] {name}: {r['ligs_per_s']:.1f} ligands/s ({r['wall']:.1f} s), lanes {res[name]['tuned_lanes']}")
+        ctx.close()
+    print(json.dumps({"metric": "ligands/sec, screening-scale batch (10,000 ligands, exhaustiveness 512, max_step 40)",
+                      "value": res["vina"]["ligands_per_s"], "unit": "ligands/s", "n_gpus": 1,
+                      "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded receptor and ligands)",
+                      "config": {"workload": "5,000-atom receptor, 10,000 ligands N(28,7) atoms / Poisson(5) "
+                                             "torsions, 22.5 A box (61^3 fp16 maps), exhaustiveness 512, max_step 40",
+                                 "global_steps": gsteps},
+                      "vina": res["vina"], "vinardo": res["vinardo"]}))
+
+
+def config3(args):
+    from oracle.oracle import Oracle
+    rec = synth.make_receptor(8000, seed=3000)
+    ligs = synth.make_ligand_set(4096, seed=3001, dist="screening")
+    types = sorted({int(t) for l in ligs for t in l.xs if t >= 0})
+    box = (30.0, 30.0, 30.0)
+    exh, gsteps = 128, global_steps_for(ligs, 20)
+    out = {}
+    ctxs = {}
+    for name, prec in (("fp16", _lib.UD_GRID_F16), ("fp32", _lib.UD_GRID_F32)):
+        ctx = _lib.Context(0)
+        ctx.set_scoring(_lib.UD_SF_VINA)
+        npts = ctx.set_box(rec.cavity, box, 0.375)
+        ctx.compute_maps(rec.xyz, rec.xs, types, prec)
+        ctx.set_ligands(synth.LigandBatch(ligs).arrays())
+        ctx.autotune(exhaustiveness=exh, global_steps=2, sample_ligands=1024)
+        r = time_form(ctx, len(ligs), exh, gsteps, 0, 1, 1, 181129)
+        out[name] = dict(ligands_per_s=r["ligs_per_s"], kernel_ms=r["kernel_ms"])
+        log(f"[config3] {name} maps ({npts}): {r['ligs_per_s']:.1f} ligands/s")
+        ctxs[name] = ctx
+    # per-pose energy error on 10^6 random poses vs the FP64 oracle (its own FP64 maps)
+    n_pose = args.poses
+    rng = np.random.default_rng(3002)
+    pl = rng.integers(0, len(ligs), n_pose).astype(np.int32)
+    o = Oracle(64, 0)
+    n, geo = o.set_grid(rec.cavity, np.array(box), 0.375)
+    lo = geo[:3]
+    hi = lo + geo[6:9] * (np.array(n) - 1)
+    P = random_poses(rng, ligs, pl, lo, hi)
+    t0 = time.perf_counter()
+    o.build_maps(rec.xyz, rec.xs, types)
+    for l in ligs:
+        o.add_ligand(l)
+    e64 = o.eval_batch(pl, P, (1000.0, 1000.0, 1000.0), min(16, os.cpu_count() or 1))
+    log(f"[config3] FP64 oracle: maps and {n_pose} poses in {time.perf_counter() - t0:.1f} s")
+    errs = {}
+    for name, ctx in ctxs.items():
+        e = np.zeros(n_pose, dtype=np.float32)
+        for a in range(0, n_pose, 1 << 17):
+            b = min(n_pose, a + (1 << 17))
+            e[a:b], _ = ctx.score_grad(pl[a:b], P[a:b])
+        d = np.abs(e.astype(np.float64) - e64)
+        # poses with an atom outside the box carry the 1e6 / A penalty: report
+        # those relative, the others in kcal/mol
+        inside = e64 < 1e4
+        rel = d[~inside] / np.abs(e64[~inside]) if np.any(~inside) else np.zeros(1)
+        low = np.abs(e64) < 100.0  # poses without steep clashes
+        errs[name] = dict(n_inside=int(inside.sum()), max_abs=float(d[inside].max()), mean_abs=float(d[inside].mean()),
+                          p99_abs=float(np.percentile(d[inside], 99)), max_rel_outside=float(rel.max()),
+                          n_below_100=int(low.sum()), max_abs_below_100=float(d[low].max()),
+                          mean_abs_below_100=float(d[low].mean()), max_rel_inside=float((d[inside] / np.maximum(1.0, np.abs(e64[inside]))).max()))
+        log(f"[config3] {name} error vs FP64 over {inside.sum()} in-box poses: max {errs[name]['max_abs']:.4f}, "
+            f"mean {errs[name]['mean_abs']:.5f}, p99 {errs[name]['p99_abs']:.4f} kcal/mol; |E| < 100: "
+            f"max {errs[name]['max_abs_below_100']:.4f} over {errs[name]['n_below_100']}; "
+            f"max relative {errs[name]['max_rel_inside']:.2e}")
+        ctx.close()
+    print(json.dumps({"metric": "ligands/sec and per-pose energy error, FP16 vs FP32 maps (81^3)",
+                      "value": out["fp16"]["ligands_per_s"], "unit": "ligands/s", "n_gpus": 1,
+                      "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded receptor and ligands)",
+                      "config": {"workload": "8,000-atom receptor, 30 A box (81^3), 4,096 ligands N(28,7) / "
+                                             "Poisson(5), Vina maps (AD4 not built), exhaustiveness 128, max_step 20",
+                                 "poses_for_error": n_pose},
+                      "docking": out, "energy_error_vs_fp64": errs}))
+
+
+def config4(args):
+    import torch
+    from oracle.oracle import Oracle
+    from unidock_tools.torch_energy import DockingEnergy
+    rec = synth.make_receptor(5000, seed=4000)
+    ligs = synth.make_ligand_set(2048, seed=4001, dist="screening")
+    box = np.array([22.5, 22.5, 22.5])
+    model = DockingEnergy(rec.xyz, rec.xs, ligs, rec.cavity, box)
+    rng = np.random.default_rng(4002)
+    pl = np.repeat(np.arange(len(ligs), dtype=np.int32), 32)
+    o = Oracle(64, 0)
+    n, geo = o.set_grid(rec.cavity, box, 0.375)
+    lo = geo[:3]
+    hi = lo + geo[6:9] * (np.array(n) - 1)
+    P = random_poses(rng, ligs, pl, lo, hi)
+    poses = torch.tensor(P, dtype=torch.float32, device="cuda", requires_grad=True)
+    w = torch.tensor(_lib.VINA_WEIGHTS, dtype=torch.float64, requires_grad=True)
+    for _ in range(args.warmup + 1):
+        e = model(poses, pl, w)
+        e.sum().backward()
+    torch.cuda.synchronize()
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        poses.grad = None
+        w.grad = None
+        e = model(poses, pl, w)
+        e.sum().backward()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    log(f"[config4] {len(pl)} poses forward + backward in {dt * 1e3:.2f} ms")
+    # central differences (h = 1e-3) of the FP64 oracle on a sample of in-box poses
+    t1 = time.perf_counter()
+    o.build_maps(rec.xyz, rec.xs, sorted({int(t) for l in ligs for t in l.xs if t >= 0}))
+    sel = rng.choice(len(pl), 64, replace=False)
+    for l in ligs:
+        o.add_ligand(l)
+    g = poses.grad.detach().cpu().numpy()
+    rel = []
+    for i in sel:
+        li = int(pl[i])
+        nt = ligs[li].n_tors
+        x = P[i].copy()
+        fd = np.zeros(7 + nt)
+        for j in range(7 + nt):
+            a, b = x.copy(), x.copy()
+            a[j] += 1e-3
+            b[j] -= 1e-3
+            qa, qb = a[3:7] / np.linalg.norm(a[3:7]), b[3:7] / np.linalg.norm(b[3:7])
+            a[3:7], b[3:7] = qa, qb
+            fd[j] = (o.eval_deriv(li, a)[0] - o.eval_deriv(li, b)[0]) / 2e-3
+        sc = max(1.0, np.abs(fd).max())
+        rel.append(float(np.abs(g[i, :7 + nt] - fd).max() / sc))
+    rel = np.array(rel)
+    log(f"[config4] FD check on {len(sel)} poses: median {np.median(rel):.2e}, 90th pct {np.percentile(rel, 90):.2e} "
+        f"(max-norm error / max(1, |g|)) in {time.perf_counter() - t1:.1f} s")
+    print(json.dumps({"metric": "poses/sec, energy + pose and weight gradients (torch.autograd)",
+                      "value": len(pl) / dt, "unit": "poses/s", "n_gpus": 1, "higher_is_better": True,
+                      "dtype": "f32", "data": "synthetic (seeded receptor, ligands and poses)", "steps": steps,
+                      "ms_per_step": dt * 1e3,
+                      "config": {"workload": "65,536 random poses over 2,048 ligands N(28,7) / Poisson(5), "
+                                             "5,000-atom receptor, 22.5 A box (61^3 fp32 maps), forward + backward"},
+                      "fd_check_h_1e-3": {"sample": len(sel), "median_rel": float(np.median(rel)),
+                                          "p90_rel": float(np.percentile(rel, 90)), "max_rel": float(rel.max())}}))
+
+
 def measured_traffic(kernel, workgroups):
     """HBM bytes of one launch from the committed counter table, or None."""
     try:
@@ -184,6 +376,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4),
+                    help="benchmark configuration (1 = the metric's; 2-4 print their own line)")
+    ap.add_argument("--poses", type=int, default=1000000, help="random poses of the --config 3 error check")
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -194,6 +389,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the ligand batch would be split across "
               "GPUs by the host with no collective in the search", file=sys.stderr)
         sys.exit(2)
+    if args.config != 1:
+        {2: config2, 3: config3, 4: config4}[args.config](args)
+        return
 
     t0 = time.perf_counter()
     rec, ligs = make_workload(args.ligands, args.seed)

@@ -35,6 +35,8 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 namespace orc {
@@ -969,8 +971,12 @@ template <class R> void build_maps(Ctx<R>& C, int n_rec, const double* rec_xyz, 
         g.maps[types[j]].assign(size_t(g.n[0]) * g.n[1] * g.n[2], R(0));
     }
     const R cutoff_sqr = R(64);
-    std::vector<R> acc(n_types);
-    for (int z = 0; z < g.n[2]; ++z)
+    // tables are built lazily: build every (receptor type, map type) pair
+    // before the z-slices are shared out among threads
+    for (int t : rt)
+        for (int j = 0; j < n_types; ++j) C.table(t, types[j]);
+    auto slice = [&](int z) {
+        std::vector<R> acc(n_types);
         for (int y = 0; y < g.n[1]; ++y)
             for (int x = 0; x < g.n[0]; ++x) {
                 V3<R> p = g.point(x, y, z);
@@ -982,6 +988,15 @@ template <class R> void build_maps(Ctx<R>& C, int n_rec, const double* rec_xyz, 
                 }
                 for (int j = 0; j < n_types; ++j) g.at(types[j], x, y, z) = acc[j];
             }
+    };
+    const int nt = std::max(1, std::min<int>(16, int(std::thread::hardware_concurrency())));
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; ++i)
+        th.emplace_back([&] {
+            for (int z = next.fetch_add(1); z < g.n[2]; z = next.fetch_add(1)) slice(z);
+        });
+    for (auto& t : th) t.join();
 }
 
 }  // namespace orc
@@ -1266,6 +1281,27 @@ extern "C" double orc_conf_independent(OrcHandle* h, double num_tors, double x) 
 
 #include <atomic>
 #include <thread>
+// energies of many poses (FP64 or FP32 per the handle), n_threads threads;
+// poses[i * 39] in the pose layout, ligs[i] the ligand of pose i
+extern "C" void orc_eval_batch(OrcHandle* h, int n, const int* ligs, const double* poses, const double* v,
+                               int n_threads, double* e_out) {
+    for (int i = 0; i < n; ++i) {  // build the lazily made pair tables first
+        if (h->prec == 64) { auto& L = h->d.ligs[ligs[i]]; for (int k = 0; k < L.n_pairs; ++k) h->d.table(L.xs[L.pa[k]], L.xs[L.pb[k]]); }
+        else { auto& L = h->f.ligs[ligs[i]]; for (int k = 0; k < L.n_pairs; ++k) h->f.table(L.xs[L.pa[k]], L.xs[L.pb[k]]); }
+    }
+    std::atomic<int> next(0);
+    auto work = [&] {
+        for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) {
+            std::vector<double> g(6 + 64);
+            e_out[i] = (h->prec == 64) ? eval_impl(h->d, ligs[i], poses + size_t(i) * 39, v, g.data(), nullptr, nullptr)
+                                       : eval_impl(h->f, ligs[i], poses + size_t(i) * 39, v, g.data(), nullptr, nullptr);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int i = 0; i < std::max(1, n_threads); ++i) th.emplace_back(work);
+    for (auto& t : th) t.join();
+}
+
 extern "C" double orc_dock_cpu(OrcHandle* h, int n_lig, const int* ligs, int exh, uint64_t seed, int gsteps,
                                const int* lsteps, int n_threads, double* best_e_out, long long* evals_out) {
     std::atomic<int> next(0);

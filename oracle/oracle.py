@@ -57,6 +57,7 @@ def load():
         "orc_score_terms": (None, [vp, ctypes.c_int, dp, dp, dp]),
         "orc_score_terms_nc": (None, [vp, ctypes.c_int, dp, dp, dp]),
         "orc_refine": (ctypes.c_double, [vp, ctypes.c_int, dp, ctypes.c_int, ip]),
+        "orc_eval_batch": (None, [vp, ctypes.c_int, ip, dp, dp, ctypes.c_int, dp]),
         "orc_nc_atom": (ctypes.c_double, [vp, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, dp]),
         "orc_eval_deriv_nc": (ctypes.c_double, [vp, ctypes.c_int, dp, dp, ctypes.c_double, dp]),
         "orc_conf_independent": (ctypes.c_double, [vp, ctypes.c_double, ctypes.c_double]),
@@ -208,6 +209,16 @@ class Oracle:
         b = ctypes.c_double()
         self.L.orc_score_terms(self.h, lig, _d(p), ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def eval_batch(self, ligs, poses, v=(1000.0, 1000.0, 1000.0), n_threads=16):
+        """Energies of many poses (poses[i] in the 39-float layout) on n_threads threads."""
+        lg = np.ascontiguousarray(ligs, dtype=np.int32)
+        P = np.zeros((len(lg), POSE))
+        P[:, :np.asarray(poses).shape[1]] = poses
+        e = np.zeros(len(lg))
+        self.L.orc_eval_batch(self.h, len(lg), _i(lg), _d(P), _d(v), n_threads,
+                              e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return e
 
     def refine(self, lig, pose, refine_step=3):
         """Refinement against the explicit receptor; returns (energy, pose, within)."""
