@@ -310,14 +310,19 @@ def test_fp16_maps_error_bound(scene):
 
 
 def test_score_terms_and_coords(scene):
+    """The rescoring kernel interpolates and sums in FP64: on the GPU's own
+    maps its inter term matches the FP64 oracle to the FP32 coordinates'
+    resolution."""
     _say("[gpu] score terms + coords")
-    ctx, o64, o32 = scene["ctx"], scene["o64"], scene["o32"]
+    ctx, o64, o32, o64m = scene["ctx"], scene["o64"], scene["o32"], scene["o64m"]
     pl, P = _poses(scene, n_per=3, seed=4)
     a, b = ctx.score_terms(pl, P)
     C = ctx.pose_coords(pl, P)
     for i in range(len(pl)):
         ra, rb = o32.score_terms(int(pl[i]), P[i])
         assert a[i] == pytest.approx(ra, rel=1e-5, abs=1e-4)
+        if _in_box(scene, o64.eval_deriv(int(pl[i]), P[i])[2]):
+            assert a[i] == pytest.approx(o64m.score_terms(int(pl[i]), P[i])[0], rel=2e-5, abs=2e-5)
         # intra uses the piecewise-constant table: a bin flip is one table step
         assert b[i] == pytest.approx(rb, rel=1e-4, abs=1e-4) or abs(b[i] - rb) < 0.05
         _, _, rc, _ = o64.eval_deriv(int(pl[i]), P[i])

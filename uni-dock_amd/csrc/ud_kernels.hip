@@ -1379,9 +1379,19 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
     }
 }
 
-// Final-score components with authentic caps (no gradient): inter = grid
-// (cache::eval), intra = pairs through the piecewise-constant table with
-// curl (eval_interacting_pairs, model.cpp:885-905).
+// curl(e, v) (curl.h:28-43) in FP64
+__device__ __forceinline__ double curl_d(double e, double v) {
+    if (e > 0.0 && v < 0.1 * double(kMaxFl)) e *= (v < double(kEpsFl)) ? 0.0 : v / (v + e);
+    return e;
+}
+
+// Final-score components with authentic caps (no gradient), the rescoring of
+// Vina::score_gpu (vina.cpp:1245-1327): inter = the receptor term of each
+// atom (cache::eval through the maps, or non_cache::eval against the
+// explicit atoms), intra = the ligand pairs through the piecewise-constant
+// table (eval_interacting_pairs, model.cpp:885-905).  Coordinates, map and
+// table values are the FP32 / FP16 inputs; interpolation, curl and every sum
+// are FP64.
 template <int L, int GF>
 __global__ __launch_bounds__(64) void terms_kernel(KArgs ka, const WorkItem* work, const int32_t* perm,
                                                    const float* poses, float v, float* inter_out, float* intra_out) {
@@ -1398,25 +1408,68 @@ __global__ __launch_bounds__(64) void terms_kernel(KArgs ka, const WorkItem* wor
     fk_frames(s, x);
     float4* coords = s.coords();
     float rg2 = 0.f;
-    float ei = atoms_grid<L, kNMax, GF>(ka, s, v, false, rg2), ea = 0.f;
+    atoms_coords(s, rg2);
     __syncthreads();
+    const double vd = double(v);
+    double ei = 0.0, ea = 0.0;
+    for (int i = s.gl; i < s.h.n_atoms; i += L) {
+        const AtomRec a = s.atoms[i];
+        const int slot = (a.meta >> 8) & 0xff;
+        if (slot == 0xff) continue;
+        const float4 c = coords[i];
+        if constexpr (GF == kGridExplicit) {
+            // non_cache::eval (non_cache.cpp:34-255): clamp into the box, the
+            // fast table of every receptor atom within the cutoff, curl, penalty
+            const RecDev& r = ka.rec;
+            const int t = int(a.meta >> 24);
+            float p[3] = {c.x, c.y, c.z};
+            double pen = 0.0;
+            int cc[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (p[d] < r.init[d]) { pen += double(r.init[d]) - double(p[d]); p[d] = r.init[d]; }
+                else if (p[d] > r.end[d]) { pen += double(p[d]) - double(r.end[d]); p[d] = r.end[d]; }
+                const float xx = (p[d] - r.init[d]) * float(r.dim[d]) / r.range[d];
+                cc[d] = (xx <= 0.f) ? 0 : (xx >= float(r.dim[d] - 1)) ? r.dim[d] - 1 : max(0, min(int(xx), r.dim[d] - 1));
+            }
+            const int cell = cc[0] + r.dim[0] * (cc[1] + r.dim[1] * cc[2]);
+            double e = 0.0;
+            for (int k = r.cell_start[cell]; k < r.cell_start[cell + 1]; ++k) {
+                const float4 b = r.atoms[k];
+                const float rx = p[0] - b.x, ry = p[1] - b.y, rz = p[2] - b.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 < 64.f) {
+                    const int t2 = __float_as_int(b.w);
+                    const int tp = t <= t2 ? t + t2 * (t2 + 1) / 2 : t2 + t * (t + 1) / 2;
+                    e += double(ka.ls.table_fast[size_t(tp) * kTableStride + int(32.f * r2)]);
+                }
+            }
+            ei += curl_d(e, vd) + double(ka.grid.slope) * pen;
+        } else {
+            // grid::evaluate_aux (grid.cpp:39-129) energy path
+            const GridPt q = grid_locate(ka.grid, slot, make_float3(c.x, c.y, c.z));
+            float f[8];
+            grid_gather<GF>(ka.grid, q.base, f);
+            const double fx = q.x, fy = q.y, fz = q.z, mx = 1.0 - fx, my = 1.0 - fy, mz = 1.0 - fz;
+            double e = f[0] * mx * my * mz + f[1] * fx * my * mz + f[2] * mx * fy * mz + f[3] * fx * fy * mz +
+                       f[4] * mx * my * fz + f[5] * fx * my * fz + f[6] * mx * fy * fz + f[7] * fx * fy * fz;
+            if (GF == UD_GRID_F16) e *= double(ka.grid.scale[slot]);
+            ei += curl_d(e, vd) + double(q.penalty);
+        }
+    }
     const PairRec* prs = s.pairs;
     for (int q = s.gl; q < s.h.n_pairs; q += L) {
         const PairRec rec = prs[q];
         const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
         const float rx = cb.x - ca.x, ry = cb.y - ca.y, rz = cb.z - ca.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < 64.f) {
-            float t = ka.ls.table_fast[size_t(rec >> 16) * kTableStride + int(32.f * r2)];
-            if (t > 0.f && v < 0.1f * kMaxFl) t *= (v < kEpsFl) ? 0.f : v / (v + t);
-            ea += t;
-        }
+        if (r2 < 64.f) ea += curl_d(double(ka.ls.table_fast[size_t(rec >> 16) * kTableStride + int(32.f * r2)]), vd);
     }
-    ei = gsum<L>(ei);
-    ea = gsum<L>(ea);
+    ei = gsum_d<L>(ei);
+    ea = gsum_d<L>(ea);
     if (active && s.gl == 0) {
-        inter_out[pi] = ei;
-        intra_out[pi] = ea;
+        inter_out[pi] = float(ei);
+        intra_out[pi] = float(ea);
     }
 }
 
