@@ -181,6 +181,19 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 // of at most 32 lanes the inverse Hessian lives in registers, otherwise in LDS
 // (full-wavefront groups are register-bound already).
 static constexpr int kNSmall = 16;
+// Compile-time switches of the search kernel (defaults measured fastest on
+// MI355X): the occupancy target of 32-lane groups (waves per SIMD), the
+// partner records prefetched before the pair pass, and the register form of
+// the small inverse Hessian.
+#ifndef UD_WAVES_L32
+#define UD_WAVES_L32 4
+#endif
+#ifndef UD_PREFETCH_RECORDS
+#define UD_PREFETCH_RECORDS 8
+#endif
+#ifndef UD_HESS_IN_REGS
+#define UD_HESS_IN_REGS 1
+#endif
 
 // LDS layout of one workgroup: the ligand's atoms and nodes, then one block
 // per lane group (coordinates, minus-forces, node frames, five pose buffers,
@@ -195,7 +208,7 @@ struct Layout {
         na = na_;
         nn = nn_;
         nc = 6 + nn - 1;
-        const bool hreg = nc <= kNSmall && lanes <= 32;  // inverse Hessian in registers
+        const bool hreg = UD_HESS_IN_REGS && nc <= kNSmall && lanes <= 32;  // inverse Hessian in registers
         ps = (7 + nn - 1 + 3) & ~3;  // pose floats (7 + torsions), >= nc
         if (hreg && ps < kNSmall) ps = kNSmall;  // broadcast vectors hold all kNSmall entries
         nh = hreg ? 0 : (nc * (nc | 1) + 3) & ~3;  // full symmetric inverse Hessian, odd row stride
@@ -381,23 +394,27 @@ __device__ __forceinline__ void grid_gather(const GridDev& g, uint32_t base, flo
     }
 }
 
-// corners f = (000, 100, 010, 110, 001, 101, 011, 111); FP16 maps apply the
-// power-of-two map scale to value and gradient before the curl
+// corners f = (000, 100, 010, 110, 001, 101, 011, 111): the trilinear value
+// of grid::evaluate_aux (grid.cpp:39-129) and its gradient, formed as nested
+// linear interpolations (x, then y, then z) so that value and gradient share
+// their partial sums; FP16 maps apply the power-of-two map scale to value and
+// gradient before the curl
 template <int GF>
 __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const GridPt& q, const float (&f)[8], float v,
                                              float3* d) {
     const float x = q.x, y = q.y, z = q.z;
-    const float mx = 1.f - x, my = 1.f - y, mz = 1.f - z;
-    float e = f[0] * mx * my * mz + f[1] * x * my * mz + f[2] * mx * y * mz + f[3] * x * y * mz +
-              f[4] * mx * my * z + f[5] * x * my * z + f[6] * mx * y * z + f[7] * x * y * z;
+    const float dx0 = f[1] - f[0], dx1 = f[3] - f[2], dx2 = f[5] - f[4], dx3 = f[7] - f[6];
+    const float a0 = f[0] + x * dx0, a1 = f[2] + x * dx1, a2 = f[4] + x * dx2, a3 = f[6] + x * dx3;
+    const float dy0 = a1 - a0, dy1 = a3 - a2;
+    const float b0 = a0 + y * dy0, b1 = a2 + y * dy1;
+    const float dz = b1 - b0;
+    float e = b0 + z * dz;
     if (GF == UD_GRID_F16) e *= g.scale[slot];
     if (d) {
-        float gx = -f[0] * my * mz + f[1] * my * mz - f[2] * y * mz + f[3] * y * mz - f[4] * my * z + f[5] * my * z -
-                   f[6] * y * z + f[7] * y * z;
-        float gy = -f[0] * mx * mz - f[1] * x * mz + f[2] * mx * mz + f[3] * x * mz - f[4] * mx * z - f[5] * x * z +
-                   f[6] * mx * z + f[7] * x * z;
-        float gz = -f[0] * mx * my - f[1] * x * my - f[2] * mx * y - f[3] * x * y + f[4] * mx * my + f[5] * x * my +
-                   f[6] * mx * y + f[7] * x * y;
+        const float ex0 = dx0 + y * (dx1 - dx0), ex1 = dx2 + y * (dx3 - dx2);
+        float gx = ex0 + z * (ex1 - ex0);
+        float gy = dy0 + z * (dy1 - dy0);
+        float gz = dz;
         if (GF == UD_GRID_F16) {
             const float sc = g.scale[slot];
             gx *= sc; gy *= sc; gz *= sc;
@@ -484,8 +501,8 @@ __device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p,
 
 // Compile-time tuning of the search kernel: load batching and the occupancy
 // target handed to the register allocator (waves per SIMD; 1 = no target).
-// Full-wavefront groups (64 lanes) always target 4 waves per SIMD: with one
-// chain per wavefront they are register-bound, and measured faster so.
+// Full-wavefront and 32-lane groups target 4 waves per SIMD (UD_WAVES_L32),
+// smaller groups UD_SEARCH_WAVES_PER_EU: measured fastest so.
 #ifndef UD_ATOM_BATCH
 #define UD_ATOM_BATCH 1
 #endif
@@ -684,7 +701,7 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
     const int N = s.h.n_atoms;
     const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
     const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
-    constexpr int kRec = 8;  // partner records fetched together
+    constexpr int kRec = UD_PREFETCH_RECORDS;  // partner records fetched together
     // partner range and first records of the lane's first atom, requested
     // before pass A so that their latency overlaps it
     const uint32_t ap0 = (s.gl < N) ? apl[s.gl] : 0u;
@@ -1073,7 +1090,7 @@ enum { ST_RGYR = 0, ST_INIT = 1, ST_TRIAL = 2, ST_IDLE = 3 };
 // MC == false runs one BFGS per group on the given poses (quasi_newton).
 // The curvature scalar y.p is accumulated in FP64.
 template <int L, int GF, bool MC, int NM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L == 64 ? 4 : UD_SEARCH_WAVES_PER_EU)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L == 64 ? 4 : L == 32 ? UD_WAVES_L32 : UD_SEARCH_WAVES_PER_EU)))
 void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* perm,
                                                     float* poses_io, float v_intra_opt, float v_inter_opt,
                                                     int max_steps_opt, float* out_pose, float* out_e,
@@ -1096,7 +1113,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
     float* BEST = s.pose(PBEST);
     float* H = s.H();
     float* pb = s.bc(1);  // search direction, read by the pose increment
-    constexpr bool HREG = NM <= kNSmall && L <= 32;
+    constexpr bool HREG = UD_HESS_IN_REGS && NM <= kNSmall && L <= 32;
     float Hr[HREG ? KV : 1][HREG ? NM : 1];
     // MC: the work item's `count` chains are taken by the groups in turn; a
     // group whose chain ends writes its result and starts the next one
@@ -1198,8 +1215,12 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         }
         __syncthreads();
         // initial scaling H = (y.p / y.y) I after the first step (bfgs.h:135-138)
-        const float yy = gdot<L>(y, y);
-        const float yp0 = gdot<L>(y, p);
+        // y.y and y.p only matter after a chain's first BFGS step
+        float yy = 0.f, yp0 = 0.f;
+        if (wave_any(cont && step_now == 0)) {
+            yy = gdot<L>(y, y);
+            yp0 = gdot<L>(y, p);
+        }
         const bool scale0 = cont && step_now == 0 && fabsf(yy) > kEpsFl;
         if constexpr (HREG) {
             const float dv = alpha * yp0 / yy;
