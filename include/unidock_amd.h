@@ -39,7 +39,12 @@ enum {
 };
 
 /* scoring functions (scoring_function.h:34) */
-enum { UD_SF_VINA = 0, UD_SF_VINARDO = 1 };
+enum { UD_SF_VINA = 0, UD_SF_VINARDO = 1, UD_SF_AD4 = 2 };
+/* AutoDock 4.2 map indices beside the AutoDock atom types 0..34
+ * (atom_constants.h:46-81): ad4cache's m_grids[AD_TYPE_SIZE] and
+ * m_grids[AD_TYPE_SIZE + 1] (ad4cache.cpp:309-318) */
+#define UD_AD4_MAP_ELEC 35
+#define UD_AD4_MAP_DSOLV 36
 /* grid-map storage: FP32, or FP16 with a power-of-two scale per map; all
  * interpolation and accumulation is FP32 either way */
 enum { UD_GRID_F32 = 0, UD_GRID_F16 = 1 };
@@ -97,6 +102,13 @@ typedef struct ud_ctx ud_ctx;
  *   pairs[2p]        intramolecular pairs (local atom indices)
  *                    (model.cpp:592-630, ligand.pairs)
  *   num_tors[l]      torsion count of conf_independent (conf_independent.cpp:79-103)
+ * For UD_SF_AD4 (read at ud_set_ligands; may be NULL otherwise):
+ *   atom_ad[a]       AutoDock type 0..34 (atom_constants.h:46-81); every atom
+ *                    is typed, hydrogens included
+ *   atom_charge[a]   partial charge (parse_pdbqt.cpp:112-114, 678-680)
+ *   torsdof[l]       TORSDOF of ad4_tors_add (conf_independent.cpp:183-187)
+ *   and `pairs` lists the pairs of AutoDock-typed atoms (model.cpp:608-612;
+ *   ud_ligset_batch_sf builds either list).
  * Replaces the per-ligand model / m_cuda_t marshalling of do_docking_base
  * (monte_carlo.cu:2487-2631). */
 typedef struct {
@@ -114,6 +126,9 @@ typedef struct {
     const float* node_axis;
     const int32_t* pairs;
     const float* num_tors;
+    const int32_t* atom_ad;
+    const float* atom_charge;
+    const float* torsdof;
 } ud_ligand_batch;
 
 /* Monte Carlo search parameters: the public fields of monte_carlo_template
@@ -143,10 +158,14 @@ const char* ud_last_error(const ud_ctx* ctx);
 
 /* Scoring function and weights.  Vina: 7 weights (gauss1, gauss2, repulsion,
  * hydrophobic, hydrogen, glue, rot); Vinardo: 6 (gauss1, repulsion,
- * hydrophobic, hydrogen, glue, rot).  Replaces Vina::set_vina_weights /
- * set_vinardo_weights + set_forcefield (vina.cpp:393-433) and builds the
+ * hydrophobic, hydrogen, glue, rot); AD4.2: 6 (vdw, hb, elec, dsolv, glue,
+ * rot).  Replaces Vina::set_vina_weights / set_vinardo_weights /
+ * set_ad4_weights + set_forcefield (vina.cpp:393-450) and builds the
  * per-type-pair tables that precalculate_parallel builds per atom pair
- * (precalculate.cu:14-167, precalculate.h:203-241). */
+ * (precalculate.cu:14-167, precalculate.h:203-241); AD4.2's charge-dependent
+ * terms become two shared radial tables times per-pair coefficients.
+ * Ligands and maps are typed for the scoring function in force when they are
+ * set: after switching between AD4.2 and Vina / Vinardo set both again. */
 int ud_set_scoring(ud_ctx* ctx, int sf, const double* weights, int n_weights);
 
 /* Search box: n_voxels = ceil(size / granularity), begin = center - span / 2
@@ -164,6 +183,17 @@ int ud_compute_maps(ud_ctx* ctx, int n_rec, const float* rec_xyz, const int32_t*
 /* Load precomputed FP32 map values (x fastest, one map per listed type) and
  * store them at `precision` (Vina::load_maps, vina.cpp:569-607). */
 int ud_set_maps(ud_ctx* ctx, int n_types, const int32_t* types, const float* values, int precision);
+
+/* AD4.2 maps on the GPU from receptor atoms with AutoDock types and partial
+ * charges: an affinity map per listed AutoDock type plus the electrostatic
+ * (UD_AD4_MAP_ELEC) and desolvation (UD_AD4_MAP_DSOLV) maps, the AutoGrid
+ * decomposition of the reference's AD4.2 pair potentials (potentials.h:
+ * 346-484), so that an atom of type t and charge q at p scores
+ * map_t(p) + q elec(p) + |q| dsolv(p).  The reference reads such maps from
+ * AutoGrid files (ad4cache::read, ad4cache.cpp:391-440; ud_set_maps takes
+ * them with both extra maps in `types`).  Needs UD_SF_AD4. */
+int ud_compute_maps_ad4(ud_ctx* ctx, int n_rec, const float* rec_xyz, const int32_t* rec_ad, const float* rec_charge,
+                        int n_types, const int32_t* types, int precision);
 
 /* Read back one map as FP32 (dequantised for FP16 storage). */
 int ud_get_map(ud_ctx* ctx, int xs_type, float* out);
@@ -193,7 +223,8 @@ int ud_score_grad_device(ud_ctx* ctx, int n_poses, const int32_t* d_pose_ligand,
  * ud_compute_maps (the per-term maps are built from its receptor on first
  * use).  No counterpart in the reference, whose eval_deriv returns the
  * change-space gradient only (ud_score_grad); its `ud_score_grad(...,
- * dE_dw_out)` seam is this call.  Host pointers; pose_ligand is on the host. */
+ * dE_dw_out)` seam is this call.  Vina and Vinardo (UD_ERR_STATE for
+ * AD4.2).  Host pointers; pose_ligand is on the host. */
 int ud_energy_grad(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* poses, const float* v,
                    float* energy_out, float* pose_grad_out, float* weight_grad_out);
 /* Same with device pointers for poses and outputs (pose_ligand stays on the
@@ -223,8 +254,9 @@ int ud_fetch_dock_results(ud_ctx* ctx, float* best_poses, float* best_e, int64_t
 /* Final-score components of poses with authentic caps (v = 1000): inter =
  * grid energy (cache::eval, cache.cpp:84-114), intra = intramolecular pairs
  * with the piecewise-constant table (eval_interacting_pairs,
- * model.cpp:885-905).  Used by the host post-processing (Vina::score_gpu,
- * vina.cpp:1245-1327). */
+ * model.cpp:885-905).  AD4.2: inter also holds the electrostatic and
+ * desolvation maps times q and |q| (ad4cache::eval, ad4cache.cpp:151-185).
+ * Used by the host post-processing (Vina::score_gpu, vina.cpp:1245-1327). */
 int ud_score_terms(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* poses, float* inter_out,
                    float* intra_out);
 
@@ -290,7 +322,9 @@ typedef struct ud_mode {
  * rescored (score_gpu, vina.cpp:1245-1327: maps, or the explicit receptor
  * after refinement), sorted by the final energy and given RMSD bounds from
  * the best mode.  modes_out[l * max_modes + k], n_modes_out[l].  The
- * reference CLI's default refine_step is 3 (main.cpp:557-558). */
+ * reference CLI's default refine_step is 3 (main.cpp:557-558).  AD4.2 modes
+ * are not refined and score inter + w_rot TORSDOF with unbound = -intra
+ * (score_gpu's AD4.2 branch, vina.cpp:1289-1320). */
 int ud_postprocess(ud_ctx* ctx, int max_modes, double min_rmsd, int refine_step, ud_mode* modes_out,
                    int32_t* n_modes_out);
 
@@ -352,6 +386,9 @@ int ud_ligset_count(const ud_ligset* set);
 /* Fill *batch with arrays owned by the set (valid until the next add or
  * destroy), ready for ud_set_ligands. */
 int ud_ligset_batch(ud_ligset* set, ud_ligand_batch* batch);
+/* Same, with the pair list of scoring function sf (UD_SF_AD4: the pairs of
+ * AutoDock-typed atoms, hydrogens included); ud_ligset_batch is sf Vina. */
+int ud_ligset_batch_sf(ud_ligset* set, int sf, ud_ligand_batch* batch);
 /* Input coordinates and atom numbers of ligand `lig` in the batch's atom
  * order (for writing poses back out); n_atoms_out first, arrays may be NULL. */
 int ud_ligset_atoms(const ud_ligset* set, int lig, int32_t* n_atoms_out, float* xyz_out, int32_t* serial_out);
@@ -385,6 +422,11 @@ int ud_ligset_write_sdf(const ud_ligset* set, int lig, int n_modes, const ud_mod
  * (or NULL arrays) nothing is written beyond it and UD_ERR_CAPACITY is
  * returned when arrays were given. */
 int ud_receptor_from_pdbqt(const char* text, int32_t capacity, int32_t* n_out, float* xyz_out, int32_t* xs_out);
+/* Every receptor atom of PDBQT text with its AutoDock type and partial charge
+ * (hydrogens included), the input of ud_compute_maps_ad4; capacity rules as
+ * above. */
+int ud_receptor_ad4_from_pdbqt(const char* text, int32_t capacity, int32_t* n_out, float* xyz_out, int32_t* ad_out,
+                               float* charge_out);
 
 /* Seeded random sequential addition of n_atoms points in [0, edge)^3 with a
  * minimum spacing (synthetic receptors); host-only. */

@@ -94,7 +94,7 @@ static bool is_glued(int a, int b) {
     return false;
 }
 
-enum { SF_VINA = 0, SF_VINARDO = 1 };
+enum { SF_VINA = 0, SF_VINARDO = 1, SF_AD4 = 2 };
 
 // ---------------------------------------------------------------- potentials
 // slope_step, potentials.h:30-39
@@ -189,9 +189,120 @@ template <class R> R sf_eval(int sf, int t1, int t2, R r, const R* w) {
     return acc;
 }
 
+// ---------------------------------------------------------------- AD4.2
+// AutoDock 4.2 atom kinds, by AD type index (atom_constants.h:46-81, 155-194):
+// vdW radius, vdW depth, H-bond depth, H-bond radius, solvation parameter,
+// atomic volume.
+enum { AD_C = 0, AD_A, AD_N, AD_O, AD_P, AD_S, AD_H, AD_F, AD_I, AD_NA, AD_OA, AD_SA, AD_HD, AD_Mg, AD_Mn,
+       AD_Zn, AD_Ca, AD_Fe, AD_Cl, AD_Br, AD_Si, AD_At, AD_G0, AD_G1, AD_G2, AD_G3, AD_CG0, AD_CG1, AD_CG2,
+       AD_CG3, AD_W, AD_OXA, AD_NXA, AD_OXD, AD_NXD, AD_SIZE };
+struct AdKind { double radius, depth, hb_depth, hb_r, solv, vol; };
+static const AdKind kAd[AD_SIZE] = {
+    {2.000, 0.150, 0, 0, -0.00143, 33.5103}, {2.000, 0.150, 0, 0, -0.00052, 33.5103},  // C, A
+    {1.750, 0.160, 0, 0, -0.00162, 22.4493}, {1.600, 0.200, 0, 0, -0.00251, 17.1573},  // N, O
+    {2.100, 0.200, 0, 0, -0.00110, 38.7924}, {2.000, 0.200, 0, 0, -0.00214, 33.5103},  // P, S
+    {1.000, 0.020, 0, 0, 0.00051, 0.0},      {1.545, 0.080, 0, 0, -0.00110, 15.4480},  // H, F
+    {2.360, 0.550, 0, 0, -0.00110, 55.0585},                                            // I
+    {1.750, 0.160, -5, 1.9, -0.00162, 22.4493}, {1.600, 0.200, -5, 1.9, -0.00251, 17.1573},  // NA, OA
+    {2.000, 0.200, -1, 2.5, -0.00214, 33.5103}, {1.000, 0.020, 1, 0, 0.00051, 0.0},          // SA, HD
+    {0.650, 0.875, 0, 0, -0.00110, 1.560},  {0.650, 0.875, 0, 0, -0.00110, 2.140},   // Mg, Mn
+    {0.740, 0.550, 0, 0, -0.00110, 1.700},  {0.990, 0.550, 0, 0, -0.00110, 2.770},   // Zn, Ca
+    {0.650, 0.010, 0, 0, -0.00110, 1.840},  {2.045, 0.276, 0, 0, -0.00110, 35.8235},  // Fe, Cl
+    {2.165, 0.389, 0, 0, -0.00110, 42.5661}, {2.300, 0.200, 0, 0, -0.00143, 50.9650},  // Br, Si
+    {2.400, 0.550, 0, 0, -0.00110, 57.9058},                                            // At
+    {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0},     // G0..G3
+    {2.000, 0.150, 0, 0, -0.00143, 33.5103}, {2.000, 0.150, 0, 0, -0.00143, 33.5103},  // CG0, CG1
+    {2.000, 0.150, 0, 0, -0.00143, 33.5103}, {2.000, 0.150, 0, 0, -0.00143, 33.5103},  // CG2, CG3
+    {0, 0, 0, 0, 0, 0},                                                                 // W
+    {1.600, 0.200, 0, 0, -0.00251, 17.1573}, {1.750, 0.160, 0, 0, -0.00162, 22.4493},  // OXA, NXA
+    {1.600, 0.200, 0, 0, -0.00251, 17.1573}, {1.750, 0.160, 0, 0, -0.00162, 22.4493},  // OXD, NXD
+};
+static const double kAd4Cutoff = 20.48;   // scoring_function.h:75-83
+static const double kAd4VdwCutoff = 8.0, kAd4Cap = 100000.0, kAd4Smoothing = 0.5;
+static const double kAd4ElecCap = 100.0, kAd4Sigma = 3.6, kAd4SolvQ = 0.01097;
+
+template <class R> R ipow(R x, int n) { R y = 1; for (int i = 0; i < n; ++i) y *= x; return y; }
+// smoothen (potentials.h:67-79)
+template <class R> R ad4_smoothen(R r, R rij, R smoothing) {
+    smoothing *= R(0.5);
+    if (r > rij + smoothing) return r - smoothing;
+    if (r < rij - smoothing) return r + smoothing;
+    return rij;
+}
+// ad4_vdw::eval (potentials.h:421-447): 12-6 with smoothing, not for H-bond pairs
+template <class R> R ad4_vdw(int t1, int t2, R r) {
+    if (r >= R(kAd4VdwCutoff)) return 0;
+    const R hb_depth = R(kAd[t1].hb_depth) * R(kAd[t2].hb_depth);
+    const R rij = R(kAd[t1].radius) + R(kAd[t2].radius);
+    const R depth = std::sqrt(R(kAd[t1].depth) * R(kAd[t2].depth));
+    if (hb_depth < 0) return 0;
+    r = ad4_smoothen(r, rij, R(kAd4Smoothing));
+    const R c12 = ipow(rij, 12) * depth, c6 = ipow(rij, 6) * depth * R(2);
+    const R r6 = ipow(r, 6), r12 = ipow(r, 12);
+    if (r12 > R(kEpsFl) && r6 > R(kEpsFl)) return std::min(R(kAd4Cap), c12 / r12 - c6 / r6);
+    return R(kAd4Cap);
+}
+// ad4_hb::eval (potentials.h:454-480): 12-10 with smoothing, H-bond pairs only
+template <class R> R ad4_hb(int t1, int t2, R r) {
+    if (r >= R(kAd4VdwCutoff)) return 0;
+    const R rij = R(kAd[t1].hb_r) + R(kAd[t2].hb_r);
+    const R hb_depth = R(kAd[t1].hb_depth) * R(kAd[t2].hb_depth);
+    if (hb_depth >= 0) return 0;
+    r = ad4_smoothen(r, rij, R(kAd4Smoothing));
+    const R c12 = ipow(rij, 12) * -hb_depth * R(10) / R(2.0), c10 = ipow(rij, 10) * -hb_depth * R(12) / R(2.0);
+    const R r10 = ipow(r, 10), r12 = ipow(r, 12);
+    if (r12 > R(kEpsFl) && r10 > R(kEpsFl)) return std::min(R(kAd4Cap), c12 / r12 - c10 / r10);
+    return R(kAd4Cap);
+}
+// ad4_electrostatic::eval (potentials.h:350-366) without the charges:
+// 332 min(cap, 1 / (r eps(r))) with the sigmoidal distance-dependent dielectric
+template <class R> R ad4_elec_r(R r) {
+    if (r >= R(kAd4Cutoff)) return 0;
+    const R B = R(78.4) + R(8.5525);
+    const R lB = -B * R(0.003627);
+    const R diel = R(-8.5525) + (B / (R(1) + R(7.7839) * std::exp(lB * r)));
+    if (r < R(kEpsFl)) return R(332.0) * R(kAd4ElecCap) / diel;
+    return R(332.0) * std::min(R(kAd4ElecCap), R(1.0) / (r * diel));
+}
+// the Gaussian of ad4_solvation::eval (potentials.h:380-400)
+template <class R> R ad4_solv_r(R r) {
+    if (r >= R(kAd4Cutoff)) return 0;
+    return std::exp(R(-0.5) * sqr(r / R(kAd4Sigma)));
+}
+// ad4_solvation's coefficient of the Gaussian for atoms of types t1, t2 and
+// charges q1, q2 (charge dependent, potentials.h:384-396)
+template <class R> R ad4_solv_coef(int t1, R q1, int t2, R q2) {
+    return (R(kAd[t1].solv) + R(kAd4SolvQ) * std::abs(q1)) * R(kAd[t2].vol) +
+           (R(kAd[t2].solv) + R(kAd4SolvQ) * std::abs(q2)) * R(kAd[t1].vol);
+}
+// ScoringFunction::eval(atom a, atom b, r) for AD4.2 (scoring_function.h:74-85):
+// w = (vdw, hb, elec, dsolv); the glue term only joins G / CG closure atoms,
+// whose pairs are never ligand pairs.
+template <class R> R sf_eval_ad4(int t1, R q1, int t2, R q2, R r, const R* w) {
+    R acc = w[0] * ad4_vdw(t1, t2, r);
+    acc += w[1] * ad4_hb(t1, t2, r);
+    acc += w[2] * q1 * q2 * ad4_elec_r(r);
+    acc += w[3] * ad4_solv_coef(t1, q1, t2, q2) * ad4_solv_r(r);
+    return acc;
+}
+// AD type remapping of ad4cache::eval_deriv (ad4cache.cpp:223-240): G types
+// carry no grid term, CG types use the C map.
+static int ad4_grid_type(int t) {
+    if (t < 0 || t >= AD_SIZE) return -1;
+    if (t >= AD_G0 && t <= AD_G3) return -1;
+    if (t >= AD_CG0 && t <= AD_CG3) return AD_C;
+    return t;
+}
+static const int kMapElec = AD_SIZE, kMapDsolv = AD_SIZE + 1;  // m_grids[AD_TYPE_SIZE (+1)]
+
 // Number of distance samples: sz(factor * max_cutoff_sqr) + 3 (precalculate.h:142-145)
 static const int kFactor = 32;
 static const int kTableN = 32 * 64 + 3;  // 2051
+static int table_n(int sf) {  // 13424 for the 20.48 A cutoff of AD4.2
+    if (sf != SF_AD4) return kTableN;
+    const float c = float(kAd4Cutoff);
+    return int(float(kFactor) * (c * c)) + 3;
+}
 
 // precalculate_element::init_from_smooth_fst (precalculate.h:66-85) applied to
 // smooth[].first = min(v, sf.eval(t1, t2, rs[i])) with v = max_fl
@@ -199,12 +310,19 @@ static const int kTableN = 32 * 64 + 3;  // 2051
 template <class R> struct PairTable {
     std::vector<R> e, dor, fast;
     void build(int sf, int t1, int t2, const R* w) {
-        const int n = kTableN;
+        build_from(kTableN, [&](R r) { return sf_eval<R>(sf, t1, t2, r, w); });
+    }
+    // precalculate_byatom (precalculate.h:206-241): the table of one atom
+    // pair, sampled from the atom-based ScoringFunction::eval
+    void build_ad4(int t1, R q1, int t2, R q2, const R* w) {
+        build_from(table_n(SF_AD4), [&](R r) { return sf_eval_ad4<R>(t1, q1, t2, q2, r, w); });
+    }
+    template <class F> void build_from(int n, F f) {
         e.assign(n, 0); dor.assign(n, 0); fast.assign(n, 0);
         std::vector<R> rs(n);
         for (int i = 0; i < n; ++i) rs[i] = std::sqrt(R(i) / R(kFactor));
         for (int i = 0; i < n; ++i) {
-            R v = sf_eval<R>(sf, t1, t2, rs[i], w);
+            R v = f(rs[i]);
             e[i] = std::min<R>(R(kMaxFl), v);
         }
         for (int i = 0; i < n; ++i) {
@@ -346,7 +464,7 @@ template <class R> struct Grid {
             factor[i] = dim_minus_1[i] / range;
             factor_inv[i] = R(1) / factor[i];
         }
-        maps.assign(XS_SIZE, std::vector<R>());
+        maps.assign(std::max<int>(XS_SIZE, AD_SIZE + 2), std::vector<R>());
     }
     R& at(int t, int x, int y, int z) { return maps[t][x + n[0] * (y + n[1] * size_t(z))]; }
     R at(int t, int x, int y, int z) const { return maps[t][x + n[0] * (y + n[1] * size_t(z))]; }
@@ -434,6 +552,11 @@ template <class R> struct Ligand {
     std::vector<V3<R>> rel_origin, rel_axis, origin0;  // per node (origin0: input-frame origin)
     std::vector<int> pa, pb, ptp;   // per pair
     R num_tors = 0;
+    // AD4.2: AutoDock types, partial charges, TORSDOF and one table per pair
+    std::vector<int> ad;
+    std::vector<R> q;
+    R torsdof = 0;
+    std::vector<PairTable<R>> ptab;
     int n_tors() const { return n_nodes - 1; }
 };
 
@@ -466,6 +589,13 @@ template <class R> struct Ctx {
         sf = sf_;
         int nw = (sf == SF_VINA) ? 7 : 6;
         for (int i = 0; i < 7; ++i) w[i] = 0;
+        if (sf == SF_AD4) {  // vdw, hb, elec, dsolv, glue, rot used as given (vina.cpp:435-450)
+            for (int i = 0; i < 6; ++i) w[i] = R(weights[i]);
+            int ntp = XS_SIZE * (XS_SIZE + 1) / 2;
+            tables.assign(ntp, PairTable<R>());
+            have.assign(ntp, 0);
+            return;
+        }
         // weights as given to set_vina_weights/set_vinardo_weights; the last
         // is weight_rot, turned into the internal 5*w_rot/0.1 - 1 (vina.cpp:404, 425)
         for (int i = 0; i < nw - 1; ++i) w[i] = R(weights[i]);
@@ -562,6 +692,25 @@ template <class R> R nc_atom(Ctx<R>& C, int t, const V3<R>& p, R slope, R v, V3<
     return e + pen;
 }
 
+// ad4cache::eval_deriv / ad4cache::eval (ad4cache.cpp:151-270) for one atom
+// of AD type t and charge q at p: the affinity map of its type, the
+// electrostatic map times q and the desolvation map times |q|, each
+// evaluated (with its own curl and out-of-box penalty) by grid::evaluate.
+template <class R> R ad4_grid_atom(const Ctx<R>& C, int t, R q, const V3<R>& p, R v, V3<R>* deriv) {
+    t = ad4_grid_type(t);
+    if (t < 0 || C.grid.maps[t].empty()) return 0;
+    V3<R> d;
+    R e = grid_eval(C.grid, t, p, C.slope, v, deriv ? &d : nullptr);
+    if (deriv) *deriv = d;
+    const R f[2] = {q, std::abs(q)};
+    const int m[2] = {kMapElec, kMapDsolv};
+    for (int k = 0; k < 2; ++k) {
+        e += grid_eval(C.grid, m[k], p, C.slope, v, deriv ? &d : nullptr) * f[k];
+        if (deriv) *deriv = *deriv + f[k] * d;
+    }
+    return e;
+}
+
 // model::eval_deriv (model.cpp:954-981) for a single ligand without flex
 // residues: grid term (cache::eval_deriv), intramolecular pairs
 // (eval_interacting_pairs_deriv, model.cpp:907-934), then the torsion-tree
@@ -573,7 +722,11 @@ template <class R> R eval_deriv(Ctx<R>& C, const Ligand<R>& L, const Pose<R>& P,
     forward_kinematics(L, P, coords, origins, axes);
     std::vector<V3<R>> mf(L.n_atoms);
     R e = 0;
-    for (int i = 0; i < L.n_atoms; ++i) {
+    for (int i = 0; i < L.n_atoms && C.sf == SF_AD4; ++i) {
+        mf[i] = V3<R>();
+        e += ad4_grid_atom(C, L.ad[i], L.q[i], coords[i], v[1], &mf[i]);
+    }
+    for (int i = 0; i < L.n_atoms && C.sf != SF_AD4; ++i) {
         int t = grid_type(L.xs[i]);
         if (C.use_nc) {  // the explicit receptor: G and untyped atoms skipped, table of the atom's own type
             V3<R> d;
@@ -588,14 +741,16 @@ template <class R> R eval_deriv(Ctx<R>& C, const Ligand<R>& L, const Pose<R>& P,
     }
     R einter = e;
     R eintra = 0;
-    const R cutoff_sqr = R(64);  // sqr(sf.get_cutoff()), scoring_function.h:56,66
+    // sqr(sf.get_cutoff()), scoring_function.h:56,66,82
+    const R cutoff_sqr = (C.sf == SF_AD4) ? sqr(R(kAd4Cutoff)) : R(64);
     for (int k = 0; k < L.n_pairs; ++k) {
         const int a = L.pa[k], b = L.pb[k];
         V3<R> r = coords[b] - coords[a];
         R r2 = norm_sqr(r);
         if (r2 < cutoff_sqr) {
             R pe, dor;
-            C.table(L.xs[a], L.xs[b]).eval_deriv(r2, pe, dor);
+            if (C.sf == SF_AD4) L.ptab[k].eval_deriv(r2, pe, dor);
+            else C.table(L.xs[a], L.xs[b]).eval_deriv(r2, pe, dor);
             V3<R> force = dor * r;
             curl(pe, force, v[0]);
             eintra += pe;
@@ -877,16 +1032,21 @@ template <class R> void score_terms(Ctx<R>& C, const Ligand<R>& L, const Pose<R>
     forward_kinematics(L, P, coords, origins, axes);
     R e = 0;
     for (int i = 0; i < L.n_atoms; ++i) {
+        if (C.sf == SF_AD4) {
+            e += ad4_grid_atom(C, L.ad[i], L.q[i], coords[i], R(1000), (V3<R>*)nullptr);
+            continue;
+        }
         int t = grid_type(L.xs[i]);
         if (t < 0 || C.grid.maps[t].empty()) continue;
         e += grid_eval(C.grid, t, coords[i], C.slope, R(1000), (V3<R>*)nullptr);
     }
     R ei = 0;
+    const R cutoff_sqr = (C.sf == SF_AD4) ? sqr(R(kAd4Cutoff)) : R(64);
     for (int k = 0; k < L.n_pairs; ++k) {
         const int a = L.pa[k], b = L.pb[k];
         R r2 = norm_sqr(coords[b] - coords[a]);
-        if (r2 < R(64)) {
-            R t = C.table(L.xs[a], L.xs[b]).eval_fast(r2);
+        if (r2 < cutoff_sqr) {
+            R t = (C.sf == SF_AD4) ? L.ptab[k].eval_fast(r2) : C.table(L.xs[a], L.xs[b]).eval_fast(r2);
             curl(t, R(1000));
             ei += t;
         }
@@ -895,6 +1055,8 @@ template <class R> void score_terms(Ctx<R>& C, const Ligand<R>& L, const Pose<R>
     *intra = ei;
 }
 template <class R> R conf_independent(const Ctx<R>& C, R num_tors, R x) {
+    // AD4.2: ad4_tors_add (conf_independent.cpp:183-187), num_tors = TORSDOF
+    if (C.sf == SF_AD4) return x + C.w[5] * num_tors;
     const R weight = R(0.1) * (C.w[6] + R(1));
     const R y = R(1) + weight * num_tors / R(5.0);
     if (std::abs(x) < R(kEpsFl)) return 0;
@@ -987,6 +1149,69 @@ template <class R> void build_maps(Ctx<R>& C, int n_rec, const double* rec_xyz, 
                         for (int j = 0; j < n_types; ++j) acc[j] += C.table(rt[i], types[j]).eval_fast(r2);
                 }
                 for (int j = 0; j < n_types; ++j) g.at(types[j], x, y, z) = acc[j];
+            }
+    };
+    const int nt = std::max(1, std::min<int>(16, int(std::thread::hardware_concurrency())));
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; ++i)
+        th.emplace_back([&] {
+            for (int z = next.fetch_add(1); z < g.n[2]; z = next.fetch_add(1)) slice(z);
+        });
+    for (auto& t : th) t.join();
+}
+
+// AutoGrid-style AD4.2 maps from the pair potentials above, so that a
+// ligand atom of type t and charge q at p scores
+//   map_t(p) + q elec(p) + |q| dsolv(p) = sum over receptor atoms j of
+//   sf_eval_ad4(t, q, t_j, q_j, r):
+//   map_t  = sum_j w_vdw vdw + w_hb hb + w_dsolv (solv_t V_j + (solv_j + Q |q_j|) V_t) G(r)
+//   elec   = sum_j w_elec q_j 332 / (r eps(r))   (capped)
+//   dsolv  = sum_j w_dsolv Q V_j G(r)
+// (the reference reads such maps from AutoGrid files, ad4cache::read,
+// ad4cache.cpp:391-440).  Brute force over every receptor atom.
+template <class R> void build_maps_ad4(Ctx<R>& C, int n_rec, const double* rec_xyz, const int* rec_ad,
+                                       const double* rec_q, int n_types, const int* types) {
+    Grid<R>& g = C.grid;
+    std::vector<V3<R>> ra;
+    std::vector<int> rt;
+    std::vector<R> rq;
+    for (int i = 0; i < n_rec; ++i) {
+        if (rec_ad[i] < 0 || rec_ad[i] >= AD_SIZE) continue;
+        ra.push_back(V3<R>(R(rec_xyz[3 * i]), R(rec_xyz[3 * i + 1]), R(rec_xyz[3 * i + 2])));
+        rt.push_back(rec_ad[i]);
+        rq.push_back(R(rec_q[i]));
+    }
+    const size_t np = size_t(g.n[0]) * g.n[1] * g.n[2];
+    for (int j = 0; j < n_types; ++j) g.maps[types[j]].assign(np, R(0));
+    g.maps[kMapElec].assign(np, R(0));
+    g.maps[kMapDsolv].assign(np, R(0));
+    const R* w = C.w;
+    const R cut2 = sqr(R(kAd4Cutoff));
+    auto slice = [&](int z) {
+        std::vector<R> acc(n_types + 2);
+        for (int y = 0; y < g.n[1]; ++y)
+            for (int x = 0; x < g.n[0]; ++x) {
+                V3<R> p = g.point(x, y, z);
+                std::fill(acc.begin(), acc.end(), R(0));
+                for (size_t i = 0; i < ra.size(); ++i) {
+                    const R r2 = norm_sqr(ra[i] - p);
+                    if (!(r2 < cut2)) continue;
+                    const R r = std::sqrt(r2);
+                    const R G = ad4_solv_r(r);
+                    const int tj = rt[i];
+                    const R Sj = R(kAd[tj].solv) + R(kAd4SolvQ) * std::abs(rq[i]);
+                    for (int j = 0; j < n_types; ++j) {
+                        const int t = types[j];
+                        acc[j] += w[0] * ad4_vdw(t, tj, r) + w[1] * ad4_hb(t, tj, r) +
+                                  w[3] * (R(kAd[t].solv) * R(kAd[tj].vol) + Sj * R(kAd[t].vol)) * G;
+                    }
+                    acc[n_types] += w[2] * rq[i] * ad4_elec_r(r);
+                    acc[n_types + 1] += w[3] * R(kAd4SolvQ) * R(kAd[tj].vol) * G;
+                }
+                for (int j = 0; j < n_types; ++j) g.at(types[j], x, y, z) = acc[j];
+                g.at(kMapElec, x, y, z) = acc[n_types];
+                g.at(kMapDsolv, x, y, z) = acc[n_types + 1];
             }
     };
     const int nt = std::max(1, std::min<int>(16, int(std::thread::hardware_concurrency())));
@@ -1270,6 +1495,50 @@ extern "C" double orc_nc_atom(OrcHandle* h, int t, const double* loc, double slo
                       deriv ? &d : nullptr);
     if (deriv) for (int k = 0; k < 3; ++k) deriv[k] = d[k];
     return e;
+}
+// AD4.2 data of ligand `lig` (AD types, charges, TORSDOF); builds the
+// per-pair tables.  The context must have been created with SF_AD4.
+template <class R> static void set_ad4(Ctx<R>& C, int lig, const int* ad, const double* q, double torsdof) {
+    Ligand<R>& L = C.ligs[lig];
+    L.ad.assign(ad, ad + L.n_atoms);
+    L.q.resize(L.n_atoms);
+    for (int i = 0; i < L.n_atoms; ++i) L.q[i] = R(q[i]);
+    L.torsdof = R(torsdof);
+    L.ptab.assign(L.n_pairs, PairTable<R>());
+    for (int k = 0; k < L.n_pairs; ++k)
+        L.ptab[k].build_ad4(L.ad[L.pa[k]], L.q[L.pa[k]], L.ad[L.pb[k]], L.q[L.pb[k]], C.w);
+}
+extern "C" int orc_set_ad4(OrcHandle* h, int lig, const int* ad, const double* q, double torsdof) {
+    if (h->d.sf != SF_AD4 || lig < 0 || lig >= int(h->d.ligs.size())) return 1;
+    set_ad4(h->f, lig, ad, q, torsdof);
+    set_ad4(h->d, lig, ad, q, torsdof);
+    return 0;
+}
+extern "C" void orc_build_maps_ad4(OrcHandle* h, int n_rec, const double* xyz, const int* ad, const double* q,
+                                   int n_types, const int* types) {
+    if (h->prec == 64) build_maps_ad4(h->d, n_rec, xyz, ad, q, n_types, types);
+    else build_maps_ad4(h->f, n_rec, xyz, ad, q, n_types, types);
+}
+// AD4.2 potentials at r for atoms (t1, q1), (t2, q2): out = vdw, hb, elec
+// (with the charges), solvation, weighted sum
+extern "C" void orc_ad4_terms(OrcHandle* h, int t1, double q1, int t2, double q2, double r, double* out) {
+    out[0] = ad4_vdw<double>(t1, t2, r);
+    out[1] = ad4_hb<double>(t1, t2, r);
+    out[2] = q1 * q2 * ad4_elec_r<double>(r);
+    out[3] = ad4_solv_coef<double>(t1, q1, t2, q2) * ad4_solv_r<double>(r);
+    out[4] = sf_eval_ad4<double>(t1, q1, t2, q2, r, h->d.w);
+}
+// the AD4.2 kind of AD type t: radius, depth, hb_depth, hb_r, solv, vol
+extern "C" void orc_ad4_kind(int t, double* out) {
+    const AdKind& k = kAd[t];
+    out[0] = k.radius; out[1] = k.depth; out[2] = k.hb_depth; out[3] = k.hb_r; out[4] = k.solv; out[5] = k.vol;
+}
+// sample tables of one AD4.2 atom pair (table_n(SF_AD4) doubles each)
+extern "C" int orc_pair_table_ad4(OrcHandle* h, int t1, double q1, int t2, double q2, double* e, double* dor, double* fast) {
+    PairTable<double> T;
+    T.build_ad4(t1, q1, t2, q2, h->d.w);
+    for (size_t i = 0; i < T.e.size(); ++i) { e[i] = T.e[i]; dor[i] = T.dor[i]; fast[i] = T.fast[i]; }
+    return int(T.e.size());
 }
 extern "C" double orc_conf_independent(OrcHandle* h, double num_tors, double x) {
     if (h->prec == 64) return conf_independent(h->d, num_tors, x);

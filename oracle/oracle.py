@@ -16,6 +16,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 POSE = 7 + 32  # pose stride shared with the product's C ABI
+SF_VINA, SF_VINARDO, SF_AD4 = 0, 1, 2
+AD4_WEIGHTS = (0.1662, 0.1209, 0.1406, 0.1322, 50.0, 0.2983)  # vina.h:136-138
+AD4_MAP_ELEC, AD4_MAP_DSOLV = 35, 36  # map indices of the electrostatic / desolvation maps
+AD4_TABLE_N = 13424  # sz(32 * 20.48^2) + 3 samples per AD4.2 pair table
 
 
 def build():
@@ -61,6 +65,13 @@ def load():
         "orc_nc_atom": (ctypes.c_double, [vp, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, dp]),
         "orc_eval_deriv_nc": (ctypes.c_double, [vp, ctypes.c_int, dp, dp, ctypes.c_double, dp]),
         "orc_conf_independent": (ctypes.c_double, [vp, ctypes.c_double, ctypes.c_double]),
+        "orc_set_ad4": (ctypes.c_int, [vp, ctypes.c_int, ip, dp, ctypes.c_double]),
+        "orc_build_maps_ad4": (None, [vp, ctypes.c_int, dp, ip, dp, ctypes.c_int, ip]),
+        "orc_ad4_terms": (None, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 dp]),
+        "orc_ad4_kind": (None, [ctypes.c_int, dp]),
+        "orc_pair_table_ad4": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, dp,
+                                              dp, dp]),
         "orc_dock_cpu": (ctypes.c_double, [vp, ctypes.c_int, ip, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ip,
                                            ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)]),
     }
@@ -80,6 +91,13 @@ def _i(a):
     return np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
+def ad4_kind(t):
+    """(radius, depth, hb_depth, hb_r, solvation, volume) of AutoDock type t."""
+    out = np.zeros(6)
+    load().orc_ad4_kind(t, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
 def philox(ctr, key):
     L = load()
     c = (ctypes.c_uint32 * 4)(*ctr)
@@ -95,12 +113,14 @@ class Oracle:
     def __init__(self, prec=64, sf=0, weights=None):
         from_w = weights
         if from_w is None:
-            from_w = (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 50.0, 0.05846) if sf == 0 else \
-                (-0.045, 0.8, -0.035, -0.600, 50.0, 0.05846)
+            from_w = {SF_VINA: (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 50.0, 0.05846),
+                      SF_VINARDO: (-0.045, 0.8, -0.035, -0.600, 50.0, 0.05846),
+                      SF_AD4: AD4_WEIGHTS}[sf]
         self.L = load()
         self.w = np.array(from_w, dtype=np.float64)
         self.h = self.L.orc_new(prec, sf, _d(self.w))
         self.prec = prec
+        self.sf = sf
         self.ligs = []
 
     def __del__(self):
@@ -149,13 +169,42 @@ class Oracle:
                                  d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if deriv else None)
         return e, d
 
-    def add_ligand(self, lig):
-        """lig: unidock_tools.synth.Ligand (or any object with the same fields)."""
+    def add_ligand(self, lig, pairs=None):
+        """lig: unidock_tools.synth.Ligand (or any object with the same fields).
+        With AD4.2 scoring the ligand's AutoDock types, charges and TORSDOF
+        (lig.ad, lig.charge, lig.torsdof) are used and `pairs` (default
+        lig.pairs) is the pair list of AD-typed atoms."""
+        pr = np.asarray(lig.pairs if pairs is None else pairs, dtype=np.int32).reshape(-1, 2)
         idx = self.L.orc_add_ligand(self.h, lig.n_atoms, _d(lig.local), _i(lig.xs), _i(lig.heavy), len(lig.parent),
                                     _i(lig.parent), _i(lig.abegin), _i(lig.aend), _d(lig.origin), _d(lig.axis),
-                                    len(lig.pairs), _i(lig.pairs), float(lig.num_tors))
+                                    len(pr), _i(pr), float(lig.num_tors))
+        if self.sf == SF_AD4:
+            rc = self.L.orc_set_ad4(self.h, idx, _i(lig.ad), _d(lig.charge), float(lig.torsdof))
+            assert rc == 0
         self.ligs.append(lig)
         return idx
+
+    def build_maps_ad4(self, rec_xyz, rec_ad, rec_charge, types):
+        """AD4.2 affinity maps of the listed AD types plus the electrostatic
+        (AD4_MAP_ELEC) and desolvation (AD4_MAP_DSOLV) maps."""
+        x = np.ascontiguousarray(rec_xyz, dtype=np.float64)
+        t = np.ascontiguousarray(rec_ad, dtype=np.int32)
+        ty = np.ascontiguousarray(types, dtype=np.int32)
+        self.L.orc_build_maps_ad4(self.h, len(t), _d(x), _i(t), _d(rec_charge), len(ty), _i(ty))
+
+    def ad4_terms(self, t1, q1, t2, q2, r):
+        """(vdw, hb, elec with charges, solvation, weighted sum) of two atoms at r."""
+        out = np.zeros(5)
+        self.L.orc_ad4_terms(self.h, t1, q1, t2, q2, r, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def pair_table_ad4(self, t1, q1, t2, q2):
+        e = np.zeros(AD4_TABLE_N); dor = np.zeros(AD4_TABLE_N); fast = np.zeros(AD4_TABLE_N)
+        n = self.L.orc_pair_table_ad4(self.h, t1, q1, t2, q2, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      dor.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      fast.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        assert n == AD4_TABLE_N
+        return e, dor, fast
 
     def eval_deriv(self, lig, pose, v=(1000.0, 1000.0, 1000.0)):
         n_t = self.ligs[lig].n_tors

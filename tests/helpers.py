@@ -76,3 +76,36 @@ def random_poses(rng, lig, center, spread, n):
         out[i, 3:7] = q / np.linalg.norm(q)
         out[i, 7:7 + lig.n_tors] = rng.uniform(-np.pi, np.pi, lig.n_tors)
     return out
+
+
+# ---------------------------------------------------------------- AutoDock 4.2 scenes
+@functools.lru_cache(maxsize=None)
+def small_receptor_ad4():
+    """small_receptor with AutoDock types and partial charges: (xyz, ad, q, cav)."""
+    xyz, xs, cav = small_receptor()
+    rec = synth.Receptor(xyz=xyz, xs=xs, edge=24.0, cavity=cav)
+    ad, q = synth.ad4_receptor(rec, seed=17)
+    return xyz, ad, q, cav
+
+
+@functools.lru_cache(maxsize=None)
+def small_ligands_ad4():
+    return tuple(synth.ad4_ligand(l, seed=100 + i) for i, l in enumerate(small_ligands()))
+
+
+def ad4_map_types(ligs):
+    from unidock_tools import _lib
+    return sorted({_lib.ad4_grid_type(int(t)) for l in ligs for t in l.ad} - {-1})
+
+
+def oracle_scene_ad4(prec=64, size=10.0):
+    """AD4.2 oracle with maps built from the small receptor over a `size` box."""
+    xyz, ad, q, cav = small_receptor_ad4()
+    ligs = small_ligands_ad4()
+    o = Oracle(prec=prec, sf=2)
+    c, s = box_of(cav, size)
+    o.set_grid(c, s, 0.375)
+    o.build_maps_ad4(xyz, ad, q, ad4_map_types(ligs))
+    for l in ligs:
+        o.add_ligand(l)
+    return o, ligs, (xyz, ad, q, cav)

@@ -18,6 +18,75 @@
 
 using namespace ud;
 
+namespace ud {
+// AutoDock 4.2 atom kinds by AD type index (atom_constants.h:155-194)
+const Ad4Params& ad4_params() {
+    static const Ad4Params P = [] {
+        const float t[kAd4Types][6] = {
+            {2.000f, 0.150f, 0.f, 0.f, -0.00143f, 33.5103f},  {2.000f, 0.150f, 0.f, 0.f, -0.00052f, 33.5103f},   // C A
+            {1.750f, 0.160f, 0.f, 0.f, -0.00162f, 22.4493f},  {1.600f, 0.200f, 0.f, 0.f, -0.00251f, 17.1573f},   // N O
+            {2.100f, 0.200f, 0.f, 0.f, -0.00110f, 38.7924f},  {2.000f, 0.200f, 0.f, 0.f, -0.00214f, 33.5103f},   // P S
+            {1.000f, 0.020f, 0.f, 0.f, 0.00051f, 0.f},        {1.545f, 0.080f, 0.f, 0.f, -0.00110f, 15.4480f},   // H F
+            {2.360f, 0.550f, 0.f, 0.f, -0.00110f, 55.0585f},                                                     // I
+            {1.750f, 0.160f, -5.f, 1.9f, -0.00162f, 22.4493f}, {1.600f, 0.200f, -5.f, 1.9f, -0.00251f, 17.1573f},  // NA OA
+            {2.000f, 0.200f, -1.f, 2.5f, -0.00214f, 33.5103f}, {1.000f, 0.020f, 1.f, 0.f, 0.00051f, 0.f},          // SA HD
+            {0.650f, 0.875f, 0.f, 0.f, -0.00110f, 1.560f},    {0.650f, 0.875f, 0.f, 0.f, -0.00110f, 2.140f},     // Mg Mn
+            {0.740f, 0.550f, 0.f, 0.f, -0.00110f, 1.700f},    {0.990f, 0.550f, 0.f, 0.f, -0.00110f, 2.770f},     // Zn Ca
+            {0.650f, 0.010f, 0.f, 0.f, -0.00110f, 1.840f},    {2.045f, 0.276f, 0.f, 0.f, -0.00110f, 35.8235f},   // Fe Cl
+            {2.165f, 0.389f, 0.f, 0.f, -0.00110f, 42.5661f},  {2.300f, 0.200f, 0.f, 0.f, -0.00143f, 50.9650f},   // Br Si
+            {2.400f, 0.550f, 0.f, 0.f, -0.00110f, 57.9058f},                                                     // At
+            {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0},                      // G0-G3
+            {2.000f, 0.150f, 0.f, 0.f, -0.00143f, 33.5103f},  {2.000f, 0.150f, 0.f, 0.f, -0.00143f, 33.5103f},   // CG0 CG1
+            {2.000f, 0.150f, 0.f, 0.f, -0.00143f, 33.5103f},  {2.000f, 0.150f, 0.f, 0.f, -0.00143f, 33.5103f},   // CG2 CG3
+            {0, 0, 0, 0, 0, 0},                                                                                  // W
+            {1.600f, 0.200f, 0.f, 0.f, -0.00251f, 17.1573f},  {1.750f, 0.160f, 0.f, 0.f, -0.00162f, 22.4493f},   // OXA NXA
+            {1.600f, 0.200f, 0.f, 0.f, -0.00251f, 17.1573f},  {1.750f, 0.160f, 0.f, 0.f, -0.00162f, 22.4493f},   // OXD NXD
+        };
+        Ad4Params p;
+        for (int i = 0; i < kAd4Types; ++i) {
+            p.radius[i] = t[i][0]; p.depth[i] = t[i][1]; p.hb_depth[i] = t[i][2];
+            p.hb_r[i] = t[i][3]; p.solv[i] = t[i][4]; p.vol[i] = t[i][5];
+        }
+        return p;
+    }();
+    return P;
+}
+}  // namespace ud
+
+namespace {
+// The charge-dependent AD4.2 pair terms as shared radial tables, sampled like
+// precalculate_byatom (precalculate.h:206-241, 66-85) at rs[k] = sqrt(k / 32),
+// k < kAd4TableN: E(r) = w_elec 332 min(100, 1 / (r eps(r))) (times q_a q_b per
+// pair) and G(r) = w_dsolv exp(-r^2 / (2 3.6^2)) (times the pair's
+// desolvation coefficient), both 0 from 20.48 A (ad4_electrostatic,
+// ad4_solvation, potentials.h:346-416).  deriv = (E, dE/r, G, dG/r) with
+// the central-difference dor and 0 at both ends; fast = midpoints.
+void ad4_radial_tables(float w_elec, float w_dsolv, std::vector<float4>& deriv, std::vector<float2>& fast) {
+    const int n = kAd4TableN;
+    std::vector<double> rs(n), E(n), G(n);
+    const double B = 78.4 + 8.5525, lB = -B * 0.003627;
+    for (int k = 0; k < n; ++k) {
+        const double r = std::sqrt(double(k) / 32.0);
+        rs[k] = r;
+        if (r >= double(kAd4Cutoff)) { E[k] = G[k] = 0.0; continue; }
+        const double diel = -8.5525 + B / (1.0 + 7.7839 * std::exp(lB * r));
+        E[k] = double(w_elec) * 332.0 * ((r < 1.1920928955078125e-07) ? 100.0 / diel : std::min(100.0, 1.0 / (r * diel)));
+        G[k] = double(w_dsolv) * std::exp(-0.5 * (r / 3.6) * (r / 3.6));
+    }
+    auto dor = [&](const std::vector<double>& e, int k) {
+        if (k == 0 || k == n - 1) return 0.0;
+        return (e[k + 1] - e[k - 1]) / ((rs[k + 1] - rs[k - 1]) * rs[k]);
+    };
+    deriv.resize(n);
+    fast.resize(n);
+    for (int k = 0; k < n; ++k) {
+        deriv[k] = make_float4(float(E[k]), float(dor(E, k)), float(G[k]), float(dor(G, k)));
+        const double e1 = (k + 1 < n) ? E[k + 1] : 0.0, g1 = (k + 1 < n) ? G[k + 1] : 0.0;
+        fast[k] = make_float2(float((E[k] + e1) / 2.0), float((G[k] + g1) / 2.0));
+    }
+}
+}  // namespace
+
 struct ud_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -28,6 +97,9 @@ struct ud_ctx {
     float rot_internal = 0.f;
     float2* d_table_deriv = nullptr;
     float* d_table_fast = nullptr;
+    // AD4.2 shared radial tables (electrostatic, desolvation Gaussian)
+    float4* d_ad4_deriv = nullptr;
+    float2* d_ad4_fast = nullptr;
     // grid
     bool box_set = false;
     float init[3], factor[3], factor_inv[3], dimm1[3];
@@ -37,7 +109,8 @@ struct ud_ctx {
     int precision = UD_GRID_F32;
     int n_maps = 0;
     int map_types[UD_MAX_MAPS];
-    int slot_of_type[36];
+    int slot_of_type[40];  // XS types 0..35, or AD types 0..34 and the AD4.2 elec / dsolv maps 35, 36
+    bool maps_ad4 = false;  // the maps are AD4.2 maps
     float scale[UD_MAX_MAPS];
     void* d_maps = nullptr;
     // receptor atoms of the last ud_compute_maps (x, y, z, XS type bits)
@@ -59,7 +132,11 @@ struct ud_ctx {
     int n_lig = 0;
     std::vector<LigHdr> hdr;
     std::vector<int> lig_nat, lig_nn;  // per ligand atoms / nodes
-    std::vector<std::vector<int>> lig_xs;
+    std::vector<std::vector<int>> lig_xs;  // per ligand the atom types (XS, or AD for AD4.2)
+    bool lig_ad4 = false;                  // ligands were set with AD4.2 typing
+    std::vector<int> h_elem;               // element class per atom (RMSD lower bound)
+    float* d_charge = nullptr;             // AD4.2 charges, per atom
+    float2* d_pair_coef = nullptr;         // AD4.2 charge coefficients, per pair
     LigHdr* d_hdr = nullptr;
     AtomRec* d_atoms = nullptr;
     NodeRec* d_nodes = nullptr;
@@ -138,6 +215,11 @@ int grid_type_of(int t) {
     return t;
 }
 
+// the scoring family of the maps / ligands must match the scoring function
+bool sf_ad4(const ud_ctx* c) { return c->sf == UD_SF_AD4; }
+int grid_format(const ud_ctx* c) { return c->precision + (sf_ad4(c) ? kGridAd4F32 : 0); }
+int map_type_of(const ud_ctx* c, int t) { return sf_ad4(c) ? ad4_grid_type(t) : grid_type_of(t); }
+
 GridDev grid_dev(const ud_ctx* c) {
     GridDev g;
     std::memset(&g, 0, sizeof(g));
@@ -152,6 +234,8 @@ GridDev grid_dev(const ud_ctx* c) {
     g.n_points = c->n_points;
     g.slope = 1e6f;  // vina.cpp:515
     for (int i = 0; i < UD_MAX_MAPS; ++i) g.scale[i] = c->scale[i];
+    g.elec_slot = c->maps_ad4 ? c->slot_of_type[kAd4MapElec] : 0;
+    g.dsolv_slot = c->maps_ad4 ? c->slot_of_type[kAd4MapDsolv] : 0;
     return g;
 }
 
@@ -165,6 +249,10 @@ LigandSetDev lig_dev(const ud_ctx* c) {
     l.atom_partners = c->d_apar;
     l.table_deriv = c->d_table_deriv;
     l.table_fast = c->d_table_fast;
+    l.charge = c->d_charge;
+    l.pair_coef = c->d_pair_coef;
+    l.ad4_deriv = c->d_ad4_deriv;
+    l.ad4_fast = c->d_ad4_fast;
     return l;
 }
 
@@ -183,9 +271,12 @@ int ready_for_eval(ud_ctx* c) {
     if (c->sf < 0) return fail(c, UD_ERR_STATE, "scoring function not set");
     if (!c->d_maps) return fail(c, UD_ERR_STATE, "grid maps not computed");
     if (c->n_lig == 0) return fail(c, UD_ERR_STATE, "no ligands set");
+    if (c->maps_ad4 != sf_ad4(c)) return fail(c, UD_ERR_STATE, "the maps were made for another scoring function");
+    if (c->lig_ad4 != sf_ad4(c)) return fail(c, UD_ERR_STATE, "the ligands were set for another scoring function");
+    // ad4cache::are_atom_types_grid_initialized (ad4cache.cpp:290-320) and its Vina counterpart
     for (int l = 0; l < c->n_lig; ++l)
         for (int t : c->lig_xs[l]) {
-            int gt = grid_type_of(t);
+            int gt = map_type_of(c, t);
             if (gt >= 0 && c->slot_of_type[gt] < 0)
                 return fail(c, UD_ERR_STATE, "affinity map for atom type " + std::to_string(gt) + " is not present");
         }
@@ -294,7 +385,7 @@ int ud_destroy(ud_ctx* c) {
                     c->d_cell_start,
                     c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs, c->d_partners, c->d_apar,
                     c->d_work, c->d_perm, c->d_poses, c->d_f1, c->d_f2, c->d_f3, c->d_i1, c->d_out_pose, c->d_out_e,
-                    c->d_out_evals};
+                    c->d_out_evals, c->d_ad4_deriv, c->d_ad4_fast, c->d_charge, c->d_pair_coef};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -312,7 +403,8 @@ const char* ud_last_error(const ud_ctx* c) { return c ? c->err.c_str() : "null c
 
 int ud_set_scoring(ud_ctx* c, int sf, const double* weights, int n_weights) {
     if (!c) return UD_ERR_INVALID;
-    if (sf != UD_SF_VINA && sf != UD_SF_VINARDO) return fail(c, UD_ERR_INVALID, "unknown scoring function");
+    if (sf != UD_SF_VINA && sf != UD_SF_VINARDO && sf != UD_SF_AD4)
+        return fail(c, UD_ERR_INVALID, "unknown scoring function");
     const int need = (sf == UD_SF_VINA) ? 7 : 6;
     if (!weights || n_weights != need) return fail(c, UD_ERR_INVALID, "wrong number of weights");
     UD_HIP(c, hipSetDevice(c->device));
@@ -320,8 +412,9 @@ int ud_set_scoring(ud_ctx* c, int sf, const double* weights, int n_weights) {
     c->terms_ok = false;
     for (int i = 0; i < 7; ++i) c->w[i] = 0.f;
     for (int i = 0; i < need - 1; ++i) c->w[i] = float(weights[i]);
-    // internal rotational weight 5 * w_rot / 0.1 - 1 (vina.cpp:404, 425)
-    c->rot_internal = float(5.0 * weights[need - 1] / 0.1 - 1.0);
+    // internal rotational weight 5 * w_rot / 0.1 - 1 (vina.cpp:404, 425); the
+    // AD4.2 torsional weight is used as given (vina.cpp:435-450)
+    c->rot_internal = (sf == UD_SF_AD4) ? float(weights[need - 1]) : float(5.0 * weights[need - 1] / 0.1 - 1.0);
     size_t cap1 = 0, cap2 = 0;
     if (!c->d_table_deriv) {
         int rc = grow(c, c->d_table_deriv, cap1, size_t(kTypePairs) * kTableStride);
@@ -330,6 +423,22 @@ int ud_set_scoring(ud_ctx* c, int sf, const double* weights, int n_weights) {
         if (rc) return rc;
     }
     UD_HIP(c, launch_tables(sf, c->w, c->d_table_deriv, c->d_table_fast, c->stream));
+    if (sf == UD_SF_AD4) {
+        std::vector<float4> deriv;
+        std::vector<float2> fast;
+        ad4_radial_tables(c->w[2], c->w[3], deriv, fast);
+        size_t cap3 = 0, cap4 = 0;
+        if (!c->d_ad4_deriv) {
+            int rc = grow(c, c->d_ad4_deriv, cap3, size_t(kAd4TableN));
+            if (rc) return rc;
+            rc = grow(c, c->d_ad4_fast, cap4, size_t(kAd4TableN));
+            if (rc) return rc;
+        }
+        UD_HIP(c, hipMemcpyAsync(c->d_ad4_deriv, deriv.data(), sizeof(float4) * kAd4TableN, hipMemcpyHostToDevice,
+                                 c->stream));
+        UD_HIP(c, hipMemcpyAsync(c->d_ad4_fast, fast.data(), sizeof(float2) * kAd4TableN, hipMemcpyHostToDevice,
+                                 c->stream));
+    }
     UD_HIP(c, hipStreamSynchronize(c->stream));
     return UD_OK;
 }
@@ -365,7 +474,7 @@ int ud_set_box(ud_ctx* c, const double* center, const double* size, double granu
         c->d_maps = nullptr;
     }
     c->n_maps = 0;
-    for (int i = 0; i < 36; ++i) c->slot_of_type[i] = -1;
+    for (int i = 0; i < 40; ++i) c->slot_of_type[i] = -1;
     return UD_OK;
 }
 
@@ -375,7 +484,8 @@ static int store_maps(ud_ctx* c, float* d_f32, int n_types, const int32_t* types
         (void)hipFree(c->d_maps);
         c->d_maps = nullptr;
     }
-    for (int i = 0; i < 36; ++i) c->slot_of_type[i] = -1;
+    for (int i = 0; i < 40; ++i) c->slot_of_type[i] = -1;
+    c->maps_ad4 = sf_ad4(c);
     c->n_maps = n_types;
     for (int m = 0; m < n_types; ++m) {
         c->map_types[m] = types[m];
@@ -504,6 +614,7 @@ static GridBuildArgs build_args(const ud_ctx* c, int n_types, const int32_t* typ
 // from the receptor of the last ud_compute_maps: the inputs of the
 // term-weight gradients.  Built once per scoring / box / map set.
 static int ensure_terms(ud_ctx* c) {
+    if (sf_ad4(c)) return fail(c, UD_ERR_STATE, "term-weight gradients are built for Vina and Vinardo");
     if (c->terms_ok) return UD_OK;
     if (!c->have_rec || c->n_maps == 0)
         return fail(c, UD_ERR_STATE, "term-weight gradients need maps built by ud_compute_maps");
@@ -531,14 +642,26 @@ static int ensure_terms(ud_ctx* c) {
     return UD_OK;
 }
 
-static int check_types(ud_ctx* c, int n_types, const int32_t* types) {
+// extra: the AD4.2 electrostatic and desolvation maps may (and, for a full
+// AD4.2 map set, must) appear in the list
+static int check_types(ud_ctx* c, int n_types, const int32_t* types, bool ad4_extra) {
     if (n_types <= 0 || n_types > UD_MAX_MAPS || !types) return fail(c, UD_ERR_INVALID, "bad map type list");
+    bool have_e = false, have_d = false;
     for (int m = 0; m < n_types; ++m) {
-        if (types[m] < 0 || types[m] >= 36 || grid_type_of(types[m]) != types[m])
+        const int t = types[m];
+        if (sf_ad4(c)) {
+            if (ad4_extra && t == kAd4MapElec) have_e = true;
+            else if (ad4_extra && t == kAd4MapDsolv) have_d = true;
+            else if (t < 0 || t >= kAd4Types || ad4_grid_type(t) != t)
+                return fail(c, UD_ERR_INVALID, "map type must be a plain AutoDock type");
+        } else if (t < 0 || t >= 36 || grid_type_of(t) != t) {
             return fail(c, UD_ERR_INVALID, "map type must be a plain XS type");
+        }
         for (int k = 0; k < m; ++k)
-            if (types[k] == types[m]) return fail(c, UD_ERR_INVALID, "duplicate map type");
+            if (types[k] == t) return fail(c, UD_ERR_INVALID, "duplicate map type");
     }
+    if (ad4_extra && !(have_e && have_d))
+        return fail(c, UD_ERR_INVALID, "AD4.2 maps need the electrostatic and desolvation maps");
     return UD_OK;
 }
 
@@ -549,7 +672,8 @@ int ud_compute_maps(ud_ctx* c, int n_rec, const float* rec_xyz, const int32_t* r
     if (c->sf < 0) return fail(c, UD_ERR_STATE, "scoring function not set");
     if (n_rec < 0 || (n_rec > 0 && (!rec_xyz || !rec_xs))) return fail(c, UD_ERR_INVALID, "bad receptor arrays");
     if (precision != UD_GRID_F32 && precision != UD_GRID_F16) return fail(c, UD_ERR_INVALID, "bad precision");
-    int rc = check_types(c, n_types, types);
+    if (sf_ad4(c)) return fail(c, UD_ERR_STATE, "AD4.2 maps are built by ud_compute_maps_ad4");
+    int rc = check_types(c, n_types, types, false);
     if (rc) return rc;
     UD_HIP(c, hipSetDevice(c->device));
     std::vector<float4> rec;
@@ -591,7 +715,8 @@ int ud_set_maps(ud_ctx* c, int n_types, const int32_t* types, const float* value
     if (!c->box_set) return fail(c, UD_ERR_STATE, "box not set");
     if (!values) return fail(c, UD_ERR_INVALID, "null map values");
     if (precision != UD_GRID_F32 && precision != UD_GRID_F16) return fail(c, UD_ERR_INVALID, "bad precision");
-    int rc = check_types(c, n_types, types);
+    if (c->sf < 0) return fail(c, UD_ERR_STATE, "scoring function not set");
+    int rc = check_types(c, n_types, types, sf_ad4(c));
     if (rc) return rc;
     UD_HIP(c, hipSetDevice(c->device));
     float* d_f32 = nullptr;
@@ -602,9 +727,64 @@ int ud_set_maps(ud_ctx* c, int n_types, const int32_t* types, const float* value
     return store_maps(c, d_f32, n_types, types, precision);
 }
 
+int ud_compute_maps_ad4(ud_ctx* c, int n_rec, const float* rec_xyz, const int32_t* rec_ad, const float* rec_charge,
+                        int n_types, const int32_t* types, int precision) {
+    if (!c) return UD_ERR_INVALID;
+    if (!c->box_set) return fail(c, UD_ERR_STATE, "box not set");
+    if (!sf_ad4(c)) return fail(c, UD_ERR_STATE, "ud_compute_maps_ad4 needs the AD4.2 scoring function");
+    if (n_rec < 0 || (n_rec > 0 && (!rec_xyz || !rec_ad || !rec_charge)))
+        return fail(c, UD_ERR_INVALID, "bad receptor arrays");
+    if (precision != UD_GRID_F32 && precision != UD_GRID_F16) return fail(c, UD_ERR_INVALID, "bad precision");
+    int rc = check_types(c, n_types, types, false);
+    if (rc) return rc;
+    if (n_types + 2 > UD_MAX_MAPS) return fail(c, UD_ERR_CAPACITY, "more than UD_MAX_MAPS maps");
+    UD_HIP(c, hipSetDevice(c->device));
+    std::vector<float4> rec;
+    std::vector<float> q;
+    for (int i = 0; i < n_rec; ++i) {
+        const int32_t t = rec_ad[i];
+        if (t < 0 || t >= kAd4Types) return fail(c, UD_ERR_INVALID, "receptor atom type out of range");
+        float tf;
+        std::memcpy(&tf, &t, 4);
+        rec.push_back(make_float4(rec_xyz[3 * i], rec_xyz[3 * i + 1], rec_xyz[3 * i + 2], tf));
+        q.push_back(rec_charge[i]);
+    }
+    c->have_rec = false;
+    float4* d_r = nullptr;
+    float* d_q = nullptr;
+    float* d_f32 = nullptr;
+    const size_t nr = std::max<size_t>(rec.size(), 1);
+    if (hipMalloc(&d_r, sizeof(float4) * nr) != hipSuccess || hipMalloc(&d_q, sizeof(float) * nr) != hipSuccess ||
+        hipMalloc(&d_f32, sizeof(float) * size_t(n_types + 2) * c->n_points) != hipSuccess) {
+        (void)hipFree(d_r);
+        (void)hipFree(d_q);
+        (void)hipFree(d_f32);
+        return fail(c, UD_ERR_NOMEM, "map allocation failed");
+    }
+    if (!rec.empty()) {
+        UD_HIP(c, hipMemcpyAsync(d_r, rec.data(), sizeof(float4) * rec.size(), hipMemcpyHostToDevice, c->stream));
+        UD_HIP(c, hipMemcpyAsync(d_q, q.data(), sizeof(float) * q.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    const GridBuildArgs a = build_args(c, n_types, types);
+    UD_HIP(c, hipEventRecord(c->ev0, c->stream));
+    UD_HIP(c, launch_grid_build_ad4(d_r, d_q, int(rec.size()), a, c->w, d_f32, c->stream));
+    UD_HIP(c, hipEventRecord(c->ev1, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    c->last_launches = 1;
+    (void)hipFree(d_r);
+    (void)hipFree(d_q);
+    std::vector<int32_t> all(types, types + n_types);
+    all.push_back(kAd4MapElec);
+    all.push_back(kAd4MapDsolv);
+    return store_maps(c, d_f32, n_types + 2, all.data(), precision);
+}
+
 int ud_get_map(ud_ctx* c, int xs_type, float* out) {
     if (!c || !out) return UD_ERR_INVALID;
-    if (xs_type < 0 || xs_type >= 36 || c->slot_of_type[xs_type] < 0) return fail(c, UD_ERR_INVALID, "no map for type");
+    if (xs_type < 0 || xs_type >= 40 || c->slot_of_type[xs_type] < 0) return fail(c, UD_ERR_INVALID, "no map for type");
     const int m = c->slot_of_type[xs_type];
     UD_HIP(c, hipSetDevice(c->device));
     if (c->precision == UD_GRID_F32) {
@@ -634,8 +814,16 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     if (!b->atom_offset || !b->node_offset || !b->pair_offset || !b->atom_local || !b->atom_xs || !b->atom_heavy ||
         !b->node_parent || !b->node_abegin || !b->node_aend || !b->node_origin || !b->node_axis || !b->num_tors)
         return fail(c, UD_ERR_INVALID, "null array in ligand batch");
+    // AD4.2 scoring types atoms by their AutoDock type and needs charges and TORSDOF
+    const bool ad = sf_ad4(c);
+    if (ad && (!b->atom_ad || !b->atom_charge || !b->torsdof))
+        return fail(c, UD_ERR_INVALID, "AD4.2 scoring needs atom_ad, atom_charge and torsdof");
     UD_HIP(c, hipSetDevice(c->device));
     const int L = b->n_ligands;
+    std::vector<float> charge;
+    std::vector<float2> coef;
+    std::vector<int> elem;
+    const Ad4Params& A4 = ad4_params();
     std::vector<LigHdr> hdr(L);
     std::vector<AtomRec> atoms;
     std::vector<NodeRec> nodes;
@@ -706,9 +894,12 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
         }
         int n_heavy = 0;
         for (int i = 0; i < na; ++i) {
-            const int t = b->atom_xs[a0 + i];
-            if (t < -1 || t >= 36) return fail(c, UD_ERR_INVALID, tag + "atom type out of range");
+            const int t = ad ? b->atom_ad[a0 + i] : b->atom_xs[a0 + i];
+            if (ad ? (t < 0 || t >= kAd4Types) : (t < -1 || t >= 36))
+                return fail(c, UD_ERR_INVALID, tag + "atom type out of range");
             xs[l].push_back(t);
+            elem.push_back(ad ? ad4_element(t) : xs_element(t));
+            charge.push_back(ad ? b->atom_charge[a0 + i] : 0.f);
             AtomRec a;
             a.x = b->atom_local[3 * (a0 + i)];
             a.y = b->atom_local[3 * (a0 + i) + 1];
@@ -721,15 +912,23 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
         for (int q = 0; q < np; ++q) {
             const int pa = b->pairs[2 * (p0 + q)], pb = b->pairs[2 * (p0 + q) + 1];
             if (pa < 0 || pb < 0 || pa >= na || pb >= na || pa == pb) return fail(c, UD_ERR_INVALID, tag + "pair index out of range");
-            const int ta = b->atom_xs[a0 + pa], tb = b->atom_xs[a0 + pb];
+            const int ta = xs[l][pa], tb = xs[l][pb];
             if (ta < 0 || tb < 0) return fail(c, UD_ERR_INVALID, tag + "pair with an untyped atom");
             pairs.push_back(uint32_t(pa) | (uint32_t(pb) << 8) | (uint32_t(tri_index(ta, tb)) << 16));
+            if (ad) {
+                // ad4_electrostatic's charge product and ad4_solvation's
+                // coefficient of the Gaussian (potentials.h:350-400)
+                const float qa = b->atom_charge[a0 + pa], qb = b->atom_charge[a0 + pb];
+                const float cs = (A4.solv[ta] + 0.01097f * std::fabs(qa)) * A4.vol[tb] +
+                                 (A4.solv[tb] + 0.01097f * std::fabs(qb)) * A4.vol[ta];
+                coef.push_back(make_float2(qa * qb, cs));
+            }
         }
         // every pair under both of its atoms, in pair-list order
         std::vector<std::vector<PartnerRec>> part(na);
         for (int q = 0; q < np; ++q) {
             const int pa = b->pairs[2 * (p0 + q)], pb = b->pairs[2 * (p0 + q) + 1];
-            const uint32_t slot21 = uint32_t(tri_index(b->atom_xs[a0 + pa], b->atom_xs[a0 + pb])) << 21;
+            const uint32_t slot21 = uint32_t(tri_index(xs[l][pa], xs[l][pb])) << 21;
             part[pa].push_back(uint32_t(pb) | (uint32_t(q) << 8) | slot21 | 0x80000000u);
             part[pb].push_back(uint32_t(pa) | (uint32_t(q) << 8) | slot21);
         }
@@ -748,7 +947,7 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
         h.n_pairs = np;
         h.max_depth = maxd;
         h.local_steps = (25 + na) / 3;  // vina.h:197
-        h.num_tors = b->num_tors[l];
+        h.num_tors = ad ? b->torsdof[l] : b->num_tors[l];  // AD4.2: TORSDOF (conf_independent.cpp:183-187)
         h.n_heavy = n_heavy;
         h.partner_begin = int32_t(pbeg);
         nat[l] = na;
@@ -784,6 +983,23 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     UD_HIP(c, hipMemcpy(c->d_hdr, hdr.data(), hdr.size() * sizeof(LigHdr), hipMemcpyHostToDevice));
     UD_HIP(c, hipMemcpy(c->d_nodes, nodes.data(), nodes.size() * sizeof(NodeRec), hipMemcpyHostToDevice));
     if (!pairs.empty()) UD_HIP(c, hipMemcpy(c->d_pairs, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
+    for (void* p : {static_cast<void*>(c->d_charge), static_cast<void*>(c->d_pair_coef)})
+        if (p) (void)hipFree(p);
+    c->d_charge = nullptr;
+    c->d_pair_coef = nullptr;
+    if (ad) {
+        cap = 0;
+        rc = grow(c, c->d_charge, cap, charge.size());
+        if (rc) return rc;
+        cap = 0;
+        rc = grow(c, c->d_pair_coef, cap, std::max<size_t>(coef.size(), 1));
+        if (rc) return rc;
+        UD_HIP(c, hipMemcpy(c->d_charge, charge.data(), charge.size() * sizeof(float), hipMemcpyHostToDevice));
+        if (!coef.empty())
+            UD_HIP(c, hipMemcpy(c->d_pair_coef, coef.data(), coef.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    c->lig_ad4 = ad;
+    c->h_elem = elem;
     c->hdr = hdr;
     c->n_lig = L;
     c->lig_nat = nat;
@@ -810,7 +1026,7 @@ int refresh_slots(ud_ctx* c) {
     bool changed = false;
     for (auto& a : atoms) {
         const uint32_t t = a.meta >> 24;
-        const int gt = (t == 0xFF) ? -1 : grid_type_of(int(t));
+        const int gt = (t == 0xFF) ? -1 : map_type_of(c, int(t));
         const uint32_t slot = (gt < 0) ? 0xFFu : uint32_t(c->slot_of_type[gt]);
         const uint32_t meta = (a.meta & ~(0xFFu << 8)) | (slot << 8);
         if (meta != a.meta) { a.meta = meta; changed = true; }
@@ -865,7 +1081,7 @@ int ud_score_grad_device(ud_ctx* c, int n_poses, const int32_t* pose_ligand, con
     rc = work_upload(c, perm, work);
     if (rc) return rc;
     UD_HIP(c, hipEventRecord(c->ev0, c->stream));
-    UD_HIP(c, launch_score(lanes, c->precision, lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
+    UD_HIP(c, launch_score(lanes, grid_format(c), lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
                            d_poses, v[0], v[1], d_energy, d_grad, c->stream));
     UD_HIP(c, hipEventRecord(c->ev1, c->stream));
     c->last_launches = 1;
@@ -982,7 +1198,7 @@ int ud_optimize(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses
     rc = work_upload(c, perm, work);
     if (rc) return rc;
     UD_HIP(c, hipEventRecord(c->ev0, c->stream));
-    UD_HIP(c, launch_optimize(lanes, c->precision, lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
+    UD_HIP(c, launch_optimize(lanes, grid_format(c), lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
                               c->d_poses, v[0], v[1], max_steps, c->d_f1, c->d_i1, c->stream));
     UD_HIP(c, hipEventRecord(c->ev1, c->stream));
     UD_HIP(c, hipMemcpyAsync(poses, c->d_poses, sizeof(float) * size_t(n_poses) * UD_POSE_FLOATS, hipMemcpyDeviceToHost, c->stream));
@@ -1117,7 +1333,7 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
         hipStream_t bs = c->bstream[k];
         UD_HIP(c, hipStreamWaitEvent(bs, c->ev0, 0));
         UD_HIP(c, hipEventRecord(c->lev0[k], bs));
-        UD_HIP(c, launch_dock(lanes_v[k], c->precision, lig_dev(c), grid_dev(c), cap_v[k], sp, c->d_work + off,
+        UD_HIP(c, launch_dock(lanes_v[k], grid_format(c), lig_dev(c), grid_dev(c), cap_v[k], sp, c->d_work + off,
                               int(works[k].size()), c->d_out_pose, c->d_out_e, c->d_out_evals, bs));
         UD_HIP(c, hipEventRecord(c->lev[k], bs));
         off += works[k].size();
@@ -1253,7 +1469,7 @@ int ud_score_terms(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const flo
     make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
     rc = work_upload(c, perm, work);
     if (rc) return rc;
-    UD_HIP(c, launch_terms(lanes, c->precision, lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
+    UD_HIP(c, launch_terms(lanes, grid_format(c), lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
                            c->d_poses, 1000.f, c->d_f1, c->d_f1 + n_poses, c->stream));
     UD_HIP(c, hipMemcpyAsync(inter_out, c->d_f1, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
     UD_HIP(c, hipMemcpyAsync(intra_out, c->d_f1 + n_poses, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
@@ -1477,7 +1693,7 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, u
             const uint32_t meta = c->h_atoms[size_t(h.atom_begin) + i].meta;
             if ((meta >> 16) & 1) {
                 heavy[l].push_back(i);
-                elem[l].push_back(xs_element(int(meta >> 24)));
+                elem[l].push_back(c->h_elem[size_t(h.atom_begin) + i]);
             }
         }
         hoff[l + 1] = hoff[l] + size_t(exh) * heavy[l].size() * 3;
@@ -1538,7 +1754,8 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, u
     const int nk_all = int(kl.size());
     // refinement against the explicit receptor when it is known (vina.h:248-272),
     // then rescoring with authentic caps against the same receptor model
-    const bool refine = refine_step > 0 && c->have_rec;
+    // (AD4.2 modes are not refined, vina.h:247-249)
+    const bool refine = refine_step > 0 && c->have_rec && !sf_ad4(c);
     std::vector<float> rkey(nk_all, 0.f), inter(nk_all), intra(nk_all);
     if (nk_all > 0) {
         if (refine) {
@@ -1578,7 +1795,7 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, u
         for (int k = 0; k < nk; ++k) ord[k] = k;
         if (refine)
             std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return rkey[q + a] < rkey[q + b]; });
-        const float intra_best = intra[q + ord[0]];
+        const float intra_best = sf_ad4(c) ? 0.f : intra[q + ord[0]];
         std::vector<int> slot_of(nk);  // mode k of `out` -> index among the ligand's kept modes
         for (int k = 0; k < nk; ++k) {
             ud_mode& m = out[k];
@@ -1586,11 +1803,19 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, u
             std::copy(kp.begin() + i * UD_POSE_FLOATS, kp.begin() + (i + 1) * UD_POSE_FLOATS, m.pose);
             m.inter = inter[i];
             m.intra = intra[i];
-            const float x = m.inter + m.intra - intra_best;
-            m.energy = conf_independent(x, c->hdr[l].num_tors, c->rot_internal);
-            m.conf_independent = m.energy - x;
+            if (sf_ad4(c)) {
+                // AD4.2 branch of score_gpu (vina.cpp:1289-1305, 1316-1320):
+                // total = inter + w_tors TORSDOF, unbound = -intra
+                m.conf_independent = c->rot_internal * c->hdr[l].num_tors;
+                m.energy = m.inter + m.conf_independent;
+                m.unbound = -m.intra;
+            } else {
+                const float x = m.inter + m.intra - intra_best;
+                m.energy = conf_independent(x, c->hdr[l].num_tors, c->rot_internal);
+                m.conf_independent = m.energy - x;
+                m.unbound = intra_best;
+            }
             m.search_energy = E[size_t(l) * exh + kchain[i]];
-            m.unbound = intra_best;
             m.chain = int32_t(size_t(l) * exh + kchain[i]);
         }
         std::stable_sort(out, out + nk, [](const ud_mode& a, const ud_mode& b) { return a.energy < b.energy; });

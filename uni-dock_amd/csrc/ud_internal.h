@@ -60,6 +60,27 @@ typedef uint32_t AtomPartners;
 static constexpr int kTableN = 2051;     // sz(32 * 64) + 3 (precalculate.h:142-145)
 // entries 0..2047 are reachable with r^2 < 64 (2048 is kept for r^2 == 64 in map builds)
 
+// AutoDock 4.2 (SF_AD42): samples of its 20.48 A cutoff, sz(32 * 20.48^2) + 3
+static constexpr int kAd4TableN = 13424;
+static constexpr float kAd4Cutoff = 20.48f;  // scoring_function.h:82-83
+static constexpr int kAd4Types = 35;         // AD_TYPE_SIZE (atom_constants.h:81)
+static constexpr int kAd4MapElec = UD_AD4_MAP_ELEC, kAd4MapDsolv = UD_AD4_MAP_DSOLV;
+// Per AutoDock type (atom_constants.h:155-194): vdW radius and depth,
+// H-bond depth and radius, solvation parameter, atomic volume.
+struct Ad4Params {
+    float radius[kAd4Types], depth[kAd4Types], hb_depth[kAd4Types], hb_r[kAd4Types], solv[kAd4Types],
+        vol[kAd4Types];
+};
+const Ad4Params& ad4_params();  // host copy
+// AD type of an atom's grid term (ad4cache.cpp:223-240): G types none, CG
+// types the C map
+inline int ad4_grid_type(int t) {
+    if (t < 0 || t >= kAd4Types) return -1;
+    if (t >= 22 && t <= 25) return -1;
+    if (t >= 26 && t <= 29) return 0;
+    return t;
+}
+
 struct GridDev {
     const void* maps;      // n_maps * n_points values (float or __half bits)
     float init[3];
@@ -70,6 +91,7 @@ struct GridDev {
     int32_t n_points;
     float slope;
     float scale[UD_MAX_MAPS];  // FP16 maps: value = half * scale
+    int32_t elec_slot, dsolv_slot;  // AD4.2: slots of the electrostatic / desolvation maps
 };
 
 struct LigandSetDev {
@@ -81,6 +103,13 @@ struct LigandSetDev {
     const AtomPartners* atom_partners;  // per atom, indexed like atoms
     const float2* table_deriv;  // (e, dor) per sample, slot * kTableStride
     const float* table_fast;    // slot * kTableUsed
+    // AD4.2: per-atom partial charges (indexed like atoms), per-pair charge
+    // coefficients (q_a q_b, desolvation coefficient; indexed like pairs), and
+    // the shared electrostatic / desolvation radial tables
+    const float* charge;
+    const float2* pair_coef;
+    const float4* ad4_deriv;  // (E e, E dor, G e, G dor) per sample, kAd4TableN
+    const float2* ad4_fast;   // (E fast, G fast) per sample
 };
 
 // Receptor atoms for scoring against the explicit receptor (non_cache,

@@ -80,11 +80,40 @@ __device__ float sf_eval(int sf, int t1, int t2, float r, const SfWeights& W) {
     return acc;
 }
 
+// AutoDock 4.2 pair terms that depend on the two AD types only:
+// w_vdw ad4_vdw + w_hb ad4_hb (potentials.h:67-79, 421-480), both with
+// smoothing 0.5, cap 100000 and cutoff 8 A; a pair is an H-bond pair (12-10)
+// when the product of the H-bond depths is negative, else a vdW pair (12-6).
+// The charge-dependent electrostatic and desolvation terms are sampled
+// separately (shared radial tables times per-pair coefficients).
+__device__ float ad4_type_pair(int t1, int t2, float r, const Ad4Params& P, float w_vdw, float w_hb) {
+    if (r >= 8.f) return 0.f;
+    const float hbd = P.hb_depth[t1] * P.hb_depth[t2];
+    const bool hb = hbd < 0.f;
+    const float rij = hb ? P.hb_r[t1] + P.hb_r[t2] : P.radius[t1] + P.radius[t2];
+    const float sm = 0.25f;  // smoothing / 2
+    const float rs = (r > rij + sm) ? r - sm : (r < rij - sm) ? r + sm : rij;
+    const float r2 = rs * rs, r4 = r2 * r2, r6 = r4 * r2, r10 = r6 * r4, r12 = r6 * r6;
+    const float q2 = rij * rij, q4 = q2 * q2, q6 = q4 * q2, q10 = q6 * q4, q12 = q6 * q6;
+    float val;
+    if (hb) {
+        const float c12 = q12 * -hbd * 10.f / 2.f, c10 = q10 * -hbd * 12.f / 2.f;
+        val = (r12 > kEpsFl && r10 > kEpsFl) ? fminf(100000.f, c12 / r12 - c10 / r10) : 100000.f;
+        return w_hb * val;
+    }
+    const float depth = sqrtf(P.depth[t1] * P.depth[t2]);
+    const float c12 = q12 * depth, c6 = q6 * depth * 2.f;
+    val = (r12 > kEpsFl && r6 > kEpsFl) ? fminf(100000.f, c12 / r12 - c6 / r6) : 100000.f;
+    return w_vdw * val;
+}
+
 // One workgroup per type pair: smooth e at rs[k] = sqrt(k / 32), dor by
 // central difference, fast = midpoint (precalculate.h:66-85, 160-171).
 // The derivative table holds (e, dor) per sample (8 B); the interpolation
 // forms the differences to the next sample itself, as eval_deriv does.
-__global__ __launch_bounds__(256) void table_kernel(int sf, SfWeights W, float2* deriv_out, float* fast_out) {
+// AD4.2 type pairs hold the vdW / H-bond part (zero beyond 8 A).
+__global__ __launch_bounds__(256) void table_kernel(int sf, SfWeights W, Ad4Params P, float2* deriv_out,
+                                                    float* fast_out) {
     __shared__ float e[kTableN];
     __shared__ float rs[kTableN];
     const int slot = blockIdx.x;
@@ -94,7 +123,10 @@ __global__ __launch_bounds__(256) void table_kernel(int sf, SfWeights W, float2*
     for (int k = threadIdx.x; k < kTableN; k += blockDim.x) {
         const float r = sqrtf(float(k) / 32.f);
         rs[k] = r;
-        e[k] = fminf(kMaxFl, sf_eval(sf, t1, t2, r, W));
+        float v;
+        if (sf == UD_SF_AD4) v = (t1 < kAd4Types && t2 < kAd4Types) ? ad4_type_pair(t1, t2, r, P, W.w[0], W.w[1]) : 0.f;
+        else v = sf_eval(sf, t1, t2, r, W);
+        e[k] = fminf(kMaxFl, v);
     }
     __syncthreads();
     auto dor = [&](int k) -> float {
@@ -155,6 +187,82 @@ __global__ __launch_bounds__(256) void grid_build_kernel(const float4* rec, int 
 #pragma unroll
         for (int m = 0; m < 8; ++m)
             if (m < nm) maps_out[size_t(m0 + m) * a.n_points + pt] = acc[m];
+    }
+}
+
+// AD4.2 maps (the AutoGrid decomposition of the pair potentials, see
+// launch_grid_build_ad4): each thread owns one sample point and up to 8
+// affinity maps (blockIdx.y picks the chunk); chunk 0 also sums the
+// electrostatic and desolvation maps.  Receptor atoms stream through LDS.
+//   affinity[t] = sum_j w_vdw vdw + w_hb hb + w_dsolv (S_t V_j + (S_j + Q|q_j|) V_t) G(r)
+//   elec        = sum_j w_elec q_j 332 min(100, 1 / (r eps(r)))
+//   dsolv       = sum_j w_dsolv Q V_j G(r)
+// with G(r) = exp(-r^2 / (2 3.6^2)), Q = 0.01097 and the sigmoidal
+// dielectric eps(r) of ad4_electrostatic (potentials.h:350-400).
+__global__ __launch_bounds__(256) void grid_build_ad4_kernel(const float4* rec, const float* rec_q, int n_rec,
+                                                             GridBuildArgs a, Ad4Params P, SfWeights W,
+                                                             float* maps_out) {
+    __shared__ float4 tile[256];
+    __shared__ float qtile[256];
+    const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int m0 = chunk * 8;
+    const int nm = max(0, min(8, a.n_maps - m0));
+    int tm[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) tm[m] = (m < nm) ? a.types[m0 + m] : 0;
+    const int x = pt % a.n[0], y = (pt / a.n[0]) % a.n[1], z = pt / (a.n[0] * a.n[1]);
+    const float px = a.init[0] + a.factor_inv[0] * float(x);
+    const float py = a.init[1] + a.factor_inv[1] * float(y);
+    const float pz = a.init[2] + a.factor_inv[2] * float(z);
+    const float cut2 = kAd4Cutoff * kAd4Cutoff;
+    const float B = 78.4f + 8.5525f, lB = -B * 0.003627f;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float el = 0.f, ds = 0.f;
+    for (int base = 0; base < n_rec; base += 256) {
+        const int j = base + threadIdx.x;
+        if (j < n_rec) {
+            tile[threadIdx.x] = rec[j];
+            qtile[threadIdx.x] = rec_q[j];
+        }
+        __syncthreads();
+        const int cnt = min(256, n_rec - base);
+        if (pt < a.n_points) {
+            for (int k = 0; k < cnt; ++k) {
+                const float4 at = tile[k];
+                const float dx = at.x - px, dy = at.y - py, dz = at.z - pz;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (!(r2 < cut2)) continue;
+                const float r = sqrtf(r2);
+                const int tj = __float_as_int(at.w);
+                const float qj = qtile[k];
+                const float G = expf(-0.5f * (r / 3.6f) * (r / 3.6f));
+                const float Sj = P.solv[tj] + 0.01097f * fabsf(qj);
+#pragma unroll
+                for (int m = 0; m < 8; ++m)
+                    if (m < nm) {
+                        const int t = tm[m];
+                        acc[m] += ad4_type_pair(t, tj, r, P, W.w[0], W.w[1]) +
+                                  W.w[3] * (P.solv[t] * P.vol[tj] + Sj * P.vol[t]) * G;
+                    }
+                if (chunk == 0) {
+                    const float diel = -8.5525f + B / (1.f + 7.7839f * expf(lB * r));
+                    const float ef = (r < kEpsFl) ? 100.f / diel : fminf(100.f, 1.f / (r * diel));
+                    el += W.w[2] * qj * 332.f * ef;
+                    ds += W.w[3] * 0.01097f * P.vol[tj] * G;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (pt < a.n_points) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+            if (m < nm) maps_out[size_t(m0 + m) * a.n_points + pt] = acc[m];
+        if (chunk == 0) {
+            maps_out[size_t(a.n_maps) * a.n_points + pt] = el;
+            maps_out[size_t(a.n_maps + 1) * a.n_points + pt] = ds;
+        }
     }
 }
 
@@ -378,10 +486,17 @@ __device__ __forceinline__ GridPt grid_locate(const GridDev& g, int slot, float3
     return q;
 }
 
+// Storage format (UD_GRID_F32 / F16) and scoring family of a grid-format
+// template value: 0 / 1 maps, 2 the explicit receptor, 3 / 4 AD4.2 maps.
+template <int GF> struct Fmt {
+    static constexpr int store = (GF >= kGridAd4F32) ? GF - kGridAd4F32 : GF;
+    static constexpr bool ad4 = GF >= kGridAd4F32;
+};
+
 template <int GF>
 __device__ __forceinline__ void grid_gather(const GridDev& g, uint32_t base, float (&f)[8]) {
     const uint32_t sy = uint32_t(g.n[0]), sz = uint32_t(g.n[0] * g.n[1]);
-    if (GF == UD_GRID_F32) {
+    if (Fmt<GF>::store == UD_GRID_F32) {
         const float* M = reinterpret_cast<const float*>(g.maps);
         f[0] = M[base]; f[1] = M[base + 1]; f[2] = M[base + sy]; f[3] = M[base + sy + 1];
         f[4] = M[base + sz]; f[5] = M[base + sz + 1]; f[6] = M[base + sy + sz]; f[7] = M[base + sy + sz + 1];
@@ -409,13 +524,13 @@ __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const G
     const float b0 = a0 + y * dy0, b1 = a2 + y * dy1;
     const float dz = b1 - b0;
     float e = b0 + z * dz;
-    if (GF == UD_GRID_F16) e *= g.scale[slot];
+    if (Fmt<GF>::store == UD_GRID_F16) e *= g.scale[slot];
     if (d) {
         const float ex0 = dx0 + y * (dx1 - dx0), ex1 = dx2 + y * (dx3 - dx2);
         float gx = ex0 + z * (ex1 - ex0);
         float gy = dy0 + z * (dy1 - dy0);
         float gz = dz;
-        if (GF == UD_GRID_F16) {
+        if (Fmt<GF>::store == UD_GRID_F16) {
             const float sc = g.scale[slot];
             gx *= sc; gy *= sc; gz *= sc;
         }
@@ -434,9 +549,39 @@ __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const G
     return e + q.penalty;
 }
 
+// ad4cache::eval_deriv (ad4cache.cpp:223-262) for an atom of charge q whose
+// affinity map is `slot`: that map, plus q times the electrostatic map and
+// |q| times the desolvation map, each through grid::evaluate with its own
+// curl and out-of-box penalty; the three share the cell and fractions.
+template <int GF>
+__device__ __forceinline__ float ad4_grid_term(const GridDev& g, int slot, float q, float3 p, float v, float3* d) {
+    const GridPt gq = grid_locate(g, slot, p);
+    const uint32_t cell = gq.base - uint32_t(slot) * uint32_t(g.n_points);
+    float f[8];
+    grid_gather<GF>(g, gq.base, f);
+    float3 dt = make_float3(0.f, 0.f, 0.f);
+    float e = grid_finish<GF>(g, slot, gq, f, v, d ? &dt : nullptr);
+    const float fac[2] = {q, fabsf(q)};
+    const int sl[2] = {g.elec_slot, g.dsolv_slot};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        grid_gather<GF>(g, cell + uint32_t(sl[k]) * uint32_t(g.n_points), f);
+        float3 dk = make_float3(0.f, 0.f, 0.f);
+        e += fac[k] * grid_finish<GF>(g, sl[k], gq, f, v, d ? &dk : nullptr);
+        dt.x += fac[k] * dk.x; dt.y += fac[k] * dk.y; dt.z += fac[k] * dk.z;
+    }
+    if (d) *d = dt;
+    return e;
+}
+
 // Grid-format value of the refinement launches: the inter term is scored
 // against the explicit receptor atoms instead of the maps.
 static constexpr int kGridExplicit = 2;
+
+// Intramolecular pair cutoff^2 (sqr(sf.get_cutoff()), scoring_function.h:56-83)
+template <int GF> __device__ __forceinline__ constexpr float pair_cut2() {
+    return Fmt<GF>::ad4 ? kAd4Cutoff * kAd4Cutoff : 64.f;
+}
 
 // non_cache::eval_deriv / non_cache::eval (non_cache.cpp:420-860, 34-255)
 // for one ligand atom of XS type t at p: the coordinate is clamped into the
@@ -515,69 +660,6 @@ __device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p,
 static constexpr int kAtomBatch = UD_ATOM_BATCH;  // atoms per lane whose corner gathers are issued together
 static constexpr int kPairBatch = UD_PAIR_BATCH;  // pairs per lane whose table loads are issued together
 
-// Grid energy (+ minus-forces into mf) of the group's atoms: each lane owns a
-// contiguous block of atoms, computes their coordinates from the node frames
-// and gathers the 8 map corners of kAtomBatch atoms at once.
-template <int L, int NM, int GF>
-__device__ __forceinline__ float atoms_grid(const KArgs& ka, const Grp<L, NM>& s, float v_inter, bool want_grad,
-                                            float& rg2) {
-    float4* coords = s.coords();
-    float4* mf = s.mf();
-    const float* fr = s.frames();
-    const int N = s.h.n_atoms;
-    const int c = (N + L - 1) / L;
-    const int i0 = s.gl * c, i1 = min(N, i0 + c);
-    float e = 0.f;
-    if constexpr (GF == kGridExplicit) {
-        for (int i = i0; i < i1; ++i) {
-            const AtomRec a = s.atoms[i];
-            const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
-            coords[i] = make_float4(p.x, p.y, p.z, 0.f);
-            if ((a.meta >> 16) & 1) {
-                const float dx = p.x - fr[0], dy = p.y - fr[1], dz = p.z - fr[2];
-                rg2 += dx * dx + dy * dy + dz * dz;
-            }
-            float3 d = make_float3(0.f, 0.f, 0.f);
-            if (((a.meta >> 8) & 0xff) != 0xff) e += explicit_term(ka, int(a.meta >> 24), p, v_inter, want_grad ? &d : nullptr);
-            if (want_grad) mf[i] = make_float4(d.x, d.y, d.z, 0.f);
-        }
-        return e;
-    }
-    for (int ib = i0; ib < i1; ib += kAtomBatch) {
-        GridPt q[kAtomBatch];
-        int slot[kAtomBatch];
-        float f[kAtomBatch][8];
-#pragma unroll
-        for (int u = 0; u < kAtomBatch; ++u) {
-            const int i = ib + u;
-            slot[u] = 0xff;
-            if (i < i1) {
-                const AtomRec a = s.atoms[i];
-                const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
-                coords[i] = make_float4(p.x, p.y, p.z, 0.f);
-                if ((a.meta >> 16) & 1) {  // heavy atom: gyration radius about the root origin
-                    const float dx = p.x - fr[0], dy = p.y - fr[1], dz = p.z - fr[2];
-                    rg2 += dx * dx + dy * dy + dz * dz;
-                }
-                slot[u] = (a.meta >> 8) & 0xff;
-                if (slot[u] != 0xff) q[u] = grid_locate(ka.grid, slot[u], p);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kAtomBatch; ++u)
-            if (slot[u] != 0xff) grid_gather<GF>(ka.grid, q[u].base, f[u]);
-#pragma unroll
-        for (int u = 0; u < kAtomBatch; ++u) {
-            const int i = ib + u;
-            if (i >= i1) continue;
-            float3 d = make_float3(0.f, 0.f, 0.f);
-            if (slot[u] != 0xff) e += grid_finish<GF>(ka.grid, slot[u], q[u], f[u], v_inter, want_grad ? &d : nullptr);
-            if (want_grad) mf[i] = make_float4(d.x, d.y, d.z, 0.f);
-        }
-    }
-    return e;
-}
-
 // Atom coordinates from the node frames (model::set, tree.h:176-214), lane
 // gl handling atoms gl, gl + L, ...; also the heavy-atom sum of squared
 // distances to the root origin (model::gyration_radius, model.cpp:871-883).
@@ -594,6 +676,18 @@ __device__ __forceinline__ void atoms_coords(const Grp<L, NM>& s, float& rg2) {
             rg2 += dx * dx + dy * dy + dz * dz;
         }
     }
+}
+
+// AD4.2 charge terms of one pair at fraction `rem` between samples i1 and
+// i1 + 1 of the shared radial tables: (q_a q_b) times the electrostatic
+// sample and the desolvation coefficient times the Gaussian sample, added to
+// the type-pair table's (e, dor).  The reference interpolates one table of
+// the pair's total (precalculate_element::eval_deriv, precalculate.h:48-65);
+// the total, its samples and its central differences are linear in these
+// parts, so the sum is the same interpolation.
+__device__ __forceinline__ void ad4_charge_terms(float2 cf, float4 u0, float4 u1, float rem, float& pe, float& dor) {
+    pe += cf.x * (u0.x + rem * (u1.x - u0.x)) + cf.y * (u0.z + rem * (u1.z - u0.z));
+    dor += cf.x * (u0.y + rem * (u1.y - u0.y)) + cf.y * (u0.w + rem * (u1.w - u0.w));
 }
 
 // Energy and gradient of each of the lane's atoms (atoms gl, gl + L, ...):
@@ -618,7 +712,7 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
         const int slot = (a.meta >> 8) & 0xff;
         GridPt gq;
         float gf[8];
-        if constexpr (GF != kGridExplicit) {
+        if constexpr (GF != kGridExplicit && !Fmt<GF>::ad4) {
             if (slot != 0xff) {
                 gq = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
                 grid_gather<GF>(ka.grid, gq.base, gf);
@@ -628,25 +722,28 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
         const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
         float gx = 0.f, gy = 0.f, gz = 0.f;
         for (int k = 0; k < kn; k += kPairBatch) {
-            int other[kPairBatch], idx[kPairBatch];
+            int other[kPairBatch], idx[kPairBatch], it[kPairBatch], pq[kPairBatch];
             float rx[kPairBatch], ry[kPairBatch], rz[kPairBatch], rem[kPairBatch];
             bool in[kPairBatch];
-            float2 t0[kPairBatch], t1[kPairBatch];
+            float2 t0[kPairBatch], t1[kPairBatch], cf[kPairBatch];
+            float4 u0[kPairBatch], u1[kPairBatch];
 #pragma unroll
             for (int u = 0; u < kPairBatch; ++u) {
                 in[u] = false;
-                other[u] = 0; idx[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
+                other[u] = 0; idx[u] = 0; it[u] = 0; pq[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
                 if (k + u < kn) {
                     const PartnerRec rec = part[k0 + k + u];
                     other[u] = int(rec & 0xff);
                     const float4 cb = coords[other[u]];
                     rx[u] = cb.x - ci.x; ry[u] = cb.y - ci.y; rz[u] = cb.z - ci.z;
                     const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
-                    if (r2 < 64.f) {
+                    if (r2 < pair_cut2<GF>()) {
                         const float rf = 32.f * r2;
                         const int i1t = int(rf);
                         rem[u] = rf - float(i1t);
                         idx[u] = int((rec >> 21) & 0x3FFu) * kTableStride + i1t;
+                        it[u] = i1t;
+                        pq[u] = int((rec >> 8) & 0x1FFFu);
                         in[u] = true;
                     }
                 }
@@ -654,14 +751,24 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
 #pragma unroll
             for (int u = 0; u < kPairBatch; ++u)
                 if (in[u]) {
-                    t0[u] = ka.ls.table_deriv[idx[u]];
-                    t1[u] = ka.ls.table_deriv[idx[u] + 1];
+                    if (!Fmt<GF>::ad4 || it[u] < kTableStride - 1) {
+                        t0[u] = ka.ls.table_deriv[idx[u]];
+                        t1[u] = ka.ls.table_deriv[idx[u] + 1];
+                    } else {
+                        t0[u] = t1[u] = make_float2(0.f, 0.f);
+                    }
+                    if constexpr (Fmt<GF>::ad4) {
+                        u0[u] = ka.ls.ad4_deriv[it[u]];
+                        u1[u] = ka.ls.ad4_deriv[it[u] + 1];
+                        cf[u] = ka.ls.pair_coef[s.h.pair_begin + pq[u]];
+                    }
                 }
 #pragma unroll
             for (int u = 0; u < kPairBatch; ++u) {
                 if (!in[u]) continue;
                 float pe = t0[u].x + rem[u] * (t1[u].x - t0[u].x);
-                const float dor = t0[u].y + rem[u] * (t1[u].y - t0[u].y);
+                float dor = t0[u].y + rem[u] * (t1[u].y - t0[u].y);
+                if constexpr (Fmt<GF>::ad4) ad4_charge_terms(cf[u], u0[u], u1[u], rem[u], pe, dor);
                 float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
                 if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
                     const float tmp = curl_factor(v_intra, pe);
@@ -676,6 +783,10 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
         float3 d = make_float3(0.f, 0.f, 0.f);
         if constexpr (GF == kGridExplicit) {
             if (slot != 0xff) e += explicit_term(ka, int(a.meta >> 24), make_float3(ci.x, ci.y, ci.z), v_inter, &d);
+        } else if constexpr (Fmt<GF>::ad4) {
+            if (slot != 0xff)
+                e += ad4_grid_term<GF>(ka.grid, slot, ka.ls.charge[s.h.atom_begin + i], make_float3(ci.x, ci.y, ci.z),
+                                       v_inter, &d);
         } else {
             if (slot != 0xff) e += grid_finish<GF>(ka.grid, slot, gq, gf, v_inter, &d);
         }
@@ -717,6 +828,9 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
             const float4 ci = coords[i];
             if constexpr (GF == kGridExplicit) {
                 e += explicit_term(ka, int(a.meta >> 24), make_float3(ci.x, ci.y, ci.z), v_inter, &d);
+            } else if constexpr (Fmt<GF>::ad4) {
+                e += ad4_grid_term<GF>(ka.grid, slot, ka.ls.charge[s.h.atom_begin + i], make_float3(ci.x, ci.y, ci.z),
+                                       v_inter, &d);
             } else {
                 const GridPt gq = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
                 float gf[8];
@@ -729,24 +843,26 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
     const int P = s.h.n_pairs;
     for (int q0 = s.gl; q0 < P; q0 += kPairBatch * L) {
         float rx[kPairBatch], ry[kPairBatch], rz[kPairBatch], rem[kPairBatch];
-        int idx[kPairBatch];
+        int idx[kPairBatch], it[kPairBatch];
         bool in[kPairBatch];
-        float2 t0[kPairBatch], t1[kPairBatch];
+        float2 t0[kPairBatch], t1[kPairBatch], cf[kPairBatch];
+        float4 u0[kPairBatch], u1[kPairBatch];
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u) {
             const int q = q0 + u * L;
             in[u] = false;
-            idx[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
+            idx[u] = 0; it[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
             if (q < P) {
                 const PairRec rec = s.pairs[q];
                 const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
                 rx[u] = cb.x - ca.x; ry[u] = cb.y - ca.y; rz[u] = cb.z - ca.z;
                 const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
-                if (r2 < 64.f) {
+                if (r2 < pair_cut2<GF>()) {
                     const float rf = 32.f * r2;
                     const int i1t = int(rf);
                     rem[u] = rf - float(i1t);
                     idx[u] = int(rec >> 16) * kTableStride + i1t;
+                    it[u] = i1t;
                     in[u] = true;
                 }
             }
@@ -754,8 +870,17 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u)
             if (in[u]) {
-                t0[u] = ka.ls.table_deriv[idx[u]];
-                t1[u] = ka.ls.table_deriv[idx[u] + 1];
+                if (!Fmt<GF>::ad4 || it[u] < kTableStride - 1) {
+                    t0[u] = ka.ls.table_deriv[idx[u]];
+                    t1[u] = ka.ls.table_deriv[idx[u] + 1];
+                } else {
+                    t0[u] = t1[u] = make_float2(0.f, 0.f);
+                }
+                if constexpr (Fmt<GF>::ad4) {
+                    u0[u] = ka.ls.ad4_deriv[it[u]];
+                    u1[u] = ka.ls.ad4_deriv[it[u] + 1];
+                    cf[u] = ka.ls.pair_coef[s.h.pair_begin + q0 + u * L];
+                }
             }
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u) {
@@ -765,6 +890,7 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
             if (in[u]) {
                 float pe = t0[u].x + rem[u] * (t1[u].x - t0[u].x);
                 d = t0[u].y + rem[u] * (t1[u].y - t0[u].y);
+                if constexpr (Fmt<GF>::ad4) ad4_charge_terms(cf[u], u0[u], u1[u], rem[u], pe, d);
                 if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
                     const float tmp = curl_factor(v_intra, pe);
                     pe *= tmp;
@@ -1467,15 +1593,24 @@ __global__ __launch_bounds__(64) void terms_kernel(KArgs ka, const WorkItem* wor
             }
             ei += curl_d(e, vd) + double(ka.grid.slope) * pen;
         } else {
-            // grid::evaluate_aux (grid.cpp:39-129) energy path
+            // grid::evaluate_aux (grid.cpp:39-129) energy path; AD4.2 adds the
+            // electrostatic map times q and the desolvation map times |q|
+            // (ad4cache::eval, ad4cache.cpp:151-185)
             const GridPt q = grid_locate(ka.grid, slot, make_float3(c.x, c.y, c.z));
-            float f[8];
-            grid_gather<GF>(ka.grid, q.base, f);
-            const double fx = q.x, fy = q.y, fz = q.z, mx = 1.0 - fx, my = 1.0 - fy, mz = 1.0 - fz;
-            double e = f[0] * mx * my * mz + f[1] * fx * my * mz + f[2] * mx * fy * mz + f[3] * fx * fy * mz +
-                       f[4] * mx * my * fz + f[5] * fx * my * fz + f[6] * mx * fy * fz + f[7] * fx * fy * fz;
-            if (GF == UD_GRID_F16) e *= double(ka.grid.scale[slot]);
-            ei += curl_d(e, vd) + double(q.penalty);
+            const int nm = Fmt<GF>::ad4 ? 3 : 1;
+            const float qa = Fmt<GF>::ad4 ? ka.ls.charge[s.h.atom_begin + i] : 0.f;
+            const uint32_t cell = q.base - uint32_t(slot) * uint32_t(ka.grid.n_points);
+            for (int m = 0; m < nm; ++m) {
+                const int sl = (m == 0) ? slot : (m == 1) ? ka.grid.elec_slot : ka.grid.dsolv_slot;
+                const double fac = (m == 0) ? 1.0 : (m == 1) ? double(qa) : fabs(double(qa));
+                float f[8];
+                grid_gather<GF>(ka.grid, cell + uint32_t(sl) * uint32_t(ka.grid.n_points), f);
+                const double fx = q.x, fy = q.y, fz = q.z, mx = 1.0 - fx, my = 1.0 - fy, mz = 1.0 - fz;
+                double e = f[0] * mx * my * mz + f[1] * fx * my * mz + f[2] * mx * fy * mz + f[3] * fx * fy * mz +
+                           f[4] * mx * my * fz + f[5] * fx * my * fz + f[6] * mx * fy * fz + f[7] * fx * fy * fz;
+                if (Fmt<GF>::store == UD_GRID_F16) e *= double(ka.grid.scale[sl]);
+                ei += fac * (curl_d(e, vd) + double(q.penalty));
+            }
         }
     }
     const PairRec* prs = s.pairs;
@@ -1484,7 +1619,18 @@ __global__ __launch_bounds__(64) void terms_kernel(KArgs ka, const WorkItem* wor
         const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
         const float rx = cb.x - ca.x, ry = cb.y - ca.y, rz = cb.z - ca.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < 64.f) ea += curl_d(double(ka.ls.table_fast[size_t(rec >> 16) * kTableStride + int(32.f * r2)]), vd);
+        if (r2 < pair_cut2<GF>()) {
+            // precalculate_element::eval_fast (precalculate.h:40-47); AD4.2:
+            // type-pair part plus the charge terms of the pair
+            const int k = int(32.f * r2);
+            double t = (k < kTableStride) ? double(ka.ls.table_fast[size_t(rec >> 16) * kTableStride + k]) : 0.0;
+            if constexpr (Fmt<GF>::ad4) {
+                const float2 cf = ka.ls.pair_coef[s.h.pair_begin + q];
+                const float2 uf = ka.ls.ad4_fast[k];
+                t += double(cf.x) * double(uf.x) + double(cf.y) * double(uf.y);
+            }
+            ea += curl_d(t, vd);
+        }
     }
     ei = gsum_d<L>(ei);
     ea = gsum_d<L>(ea);
@@ -1682,10 +1828,29 @@ static KArgs make_kargs(const LigandSetDev& ls, const GridDev& g, const LaunchCa
         default: { constexpr int L = 64; __VA_ARGS__; } break; \
     }
 
+// grid-format template values dispatched at run time: FP32 / FP16 maps and
+// AD4.2 over FP32 / FP16 maps
+#define UD_DISPATCH_GF(gf, ...)                                                      \
+    switch (gf) {                                                                    \
+        case UD_GRID_F16: { constexpr int GFV = UD_GRID_F16; __VA_ARGS__; } break;   \
+        case kGridAd4F32: { constexpr int GFV = kGridAd4F32; __VA_ARGS__; } break;   \
+        case kGridAd4F16: { constexpr int GFV = kGridAd4F16; __VA_ARGS__; } break;   \
+        default: { constexpr int GFV = UD_GRID_F32; __VA_ARGS__; } break;            \
+    }
+
 hipError_t launch_tables(int sf, const float* w, float2* deriv, float* fast, hipStream_t st) {
     SfWeights W;
     for (int i = 0; i < 7; ++i) W.w[i] = w[i];
-    hipLaunchKernelGGL(table_kernel, dim3(kTypePairs), dim3(256), 0, st, sf, W, deriv, fast);
+    hipLaunchKernelGGL(table_kernel, dim3(kTypePairs), dim3(256), 0, st, sf, W, ad4_params(), deriv, fast);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_build_ad4(const float4* rec, const float* rec_q, int n_rec, const GridBuildArgs& a,
+                                 const float* w, float* maps_out, hipStream_t st) {
+    SfWeights W;
+    for (int i = 0; i < 7; ++i) W.w[i] = (i < 4) ? w[i] : 0.f;
+    dim3 grid((a.n_points + 255) / 256, std::max(1, (a.n_maps + 7) / 8));
+    hipLaunchKernelGGL(grid_build_ad4_kernel, grid, dim3(256), 0, st, rec, rec_q, n_rec, a, ad4_params(), W, maps_out);
     return hipGetLastError();
 }
 
@@ -1714,12 +1879,8 @@ hipError_t launch_score(int lanes, int gf, const LigandSetDev& ls, const GridDev
     const KArgs ka = make_kargs(ls, g, cap);
     const size_t shm = lds_bytes(lanes, cap);
     UD_DISPATCH_L(lanes, {
-        if (gf == UD_GRID_F16)
-            hipLaunchKernelGGL((score_kernel<L, UD_GRID_F16>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
-                               v_intra, v_inter, e_out, g_out);
-        else
-            hipLaunchKernelGGL((score_kernel<L, UD_GRID_F32>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
-                               v_intra, v_inter, e_out, g_out);
+        UD_DISPATCH_GF(gf, hipLaunchKernelGGL((score_kernel<L, GFV>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm,
+                                              poses, v_intra, v_inter, e_out, g_out));
     });
     return hipGetLastError();
 }
@@ -1742,21 +1903,14 @@ static void launch_search_nm(int gf, const LaunchCap& cap, const KArgs& ka, cons
                              float* poses, float v_intra, float v_inter, int max_steps, float* out_pose, float* out_e,
                              long long* out_evals, int32_t* evals32) {
     const bool small = 6 + cap.n_nodes - 1 <= kNSmall;
-    if (gf == UD_GRID_F16) {
+    UD_DISPATCH_GF(gf, {
         if (small)
-            launch_search<L, UD_GRID_F16, MC, kNSmall>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter,
-                                                        max_steps, out_pose, out_e, out_evals, evals32);
+            launch_search<L, GFV, MC, kNSmall>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter, max_steps,
+                                               out_pose, out_e, out_evals, evals32);
         else
-            launch_search<L, UD_GRID_F16, MC, kNMax>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter,
-                                                      max_steps, out_pose, out_e, out_evals, evals32);
-    } else {
-        if (small)
-            launch_search<L, UD_GRID_F32, MC, kNSmall>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter,
-                                                        max_steps, out_pose, out_e, out_evals, evals32);
-        else
-            launch_search<L, UD_GRID_F32, MC, kNMax>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter,
-                                                      max_steps, out_pose, out_e, out_evals, evals32);
-    }
+            launch_search<L, GFV, MC, kNMax>(ka, sp, work, n_work, shm, st, perm, poses, v_intra, v_inter, max_steps,
+                                             out_pose, out_e, out_evals, evals32);
+    });
 }
 
 hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
@@ -1837,12 +1991,8 @@ hipError_t launch_terms(int lanes, int gf, const LigandSetDev& ls, const GridDev
     const KArgs ka = make_kargs(ls, g, cap);
     const size_t shm = lds_bytes(lanes, cap);
     UD_DISPATCH_L(lanes, {
-        if (gf == UD_GRID_F16)
-            hipLaunchKernelGGL((terms_kernel<L, UD_GRID_F16>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
-                               v, inter_out, intra_out);
-        else
-            hipLaunchKernelGGL((terms_kernel<L, UD_GRID_F32>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses,
-                               v, inter_out, intra_out);
+        UD_DISPATCH_GF(gf, hipLaunchKernelGGL((terms_kernel<L, GFV>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm,
+                                              poses, v, inter_out, intra_out));
     });
     return hipGetLastError();
 }

@@ -6,8 +6,11 @@
 
 namespace ud {
 
-static constexpr int kTableStride = 2049;  // table entries stored per type pair
-static constexpr int kTypePairs = 36 * 37 / 2;
+static constexpr int kTableStride = 2050;  // table entries stored per type pair
+static constexpr int kTypePairs = 36 * 37 / 2;  // XS types; the 35 AD types need 630
+// grid-format template values of the kernels: FP32 / FP16 maps (UD_GRID_*),
+// the explicit receptor, and AD4.2 over FP32 / FP16 maps
+static constexpr int kGridAd4F32 = 3, kGridAd4F16 = 4;
 
 struct GridBuildArgs {
     float init[3];
@@ -23,6 +26,10 @@ size_t lds_bytes(int lanes, const LaunchCap& cap);
 hipError_t launch_tables(int sf, const float* w, float2* deriv, float* fast, hipStream_t st);
 hipError_t launch_grid_build(const float4* rec, int n_rec, const GridBuildArgs& a, const float* table_fast,
                              float* maps_out, hipStream_t st);
+// AD4.2 maps: a.n_maps affinity maps, then the electrostatic and desolvation
+// maps (rec.w holds the AD type bits); w = (vdw, hb, elec, dsolv)
+hipError_t launch_grid_build_ad4(const float4* rec, const float* rec_q, int n_rec, const GridBuildArgs& a,
+                                 const float* w, float* maps_out, hipStream_t st);
 hipError_t launch_grid_absmax(const float* maps, int n_points, int n_maps, unsigned int* out, hipStream_t st);
 hipError_t launch_grid_to_half(const float* maps, int n_points, int n_maps, const float* inv_scale, void* out,
                                hipStream_t st);

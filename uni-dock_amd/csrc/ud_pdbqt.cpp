@@ -71,6 +71,7 @@ struct Atom {
     float x = 0, y = 0, z = 0;
     int node = 0;
     int xs = -1;
+    float charge = 0.f;  // partial charge (AD4.2 scoring)
 };
 
 std::string trim(const std::string& s) {
@@ -95,6 +96,13 @@ bool parse_atom_line(const std::string& ln, Atom& a, std::string& err) {
         if (c.empty() || *end) { err = "bad coordinate: " + ln; return false; }
     }
     a.x = v[0]; a.y = v[1]; a.z = v[2];
+    // partial charge, columns 69-76 when not blank (parse_pdbqt_atom_string, parse_pdbqt.cpp:112-114)
+    const std::string q = trim(cols(ln, 69, 76));
+    a.charge = 0.f;
+    if (!q.empty()) {
+        a.charge = std::strtof(q.c_str(), &end);
+        if (*end) { err = "Charge \"" + q + "\" is not valid: " + ln; return false; }
+    }
     const std::string t = trim(cols(ln, 78, 79));
     a.ad = ad_of(t);
     if (a.ad < 0) { err = "atom type " + t + " is not a valid AutoDock type: " + ln; return false; }
@@ -231,7 +239,8 @@ struct Ligand {
     std::vector<Atom> atoms;  // reference order (nodes in pre-order, input order inside)
     std::vector<int> node_parent, node_abegin, node_aend;
     std::vector<float> node_origin, node_axis;
-    std::vector<int> pairs;
+    std::vector<int> pairs;      // X-Score typed atoms (Vina, Vinardo)
+    std::vector<int> pairs_ad4;  // AutoDock typed atoms, hydrogens included (AD4.2)
     float num_tors = 0;
     int torsdof = 0;
 };
@@ -417,14 +426,18 @@ bool finish_ligand(const Tree& T, Ligand& L, std::string& err) {
                 if (is_cg(L.atoms[u].ad, k) && has[k]) return true;
         return false;
     };
+    // the typing in use decides which atoms are typed (model.cpp:608-612):
+    // every atom has an AutoDock type; hydrogens have no X-Score type
     for (int i = 0; i < n; ++i) {
         const std::vector<int> near = within3(i);
         for (int j = i + 1; j < n; ++j) {
             if (fixed(i, j)) continue;
             if (std::find(near.begin(), near.end(), j) != near.end()) continue;
             if (closure_clash(i, j)) continue;
-            if (L.atoms[i].xs < 0 || L.atoms[j].xs < 0) continue;
             if (glue(i, j)) continue;  // glue pairs are not ligand pairs
+            L.pairs_ad4.push_back(i);
+            L.pairs_ad4.push_back(j);
+            if (L.atoms[i].xs < 0 || L.atoms[j].xs < 0) continue;
             L.pairs.push_back(i);
             L.pairs.push_back(j);
         }
@@ -551,6 +564,11 @@ bool build_ligand_sdf(const std::vector<std::string>& lines, Ligand& L, std::str
                 const int ad = ad_of(name);
                 if (ad < 0) { err = "atom type " + name + " is not a valid AutoDock type: " + ln; return false; }
                 all[id - 1].ad = ad;
+                // partial charge in columns 4-13 (parse_sdf_aux, parse_pdbqt.cpp:678-680)
+                const std::string qs = trim(cols(ln, 4, 13));
+                const float qv = std::strtof(qs.c_str(), &end);
+                if (qs.empty() || *end) { err = "bad partial charge in atomInfo line: " + ln; return false; }
+                all[id - 1].charge = qv;
             } else if (tor || frag) {
                 if (!sdf_ints(ln, ints) || ints.empty()) { err = "bad SDF data line: " + ln; return false; }
                 if (tor && ints.size() != 4) { err = "torsionInfo lines hold 4 numbers: " + ln; return false; }
@@ -645,8 +663,8 @@ struct ud_ligset {
     std::string err;
     // flattened batch arrays (rebuilt by ud_ligset_batch)
     std::vector<int32_t> atom_offset, node_offset, pair_offset, atom_xs, atom_heavy, node_parent, node_abegin,
-        node_aend, pairs;
-    std::vector<float> atom_local, node_origin, node_axis, num_tors;
+        node_aend, pairs, atom_ad;
+    std::vector<float> atom_local, node_origin, node_axis, num_tors, atom_charge, torsdof;
 };
 
 extern "C" {
@@ -673,7 +691,7 @@ int ud_ligset_add_pdbqt(ud_ligset* s, const char* text, int32_t* index_out) {
         return UD_ERR_INVALID;
     }
     if (int(L.atoms.size()) > UD_MAX_ATOMS || int(L.node_parent.size()) - 1 > UD_MAX_TORSIONS ||
-        int(L.pairs.size() / 2) > UD_MAX_PAIRS) {
+        int(L.pairs_ad4.size() / 2) > UD_MAX_PAIRS) {
         s->err = "ligand exceeds the library's capacities";
         return UD_ERR_CAPACITY;
     }
@@ -715,7 +733,7 @@ int ud_ligset_add_sdf(ud_ligset* s, const char* text, int32_t* first_index_out, 
         }
         const Ligand& L = parsed[r];
         if (int(L.atoms.size()) > UD_MAX_ATOMS || int(L.node_parent.size()) - 1 > UD_MAX_TORSIONS ||
-            int(L.pairs.size() / 2) > UD_MAX_PAIRS) {
+            int(L.pairs_ad4.size() / 2) > UD_MAX_PAIRS) {
             s->err = "SDF record " + std::to_string(r + 1) + ": ligand exceeds the library's capacities";
             return UD_ERR_CAPACITY;
         }
@@ -728,20 +746,24 @@ int ud_ligset_add_sdf(ud_ligset* s, const char* text, int32_t* first_index_out, 
 
 int ud_ligset_count(const ud_ligset* s) { return s ? int(s->ligs.size()) : 0; }
 
-int ud_ligset_batch(ud_ligset* s, ud_ligand_batch* b) {
-    if (!s || !b) return UD_ERR_INVALID;
+int ud_ligset_batch(ud_ligset* s, ud_ligand_batch* b) { return ud_ligset_batch_sf(s, UD_SF_VINA, b); }
+
+int ud_ligset_batch_sf(ud_ligset* s, int sf, ud_ligand_batch* b) {
+    if (!s || !b || (sf != UD_SF_VINA && sf != UD_SF_VINARDO && sf != UD_SF_AD4)) return UD_ERR_INVALID;
     s->atom_offset.assign(1, 0);
     s->node_offset.assign(1, 0);
     s->pair_offset.assign(1, 0);
     s->atom_xs.clear(); s->atom_heavy.clear(); s->node_parent.clear(); s->node_abegin.clear();
     s->node_aend.clear(); s->pairs.clear(); s->atom_local.clear(); s->node_origin.clear();
-    s->node_axis.clear(); s->num_tors.clear();
+    s->node_axis.clear(); s->num_tors.clear(); s->atom_ad.clear(); s->atom_charge.clear(); s->torsdof.clear();
     for (const Ligand& L : s->ligs) {
         for (const Atom& a : L.atoms) {
             const float* o = &L.node_origin[3 * a.node];
             s->atom_local.insert(s->atom_local.end(), {a.x - o[0], a.y - o[1], a.z - o[2]});
             s->atom_xs.push_back(a.xs);
             s->atom_heavy.push_back(a.el != EL_H ? 1 : 0);
+            s->atom_ad.push_back(a.ad);
+            s->atom_charge.push_back(a.charge);
         }
         for (size_t k = 0; k < L.node_parent.size(); ++k) {
             s->node_parent.push_back(L.node_parent[k]);
@@ -750,11 +772,13 @@ int ud_ligset_batch(ud_ligset* s, ud_ligand_batch* b) {
         }
         s->node_origin.insert(s->node_origin.end(), L.node_origin.begin(), L.node_origin.end());
         s->node_axis.insert(s->node_axis.end(), L.node_axis.begin(), L.node_axis.end());
-        s->pairs.insert(s->pairs.end(), L.pairs.begin(), L.pairs.end());
+        const std::vector<int>& pr = (sf == UD_SF_AD4) ? L.pairs_ad4 : L.pairs;
+        s->pairs.insert(s->pairs.end(), pr.begin(), pr.end());
         s->num_tors.push_back(L.num_tors);
+        s->torsdof.push_back(float(L.torsdof));
         s->atom_offset.push_back(s->atom_offset.back() + int32_t(L.atoms.size()));
         s->node_offset.push_back(s->node_offset.back() + int32_t(L.node_parent.size()));
-        s->pair_offset.push_back(s->pair_offset.back() + int32_t(L.pairs.size() / 2));
+        s->pair_offset.push_back(s->pair_offset.back() + int32_t(pr.size() / 2));
     }
     b->n_ligands = int32_t(s->ligs.size());
     b->atom_offset = s->atom_offset.data();
@@ -770,6 +794,9 @@ int ud_ligset_batch(ud_ligset* s, ud_ligand_batch* b) {
     b->node_axis = s->node_axis.data();
     b->pairs = s->pairs.data();
     b->num_tors = s->num_tors.data();
+    b->atom_ad = s->atom_ad.data();
+    b->atom_charge = s->atom_charge.data();
+    b->torsdof = s->torsdof.data();
     return UD_OK;
 }
 
@@ -885,6 +912,32 @@ int ud_ligset_write_sdf(const ud_ligset* s, int lig, int n_modes, const ud_mode*
     if (!out) return UD_OK;
     if (cap < txt.size() + 1) return UD_ERR_CAPACITY;
     std::memcpy(out, txt.c_str(), txt.size() + 1);
+    return UD_OK;
+}
+
+int ud_receptor_ad4_from_pdbqt(const char* text, int32_t capacity, int32_t* n_out, float* xyz_out, int32_t* ad_out,
+                               float* charge_out) {
+    if (!text || !n_out) return UD_ERR_INVALID;
+    std::istringstream in(text);
+    std::string ln, err;
+    int n = 0;
+    const bool write = xyz_out && ad_out && charge_out;
+    while (std::getline(in, ln)) {
+        if (!ln.empty() && ln.back() == '\r') ln.pop_back();
+        if (ln.rfind("ATOM", 0) != 0 && ln.rfind("HETATM", 0) != 0) continue;
+        Atom a;
+        if (!parse_atom_line(ln, a, err)) return UD_ERR_INVALID;
+        if (write && n < capacity) {
+            xyz_out[3 * n] = a.x;
+            xyz_out[3 * n + 1] = a.y;
+            xyz_out[3 * n + 2] = a.z;
+            ad_out[n] = a.ad;
+            charge_out[n] = a.charge;
+        }
+        ++n;
+    }
+    *n_out = n;
+    if ((xyz_out || ad_out || charge_out) && n > capacity) return UD_ERR_CAPACITY;
     return UD_OK;
 }
 

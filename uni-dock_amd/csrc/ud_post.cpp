@@ -91,4 +91,17 @@ int xs_element(int t) {
     }
 }
 
+// element class of an AutoDock type (ad_type_to_el_type, atom_constants.h
+// and atom_type.h): the same classes as xs_element for the same atoms
+int ad4_element(int t) {
+    switch (t) {
+        case 0: case 1: case 26: case 27: case 28: case 29: return 6;   // C A CG0-3
+        case 2: case 9: case 32: case 34: return 7;                    // N NA NXA NXD
+        case 3: case 10: case 31: case 33: return 8;                   // O OA OXA OXD
+        case 5: case 11: return 110;                                   // S SA
+        case 13: case 14: case 15: case 16: case 17: return 118;       // metals
+        default: return 200 + t;
+    }
+}
+
 }  // namespace ud

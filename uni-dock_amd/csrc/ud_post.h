@@ -12,5 +12,6 @@ void cluster_modes(int n_cand, int n_heavy, const float* xyz, const float* energ
                    std::vector<int>& kept);
 float conf_independent(float x, float num_tors, float rot_internal);
 int xs_element(int xs_type);
+int ad4_element(int ad_type);
 
 }  // namespace ud

@@ -13,7 +13,8 @@ import numpy as np
 from . import _build
 
 UD_OK, UD_ERR_INVALID, UD_ERR_CAPACITY, UD_ERR_HIP, UD_ERR_STATE, UD_ERR_NOMEM = range(6)
-UD_SF_VINA, UD_SF_VINARDO = 0, 1
+UD_SF_VINA, UD_SF_VINARDO, UD_SF_AD4 = 0, 1, 2
+UD_AD4_MAP_ELEC, UD_AD4_MAP_DSOLV = 35, 36  # AD4.2 electrostatic / desolvation map indices
 UD_GRID_F32, UD_GRID_F16 = 0, 1
 UD_MAX_ATOMS = 128
 UD_MAX_TORSIONS = 32
@@ -25,6 +26,12 @@ BUCKET_ATOMS = (16, 32, 48, 64, 96, 128)
 
 VINA_WEIGHTS = (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 50.0, 0.05846)  # vina.h:129-132
 VINARDO_WEIGHTS = (-0.045, 0.8, -0.035, -0.600, 50.0, 0.05846)                      # vina.h:133-135
+AD4_WEIGHTS = (0.1662, 0.1209, 0.1406, 0.1322, 50.0, 0.2983)                         # vina.h:136-138
+DEFAULT_WEIGHTS = {UD_SF_VINA: VINA_WEIGHTS, UD_SF_VINARDO: VINARDO_WEIGHTS, UD_SF_AD4: AD4_WEIGHTS}
+# AutoDock 4.2 type names by index (atom_constants.h:46-81)
+AD_TYPE_NAMES = ("C", "A", "N", "O", "P", "S", "H", "F", "I", "NA", "OA", "SA", "HD", "Mg", "Mn", "Zn", "Ca", "Fe",
+                 "Cl", "Br", "Si", "At", "G0", "G1", "G2", "G3", "CG0", "CG1", "CG2", "CG3", "W", "OXA", "NXA",
+                 "OXD", "NXD")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int32)
 _c_float_p = ctypes.POINTER(ctypes.c_float)
@@ -36,7 +43,8 @@ class UdLigandBatch(ctypes.Structure):
                 ("atom_local", _c_float_p), ("atom_xs", _c_int_p), ("atom_heavy", _c_int_p),
                 ("node_parent", _c_int_p), ("node_abegin", _c_int_p), ("node_aend", _c_int_p),
                 ("node_origin", _c_float_p), ("node_axis", _c_float_p),
-                ("pairs", _c_int_p), ("num_tors", _c_float_p)]
+                ("pairs", _c_int_p), ("num_tors", _c_float_p),
+                ("atom_ad", _c_int_p), ("atom_charge", _c_float_p), ("torsdof", _c_float_p)]
 
 
 class UdLaunchInfo(ctypes.Structure):
@@ -121,6 +129,9 @@ def load() -> ctypes.CDLL:
         "ud_ligset_add_sdf": (ctypes.c_int, [vp, ctypes.c_char_p, ip, ip]),
         "ud_ligset_count": (ctypes.c_int, [vp]),
         "ud_ligset_batch": (ctypes.c_int, [vp, ctypes.POINTER(UdLigandBatch)]),
+        "ud_ligset_batch_sf": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(UdLigandBatch)]),
+        "ud_receptor_ad4_from_pdbqt": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ip, fp, ip, fp]),
+        "ud_compute_maps_ad4": (ctypes.c_int, [vp, ctypes.c_int, fp, ip, fp, ctypes.c_int, ip, ctypes.c_int]),
         "ud_ligset_atoms": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, ip]),
         "ud_receptor_from_pdbqt": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ip, fp, ip]),
         "ud_ligset_write_sdf": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(UdMode), fp, ctypes.c_int,
@@ -198,12 +209,13 @@ class LigandSet:
     def __len__(self):
         return self.L.ud_ligset_count(self.h)
 
-    def arrays(self) -> dict:
-        """The batch in the dict layout Context.set_ligands takes (copies)."""
+    def arrays(self, sf: int = UD_SF_VINA) -> dict:
+        """The batch in the dict layout Context.set_ligands takes (copies),
+        with the pair list of scoring function sf."""
         b = UdLigandBatch()
-        rc = self.L.ud_ligset_batch(self.h, ctypes.byref(b))
+        rc = self.L.ud_ligset_batch_sf(self.h, sf, ctypes.byref(b))
         if rc != UD_OK:
-            raise UdError(rc, "ud_ligset_batch failed")
+            raise UdError(rc, "ud_ligset_batch_sf failed")
         n = b.n_ligands
         ao = np.ctypeslib.as_array(b.atom_offset, (n + 1,)).copy()
         no = np.ctypeslib.as_array(b.node_offset, (n + 1,)).copy()
@@ -217,6 +229,7 @@ class LigandSet:
             node_abegin=get(b.node_abegin, nn), node_aend=get(b.node_aend, nn),
             node_origin=get(b.node_origin, 3 * nn).reshape(-1, 3), node_axis=get(b.node_axis, 3 * nn).reshape(-1, 3),
             pairs=get(b.pairs, 2 * npr).reshape(-1, 2), num_tors=get(b.num_tors, n),
+            atom_ad=get(b.atom_ad, na), atom_charge=get(b.atom_charge, na), torsdof=get(b.torsdof, n),
         )
 
     def _write(self, fn, lig, modes, coords, how_many, energy_range) -> str:
@@ -250,8 +263,8 @@ class LigandSet:
         mode with the "> <Uni-Dock RESULT>" field."""
         return self._write("ud_ligset_write_sdf", lig, modes, coords, how_many, energy_range)
 
-    def ligands(self) -> list:
-        a = self.arrays()
+    def ligands(self, sf: int = UD_SF_VINA) -> list:
+        a = self.arrays(sf)
         out = []
         for l in range(a["n_ligands"]):
             a0, a1 = a["atom_offset"][l], a["atom_offset"][l + 1]
@@ -268,7 +281,8 @@ class LigandSet:
                 abegin=a["node_abegin"][k0:k1], aend=a["node_aend"][k0:k1],
                 origin=a["node_origin"][k0:k1].astype(np.float64), axis=a["node_axis"][k0:k1].astype(np.float64),
                 pairs=a["pairs"][p0:p1], num_tors=float(a["num_tors"][l]), coords=xyz.astype(np.float64),
-                serial=serial))
+                serial=serial, ad=a["atom_ad"][a0:a1], charge=a["atom_charge"][a0:a1].astype(np.float64),
+                torsdof=float(a["torsdof"][l])))
         return out
 
 
@@ -285,6 +299,65 @@ def receptor_from_pdbqt(text: str):
     if rc != UD_OK:
         raise UdError(rc, "receptor PDBQT could not be parsed")
     return xyz, xs
+
+
+def receptor_ad4_from_pdbqt(text: str):
+    """Every receptor atom with its AutoDock type and partial charge:
+    (xyz float32 [n, 3], ad int32 [n], charge float32 [n])."""
+    L = load()
+    n = ctypes.c_int32()
+    rc = L.ud_receptor_ad4_from_pdbqt(text.encode(), 0, ctypes.byref(n), None, None, None)
+    if rc != UD_OK:
+        raise UdError(rc, "receptor PDBQT could not be parsed")
+    xyz = np.zeros((n.value, 3), dtype=np.float32)
+    ad = np.zeros(n.value, dtype=np.int32)
+    q = np.zeros(n.value, dtype=np.float32)
+    rc = L.ud_receptor_ad4_from_pdbqt(text.encode(), n.value, ctypes.byref(n), _p(xyz, ctypes.c_float),
+                                      _p(ad, ctypes.c_int32), _p(q, ctypes.c_float))
+    if rc != UD_OK:
+        raise UdError(rc, "receptor PDBQT could not be parsed")
+    return xyz, ad, q
+
+
+def ad4_grid_type(t: int) -> int:
+    """AutoDock type whose affinity map an atom of type t reads: CG carbons
+    the C map, G atoms none (ad4cache.cpp:223-240)."""
+    if t < 0 or t >= 35 or 22 <= t <= 25:
+        return -1
+    if 26 <= t <= 29:
+        return 0
+    return t
+
+
+def read_ad4_map(path):
+    """One AutoGrid map file (read_ad4_map, ad4cache.cpp:332-389): returns
+    (center[3], n_voxels[3], spacing, values x-fastest).  The values start on
+    line 7; line 4 holds SPACING, line 5 NELEMENTS (even counts), line 6
+    CENTER."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    spacing = float(lines[3].split()[1])
+    nv = [int(x) for x in lines[4].split()[1:4]]
+    if any(n % 2 for n in nv):
+        raise ValueError(f"{path}: number of voxels (NELEMENTS) must be even")
+    center = [float(x) for x in lines[5].split()[1:4]]
+    vals = np.array([float(x) for x in lines[6:] if x.strip()], dtype=np.float64)
+    if vals.size != (nv[0] + 1) * (nv[1] + 1) * (nv[2] + 1):
+        raise ValueError(f"{path}: {vals.size} values for {nv} voxels")
+    return np.array(center), np.array(nv), spacing, vals
+
+
+def write_ad4_map(path, center, n_voxels, spacing, values, receptor="receptor.pdbqt"):
+    """An AutoGrid-format map file (the layout ad4cache::write writes,
+    ad4cache.cpp:470-520): six header lines, then one value per line."""
+    nv = [int(x) for x in n_voxels]
+    with open(path, "w") as f:
+        f.write("GRID_PARAMETER_FILE unidock_amd.gpf\nGRID_DATA_FILE unidock_amd.maps.fld\n")
+        f.write(f"MACROMOLECULE {receptor}\nSPACING {spacing:.3f}\n")
+        f.write(f"NELEMENTS {nv[0]} {nv[1]} {nv[2]}\n")
+        f.write(f"CENTER {center[0]:.3f} {center[1]:.3f} {center[2]:.3f}\n")
+        for v in np.asarray(values).reshape(-1):
+            f.write(f"{v:.4g}\n")  # std::setprecision(4)
 
 
 def cluster_modes(heavy_xyz, energy, min_rmsd=1.0, max_size=9):
@@ -354,10 +427,10 @@ class Context:
 
     # ---- set-up
     def set_scoring(self, sf: int = UD_SF_VINA, weights=None):
-        w = np.array(weights if weights is not None else (VINA_WEIGHTS if sf == UD_SF_VINA else VINARDO_WEIGHTS),
-                     dtype=np.float64)
+        w = np.array(weights if weights is not None else DEFAULT_WEIGHTS[sf], dtype=np.float64)
         self._ck(self.L.ud_set_scoring(self.h, sf, _p(w, ctypes.c_double), len(w)))
         self.weights = tuple(float(x) for x in w)
+        self.sf = sf
 
     def set_box(self, center, size, granularity=0.375):
         c = np.array(center, dtype=np.float64)
@@ -375,6 +448,35 @@ class Context:
         self._ck(self.L.ud_compute_maps(self.h, len(t), _p(x, ctypes.c_float), _p(t, ctypes.c_int32), len(ty),
                                         _p(ty, ctypes.c_int32), precision))
         self.map_types = [int(v) for v in ty]
+
+    def compute_maps_ad4(self, rec_xyz, rec_ad, rec_charge, types, precision=UD_GRID_F32):
+        """AD4.2 affinity maps of the listed AutoDock types plus the
+        electrostatic and desolvation maps (UD_SF_AD4 scoring)."""
+        x = np.ascontiguousarray(rec_xyz, dtype=np.float32).reshape(-1)
+        t = np.ascontiguousarray(rec_ad, dtype=np.int32)
+        q = np.ascontiguousarray(rec_charge, dtype=np.float32)
+        ty = np.ascontiguousarray(types, dtype=np.int32)
+        self._ck(self.L.ud_compute_maps_ad4(self.h, len(t), _p(x, ctypes.c_float), _p(t, ctypes.c_int32),
+                                            _p(q, ctypes.c_float), len(ty), _p(ty, ctypes.c_int32), precision))
+        self.map_types = [int(v) for v in ty] + [UD_AD4_MAP_ELEC, UD_AD4_MAP_DSOLV]
+
+    def load_ad4_maps(self, prefix, ad_types, precision=UD_GRID_F32):
+        """AutoGrid maps <prefix>.<type>.map of the needed AutoDock types plus
+        <prefix>.e.map and <prefix>.d.map (ad4cache::read, ad4cache.cpp:
+        391-440): sets the box from the files and uploads the maps."""
+        types = sorted({ad4_grid_type(int(t)) for t in ad_types} - {-1})
+        names = [f"{prefix}.{AD_TYPE_NAMES[t]}.map" for t in types] + [f"{prefix}.e.map", f"{prefix}.d.map"]
+        geo, vals = None, []
+        for nm in names:
+            center, nv, spacing, v = read_ad4_map(nm)
+            if geo is None:
+                geo = (center, nv, spacing)
+            elif not (np.allclose(center, geo[0]) and np.array_equal(nv, geo[1]) and spacing == geo[2]):
+                raise ValueError(f"{nm}: grid differs from {names[0]}")
+            vals.append(v)
+        center, nv, spacing = geo
+        self.set_box(center, nv * spacing, spacing)
+        self.set_maps(types + [UD_AD4_MAP_ELEC, UD_AD4_MAP_DSOLV], np.concatenate(vals), precision)
 
     def set_maps(self, types, values, precision=UD_GRID_F32):
         ty = np.ascontiguousarray(types, dtype=np.int32)
@@ -398,6 +500,14 @@ class Context:
         for k in ("atom_local", "node_origin", "node_axis", "num_tors"):
             a[k] = np.ascontiguousarray(a[k], dtype=np.float32).reshape(-1)
             setattr(b, k, _p(a[k], ctypes.c_float))
+        # AD4.2 typing (optional for Vina / Vinardo)
+        if a.get("atom_ad") is not None:
+            a["atom_ad"] = np.ascontiguousarray(a["atom_ad"], dtype=np.int32).reshape(-1)
+            b.atom_ad = _p(a["atom_ad"], ctypes.c_int32)
+        for k in ("atom_charge", "torsdof"):
+            if a.get(k) is not None:
+                a[k] = np.ascontiguousarray(a[k], dtype=np.float32).reshape(-1)
+                setattr(b, k, _p(a[k], ctypes.c_float))
         self._ck(self.L.ud_set_ligands(self.h, ctypes.byref(b)))
         self._keep = a
         self.n_ligands = a["n_ligands"]

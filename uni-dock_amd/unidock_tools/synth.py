@@ -110,6 +110,10 @@ class Ligand:
     bonds: np.ndarray           # (B, 2) int32 (node-ordered atom ids)
     rotatable: np.ndarray       # (B,) bool
     num_tors: float = 0.0
+    # AutoDock 4.2 view (ad4_ligand): AutoDock types, partial charges, TORSDOF
+    ad: np.ndarray | None = None
+    charge: np.ndarray | None = None
+    torsdof: float = 0.0
 
     @property
     def n_atoms(self) -> int:
@@ -348,6 +352,59 @@ def num_tors_vina(lig: Ligand) -> float:
     return acc
 
 
+# ---------------------------------------------------------------- AutoDock 4.2
+# AutoDock type indices (atom_constants.h:46-81)
+AD_C, AD_A, AD_N, AD_O, AD_P, AD_S, AD_H, AD_F, AD_I, AD_NA, AD_OA, AD_SA, AD_HD = range(13)
+# synthetic AutoDock type per X-Score type: acceptor N -> NA, O -> OA, S -> SA
+_XS_TO_AD = {XS_C_H: AD_C, XS_C_P: AD_C, XS_N_P: AD_N, XS_N_D: AD_N, XS_N_A: AD_NA, XS_N_DA: AD_NA,
+             XS_O_P: AD_OA, XS_O_D: AD_OA, XS_O_A: AD_OA, XS_O_DA: AD_OA, XS_S_P: AD_SA}
+# partial-charge mean and spread per AutoDock type (Gasteiger-like magnitudes)
+_AD_CHARGE = {AD_C: (0.05, 0.08), AD_A: (0.0, 0.04), AD_N: (-0.25, 0.08), AD_NA: (-0.35, 0.08),
+              AD_OA: (-0.40, 0.08), AD_SA: (-0.15, 0.05), AD_HD: (0.22, 0.04)}
+
+
+def _ad_charges(rng, ad):
+    return np.array([rng.normal(*_AD_CHARGE[int(t)]) for t in ad], dtype=np.float64)
+
+
+def ad4_ligand(lig: Ligand, seed: int) -> Ligand:
+    """The AutoDock 4.2 view of a synthetic ligand: AutoDock types from the
+    X-Score types (30 % of the carbons aromatic "A"), half of the terminal
+    atoms bonded to an N or O turned into polar hydrogens "HD" (not heavy),
+    seeded partial charges and TORSDOF = torsions.  Coordinates, tree and
+    pairs are the ligand's own (every atom is AutoDock-typed)."""
+    from dataclasses import replace
+    rng = np.random.default_rng(seed)
+    ad = np.array([_XS_TO_AD.get(int(t), AD_C) for t in lig.xs], dtype=np.int32)
+    ad[(ad == AD_C) & (rng.random(lig.n_atoms) < 0.3)] = AD_A
+    deg = np.zeros(lig.n_atoms, dtype=np.int64)
+    for a, b in lig.bonds:
+        deg[a] += 1
+        deg[b] += 1
+    heavy = lig.heavy.copy()
+    xs = lig.xs.copy()
+    polar = {AD_N, AD_NA, AD_OA}
+    for a, b in lig.bonds:
+        for h, x in ((a, b), (b, a)):
+            if deg[h] == 1 and int(ad[x]) in polar and ad[h] != AD_HD and rng.random() < 0.5:
+                ad[h] = AD_HD
+                heavy[h] = 0
+                xs[h] = -1
+    return replace(lig, ad=ad, charge=_ad_charges(rng, ad), torsdof=float(lig.n_tors), heavy=heavy, xs=xs)
+
+
+def ad4_receptor(rec: Receptor, seed: int, hd_fraction: float = 0.08):
+    """AutoDock types and partial charges for a synthetic receptor (X-Score
+    types mapped as for ligands, carbons 30 % "A", a fraction of the atoms
+    polar hydrogens "HD"): (ad int32 [n], charge float64 [n])."""
+    rng = np.random.default_rng(seed)
+    n = len(rec.xs)
+    ad = np.array([_XS_TO_AD.get(int(t), AD_C) for t in rec.xs], dtype=np.int32)
+    ad[(ad == AD_C) & (rng.random(n) < 0.3)] = AD_A
+    ad[rng.random(n) < hd_fraction] = AD_HD
+    return ad, _ad_charges(rng, ad)
+
+
 @dataclass
 class LigandBatch:
     """Flat arrays in the C-ABI layout (see include/unidock_amd.h)."""
@@ -372,6 +429,10 @@ class LigandBatch:
             node_axis=np.concatenate([l.axis for l in ls]).astype(np.float32).reshape(-1, 3),
             pairs=np.concatenate([l.pairs for l in ls]).astype(np.int32).reshape(-1, 2),
             num_tors=np.array([l.num_tors for l in ls], dtype=np.float32),
+            **({} if any(l.ad is None for l in ls) else dict(
+                atom_ad=np.concatenate([l.ad for l in ls]).astype(np.int32),
+                atom_charge=np.concatenate([l.charge for l in ls]).astype(np.float32),
+                torsdof=np.array([l.torsdof for l in ls], dtype=np.float32))),
         )
 
 
