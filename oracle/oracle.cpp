@@ -1505,8 +1505,15 @@ template <class R> static void set_ad4(Ctx<R>& C, int lig, const int* ad, const 
     for (int i = 0; i < L.n_atoms; ++i) L.q[i] = R(q[i]);
     L.torsdof = R(torsdof);
     L.ptab.assign(L.n_pairs, PairTable<R>());
-    for (int k = 0; k < L.n_pairs; ++k)
-        L.ptab[k].build_ad4(L.ad[L.pa[k]], L.q[L.pa[k]], L.ad[L.pb[k]], L.q[L.pb[k]], C.w);
+    std::atomic<int> next(0);
+    auto work = [&] {
+        for (int k = next.fetch_add(1); k < L.n_pairs; k = next.fetch_add(1))
+            L.ptab[k].build_ad4(L.ad[L.pa[k]], L.q[L.pa[k]], L.ad[L.pb[k]], L.q[L.pb[k]], C.w);
+    };
+    std::vector<std::thread> th;
+    const int nt = std::max(1, std::min<int>(16, int(std::thread::hardware_concurrency())));
+    for (int i = 0; i < nt; ++i) th.emplace_back(work);
+    for (auto& t : th) t.join();
 }
 extern "C" int orc_set_ad4(OrcHandle* h, int lig, const int* ad, const double* q, double torsdof) {
     if (h->d.sf != SF_AD4 || lig < 0 || lig >= int(h->d.ligs.size())) return 1;
@@ -1554,7 +1561,7 @@ extern "C" double orc_conf_independent(OrcHandle* h, double num_tors, double x) 
 // poses[i * 39] in the pose layout, ligs[i] the ligand of pose i
 extern "C" void orc_eval_batch(OrcHandle* h, int n, const int* ligs, const double* poses, const double* v,
                                int n_threads, double* e_out) {
-    for (int i = 0; i < n; ++i) {  // build the lazily made pair tables first
+    for (int i = 0; i < n && h->d.sf != SF_AD4; ++i) {  // build the lazily made type-pair tables first
         if (h->prec == 64) { auto& L = h->d.ligs[ligs[i]]; for (int k = 0; k < L.n_pairs; ++k) h->d.table(L.xs[L.pa[k]], L.xs[L.pb[k]]); }
         else { auto& L = h->f.ligs[ligs[i]]; for (int k = 0; k < L.n_pairs; ++k) h->f.table(L.xs[L.pa[k]], L.xs[L.pb[k]]); }
     }

@@ -23,7 +23,7 @@ def test_unbuilt_options_are_refused(tmp_path):
                 ligands=[os.path.join(GOLDEN, "1iep_ligand.pdbqt")], output_dir=tmp_path,
                 center_x=15.19, center_y=53.903, center_z=16.917)
     with pytest.raises(NotImplementedError):
-        docking.run_unidock(scoring="ad4", **args)
+        docking.run_unidock(scoring="dkoes", **args)
     with pytest.raises(NotImplementedError):
         docking.run_unidock(bias_file="x.bpf", **args)
     with pytest.raises(ValueError):
@@ -61,4 +61,22 @@ def test_run_unidock_on_reference_fixtures(tmp_path):
                                   15.19, 53.903, 16.917, 20, 20, 20, score_only=True)
     txt = (tmp_path / "scores.txt").read_text().split()
     assert txt[0] == "REMARK" and txt[1] == "1iep_ligand.pdbqt" and float(txt[2]) == pytest.approx(sc[0][0], abs=1e-3)
+    assert -20 <= sc[0][0] <= -1  # the crystal pose scores well
+
+
+@pytest.mark.gpu
+def test_run_unidock_ad4(tmp_path):
+    """scoring="ad4": AD4.2 maps built on the GPU from the receptor (the
+    reference runs AutoGrid for them), docking, AD4.2 final scores."""
+    ligs = [os.path.join(GOLDEN, "1iep_ligand.pdbqt")]
+    out, scores = docking.run_unidock(os.path.join(GOLDEN, "1iep_receptor.pdbqt"), ligs, tmp_path,
+                                      15.19, 53.903, 16.917, 20, 20, 20, scoring="ad4", search_mode="fast",
+                                      num_modes=9)
+    print(f"[gpu] run_unidock 1iep ad4: {scores}", flush=True)
+    assert [p.name for p in out] == ["1iep_ligand_out.pdbqt"]
+    assert 1 <= len(scores[0]) <= 9 and -20 <= scores[0][0] <= -1
+    assert scores[0] == sorted(scores[0])
+    inp, sc = docking.run_unidock(os.path.join(GOLDEN, "1iep_receptor.pdbqt"), ligs, tmp_path,
+                                  15.19, 53.903, 16.917, 20, 20, 20, scoring="ad4", score_only=True)
+    print(f"[gpu] run_unidock 1iep ad4 score_only: {sc}", flush=True)
     assert -20 <= sc[0][0] <= -1  # the crystal pose scores well

@@ -241,3 +241,19 @@ def test_ligand_view():
         hd = lig.ad == HD
         assert np.all(lig.heavy[hd] == 0) and np.all(lig.xs[hd] == -1)
         assert lig.torsdof == lig.n_tors
+
+
+def test_eval_batch_with_hydrogens(scene):
+    """The threaded batch evaluation (the bench's FP64 error leg) on ligands
+    whose polar hydrogens have no X-Score type."""
+    o, ligs, (xyz, ad, q, cav) = scene
+    assert any(np.any(l.xs < 0) for l in ligs)
+    rng = np.random.default_rng(12)
+    pl, P = [], []
+    for li, lig in enumerate(ligs):
+        for p in random_poses(rng, lig, cav, 1.0, 2):
+            pl.append(li)
+            P.append(p)
+    e = o.eval_batch(np.array(pl), np.array(P), n_threads=4)
+    for i, li in enumerate(pl):
+        assert e[i] == o.eval_deriv(li, P[i])[0]

@@ -3,7 +3,9 @@ run_unidock (unidock_tools/modules/docking/unidock.py:155-201).
 
 The reference builds an `unidock` command line and runs the binary in a
 subprocess; here the same arguments drive the library directly on one GPU:
-receptor and ligands are parsed (PDBQT / SDF), maps are built on the GPU,
+receptor and ligands are parsed (PDBQT / SDF), maps are built on the GPU
+(for scoring "ad4" the AutoDock 4.2 maps the reference has AutoGrid write,
+unidock.py:47-53, gen_grid.py:37-68),
 every ligand is docked in one batch, the modes are clustered, refined and
 rescored, and each input file gets `<stem>_out.sdf` / `<stem>_out.pdbqt` in
 output_dir with the reference's result fields.  Returns the written files
@@ -22,7 +24,7 @@ from . import _lib
 # --search_mode presets (main.cpp:640-652)
 SEARCH_MODES = {"fast": (128, 20), "balance": (384, 40), "balanced": (384, 40), "detail": (512, 40),
                 "detailed": (512, 40)}
-SCORING = {"vina": _lib.UD_SF_VINA, "vinardo": _lib.UD_SF_VINARDO}
+SCORING = {"vina": _lib.UD_SF_VINA, "vinardo": _lib.UD_SF_VINARDO, "ad4": _lib.UD_SF_AD4}
 
 
 def global_steps_for(ligands, max_step: int) -> int:
@@ -76,8 +78,8 @@ def run_unidock(
     """Dock `ligands` (PDBQT or SDF files) into `receptor` (PDBQT) and write
     `<stem>_out.<ext>` per ligand file into output_dir.
 
-    Same arguments and defaults as the reference; AD4 scoring and biases are
-    not built (NotImplementedError).  score_only writes output_dir/scores.txt
+    Same arguments and defaults as the reference; biases are not built
+    (NotImplementedError).  score_only writes output_dir/scores.txt
     ("REMARK <file> <affinity>" per ligand) and returns the inputs with their
     scores; local_only optimises the input pose instead of searching."""
     if scoring.lower() not in SCORING:
@@ -102,15 +104,21 @@ def run_unidock(
             owner.append((path, [ls.add_pdbqt(text)]))
         else:
             raise ValueError(f"{path}: ligands are read from .pdbqt or .sdf files")
-    parsed = ls.ligands()
-    rec_xyz, rec_xs = _lib.receptor_from_pdbqt(Path(receptor).read_text())
+    sf = SCORING[scoring.lower()]
+    parsed = ls.ligands(sf)
     ctx = _lib.Context(device)
     try:
-        ctx.set_scoring(SCORING[scoring.lower()])
+        ctx.set_scoring(sf)
         ctx.set_box((center_x, center_y, center_z), (size_x, size_y, size_z), 0.375)
-        types = sorted({_lib.grid_type(int(t)) for l in parsed for t in l.xs} - {-1})
-        ctx.compute_maps(rec_xyz, rec_xs, types, precision)
-        ctx.set_ligands(ls.arrays())
+        if sf == _lib.UD_SF_AD4:
+            rec_xyz, rec_ad, rec_q = _lib.receptor_ad4_from_pdbqt(Path(receptor).read_text())
+            types = sorted({_lib.ad4_grid_type(int(t)) for l in parsed for t in l.ad} - {-1})
+            ctx.compute_maps_ad4(rec_xyz, rec_ad, rec_q, types, precision)
+        else:
+            rec_xyz, rec_xs = _lib.receptor_from_pdbqt(Path(receptor).read_text())
+            types = sorted({_lib.grid_type(int(t)) for l in parsed for t in l.xs} - {-1})
+            ctx.compute_maps(rec_xyz, rec_xs, types, precision)
+        ctx.set_ligands(ls.arrays(sf))
         if score_only:
             return _score_only(ctx, parsed, owner, output_dir)
         if local_only:
@@ -156,19 +164,29 @@ def _input_poses(parsed):
     return P
 
 
+def _affinity(ctx, lig, inter, intra):
+    """Final score of a pose scored on its own: Vina / Vinardo divide inter +
+    intra - intra by 1 + w (num_tors) (vina.cpp:404, conf_independent.cpp:
+    153-156); AD4.2 adds w_rot TORSDOF to inter (vina.cpp:1210-1227,
+    conf_independent.cpp:183-187).  Returns (energy, conf_independent)."""
+    if ctx.sf == _lib.UD_SF_AD4:
+        c = ctx.weights[-1] * float(lig.torsdof)
+        return inter + c, c
+    rot = 5 * ctx.weights[-1] / 0.1 - 1  # internal rotational weight (vina.cpp:404)
+    y = 1.0 + 0.1 * (rot + 1.0) * float(lig.num_tors) / 5.0
+    return inter / y, inter / y - inter
+
+
 def _score_only(ctx, parsed, owner, output_dir):
-    """Vina::score (vina.cpp:1329-1353) of the input poses: affinity =
-    conf-independent form of inter + intra - intra = inter."""
+    """Vina::score (vina.cpp:1168-1243) of the input poses."""
     n = len(parsed)
     pl = np.arange(n, dtype=np.int32)
     inter, intra = ctx.score_terms(pl, _input_poses(parsed))
-    rot = 5 * ctx.weights[-1] / 0.1 - 1  # internal rotational weight (vina.cpp:404)
     lines, scores = [], []
     for path, idx in owner:
         s = []
         for li in idx:
-            y = 1.0 + 0.1 * (rot + 1.0) * float(parsed[li].num_tors) / 5.0
-            s.append(float(inter[li]) / y)
+            s.append(_affinity(ctx, parsed[li], float(inter[li]), float(intra[li]))[0])
         scores.append(s)
         lines.append(f"REMARK {path.name} {s[0]:.3f}")
     (output_dir / "scores.txt").write_text("\n".join(lines) + "\n")
@@ -182,12 +200,11 @@ def _local_only(ctx, parsed):
     pl = np.arange(n, dtype=np.int32)
     P, e, _ = ctx.optimize(pl, _input_poses(parsed), (1000.0, 1000.0, 1000.0))
     inter, intra = ctx.score_terms(pl, P)
-    rot = 5 * ctx.weights[-1] / 0.1 - 1
     modes = []
     for i in range(n):
-        y = 1.0 + 0.1 * (rot + 1.0) * float(parsed[i].num_tors) / 5.0
-        x = float(inter[i])
-        modes.append([dict(pose=P[i], energy=x / y, inter=float(inter[i]), intra=float(intra[i]),
-                           conf_independent=x / y - x, search_energy=float(e[i]), rmsd_lb=0.0, rmsd_ub=0.0,
-                           unbound=float(intra[i]), chain=i)])
+        energy, ci = _affinity(ctx, parsed[i], float(inter[i]), float(intra[i]))
+        unbound = -float(intra[i]) if ctx.sf == _lib.UD_SF_AD4 else float(intra[i])
+        modes.append([dict(pose=P[i], energy=energy, inter=float(inter[i]), intra=float(intra[i]),
+                           conf_independent=ci, search_energy=float(e[i]), rmsd_lb=0.0, rmsd_ub=0.0,
+                           unbound=unbound, chain=i)])
     return modes
