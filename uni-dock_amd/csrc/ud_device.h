@@ -101,8 +101,10 @@ template <int L> __device__ __forceinline__ double gsum_d(double v) { return sub
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
 
 // ------------------------------------------------------------ angles / quaternions
-// normalize_angle (common.h:260-281)
-__device__ __forceinline__ float normalize_angle(float x) {
+// normalize_angle (common.h:260-281).  Angles beyond +-3 pi take the
+// reference's "how many 2 pi's" steps (with its IEEE division) in a function
+// of their own, so that the common path inlined everywhere is two compares.
+__device__ __attribute__((noinline)) float normalize_angle_far(float x) {
     for (int it = 0; it < 8; ++it) {
         if (x > 3.f * kPi) {
             float n = (x - kPi) / (2.f * kPi);
@@ -111,12 +113,39 @@ __device__ __forceinline__ float normalize_angle(float x) {
             float n = (-x - kPi) / (2.f * kPi);
             x += 2.f * kPi * ceilf(n);
         } else {
-            if (x > kPi) x -= 2.f * kPi;
-            else if (x < -kPi) x += 2.f * kPi;
             break;
         }
     }
     return x;
+}
+__device__ __forceinline__ float normalize_angle(float x) {
+    if (!(fabsf(x) <= 3.f * kPi)) x = normalize_angle_far(x);
+    if (x > kPi) x -= 2.f * kPi;
+    else if (x < -kPi) x += 2.f * kPi;
+    return x;
+}
+// sin and cos of x in [-pi/2, pi/2] (half of a normalised angle) from their
+// Taylor series through x^13 / x^14: the first omitted terms are below
+// 7e-10 there, under half an ulp of the results, so the error is the FP32
+// evaluation's (a few ulp).  No range reduction is needed for these
+// arguments.
+__device__ __forceinline__ void sincos_half(float x, float* s, float* c) {
+    const float z = x * x;
+    float ps = 1.f / 6227020800.f;                     // 1/13!
+    ps = fmaf(ps, z, -1.f / 39916800.f);               // 1/11!
+    ps = fmaf(ps, z, 1.f / 362880.f);                  // 1/9!
+    ps = fmaf(ps, z, -1.f / 5040.f);                   // 1/7!
+    ps = fmaf(ps, z, 1.f / 120.f);
+    ps = fmaf(ps, z, -1.f / 6.f);
+    *s = fmaf(ps * z, x, x);
+    float pc = -1.f / 87178291200.f;                   // 1/14!
+    pc = fmaf(pc, z, 1.f / 479001600.f);               // 1/12!
+    pc = fmaf(pc, z, -1.f / 3628800.f);                // 1/10!
+    pc = fmaf(pc, z, 1.f / 40320.f);                   // 1/8!
+    pc = fmaf(pc, z, -1.f / 720.f);
+    pc = fmaf(pc, z, 1.f / 24.f);
+    pc = fmaf(pc, z, -0.5f);
+    *c = fmaf(pc, z, 1.f);
 }
 // Hamilton product p * q, (w, x, y, z)
 __device__ __forceinline__ float4 qmul(float4 p, float4 q) {
@@ -146,14 +175,18 @@ __device__ __forceinline__ float4 q_normalize_approx(float4 q) {
 __device__ __forceinline__ float4 axis_angle_q(float ax, float ay, float az, float angle) {
     angle = normalize_angle(angle);
     float s, c;
-    sincosf(angle * 0.5f, &s, &c);
+    sincos_half(angle * 0.5f, &s, &c);
     return make_float4(c, s * ax, s * ay, s * az);
 }
-// quaternion_increment (quaternion.cpp:128-133) with angle_to_quaternion(rotation)
+// quaternion_increment (quaternion.cpp:128-133) with angle_to_quaternion(rotation);
+// the axis is the rotation times the angle's reciprocal
 __device__ __forceinline__ float4 q_increment(float4 q, float rx, float ry, float rz) {
     const float angle = sqrtf(rx * rx + ry * ry + rz * rz);
     float4 r = make_float4(1.f, 0.f, 0.f, 0.f);
-    if (angle > kEpsFl) r = axis_angle_q(rx / angle, ry / angle, rz / angle, angle);
+    if (angle > kEpsFl) {
+        const float inv = 1.f / angle;
+        r = axis_angle_q(rx * inv, ry * inv, rz * inv, angle);
+    }
     return q_normalize_approx(qmul(r, q));
 }
 

@@ -410,7 +410,7 @@ __device__ void fk_frames(const Grp<L, NM>& s, const float* pose) {
         const int k = 1 + s.gl + m * L;
         hc[m] = 1.f;
         hs[m] = 0.f;
-        if (k < s.h.n_nodes) sincosf(normalize_angle(pose[7 + k - 1]) * 0.5f, &hs[m], &hc[m]);
+        if (k < s.h.n_nodes) sincos_half(normalize_angle(pose[7 + k - 1]) * 0.5f, &hs[m], &hc[m]);
     }
     __syncthreads();
     for (int d = 1; d <= s.h.max_depth; ++d) {
@@ -493,19 +493,31 @@ template <int GF> struct Fmt {
     static constexpr bool ad4 = GF >= kGridAd4F32;
 };
 
+// Load of a T at a 32-bit byte offset from a wave-uniform base: the address
+// is the base (scalar registers) plus a zero-extended 32-bit offset, which
+// the global load instructions take directly (constant parts become their
+// immediate offset), instead of 64-bit address arithmetic per load.  Every
+// buffer read this way is below 4 GiB (maps at most 2^24 points x 32 maps x 4
+// B, tables and ligand records far less).
+template <class T> __device__ __forceinline__ T ld_at(const void* base, uint32_t byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
 template <int GF>
 __device__ __forceinline__ void grid_gather(const GridDev& g, uint32_t base, float (&f)[8]) {
-    const uint32_t sy = uint32_t(g.n[0]), sz = uint32_t(g.n[0] * g.n[1]);
+    constexpr uint32_t B = (Fmt<GF>::store == UD_GRID_F32) ? 4u : 2u;  // bytes per value
+    const uint32_t sy = uint32_t(g.n[0]) * B, sz = uint32_t(g.n[0] * g.n[1]) * B;
+    const uint32_t o0 = base * B, o1 = o0 + sy, o2 = o0 + sz, o3 = o2 + sy;
     if (Fmt<GF>::store == UD_GRID_F32) {
-        const float* M = reinterpret_cast<const float*>(g.maps);
-        f[0] = M[base]; f[1] = M[base + 1]; f[2] = M[base + sy]; f[3] = M[base + sy + 1];
-        f[4] = M[base + sz]; f[5] = M[base + sz + 1]; f[6] = M[base + sy + sz]; f[7] = M[base + sy + sz + 1];
+        f[0] = ld_at<float>(g.maps, o0); f[1] = ld_at<float>(g.maps, o0 + 4u);
+        f[2] = ld_at<float>(g.maps, o1); f[3] = ld_at<float>(g.maps, o1 + 4u);
+        f[4] = ld_at<float>(g.maps, o2); f[5] = ld_at<float>(g.maps, o2 + 4u);
+        f[6] = ld_at<float>(g.maps, o3); f[7] = ld_at<float>(g.maps, o3 + 4u);
     } else {
-        const __half* M = reinterpret_cast<const __half*>(g.maps);
-        f[0] = __half2float(M[base]); f[1] = __half2float(M[base + 1]);
-        f[2] = __half2float(M[base + sy]); f[3] = __half2float(M[base + sy + 1]);
-        f[4] = __half2float(M[base + sz]); f[5] = __half2float(M[base + sz + 1]);
-        f[6] = __half2float(M[base + sy + sz]); f[7] = __half2float(M[base + sy + sz + 1]);
+        f[0] = __half2float(ld_at<__half>(g.maps, o0)); f[1] = __half2float(ld_at<__half>(g.maps, o0 + 2u));
+        f[2] = __half2float(ld_at<__half>(g.maps, o1)); f[3] = __half2float(ld_at<__half>(g.maps, o1 + 2u));
+        f[4] = __half2float(ld_at<__half>(g.maps, o2)); f[5] = __half2float(ld_at<__half>(g.maps, o2 + 2u));
+        f[6] = __half2float(ld_at<__half>(g.maps, o3)); f[7] = __half2float(ld_at<__half>(g.maps, o3 + 2u));
     }
 }
 
@@ -732,7 +744,7 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
                 in[u] = false;
                 other[u] = 0; idx[u] = 0; it[u] = 0; pq[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
                 if (k + u < kn) {
-                    const PartnerRec rec = part[k0 + k + u];
+                    const PartnerRec rec = ld_at<uint32_t>(part, uint32_t(k0 + k + u) * 4u);
                     other[u] = int(rec & 0xff);
                     const float4 cb = coords[other[u]];
                     rx[u] = cb.x - ci.x; ry[u] = cb.y - ci.y; rz[u] = cb.z - ci.z;
@@ -752,14 +764,14 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
             for (int u = 0; u < kPairBatch; ++u)
                 if (in[u]) {
                     if (!Fmt<GF>::ad4 || it[u] < kTableStride - 1) {
-                        t0[u] = ka.ls.table_deriv[idx[u]];
-                        t1[u] = ka.ls.table_deriv[idx[u] + 1];
+                        t0[u] = ld_at<float2>(ka.ls.table_deriv, uint32_t(idx[u]) * 8u);
+                        t1[u] = ld_at<float2>(ka.ls.table_deriv, uint32_t(idx[u]) * 8u + 8u);
                     } else {
                         t0[u] = t1[u] = make_float2(0.f, 0.f);
                     }
                     if constexpr (Fmt<GF>::ad4) {
-                        u0[u] = ka.ls.ad4_deriv[it[u]];
-                        u1[u] = ka.ls.ad4_deriv[it[u] + 1];
+                        u0[u] = ld_at<float4>(ka.ls.ad4_deriv, uint32_t(it[u]) * 16u);
+                        u1[u] = ld_at<float4>(ka.ls.ad4_deriv, uint32_t(it[u]) * 16u + 16u);
                         cf[u] = ka.ls.pair_coef[s.h.pair_begin + pq[u]];
                     }
                 }
@@ -818,7 +830,7 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
     const uint32_t ap0 = (s.gl < N) ? apl[s.gl] : 0u;
     uint32_t pre[kRec];
 #pragma unroll
-    for (int u = 0; u < kRec; ++u) pre[u] = (u < int(ap0 >> 20)) ? part[int(ap0 & 0xFFFFFu) + u] : 0u;
+    for (int u = 0; u < kRec; ++u) pre[u] = (u < int(ap0 >> 20)) ? ld_at<uint32_t>(part, ((ap0 & 0xFFFFFu) + uint32_t(u)) * 4u) : 0u;
     float e = 0.f;
     for (int i = s.gl; i < N; i += L) {
         const AtomRec a = s.atoms[i];
@@ -871,14 +883,14 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         for (int u = 0; u < kPairBatch; ++u)
             if (in[u]) {
                 if (!Fmt<GF>::ad4 || it[u] < kTableStride - 1) {
-                    t0[u] = ka.ls.table_deriv[idx[u]];
-                    t1[u] = ka.ls.table_deriv[idx[u] + 1];
+                    t0[u] = ld_at<float2>(ka.ls.table_deriv, uint32_t(idx[u]) * 8u);
+                    t1[u] = ld_at<float2>(ka.ls.table_deriv, uint32_t(idx[u]) * 8u + 8u);
                 } else {
                     t0[u] = t1[u] = make_float2(0.f, 0.f);
                 }
                 if constexpr (Fmt<GF>::ad4) {
-                    u0[u] = ka.ls.ad4_deriv[it[u]];
-                    u1[u] = ka.ls.ad4_deriv[it[u] + 1];
+                    u0[u] = ld_at<float4>(ka.ls.ad4_deriv, uint32_t(it[u]) * 16u);
+                    u1[u] = ld_at<float4>(ka.ls.ad4_deriv, uint32_t(it[u]) * 16u + 16u);
                     cf[u] = ka.ls.pair_coef[s.h.pair_begin + q0 + u * L];
                 }
             }
@@ -912,7 +924,7 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
             uint32_t rec[kRec];
 #pragma unroll
             for (int u = 0; u < kRec; ++u)
-                rec[u] = (first && k == 0) ? pre[u] : ((k + u < kn) ? part[k0 + k + u] : 0u);
+                rec[u] = (first && k == 0) ? pre[u] : ((k + u < kn) ? ld_at<uint32_t>(part, uint32_t(k0 + k + u) * 4u) : 0u);
 #pragma unroll
             for (int u = 0; u < kRec; ++u) {
                 if (k + u >= kn) continue;
@@ -1161,9 +1173,12 @@ __device__ void random_start(float* pose, int T, const SearchDev& sp, uint32_t c
         const float u1 = 1.f - u01(ov[2 * j]);
         const float u2 = u01(ov[2 * j + 1]);
         const float rad = sqrtf(-2.f * logf(u1));
-        const float ang = 2.f * kPi * u2;
-        nq[2 * j] = rad * cosf(ang);
-        nq[2 * j + 1] = rad * sinf(ang);
+        // cos / sin of 2 pi u2 through h = pi (u2 - 1/2) in [-pi/2, pi/2):
+        // cos(2h + pi) = 2 sin^2 h - 1, sin(2h + pi) = -2 sin h cos h
+        float sh, ch;
+        sincos_half(kPi * (u2 - 0.5f), &sh, &ch);
+        nq[2 * j] = rad * (2.f * sh * sh - 1.f);
+        nq[2 * j + 1] = rad * (-2.f * sh * ch);
     }
     float nrm = sqrtf(nq[0] * nq[0] + nq[1] * nq[1] + nq[2] * nq[2] + nq[3] * nq[3]);
     if (!(nrm > kEpsFl)) { nq[0] = 1.f; nq[1] = nq[2] = nq[3] = 0.f; nrm = 1.f; }
@@ -1184,10 +1199,12 @@ __device__ void mutate(float* pose, int T, float rgyr, const SearchDev& sp, uint
     const uint32_t count = 2u + uint32_t(T);
     const uint32_t which = uint32_t((uint64_t(o.x) * count) >> 32);
     const float z = 2.f * u01(o.y) - 1.f;
-    const float phi = 2.f * kPi * u01(o.z);
+    // phi = 2 pi u: cos phi = 2 sin^2 h - 1, sin phi = -2 sin h cos h, h = pi (u - 1/2)
+    float sh, ch;
+    sincos_half(kPi * (u01(o.z) - 0.5f), &sh, &ch);
     const float rr = cbrtf(u01(o.w));
     const float sxy = sqrtf(fmaxf(0.f, 1.f - z * z));
-    const float ux = rr * sxy * cosf(phi), uy = rr * sxy * sinf(phi), uz = rr * z;
+    const float ux = rr * sxy * (2.f * sh * sh - 1.f), uy = rr * sxy * (-2.f * sh * ch), uz = rr * z;
     const float amp = sp.amplitude;
     if (which == 0) {
         pose[0] += amp * ux; pose[1] += amp * uy; pose[2] += amp * uz;
