@@ -502,6 +502,14 @@ template <int GF> struct Fmt {
 template <class T> __device__ __forceinline__ T ld_at(const void* base, uint32_t byte_off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
 }
+// two consecutive (e, dor) samples of a pair table in one 16-byte load (the
+// table's entries are 8-byte aligned; global memory takes unaligned vectors)
+typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+__device__ __forceinline__ void ld_samples(const float2* table, uint32_t idx, float2& t0, float2& t1) {
+    const f4a8 v = ld_at<f4a8>(table, idx * 8u);
+    t0 = make_float2(v.x, v.y);
+    t1 = make_float2(v.z, v.w);
+}
 
 template <int GF>
 __device__ __forceinline__ void grid_gather(const GridDev& g, uint32_t base, float (&f)[8]) {
@@ -764,8 +772,7 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
             for (int u = 0; u < kPairBatch; ++u)
                 if (in[u]) {
                     if (!Fmt<GF>::ad4 || it[u] < kTableStride - 1) {
-                        t0[u] = ld_at<float2>(ka.ls.table_deriv, uint32_t(idx[u]) * 8u);
-                        t1[u] = ld_at<float2>(ka.ls.table_deriv, uint32_t(idx[u]) * 8u + 8u);
+                        ld_samples(ka.ls.table_deriv, uint32_t(idx[u]), t0[u], t1[u]);
                     } else {
                         t0[u] = t1[u] = make_float2(0.f, 0.f);
                     }
@@ -831,6 +838,77 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
     uint32_t pre[kRec];
 #pragma unroll
     for (int u = 0; u < kRec; ++u) pre[u] = (u < int(ap0 >> 20)) ? ld_at<uint32_t>(part, ((ap0 & 0xFFFFFu) + uint32_t(u)) * 4u) : 0u;
+    // pass A's pair batches: r, table index and fraction, then the sample
+    // loads (issue), then energy and force scale into pf (finish); the first
+    // batch is issued before the grid term so that its table loads and the
+    // map gathers are in flight together
+    const int P = s.h.n_pairs;
+    struct Batch {
+        float rem[kPairBatch];
+        int idx[kPairBatch], it[kPairBatch];
+        bool in[kPairBatch];
+        float2 t0[kPairBatch], t1[kPairBatch], cf[kPairBatch];
+        float4 u0[kPairBatch], u1[kPairBatch];
+    } b;
+    auto issue = [&](int q0) {
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u) {
+            const int q = q0 + u * L;
+            b.in[u] = false;
+            b.idx[u] = 0; b.it[u] = 0; b.rem[u] = 0.f;
+            if (q < P) {
+                const PairRec rec = s.pairs[q];
+                const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
+                const float rx = cb.x - ca.x, ry = cb.y - ca.y, rz = cb.z - ca.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 < pair_cut2<GF>()) {
+                    const float rf = 32.f * r2;
+                    const int i1t = int(rf);
+                    b.rem[u] = rf - float(i1t);
+                    b.idx[u] = int(rec >> 16) * kTableStride + i1t;
+                    b.it[u] = i1t;
+                    b.in[u] = true;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u)
+            if (b.in[u]) {
+                if (!Fmt<GF>::ad4 || b.it[u] < kTableStride - 1) {
+                    ld_samples(ka.ls.table_deriv, uint32_t(b.idx[u]), b.t0[u], b.t1[u]);
+                } else {
+                    b.t0[u] = b.t1[u] = make_float2(0.f, 0.f);
+                }
+                if constexpr (Fmt<GF>::ad4) {
+                    b.u0[u] = ld_at<float4>(ka.ls.ad4_deriv, uint32_t(b.it[u]) * 16u);
+                    b.u1[u] = ld_at<float4>(ka.ls.ad4_deriv, uint32_t(b.it[u]) * 16u + 16u);
+                    b.cf[u] = ka.ls.pair_coef[s.h.pair_begin + q0 + u * L];
+                }
+            }
+    };
+    auto finish = [&](int q0) {
+        float ep = 0.f;
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u) {
+            const int q = q0 + u * L;
+            if (q >= P) continue;
+            float d = 0.f;
+            if (b.in[u]) {
+                float pe = b.t0[u].x + b.rem[u] * (b.t1[u].x - b.t0[u].x);
+                d = b.t0[u].y + b.rem[u] * (b.t1[u].y - b.t0[u].y);
+                if constexpr (Fmt<GF>::ad4) ad4_charge_terms(b.cf[u], b.u0[u], b.u1[u], b.rem[u], pe, d);
+                if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
+                    const float tmp = curl_factor(v_intra, pe);
+                    pe *= tmp;
+                    d *= tmp * tmp;
+                }
+                ep += pe;
+            }
+            pf[q] = d;
+        }
+        return ep;
+    };
+    issue(s.gl);
     float e = 0.f;
     for (int i = s.gl; i < N; i += L) {
         const AtomRec a = s.atoms[i];
@@ -852,66 +930,9 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         }
         mf[i] = make_float4(d.x, d.y, d.z, 0.f);
     }
-    const int P = s.h.n_pairs;
     for (int q0 = s.gl; q0 < P; q0 += kPairBatch * L) {
-        float rx[kPairBatch], ry[kPairBatch], rz[kPairBatch], rem[kPairBatch];
-        int idx[kPairBatch], it[kPairBatch];
-        bool in[kPairBatch];
-        float2 t0[kPairBatch], t1[kPairBatch], cf[kPairBatch];
-        float4 u0[kPairBatch], u1[kPairBatch];
-#pragma unroll
-        for (int u = 0; u < kPairBatch; ++u) {
-            const int q = q0 + u * L;
-            in[u] = false;
-            idx[u] = 0; it[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
-            if (q < P) {
-                const PairRec rec = s.pairs[q];
-                const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
-                rx[u] = cb.x - ca.x; ry[u] = cb.y - ca.y; rz[u] = cb.z - ca.z;
-                const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
-                if (r2 < pair_cut2<GF>()) {
-                    const float rf = 32.f * r2;
-                    const int i1t = int(rf);
-                    rem[u] = rf - float(i1t);
-                    idx[u] = int(rec >> 16) * kTableStride + i1t;
-                    it[u] = i1t;
-                    in[u] = true;
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kPairBatch; ++u)
-            if (in[u]) {
-                if (!Fmt<GF>::ad4 || it[u] < kTableStride - 1) {
-                    t0[u] = ld_at<float2>(ka.ls.table_deriv, uint32_t(idx[u]) * 8u);
-                    t1[u] = ld_at<float2>(ka.ls.table_deriv, uint32_t(idx[u]) * 8u + 8u);
-                } else {
-                    t0[u] = t1[u] = make_float2(0.f, 0.f);
-                }
-                if constexpr (Fmt<GF>::ad4) {
-                    u0[u] = ld_at<float4>(ka.ls.ad4_deriv, uint32_t(it[u]) * 16u);
-                    u1[u] = ld_at<float4>(ka.ls.ad4_deriv, uint32_t(it[u]) * 16u + 16u);
-                    cf[u] = ka.ls.pair_coef[s.h.pair_begin + q0 + u * L];
-                }
-            }
-#pragma unroll
-        for (int u = 0; u < kPairBatch; ++u) {
-            const int q = q0 + u * L;
-            if (q >= P) continue;
-            float d = 0.f;
-            if (in[u]) {
-                float pe = t0[u].x + rem[u] * (t1[u].x - t0[u].x);
-                d = t0[u].y + rem[u] * (t1[u].y - t0[u].y);
-                if constexpr (Fmt<GF>::ad4) ad4_charge_terms(cf[u], u0[u], u1[u], rem[u], pe, d);
-                if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
-                    const float tmp = curl_factor(v_intra, pe);
-                    pe *= tmp;
-                    d *= tmp * tmp;
-                }
-                e += pe;
-            }
-            pf[q] = d;
-        }
+        if (q0 != s.gl) issue(q0);
+        e += finish(q0);
     }
     __syncthreads();
     for (int i = s.gl; i < N; i += L) {
