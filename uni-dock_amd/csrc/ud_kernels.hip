@@ -1081,9 +1081,20 @@ __device__ __forceinline__ void minus_hmatvec(const Grp<L, NM>& s, const float (
 // gl + j L, all NM columns, so every index is a compile-time constant.
 // Vectors are broadcast through the group's bc(0) with entries n..NM-1
 // zero, which keeps the columns beyond n at zero.
-template <int L, int NM, int KV>
-__device__ __forceinline__ void minus_hmatvec_reg(const Grp<L, NM>& s, const float (&Hr)[KV][NM],
+// Register form of the inverse Hessian.  HS = 1: lane gl + j L holds row
+// gl + j L.  HS = 2 (groups of 2 NM lanes): lanes gl and gl + NM share row gl,
+// columns [0, NM/2) and [NM/2, NM), and their partial products are added
+// across the pair; half the registers per lane.
+template <int L, int NM, int HS>
+__device__ __forceinline__ int hess_row(int gl, int j) { return HS > 1 ? gl % NM : gl + j * L; }
+template <int L, int NM, int HS>
+__device__ __forceinline__ int hess_col0(int gl) { return HS > 1 ? (gl / NM) * (NM / HS) : 0; }
+
+template <int L, int NM, int KV, int HS>
+__device__ __forceinline__ void minus_hmatvec_reg(const Grp<L, NM>& s, const float (&Hr)[KV][NM / HS],
                                                   const float (&in)[KV], float (&out)[KV], int n) {
+    static_assert(HS == 1 || (HS == 2 && KV == 1 && L == 2 * NM), "row split of two lanes per row");
+    constexpr int HC = NM / HS;
     float* bc = s.bc(0);
 #pragma unroll
     for (int j = 0; j < KV; ++j) {
@@ -1091,15 +1102,17 @@ __device__ __forceinline__ void minus_hmatvec_reg(const Grp<L, NM>& s, const flo
         if (idx < NM) bc[idx] = (idx < n) ? in[j] : 0.f;
     }
     __syncthreads();
-    float v[NM];
+    const int c0 = hess_col0<L, NM, HS>(s.gl);
+    float v[HC];
 #pragma unroll
-    for (int c = 0; c < NM; ++c) v[c] = bc[c];
+    for (int c = 0; c < HC; ++c) v[c] = bc[c0 + c];
 #pragma unroll
     for (int j = 0; j < KV; ++j) {
         const int i = s.gl + j * L;
         float sum = 0.f;
 #pragma unroll
-        for (int c = 0; c < NM; ++c) sum += Hr[j][c] * v[c];
+        for (int c = 0; c < HC; ++c) sum += Hr[j][c] * v[c];
+        if constexpr (HS > 1) sum += __shfl_xor(sum, NM);  // the row's other half (lane gl ^ NM)
         out[j] = (i < n) ? -sum : 0.f;
     }
     __syncthreads();
@@ -1108,18 +1121,21 @@ __device__ __forceinline__ void minus_hmatvec_reg(const Grp<L, NM>& s, const flo
 // rank-2 update of the register rows (bfgs_update, bfgs.h:46-63).  The mixed
 // term ma_i p_c + ma_c p_i is formed without contraction, so entries (i, c)
 // and (c, i) get identical values and H stays exactly symmetric.
-template <int L, int NM, int KV>
-__device__ __forceinline__ void hess_update_reg(const Grp<L, NM>& s, float (&Hr)[KV][NM], const float (&mhy)[KV],
+template <int L, int NM, int KV, int HS>
+__device__ __forceinline__ void hess_update_reg(const Grp<L, NM>& s, float (&Hr)[KV][NM / HS], const float (&mhy)[KV],
                                                 const float (&p)[KV], bool pred, float c1, float c2,
                                                 const float* ma, const float* pb) {
-    float mav[NM], pbv[NM];
+    constexpr int HC = NM / HS;
+    const int c0 = hess_col0<L, NM, HS>(s.gl);
+    float mav[HC], pbv[HC];
 #pragma unroll
-    for (int c = 0; c < NM; ++c) { mav[c] = ma[c]; pbv[c] = pb[c]; }
+    for (int c = 0; c < HC; ++c) { mav[c] = ma[c0 + c]; pbv[c] = pb[c0 + c]; }
 #pragma unroll
     for (int j = 0; j < KV; ++j) {
-        const float mi = mhy[j], pi = p[j];
+        // the row's entries of mhy and p (the broadcast buffers hold them for lanes of the second half)
+        const float mi = HS > 1 ? ma[s.gl % NM] : mhy[j], pi = HS > 1 ? pb[s.gl % NM] : p[j];
 #pragma unroll
-        for (int c = 0; c < NM; ++c) {
+        for (int c = 0; c < HC; ++c) {
             const float t = __fadd_rn(__fmul_rn(mi, pbv[c]), __fmul_rn(mav[c], pi));
             const float nv = Hr[j][c] + (c1 * t + c2 * (pi * pbv[c]));
             Hr[j][c] = pred ? nv : Hr[j][c];
@@ -1278,7 +1294,8 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
     float* H = s.H();
     float* pb = s.bc(1);  // search direction, read by the pose increment
     constexpr bool HREG = UD_HESS_IN_REGS && NM <= kNSmall && L <= 32;
-    float Hr[HREG ? KV : 1][HREG ? NM : 1];
+    constexpr int HS = (HREG && L == 2 * NM) ? 2 : 1;  // lanes per Hessian row
+    float Hr[HREG ? KV : 1][HREG ? NM / HS : 1];
     // MC: the work item's `count` chains are taken by the groups in turn; a
     // group whose chain ends writes its result and starts the next one
     uint32_t chain = uint32_t(w.first + (active ? s.g : 0));
@@ -1372,7 +1389,9 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
 #pragma unroll
             for (int j = 0; j < KV; ++j)
 #pragma unroll
-                for (int c = 0; c < NM; ++c) Hr[j][c] = is_init ? ((c == s.gl + j * L) ? 1.f : 0.f) : Hr[j][c];
+                for (int c = 0; c < NM / HS; ++c)
+                    Hr[j][c] = is_init ? ((hess_col0<L, NM, HS>(s.gl) + c == hess_row<L, NM, HS>(s.gl, j)) ? 1.f : 0.f)
+                                       : Hr[j][c];
         } else if (is_init) {
             for (int idx = s.gl; idx < nh; idx += L) H[idx] = 0.f;
             for (int i = s.gl; i < n; i += L) H[i * hs + i] = 1.f;
@@ -1391,7 +1410,8 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
 #pragma unroll
             for (int j = 0; j < KV; ++j)
 #pragma unroll
-                for (int c = 0; c < NM; ++c) Hr[j][c] = (scale0 && c == s.gl + j * L) ? dv : Hr[j][c];
+                for (int c = 0; c < NM / HS; ++c)
+                    Hr[j][c] = (scale0 && hess_col0<L, NM, HS>(s.gl) + c == hess_row<L, NM, HS>(s.gl, j)) ? dv : Hr[j][c];
         } else if (scale0) {
             const float dv = alpha * yp0 / yy;
             for (int i = s.gl; i < n; i += L) H[i * hs + i] = dv;
@@ -1402,7 +1422,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         const bool upd_pre = cont && !finished;
         const float yp = float(gdot_d<L>(y, p));
         const bool do_upd = upd_pre && !(alpha * yp < kEpsFl);
-        if constexpr (HREG) minus_hmatvec_reg<L, NM, KV>(s, Hr, y, mhy, n);
+        if constexpr (HREG) minus_hmatvec_reg<L, NM, KV, HS>(s, Hr, y, mhy, n);
         else minus_hmatvec<L, NM>(s, y, mhy, n);
         const float yhy = -gdot<L>(y, mhy);
         {
@@ -1418,7 +1438,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
                 if (wave_any(do_upd)) {
                     const float r = 1.f / (alpha * yp);
                     const float c1 = alpha * r, c2 = alpha * alpha * (r * r * yhy + r);
-                    hess_update_reg<L, NM, KV>(s, Hr, mhy, p, do_upd, c1, c2, ma, pb);
+                    hess_update_reg<L, NM, KV, HS>(s, Hr, mhy, p, do_upd, c1, c2, ma, pb);
                 }
             } else if (do_upd) {
                 const float r = 1.f / (alpha * yp);
@@ -1437,7 +1457,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         }
         // new search direction p = -H g (minus_mat_vec_product, bfgs.h:30-38)
         const bool new_dir = !finished && (is_init || cont);
-        if constexpr (HREG) minus_hmatvec_reg<L, NM, KV>(s, Hr, g, t, n);
+        if constexpr (HREG) minus_hmatvec_reg<L, NM, KV, HS>(s, Hr, g, t, n);
         else minus_hmatvec<L, NM>(s, g, t, n);
         if (new_dir) {
 #pragma unroll
