@@ -177,6 +177,13 @@ def test_dock_and_postprocess(sc):
         pl = np.full(len(got), li, dtype=np.int32)
         Pm = np.array([m["pose"] for m in got])
         inter, intra = ctx.score_terms(pl, Pm)
+        # atoms on or beyond the box faces pay the 1e6 / Å penalty, where the
+        # FP32 and FP64 coordinates of the same pose differ by ~1e-6 Å
+        C = ctx.pose_coords(pl, Pm)[:, :lig.n_atoms]
+        c0, s0 = box_of(sc["cav"], EDGE)
+        span = 0.375 * np.ceil(s0 / 0.375)
+        lo, hi = c0 - span / 2, c0 + span / 2
+        near_face = ((C < lo + 1e-3) | (C > hi - 1e-3)).any(axis=(1, 2))
         for k, m in enumerate(got):
             np.testing.assert_array_equal(m["pose"], P[m["chain"]])  # not refined
             assert m["inter"] == inter[k] and m["intra"] == intra[k]
@@ -184,7 +191,8 @@ def test_dock_and_postprocess(sc):
             assert m["unbound"] == pytest.approx(-m["intra"], abs=0)
             a, b = o.score_terms(li, m["pose"].astype(np.float64))
             # poses of ligands that do not fit the small box carry the box penalty
-            allow = 0.0 if abs(a) < 100 else 1e6 * 4e-6 * lig.n_atoms * (1.0 + 2.0 * np.abs(lig.charge).max())
+            boxed = abs(a) >= 100 or near_face[k]
+            allow = 1e6 * 4e-6 * lig.n_atoms * (1.0 + 2.0 * np.abs(lig.charge).max()) if boxed else 0.0
             assert abs(m["inter"] - a) <= 1e-5 * abs(a) + 1e-4 + allow, (li, m["inter"], a)
 
 

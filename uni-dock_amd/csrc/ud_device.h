@@ -96,6 +96,25 @@ template <class T> __device__ __forceinline__ T sub_sum(T v, int S) {
     }
     return v;
 }
+// Inclusive prefix sum over the L-lane group (lane gl gets the sum of group
+// lanes 0..gl): row_shr steps within 16-lane rows (lanes whose source lies
+// before the row start receive 0; for groups narrower than a row the lane's
+// group index masks the steps that would cross into the previous group), then
+// row_bcast:15 / row_bcast:31 carry row totals into the rows above.
+template <int K, int L> __device__ __forceinline__ float scan_step(float v, int gl) {
+    const float t = dpp_f<0x110 + K>(v);  // row_shr:K
+    return v + ((L >= 16 || gl >= K) ? t : 0.f);
+}
+template <int L> __device__ __forceinline__ float gscan(float v, int gl) {
+    if constexpr (L >= 2) v = scan_step<1, L>(v, gl);
+    if constexpr (L >= 4) v = scan_step<2, L>(v, gl);
+    if constexpr (L >= 8) v = scan_step<4, L>(v, gl);
+    if constexpr (L >= 16) v = scan_step<8, L>(v, gl);
+    if constexpr (L >= 32) v += dpp_f<kDppRowBcast15, 0xA>(v);  // rows 1 and 3 add rows 0 and 2
+    if constexpr (L == 64) v += dpp_f<kDppRowBcast31, 0xC>(v);  // rows 2 and 3 add lane 31
+    return v;
+}
+
 template <int L> __device__ __forceinline__ float gsum(float v) { return sub_sum<float>(v, L); }
 template <int L> __device__ __forceinline__ double gsum_d(double v) { return sub_sum<double>(v, L); }
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }

@@ -291,8 +291,8 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 static constexpr int kNSmall = 16;
 // Compile-time switches of the search kernel (defaults measured fastest on
 // MI355X): the occupancy target of 32-lane groups (waves per SIMD), the
-// partner records prefetched before the pair pass, and the register form of
-// the small inverse Hessian.
+// partner records prefetched before the pair pass, the register form of
+// the small inverse Hessian, and subtree sums from prefix sums.
 #ifndef UD_WAVES_L32
 #define UD_WAVES_L32 4
 #endif
@@ -301,6 +301,9 @@ static constexpr int kNSmall = 16;
 #endif
 #ifndef UD_HESS_IN_REGS
 #define UD_HESS_IN_REGS 1
+#endif
+#ifndef UD_TORQUE_SCAN
+#define UD_TORQUE_SCAN 1
 #endif
 
 // LDS layout of one workgroup: the ligand's atoms and nodes, then one block
@@ -964,11 +967,13 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
 // Energy (+ gradient) of the group's pose: model::eval_deriv (model.cpp:954-981)
 // = grid term and intramolecular pairs per atom (atoms_forces), then the
 // torsion-tree derivative
-// (tree.h:75-84, 152-193).  Each node's force and torque are summed directly
-// over the atoms of its subtree (a contiguous range in pre-order), which is
-// the reference's bottom-up accumulation written out:
+// (tree.h:75-84, 152-193).  Each node's force and torque are sums over the
+// atoms of its subtree (a contiguous range in pre-order), which is the
+// reference's bottom-up accumulation written out:
 //   F_k = sum f_i,  T_k = sum (c_i - o_k) x f_i  over atoms i below node k,
 //   dE/dtorsion_k = T_k . axis_k,  root: (F_0, T_0).
+// With at most one atom per lane they come from group prefix sums; larger
+// ligands sum each subtree directly.
 // gout[j] receives gradient element gl + j * L.
 template <int L, int NM, int GF>
 __device__ float eval_group(const KArgs& ka, const Grp<L, NM>& s, const float* pose, float v_intra, float v_inter,
@@ -992,9 +997,54 @@ __device__ float eval_group(const KArgs& ka, const Grp<L, NM>& s, const float* p
     __syncthreads();
     if (!want_grad) return e;
     float* gbuf = s.bc(2);
+    const int nn = s.h.n_nodes;
+#if UD_TORQUE_SCAN
+    if (s.h.n_atoms <= L) {
+        // One atom per lane: prefix sums over the pre-order atom list of the
+        // force and of its moment about the root origin o0, m_i = (c_i - o0) x f_i.
+        // Node k's subtree is atoms [ab, se), so F_k = PF[se-1] - PF[ab-1] and
+        // T_k = (PM[se-1] - PM[ab-1]) + (o0 - o_k) x F_k; each node's lane
+        // reads the two prefix lanes it needs.
+        float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (s.gl < s.h.n_atoms) {
+            const float4 fi = mf[s.gl], ci = coords[s.gl];
+            const float dx = ci.x - fr[0], dy = ci.y - fr[1], dz = ci.z - fr[2];
+            v[0] = fi.x; v[1] = fi.y; v[2] = fi.z;
+            v[3] = dy * fi.z - dz * fi.y;
+            v[4] = dz * fi.x - dx * fi.z;
+            v[5] = dx * fi.y - dy * fi.x;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v[c] = gscan<L>(v[c], s.gl);
+        for (int k0 = 0; k0 < nn; k0 += L) {
+            const int k = k0 + s.gl;
+            const NodeRec nd = s.nodes[min(k, nn - 1)];
+            const int ab = nd.packed & 0xff, se = (nd.packed >> 16) & 0xff;
+            float d[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const float hi = __shfl(v[c], se - 1, L), lo = __shfl(v[c], max(ab - 1, 0), L);
+                d[c] = hi - (ab > 0 ? lo : 0.f);
+            }
+            if (k < nn) {
+                if (k == 0) {
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) gbuf[c] = d[c];
+                } else {
+                    const float* f = fr + 20 * k;
+                    const float ex = fr[0] - f[0], ey = fr[1] - f[1], ez = fr[2] - f[2];
+                    const float T0 = d[3] + (ey * d[2] - ez * d[1]);
+                    const float T1 = d[4] + (ez * d[0] - ex * d[2]);
+                    const float T2 = d[5] + (ex * d[1] - ey * d[0]);
+                    gbuf[5 + k] = T0 * f[3] + T1 * f[4] + T2 * f[5];
+                }
+            }
+        }
+    } else
+#endif
+    {
     // S lanes share one node's subtree sum (strided over its atoms, then a
     // shuffle reduction); L / S nodes are handled per pass
-    const int nn = s.h.n_nodes;
     int S = L;
     while (S > 1 && S * nn > L) S >>= 1;
     const int per = L / S;
@@ -1025,6 +1075,7 @@ __device__ float eval_group(const KArgs& ka, const Grp<L, NM>& s, const float* p
                 gbuf[5 + k] = T0 * f[3] + T1 * f[4] + T2 * f[5];
             }
         }
+    }
     }
     __syncthreads();
     const int n = s.ndof();
