@@ -143,6 +143,10 @@ typedef struct {
     float authentic_v[3];        /* caps of the refinement BFGS (1000, 1000, 1000) */
     float temperature;           /* Metropolis temperature (1.2) */
     float mutation_amplitude;    /* 2 */
+    int32_t chains_per_group;    /* chains each lane group of a workgroup runs in turn, taking
+                                    the workgroup's next chain when one ends; 0: 1 (one chain
+                                    per group, the finest workgroups).  No counterpart in the
+                                    reference, whose blocks each run one chain. */
 } ud_search_params;
 
 /* fill *p with the reference defaults (exhaustiveness 8, global_steps 20) */

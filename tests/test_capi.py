@@ -32,6 +32,7 @@ def test_default_search_params():
     assert p.exhaustiveness == 8 and p.global_steps == 20 and p.seed == 181129
     assert list(p.hunt_cap) == [10, 10, 10] and list(p.authentic_v) == [1000, 1000, 1000]
     assert p.temperature == pytest.approx(1.2) and p.mutation_amplitude == 2
+    assert p.chains_per_group == 0
 
 
 def test_null_and_invalid_arguments_return_codes():

@@ -16,6 +16,8 @@ Tolerances
   MC           identical random starts; per-ligand best energies and the
                pooled best-score distribution (KS test) vs the oracle
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -206,6 +208,22 @@ def test_dock_is_deterministic_and_consistent(scene):
         # the reported best energy is the authentic-cap energy of the reported pose
         eg, _ = ctx.score_grad(pl, P1)
         np.testing.assert_allclose(eg, e1, rtol=1e-4, atol=1e-3)
+
+
+def test_chain_refill_gives_identical_chains(scene):
+    """A chain's trajectory depends only on its index and the seed, not on
+    which lane group runs it: workgroups whose groups run several chains in
+    turn (refilled as chains end) report exactly the one-chain-per-group
+    results."""
+    _say("[gpu] dock chain refill")
+    ctx = scene["ctx"]
+    for lanes in (32, 16):
+        P1, e1, ev1 = ctx.dock(exhaustiveness=12, global_steps=3, seed=7, lanes=lanes)
+        for cpg in (3, 8):
+            P2, e2, ev2 = ctx.dock(exhaustiveness=12, global_steps=3, seed=7, lanes=lanes, chains_per_group=cpg)
+            assert np.array_equal(P1, P2) and np.array_equal(e1, e2) and np.array_equal(ev1, ev2), (lanes, cpg)
+    p = ctx._params(8, 3, 7, 0, 0, 65)
+    assert ctx.L.ud_dock(ctx.h, ctypes.byref(p), None, None, None) == _lib.UD_ERR_INVALID
 
 
 def test_dock_lane_widths_agree(scene):

@@ -351,6 +351,7 @@ void ud_default_search_params(ud_search_params* p) {
     }
     p->temperature = 1.2f;
     p->mutation_amplitude = 2.f;
+    p->chains_per_group = 0;
 }
 
 int ud_create(int device_id, void* stream, ud_ctx** out) {
@@ -1217,7 +1218,10 @@ int ud_optimize(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses
 namespace {
 
 const int kBucketCaps[UD_N_BUCKETS] = {16, 32, 48, 64, 96, 128};
-const int kChainsPerGroup = 4;
+// one chain per lane group unless the caller asks for more: many small
+// workgroups balance the end of a launch (and of concurrent bucket launches)
+// better than fewer workgroups whose groups refill
+const int kChainsPerGroup = 1;
 
 int bucket_of(int n_atoms) {
     int bi = 0;
@@ -1231,6 +1235,8 @@ int check_search_params(ud_ctx* c, const ud_search_params* p) {
     if (p->lanes_per_chain != 0 && p->lanes_per_chain != 8 && p->lanes_per_chain != 16 && p->lanes_per_chain != 32 &&
         p->lanes_per_chain != 64)
         return fail(c, UD_ERR_INVALID, "lanes_per_chain must be 0, 8, 16, 32 or 64");
+    if (p->chains_per_group < 0 || p->chains_per_group > 64)
+        return fail(c, UD_ERR_INVALID, "chains_per_group must be in 0..64");
     return UD_OK;
 }
 
@@ -1304,9 +1310,9 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
         std::stable_sort(order.begin(), order.end(), [&](int a, int b2) {
             return c->lig_nat[a] * c->lig_nn[a] > c->lig_nat[b2] * c->lig_nn[b2];
         });
-        // a work item holds up to kChainsPerGroup chains per lane group; a
+        // a work item holds up to chains_per_group chains per lane group; a
         // group that finishes a chain takes the item's next one
-        const int per_item = G * kChainsPerGroup;
+        const int per_item = G * (p->chains_per_group > 0 ? p->chains_per_group : kChainsPerGroup);
         std::vector<WorkItem> w;
         for (int l : order)
             for (int c0 = 0; c0 < exh; c0 += per_item)

@@ -64,7 +64,8 @@ class UdSearchParams(ctypes.Structure):
                 ("local_steps", ctypes.c_int32), ("lanes_per_chain", ctypes.c_int32),
                 ("seed", ctypes.c_uint64),
                 ("hunt_cap", ctypes.c_float * 3), ("authentic_v", ctypes.c_float * 3),
-                ("temperature", ctypes.c_float), ("mutation_amplitude", ctypes.c_float)]
+                ("temperature", ctypes.c_float), ("mutation_amplitude", ctypes.c_float),
+                ("chains_per_group", ctypes.c_int32)]
 
 
 _LIB = None
@@ -565,7 +566,7 @@ class Context:
                                     _p(vv, ctypes.c_float), max_steps, _p(e, ctypes.c_float), _p(ev, ctypes.c_int32)))
         return P, e, ev
 
-    def _params(self, exhaustiveness, global_steps, seed, local_steps, lanes):
+    def _params(self, exhaustiveness, global_steps, seed, local_steps, lanes, chains_per_group=0):
         p = UdSearchParams()
         self.L.ud_default_search_params(ctypes.byref(p))
         p.exhaustiveness = exhaustiveness
@@ -573,10 +574,12 @@ class Context:
         p.seed = seed
         p.local_steps = local_steps
         p.lanes_per_chain = lanes
+        p.chains_per_group = chains_per_group
         return p
 
-    def dock(self, exhaustiveness=8, global_steps=20, seed=181129, local_steps=0, lanes=0, fetch=True):
-        p = self._params(exhaustiveness, global_steps, seed, local_steps, lanes)
+    def dock(self, exhaustiveness=8, global_steps=20, seed=181129, local_steps=0, lanes=0, fetch=True,
+             chains_per_group=0):
+        p = self._params(exhaustiveness, global_steps, seed, local_steps, lanes, chains_per_group)
         self._ck(self.L.ud_dock(self.h, ctypes.byref(p), None, None, None))
         self._last_dock = (self.n_ligands, exhaustiveness)
         if fetch:
