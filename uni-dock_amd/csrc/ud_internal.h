@@ -52,8 +52,13 @@ typedef uint32_t PairRec;
 // order): bits 0..7 other atom, 8..20 pair index, 21..30 table slot, bit 31
 // set when the atom is the pair's first atom.  Per atom, AtomPartners =
 // first partner (relative to the ligand's partner_begin) | count << 20.
+// Each atom's list is padded to a multiple of kPartnerPad records with
+// records naming the atom itself and pair 0 (count excludes them), so that
+// record batches of that size need no per-record bound; a self record adds
+// d (c_i - c_i) = 0 to the atom's force.
 typedef uint32_t PartnerRec;
 typedef uint32_t AtomPartners;
+static constexpr int kPartnerPad = 8;
 
 // Pair-table entry for the derivative path: e_k, dor_k, e_{k+1} - e_k,
 // dor_{k+1} - dor_k (precalculate_element::eval_deriv, precalculate.h:48-65)

@@ -291,13 +291,10 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 static constexpr int kNSmall = 16;
 // Compile-time switches of the search kernel (defaults measured fastest on
 // MI355X): the occupancy target of 32-lane groups (waves per SIMD), the
-// partner records prefetched before the pair pass, the register form of
-// the small inverse Hessian, and subtree sums from prefix sums.
+// register form of the small inverse Hessian, and subtree sums from prefix
+// sums.
 #ifndef UD_WAVES_L32
 #define UD_WAVES_L32 4
-#endif
-#ifndef UD_PREFETCH_RECORDS
-#define UD_PREFETCH_RECORDS 8
 #endif
 #ifndef UD_HESS_IN_REGS
 #define UD_HESS_IN_REGS 1
@@ -834,13 +831,13 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
     const int N = s.h.n_atoms;
     const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
     const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
-    constexpr int kRec = UD_PREFETCH_RECORDS;  // partner records fetched together
+    constexpr int kRec = kPartnerPad;  // partner records fetched together (lists are padded to this)
     // partner range and first records of the lane's first atom, requested
     // before pass A so that their latency overlaps it
     const uint32_t ap0 = (s.gl < N) ? apl[s.gl] : 0u;
     uint32_t pre[kRec];
 #pragma unroll
-    for (int u = 0; u < kRec; ++u) pre[u] = (u < int(ap0 >> 20)) ? ld_at<uint32_t>(part, ((ap0 & 0xFFFFFu) + uint32_t(u)) * 4u) : 0u;
+    for (int u = 0; u < kRec; ++u) pre[u] = (ap0 >> 20) ? ld_at<uint32_t>(part, ((ap0 & 0xFFFFFu) + uint32_t(u)) * 4u) : 0u;
     // pass A's pair batches: r, table index and fraction, then the sample
     // loads (issue), then energy and force scale into pf (finish); the first
     // batch is issued before the grid term so that its table loads and the
@@ -945,13 +942,13 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         const uint32_t ap = first ? ap0 : apl[i];
         const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
         for (int k = 0; k < kn; k += kRec) {
+            // whole batches: the padding records add zero
             uint32_t rec[kRec];
 #pragma unroll
             for (int u = 0; u < kRec; ++u)
-                rec[u] = (first && k == 0) ? pre[u] : ((k + u < kn) ? ld_at<uint32_t>(part, uint32_t(k0 + k + u) * 4u) : 0u);
+                rec[u] = (first && k == 0) ? pre[u] : ld_at<uint32_t>(part, uint32_t(k0 + k + u) * 4u);
 #pragma unroll
             for (int u = 0; u < kRec; ++u) {
-                if (k + u >= kn) continue;
                 const float d = pf[(rec[u] >> 8) & 0x1FFFu];
                 const float4 co = coords[rec[u] & 0xFFu];
                 g.x += d * (ci.x - co.x);
