@@ -516,16 +516,27 @@ __device__ __forceinline__ void grid_gather(const GridDev& g, uint32_t base, flo
     constexpr uint32_t B = (Fmt<GF>::store == UD_GRID_F32) ? 4u : 2u;  // bytes per value
     const uint32_t sy = uint32_t(g.n[0]) * B, sz = uint32_t(g.n[0] * g.n[1]) * B;
     const uint32_t o0 = base * B, o1 = o0 + sy, o2 = o0 + sz, o3 = o2 + sy;
+    // the x-neighbours of each corner pair are adjacent values: one load per
+    // pair (at the value's alignment, which global loads accept)
     if (Fmt<GF>::store == UD_GRID_F32) {
-        f[0] = ld_at<float>(g.maps, o0); f[1] = ld_at<float>(g.maps, o0 + 4u);
-        f[2] = ld_at<float>(g.maps, o1); f[3] = ld_at<float>(g.maps, o1 + 4u);
-        f[4] = ld_at<float>(g.maps, o2); f[5] = ld_at<float>(g.maps, o2 + 4u);
-        f[6] = ld_at<float>(g.maps, o3); f[7] = ld_at<float>(g.maps, o3 + 4u);
+        typedef float f2a4 __attribute__((ext_vector_type(2), aligned(4)));
+        const uint32_t o[4] = {o0, o1, o2, o3};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const f2a4 v = ld_at<f2a4>(g.maps, o[k]);
+            f[2 * k] = v.x;
+            f[2 * k + 1] = v.y;
+        }
     } else {
-        f[0] = __half2float(ld_at<__half>(g.maps, o0)); f[1] = __half2float(ld_at<__half>(g.maps, o0 + 2u));
-        f[2] = __half2float(ld_at<__half>(g.maps, o1)); f[3] = __half2float(ld_at<__half>(g.maps, o1 + 2u));
-        f[4] = __half2float(ld_at<__half>(g.maps, o2)); f[5] = __half2float(ld_at<__half>(g.maps, o2 + 2u));
-        f[6] = __half2float(ld_at<__half>(g.maps, o3)); f[7] = __half2float(ld_at<__half>(g.maps, o3 + 2u));
+        typedef uint32_t u32a2 __attribute__((aligned(2)));
+        const uint32_t w0 = ld_at<u32a2>(g.maps, o0), w1 = ld_at<u32a2>(g.maps, o1);
+        const uint32_t w2 = ld_at<u32a2>(g.maps, o2), w3 = ld_at<u32a2>(g.maps, o3);
+        const uint32_t w[4] = {w0, w1, w2, w3};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            f[2 * k] = __half2float(__ushort_as_half(uint16_t(w[k] & 0xFFFFu)));
+            f[2 * k + 1] = __half2float(__ushort_as_half(uint16_t(w[k] >> 16)));
+        }
     }
 }
 
