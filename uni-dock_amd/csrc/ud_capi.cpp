@@ -937,7 +937,7 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
         for (int i = 0; i < na; ++i) {
             apar.push_back(uint32_t(partners.size() - pbeg) | (uint32_t(part[i].size()) << 20));
             partners.insert(partners.end(), part[i].begin(), part[i].end());
-            while ((partners.size() - pbeg) % kPartnerPad) partners.push_back(uint32_t(i));
+            while ((partners.size() - pbeg) % kPartnerPad) partners.push_back(uint32_t(i) | (uint32_t(np) << 8));
         }
         LigHdr& h = hdr[l];
         std::memset(&h, 0, sizeof(h));

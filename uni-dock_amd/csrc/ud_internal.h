@@ -53,9 +53,9 @@ typedef uint32_t PairRec;
 // set when the atom is the pair's first atom.  Per atom, AtomPartners =
 // first partner (relative to the ligand's partner_begin) | count << 20.
 // Each atom's list is padded to a multiple of kPartnerPad records with
-// records naming the atom itself and pair 0 (count excludes them), so that
-// record batches of that size need no per-record bound; a self record adds
-// d (c_i - c_i) = 0 to the atom's force.
+// records naming the atom itself and pair n_pairs, whose force-scale slot
+// holds 0 (count excludes them), so that record batches of that size need no
+// per-record bound.
 typedef uint32_t PartnerRec;
 typedef uint32_t AtomPartners;
 static constexpr int kPartnerPad = 8;

@@ -311,7 +311,7 @@ static constexpr int kNSmall = 16;
 struct Layout {
     int na, nn, nc, nh, ps;
     int o_coords, o_mf, o_frames, o_poses, o_bc, o_H, o_pf, group_floats, shared_floats;
-    // pf: per-pair buffer (one float per pair) for pair-parallel forces
+    // pf: per-pair buffer (one float per pair, then a zero) for pair-parallel forces
     __host__ __device__ void init(int na_, int nn_, int np_, bool pf, int lanes) {
         na = na_;
         nn = nn_;
@@ -327,7 +327,7 @@ struct Layout {
         o_bc = o_poses + 5 * ps;
         o_H = o_bc + 3 * ps;
         o_pf = pf ? o_H + nh : -1;
-        const int end = pf ? o_pf + ((np_ + 3) & ~3) : o_H + nh;
+        const int end = pf ? o_pf + ((np_ + 1 + 3) & ~3) : o_H + nh;  // + a zero entry after the pairs
         // group blocks start 4 floats apart modulo 32 banks, so that the
         // same offset in up to 8 groups falls into different LDS banks
         group_floats = ((end + 31) & ~31) + 4;
@@ -384,6 +384,7 @@ __device__ Grp<L, NM> setup_group(const KArgs& ka, int lig, float* lds) {
     s.g = threadIdx.x / L;
     s.gl = threadIdx.x % L;
     s.gb = lds + s.lay.shared_floats + s.g * s.lay.group_floats;
+    if (s.lay.o_pf >= 0 && s.gl == 0) s.pf()[s.h.n_pairs] = 0.f;  // the padding records' pair
     __syncthreads();
     return s;
 }
@@ -952,8 +953,10 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         const bool first = i == s.gl;
         const uint32_t ap = first ? ap0 : apl[i];
         const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
+        // sum d (c_i - c_o) = c_i sum d - sum d c_o over the atom's records,
+        // in whole batches (padding records carry d = 0)
+        float sd = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
         for (int k = 0; k < kn; k += kRec) {
-            // whole batches: the padding records add zero
             uint32_t rec[kRec];
 #pragma unroll
             for (int u = 0; u < kRec; ++u)
@@ -962,11 +965,15 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
             for (int u = 0; u < kRec; ++u) {
                 const float d = pf[(rec[u] >> 8) & 0x1FFFu];
                 const float4 co = coords[rec[u] & 0xFFu];
-                g.x += d * (ci.x - co.x);
-                g.y += d * (ci.y - co.y);
-                g.z += d * (ci.z - co.z);
+                sd += d;
+                sx += d * co.x;
+                sy += d * co.y;
+                sz += d * co.z;
             }
         }
+        g.x += ci.x * sd - sx;
+        g.y += ci.y * sd - sy;
+        g.z += ci.z * sd - sz;
         mf[i] = g;
     }
     return e;
