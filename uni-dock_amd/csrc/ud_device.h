@@ -187,7 +187,7 @@ __device__ __forceinline__ void q_to_mat(float4 q, float* m) {
 __device__ __forceinline__ float4 q_normalize_approx(float4 q) {
     const float s = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
     if (fabsf(s - 1.f) < 1e-6f) return q;
-    const float inv = 1.f / sqrtf(s);
+    const float inv = __builtin_amdgcn_rsqf(s);
     return make_float4(q.x * inv, q.y * inv, q.z * inv, q.w * inv);
 }
 // angle_to_quaternion(axis, angle) (quaternion.cpp:37-47)
@@ -198,12 +198,14 @@ __device__ __forceinline__ float4 axis_angle_q(float ax, float ay, float az, flo
     return make_float4(c, s * ax, s * ay, s * az);
 }
 // quaternion_increment (quaternion.cpp:128-133) with angle_to_quaternion(rotation);
-// the axis is the rotation times the angle's reciprocal
+// the axis is the rotation times the angle's reciprocal.  Square roots and
+// reciprocals on the per-iteration path are the hardware's (about 1 ulp), as
+// the reference's GPU build computes them (--use_fast_math).
 __device__ __forceinline__ float4 q_increment(float4 q, float rx, float ry, float rz) {
-    const float angle = sqrtf(rx * rx + ry * ry + rz * rz);
+    const float angle = __builtin_amdgcn_sqrtf(rx * rx + ry * ry + rz * rz);
     float4 r = make_float4(1.f, 0.f, 0.f, 0.f);
     if (angle > kEpsFl) {
-        const float inv = 1.f / angle;
+        const float inv = __builtin_amdgcn_rcpf(angle);
         r = axis_angle_q(rx * inv, ry * inv, rz * inv, angle);
     }
     return q_normalize_approx(qmul(r, q));

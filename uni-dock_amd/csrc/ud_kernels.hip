@@ -1007,7 +1007,7 @@ __device__ float eval_group(const KArgs& ka, const Grp<L, NM>& s, const float* p
     if (rgyr) {
         // model::gyration_radius (model.cpp:871-883): heavy atoms about the root origin
         rg2 = gsum<L>(rg2);
-        *rgyr = s.h.n_heavy > 0 ? sqrtf(rg2 / float(s.h.n_heavy)) : 0.f;
+        *rgyr = s.h.n_heavy > 0 ? __builtin_amdgcn_sqrtf(rg2 * __builtin_amdgcn_rcpf(float(s.h.n_heavy))) : 0.f;
     }
     __syncthreads();
     if (!want_grad) return e;
@@ -1430,7 +1430,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
             f0 = f;
             rg_x = rg;
         }
-        const bool conv = ls_end && !(sqrtf(gg) >= 1e-5f);  // uses g before g = g_new (bfgs.h:131-132)
+        const bool conv = ls_end && !(__builtin_amdgcn_sqrtf(gg) >= 1e-5f);  // uses g before g = g_new (bfgs.h:131-132)
         const bool cont = ls_end && !conv;
         if (cont) {
 #pragma unroll
@@ -1472,14 +1472,14 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         }
         const bool scale0 = cont && step_now == 0 && fabsf(yy) > kEpsFl;
         if constexpr (HREG) {
-            const float dv = alpha * yp0 / yy;
+            const float dv = alpha * yp0 * __builtin_amdgcn_rcpf(yy);
 #pragma unroll
             for (int j = 0; j < KV; ++j)
 #pragma unroll
                 for (int c = 0; c < NM / HS; ++c)
                     Hr[j][c] = (scale0 && hess_col0<L, NM, HS>(s.gl) + c == hess_row<L, NM, HS>(s.gl, j)) ? dv : Hr[j][c];
         } else if (scale0) {
-            const float dv = alpha * yp0 / yy;
+            const float dv = alpha * yp0 * __builtin_amdgcn_rcpf(yy);
             for (int i = s.gl; i < n; i += L) H[i * hs + i] = dv;
         }
         __syncthreads();
@@ -1502,12 +1502,12 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
             __syncthreads();
             if constexpr (HREG) {
                 if (wave_any(do_upd)) {
-                    const float r = 1.f / (alpha * yp);
+                    const float r = __builtin_amdgcn_rcpf(alpha * yp);
                     const float c1 = alpha * r, c2 = alpha * alpha * (r * r * yhy + r);
                     hess_update_reg<L, NM, KV, HS>(s, Hr, mhy, p, do_upd, c1, c2, ma, pb);
                 }
             } else if (do_upd) {
-                const float r = 1.f / (alpha * yp);
+                const float r = __builtin_amdgcn_rcpf(alpha * yp);
                 const float c1 = alpha * r, c2 = alpha * alpha * (r * r * yhy + r);
                 // one row per lane; each entry uses the (min, max) index order of
                 // the reference's upper-triangle update, so H stays exactly symmetric
