@@ -290,11 +290,14 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 // (full-wavefront groups are register-bound already).
 static constexpr int kNSmall = 16;
 // Compile-time switches of the search kernel (defaults measured fastest on
-// MI355X): the occupancy target of 32-lane groups (waves per SIMD), the
+// MI355X): the occupancy targets of 32- and 16-lane groups (waves per SIMD), the
 // register form of the small inverse Hessian, and subtree sums from prefix
 // sums.
 #ifndef UD_WAVES_L32
 #define UD_WAVES_L32 4
+#endif
+#ifndef UD_WAVES_L16
+#define UD_WAVES_L16 4
 #endif
 #ifndef UD_HESS_IN_REGS
 #define UD_HESS_IN_REGS 1
@@ -678,13 +681,14 @@ __device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p,
 
 // Compile-time tuning of the search kernel: load batching and the occupancy
 // target handed to the register allocator (waves per SIMD; 1 = no target).
-// Full-wavefront and 32-lane groups target 4 waves per SIMD (UD_WAVES_L32),
-// smaller groups UD_SEARCH_WAVES_PER_EU: measured fastest so.
+// Full-wavefront, 32- and 16-lane groups target 4 waves per SIMD (UD_WAVES_L32,
+// UD_WAVES_L16), 8-lane groups UD_SEARCH_WAVES_PER_EU; one pair per lane per
+// table-load batch: measured fastest so.
 #ifndef UD_ATOM_BATCH
 #define UD_ATOM_BATCH 1
 #endif
 #ifndef UD_PAIR_BATCH
-#define UD_PAIR_BATCH 2
+#define UD_PAIR_BATCH 1
 #endif
 #ifndef UD_SEARCH_WAVES_PER_EU
 #define UD_SEARCH_WAVES_PER_EU 3
@@ -1336,7 +1340,7 @@ enum { ST_RGYR = 0, ST_INIT = 1, ST_TRIAL = 2, ST_IDLE = 3 };
 // MC == false runs one BFGS per group on the given poses (quasi_newton).
 // The curvature scalar y.p is accumulated in FP64.
 template <int L, int GF, bool MC, int NM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L == 64 ? 4 : L == 32 ? UD_WAVES_L32 : UD_SEARCH_WAVES_PER_EU)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L == 64 ? 4 : L == 32 ? UD_WAVES_L32 : L == 16 ? UD_WAVES_L16 : UD_SEARCH_WAVES_PER_EU)))
 void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* perm,
                                                     float* poses_io, float v_intra_opt, float v_inter_opt,
                                                     int max_steps_opt, float* out_pose, float* out_e,
