@@ -299,6 +299,9 @@ static constexpr int kNSmall = 16;
 #ifndef UD_WAVES_L16
 #define UD_WAVES_L16 4
 #endif
+#ifndef UD_WAVES_L64
+#define UD_WAVES_L64 4
+#endif
 #ifndef UD_HESS_IN_REGS
 #define UD_HESS_IN_REGS 1
 #endif
@@ -1340,7 +1343,7 @@ enum { ST_RGYR = 0, ST_INIT = 1, ST_TRIAL = 2, ST_IDLE = 3 };
 // MC == false runs one BFGS per group on the given poses (quasi_newton).
 // The curvature scalar y.p is accumulated in FP64.
 template <int L, int GF, bool MC, int NM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L == 64 ? 4 : L == 32 ? UD_WAVES_L32 : L == 16 ? UD_WAVES_L16 : UD_SEARCH_WAVES_PER_EU)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L == 64 ? UD_WAVES_L64 : L == 32 ? UD_WAVES_L32 : L == 16 ? UD_WAVES_L16 : UD_SEARCH_WAVES_PER_EU)))
 void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* perm,
                                                     float* poses_io, float v_intra_opt, float v_inter_opt,
                                                     int max_steps_opt, float* out_pose, float* out_e,
