@@ -478,14 +478,20 @@ __device__ __forceinline__ GridPt grid_locate(const GridDev& g, int slot, float3
     GridPt q;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        if (s[i] < 0.f) {
-            miss[i] = -s[i]; q.reg[i] = -1; a[i] = 0; s[i] = 0.f;
-        } else if (s[i] >= g.dimm1[i]) {
-            miss[i] = s[i] - g.dimm1[i]; q.reg[i] = 1; a[i] = g.n[i] - 2; s[i] = 1.f;
-        } else {
-            miss[i] = 0.f; q.reg[i] = 0; a[i] = int(s[i]); s[i] -= float(a[i]);
-        }
-        a[i] = min(max(a[i], 0), g.n[i] - 2);  // a NaN coordinate must not index outside the map
+        // below the box: corner 0, fraction 0; at or beyond its last sample:
+        // cell n - 2, fraction 1.  Branch-free: a clamp (v_med3) and selects.
+        // The out-of-box distance is selected, not formed as |s - clamp(s)|,
+        // which FMA contraction with s's product would make a rounding
+        // residue (times the 1e6 penalty slope) inside the box.
+        const float si = s[i];
+        const float sc = __builtin_amdgcn_fmed3f(si, 0.f, g.dimm1[i]);
+        const bool hi = si >= g.dimm1[i];
+        miss[i] = hi ? si - g.dimm1[i] : (si < 0.f ? -si : 0.f);
+        q.reg[i] = hi ? 1 : (si < 0.f ? -1 : 0);
+        a[i] = min(max(int(sc), 0), g.n[i] - 2);  // a NaN coordinate must not index outside the map
+        // the fraction is formed from s itself, so that a NaN coordinate
+        // still gives a NaN energy (which the line search rejects)
+        s[i] = hi ? 1.f : (si < 0.f ? 0.f : si - float(a[i]));
     }
     q.penalty = g.slope * (miss[0] * g.factor_inv[0] + miss[1] * g.factor_inv[1] + miss[2] * g.factor_inv[2]);
     q.base = uint32_t(slot) * uint32_t(g.n_points) + uint32_t(a[0] + g.n[0] * (a[1] + g.n[1] * a[2]));
