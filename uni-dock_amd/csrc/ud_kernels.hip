@@ -860,9 +860,21 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
     // partner range and first records of the lane's first atom, requested
     // before pass A so that their latency overlaps it
     const uint32_t ap0 = (s.gl < N) ? apl[s.gl] : 0u;
+    // (each atom's list starts at a multiple of kPartnerPad records, so a
+    // batch is two aligned 16-byte loads)
+    static_assert(kPartnerPad == 8, "record batches are loaded as two uint4");
+    auto load_batch = [&](uint32_t first_rec, uint32_t (&r)[kRec]) {
+        const uint4 lo = ld_at<uint4>(part, first_rec * 4u), hi = ld_at<uint4>(part, first_rec * 4u + 16u);
+        r[0] = lo.x; r[1] = lo.y; r[2] = lo.z; r[3] = lo.w;
+        r[4] = hi.x; r[5] = hi.y; r[6] = hi.z; r[7] = hi.w;
+    };
     uint32_t pre[kRec];
+    if (ap0 >> 20) {
+        load_batch(ap0 & 0xFFFFFu, pre);
+    } else {
 #pragma unroll
-    for (int u = 0; u < kRec; ++u) pre[u] = (ap0 >> 20) ? ld_at<uint32_t>(part, ((ap0 & 0xFFFFFu) + uint32_t(u)) * 4u) : 0u;
+        for (int u = 0; u < kRec; ++u) pre[u] = 0u;
+    }
     // pass A's pair batches: r, table index and fraction, then the sample
     // loads (issue), then energy and force scale into pf (finish); the first
     // batch is issued before the grid term so that its table loads and the
@@ -971,9 +983,12 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         float sd = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
         for (int k = 0; k < kn; k += kRec) {
             uint32_t rec[kRec];
+            if (first && k == 0) {
 #pragma unroll
-            for (int u = 0; u < kRec; ++u)
-                rec[u] = (first && k == 0) ? pre[u] : ld_at<uint32_t>(part, uint32_t(k0 + k + u) * 4u);
+                for (int u = 0; u < kRec; ++u) rec[u] = pre[u];
+            } else {
+                load_batch(uint32_t(k0 + k), rec);
+            }
 #pragma unroll
             for (int u = 0; u < kRec; ++u) {
                 const float d = pf[(rec[u] >> 8) & 0x1FFFu];
