@@ -291,8 +291,9 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 static constexpr int kNSmall = 16;
 // Compile-time switches of the search kernel (defaults measured fastest on
 // MI355X): the occupancy targets of 32- and 16-lane groups (waves per SIMD), the
-// register form of the small inverse Hessian, and subtree sums from prefix
-// sums.
+// register form of the small inverse Hessian, subtree sums from prefix
+// sums, and whether pass B's first record batch is loaded before pass A (off:
+// the 8 registers it holds through pass A cost more than the latency saved).
 #ifndef UD_WAVES_L32
 #define UD_WAVES_L32 4
 #endif
@@ -307,6 +308,9 @@ static constexpr int kNSmall = 16;
 #endif
 #ifndef UD_TORQUE_SCAN
 #define UD_TORQUE_SCAN 1
+#endif
+#ifndef UD_PREFETCH_RECORDS
+#define UD_PREFETCH_RECORDS 0
 #endif
 
 // LDS layout of one workgroup: the ligand's atoms and nodes, then one block
@@ -857,8 +861,9 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
     const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
     const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
     constexpr int kRec = kPartnerPad;  // partner records fetched together (lists are padded to this)
-    // partner range and first records of the lane's first atom, requested
-    // before pass A so that their latency overlaps it
+    // partner range of the lane's first atom (and, with UD_PREFETCH_RECORDS,
+    // its first record batch), requested before pass A so that the latency
+    // overlaps it
     const uint32_t ap0 = (s.gl < N) ? apl[s.gl] : 0u;
     // (each atom's list starts at a multiple of kPartnerPad records, so a
     // batch is two aligned 16-byte loads)
@@ -868,12 +873,14 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         r[0] = lo.x; r[1] = lo.y; r[2] = lo.z; r[3] = lo.w;
         r[4] = hi.x; r[5] = hi.y; r[6] = hi.z; r[7] = hi.w;
     };
-    uint32_t pre[kRec];
-    if (ap0 >> 20) {
-        load_batch(ap0 & 0xFFFFFu, pre);
-    } else {
+    uint32_t pre[UD_PREFETCH_RECORDS ? kRec : 1];
+    if constexpr (UD_PREFETCH_RECORDS) {
+        if (ap0 >> 20) {
+            load_batch(ap0 & 0xFFFFFu, pre);
+        } else {
 #pragma unroll
-        for (int u = 0; u < kRec; ++u) pre[u] = 0u;
+            for (int u = 0; u < kRec; ++u) pre[u] = 0u;
+        }
     }
     // pass A's pair batches: r, table index and fraction, then the sample
     // loads (issue), then energy and force scale into pf (finish); the first
@@ -983,9 +990,9 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         float sd = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
         for (int k = 0; k < kn; k += kRec) {
             uint32_t rec[kRec];
-            if (first && k == 0) {
+            if (UD_PREFETCH_RECORDS && first && k == 0) {
 #pragma unroll
-                for (int u = 0; u < kRec; ++u) rec[u] = pre[u];
+                for (int u = 0; u < kRec; ++u) rec[u] = pre[UD_PREFETCH_RECORDS ? u : 0];
             } else {
                 load_batch(uint32_t(k0 + k), rec);
             }
