@@ -934,6 +934,8 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
             part[pb].push_back(uint32_t(pa) | (uint32_t(q) << 8) | slot21);
         }
         const size_t pbeg = partners.size();
+        // the search kernel loads record batches as aligned 16-byte pairs
+        if (pbeg % kPartnerPad) return fail(c, UD_ERR_INVALID, "internal: partner list misaligned");
         for (int i = 0; i < na; ++i) {
             apar.push_back(uint32_t(partners.size() - pbeg) | (uint32_t(part[i].size()) << 20));
             partners.insert(partners.end(), part[i].begin(), part[i].end());
