@@ -292,8 +292,10 @@ static constexpr int kNSmall = 16;
 // Compile-time switches of the search kernel (defaults measured fastest on
 // MI355X): the occupancy targets of 32- and 16-lane groups (waves per SIMD), the
 // register form of the small inverse Hessian, subtree sums from prefix
-// sums, and whether pass B's first record batch is loaded before pass A (off:
-// the 8 registers it holds through pass A cost more than the latency saved).
+// sums, whether pass B's first record batch is loaded before pass A (off:
+// the 8 registers it holds through pass A cost more than the latency saved),
+// and the lane's torsion-tree node record kept in registers across the
+// forward-kinematics levels.
 #ifndef UD_WAVES_L32
 #define UD_WAVES_L32 4
 #endif
@@ -311,6 +313,9 @@ static constexpr int kNSmall = 16;
 #endif
 #ifndef UD_PREFETCH_RECORDS
 #define UD_PREFETCH_RECORDS 0
+#endif
+#ifndef UD_FK_NODE_REGS
+#define UD_FK_NODE_REGS 1
 #endif
 
 // LDS layout of one workgroup: the ligand's atoms and nodes, then one block
@@ -423,13 +428,16 @@ __device__ void fk_frames(const Grp<L, NM>& s, const float* pose) {
         hs[m] = 0.f;
         if (k < s.h.n_nodes) sincos_half(normalize_angle(pose[7 + k - 1]) * 0.5f, &hs[m], &hc[m]);
     }
+    // with one node per lane its record is read once, not once per level
+    NodeRec nd1;
+    if (UD_FK_NODE_REGS && KN == 1) nd1 = s.nodes[min(1 + s.gl, s.h.n_nodes - 1)];
     __syncthreads();
     for (int d = 1; d <= s.h.max_depth; ++d) {
 #pragma unroll
         for (int m = 0; m < KN; ++m) {
             const int k = 1 + s.gl + m * L;
             if (k >= s.h.n_nodes) continue;
-            const NodeRec nd = s.nodes[k];
+            const NodeRec nd = (UD_FK_NODE_REGS && KN == 1) ? nd1 : s.nodes[k];
             if (int(nd.packed >> 24) != d) continue;
             const float* pf = fr + 20 * nd.parent;
             const float* R = pf + 10;
