@@ -147,6 +147,10 @@ typedef struct {
                                     the workgroup's next chain when one ends; 0: 1 (one chain
                                     per group, the finest workgroups).  No counterpart in the
                                     reference, whose blocks each run one chain. */
+    int32_t first_ls_f0_zero;    /* 1: the first line search of each BFGS compares with
+                                    f0 = 0, as the reference's GPU BFGS does (bfgs_warp,
+                                    warp_ops.cuh:1062-1076); 0: with f0 = f(x), the CPU rule
+                                    (bfgs.h:107) */
 } ud_search_params;
 
 /* fill *p with the reference defaults (exhaustiveness 8, global_steps 20) */

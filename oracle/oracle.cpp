@@ -580,6 +580,9 @@ template <class R> struct Ctx {
     std::vector<int> rec_t;
     bool use_nc = false;   // eval_deriv scores against rec (non_cache) instead of the maps
     R nc_slope = R(1e6);
+    // first line search of every BFGS against f0 = 0 (the reference GPU's
+    // bfgs_warp, warp_ops.cuh:1062-1076) instead of f(x) (bfgs.h:107)
+    bool ls_f0_zero = false;
     const PairTable<R>& table(int t1, int t2) {
         int k = tri_index(t1, t2);
         if (!have[k]) { tables[k].build(sf, t1, t2, w); have[k] = 1; }
@@ -797,75 +800,118 @@ template <class R> void pose_increment(Pose<R>& x, const R* p, R alpha) {
     }
 }
 
-// bfgs() with line_search() and bfgs_update() (bfgs.h:30-147), quasi_newton
-// driver (quasi_newton.cpp:43-53).
-template <class R> R bfgs(Ctx<R>& C, const Ligand<R>& L, Pose<R>& x, const R v[3], int max_steps, int* evalcount) {
-    const int n = 6 + L.n_tors();
-    std::vector<R> h(n * (n + 1) / 2, R(0));
+// Per-step record of a BFGS run (test hook): the accepted step length, the
+// energy after the line search, the number of line-search evaluations, and
+// after the step's update the inverse Hessian (full n x n) with the step's
+// s = alpha p and y = g_new - g.
+struct BfgsTrace {
+    int steps = 0;
+    std::vector<double> alpha, f, H, s, y;
+    std::vector<int> trials;
+};
+
+// bfgs() with line_search() and bfgs_update() (bfgs.h:30-147) over any
+// objective: eval(x, g) returns f(x) and fills g (n entries); incr(x, p, a)
+// moves x by a p (conf::increment for poses, conf.h:98-104).  The line search
+// compares with f0 = f(x) (bfgs.h:107); with f0_zero_first, the first line
+// search compares with f0 = 0 instead, as the reference's GPU BFGS does
+// (bfgs_warp, warp_ops.cuh:1062-1076: f0 starts at 0 and f_orig is kept
+// separately).  Packed upper-triangular inverse Hessian (triangular_matrix,
+// matrix.h), index tri_index.
+template <class R, class X, class Eval, class Incr>
+R bfgs_core(Eval& eval, Incr& incr, X& x, int n, int max_steps, int* evalcount, bool f0_zero_first,
+            BfgsTrace* tr = nullptr) {
+    std::vector<R> h(size_t(n) * (n + 1) / 2, R(0));
     auto H = [&](int i, int j) -> R& { return h[tri_index(i, j)]; };
     for (int i = 0; i < n; ++i) H(i, i) = 1;
     std::vector<R> g(n), g_new(n), p(n), y(n), minus_hy(n);
-    Pose<R> x_new = x;
-    R f0 = eval_deriv(C, L, x, v, g.data());
+    X x_new = x;
+    const R f_orig = eval(x, g.data());
     ++*evalcount;
-    const R f_orig = f0;
-    const std::vector<R> g_orig = g;
-    const Pose<R> x_orig = x;
+    R f0 = f0_zero_first ? R(0) : f_orig;
+    const X x_orig = x;
     auto dotn = [&](const std::vector<R>& a, const std::vector<R>& b) {
         R s = 0;
         for (int i = 0; i < n; ++i) s += a[i] * b[i];
         return s;
     };
     for (int step = 0; step < max_steps; ++step) {
-        for (int i = 0; i < n; ++i) {  // minus_mat_vec_product
+        for (int i = 0; i < n; ++i) {  // minus_mat_vec_product (bfgs.h:30-38)
             R sum = 0;
             for (int j = 0; j < n; ++j) sum += H(i, j) * g[j];
             p[i] = -sum;
         }
-        // line_search
+        // line_search (bfgs.h:65-85): c0 = 1e-4, multiplier 0.5, 10 trials
         R f1 = 0;
         R alpha = 1;
         const R pg = dotn(p, g);
+        int trials = 0;
         for (int trial = 0; trial < 10; ++trial) {
             x_new = x;
-            pose_increment(x_new, p.data(), alpha);
-            f1 = eval_deriv(C, L, x_new, v, g_new.data());
+            incr(x_new, p.data(), alpha);
+            f1 = eval(x_new, g_new.data());
             ++*evalcount;
+            ++trials;
             if (f1 - f0 < R(0.0001) * alpha * pg) break;
             alpha *= R(0.5);
         }
         for (int i = 0; i < n; ++i) y[i] = g_new[i] - g[i];
         f0 = f1;
         x = x_new;
-        if (!(std::sqrt(dotn(g, g)) >= R(1e-5))) break;
-        g = g_new;
-        if (step == 0) {
-            const R yy = dotn(y, y);
-            if (std::abs(yy) > R(kEpsFl)) {
-                const R d = alpha * dotn(y, p) / yy;
-                for (int i = 0; i < n; ++i) H(i, i) = d;
+        if (tr) {
+            tr->steps = step + 1;
+            tr->alpha.push_back(double(alpha));
+            tr->f.push_back(double(f1));
+            tr->trials.push_back(trials);
+            for (int i = 0; i < n; ++i) tr->s.push_back(double(alpha * p[i]));
+            for (int i = 0; i < n; ++i) tr->y.push_back(double(y[i]));
+        }
+        // the convergence test reads g before g = g_new (bfgs.h:131-132)
+        const bool stop = !(std::sqrt(dotn(g, g)) >= R(1e-5));
+        if (!stop) {
+            g = g_new;
+            if (step == 0) {  // initial scaling (bfgs.h:135-138)
+                const R yy = dotn(y, y);
+                if (std::abs(yy) > R(kEpsFl)) {
+                    const R d = alpha * dotn(y, p) / yy;
+                    for (int i = 0; i < n; ++i) H(i, i) = d;
+                }
+            }
+            // bfgs_update (bfgs.h:46-63)
+            const R yp = dotn(y, p);
+            if (!(alpha * yp < R(kEpsFl))) {
+                for (int i = 0; i < n; ++i) {
+                    R sum = 0;
+                    for (int j = 0; j < n; ++j) sum += H(i, j) * y[j];
+                    minus_hy[i] = -sum;
+                }
+                const R yhy = -dotn(y, minus_hy);
+                const R r = R(1) / (alpha * yp);
+                for (int i = 0; i < n; ++i)
+                    for (int j = i; j < n; ++j)
+                        H(i, j) += alpha * r * (minus_hy[i] * p[j] + minus_hy[j] * p[i]) +
+                                   alpha * alpha * (r * r * yhy + r) * p[i] * p[j];
             }
         }
-        // bfgs_update (bfgs.h:46-63)
-        const R yp = dotn(y, p);
-        if (alpha * yp < R(kEpsFl)) continue;
-        for (int i = 0; i < n; ++i) {
-            R sum = 0;
-            for (int j = 0; j < n; ++j) sum += H(i, j) * y[j];
-            minus_hy[i] = -sum;
-        }
-        const R yhy = -dotn(y, minus_hy);
-        const R r = R(1) / (alpha * yp);
-        for (int i = 0; i < n; ++i)
-            for (int j = i; j < n; ++j)
-                H(i, j) += alpha * r * (minus_hy[i] * p[j] + minus_hy[j] * p[i]) +
-                           alpha * alpha * (r * r * yhy + r) * p[i] * p[j];
+        if (tr)
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) tr->H.push_back(double(H(i, j)));
+        if (stop) break;
     }
+    // revert when nothing improved (bfgs.h:140-144)
     if (!(f0 <= f_orig)) {
         f0 = f_orig;
         x = x_orig;
     }
     return f0;
+}
+
+// quasi_newton (quasi_newton.cpp:43-53) on a pose: eval_deriv with caps v,
+// conf::increment.  C.ls_f0_zero selects the GPU's first-line-search rule.
+template <class R> R bfgs(Ctx<R>& C, const Ligand<R>& L, Pose<R>& x, const R v[3], int max_steps, int* evalcount) {
+    auto ev = [&](const Pose<R>& P, R* g) { return eval_deriv(C, L, P, v, g); };
+    auto inc = [&](Pose<R>& P, const R* p, R a) { pose_increment(P, p, a); };
+    return bfgs_core<R>(ev, inc, x, 6 + L.n_tors(), max_steps, evalcount, C.ls_f0_zero);
 }
 
 // ---------------------------------------------------------------- Philox
@@ -1394,6 +1440,61 @@ extern "C" double orc_bfgs(OrcHandle* h, int lig, double* pose, const double* v,
     return bfgs_impl(h->f, lig, pose, v, max_steps, evals);
 }
 
+// BFGS on the quadratic f(x) = 1/2 x^T A x - b^T x + c (A row-major n x n),
+// x moved by x + alpha p: the search itself, free of poses, for tests with
+// hand-computable iterates.  prec 32 or 64.  Per step s (up to max_steps):
+// alpha[s], f[s], trials[s], H[s][n][n] after the step's update, sv[s][n] =
+// alpha p, yv[s][n] = g_new - g.  Returns the final f; x in place.
+template <class R> static double bfgs_quad_impl(int n, const double* A, const double* b, double c, double* x,
+                                                int max_steps, int f0_zero, int* evals, int* steps, double* alpha,
+                                                double* f, int* trials, double* H, double* sv, double* yv) {
+    std::vector<R> xv(x, x + n);
+    auto ev = [&](const std::vector<R>& z, R* g) {
+        R fz = R(c);
+        for (int i = 0; i < n; ++i) {
+            R ai = 0;
+            for (int j = 0; j < n; ++j) ai += R(A[i * n + j]) * z[j];
+            g[i] = ai - R(b[i]);
+            fz += R(0.5) * z[i] * ai - R(b[i]) * z[i];
+        }
+        return fz;
+    };
+    auto inc = [&](std::vector<R>& z, const R* p, R a) {
+        for (int i = 0; i < n; ++i) z[i] += a * p[i];
+    };
+    BfgsTrace tr;
+    int ec = 0;
+    R e = bfgs_core<R>(ev, inc, xv, n, max_steps, &ec, f0_zero != 0, &tr);
+    for (int i = 0; i < n; ++i) x[i] = double(xv[i]);
+    if (evals) *evals = ec;
+    if (steps) *steps = tr.steps;
+    for (int k = 0; k < tr.steps; ++k) {
+        if (alpha) alpha[k] = tr.alpha[k];
+        if (f) f[k] = tr.f[k];
+        if (trials) trials[k] = tr.trials[k];
+    }
+    if (H) std::copy(tr.H.begin(), tr.H.end(), H);
+    if (sv) std::copy(tr.s.begin(), tr.s.end(), sv);
+    if (yv) std::copy(tr.y.begin(), tr.y.end(), yv);
+    return double(e);
+}
+extern "C" double orc_bfgs_quadratic(int prec, int n, const double* A, const double* b, double c, double* x,
+                                     int max_steps, int f0_zero, int* evals, int* steps, double* alpha, double* f,
+                                     int* trials, double* H, double* sv, double* yv) {
+    if (prec == 64) return bfgs_quad_impl<double>(n, A, b, c, x, max_steps, f0_zero, evals, steps, alpha, f, trials, H, sv, yv);
+    return bfgs_quad_impl<float>(n, A, b, c, x, max_steps, f0_zero, evals, steps, alpha, f, trials, H, sv, yv);
+}
+// first-line-search rule of every BFGS on this handle: 0 = f0 = f(x)
+// (bfgs.h:107), 1 = f0 = 0 (bfgs_warp, warp_ops.cuh:1062-1076)
+extern "C" void orc_set_ls_f0_zero(OrcHandle* h, int on) {
+    h->d.ls_f0_zero = on != 0;
+    h->f.ls_f0_zero = on != 0;
+}
+// metropolis_accept (monte_carlo.cu:92-97) with uniform u
+extern "C" int orc_metropolis(double old_f, double new_f, double temperature, double u) {
+    return metropolis<double>(old_f, new_f, temperature, u) ? 1 : 0;
+}
+
 template <class R> static double mc_impl(Ctx<R>& C, int lig, uint64_t seed, uint32_t chain, int gsteps, int lsteps, double* best_pose, int* evals, double* trace) {
     const Ligand<R>& L = C.ligs[lig];
     Pose<R> best;
@@ -1578,8 +1679,12 @@ extern "C" void orc_eval_batch(OrcHandle* h, int n, const int* ligs, const doubl
     for (auto& t : th) t.join();
 }
 
+// Every chain of n_lig ligands (chain ids ligand_position * exh + k), on
+// n_threads threads: best energies, evaluation count and (when best_pose_out
+// is not NULL) the best pose of each chain, 39 values per chain.
 extern "C" double orc_dock_cpu(OrcHandle* h, int n_lig, const int* ligs, int exh, uint64_t seed, int gsteps,
-                               const int* lsteps, int n_threads, double* best_e_out, long long* evals_out) {
+                               const int* lsteps, int n_threads, double* best_e_out, long long* evals_out,
+                               double* best_pose_out) {
     std::atomic<int> next(0);
     const int total = n_lig * exh;
     std::vector<long long> evals(n_threads, 0);
@@ -1597,6 +1702,7 @@ extern "C" double orc_dock_cpu(OrcHandle* h, int n_lig, const int* ligs, int exh
             else e = mc_impl(h->f, lig, seed, uint32_t(li * exh + ci), gsteps, lsteps[li], pose.data(), &ec, nullptr);
             be[t] = e;
             evals[tid] += ec;
+            if (best_pose_out) std::copy(pose.begin(), pose.begin() + 39, best_pose_out + size_t(t) * 39);
         }
     };
     // tables are built lazily; build them for all pairs used before threading

@@ -72,8 +72,12 @@ def load():
         "orc_ad4_kind": (None, [ctypes.c_int, dp]),
         "orc_pair_table_ad4": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, dp,
                                               dp, dp]),
+        "orc_bfgs_quadratic": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_double, dp,
+                                                 ctypes.c_int, ctypes.c_int, ip, ip, dp, dp, ip, dp, dp, dp]),
+        "orc_set_ls_f0_zero": (None, [vp, ctypes.c_int]),
+        "orc_metropolis": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "orc_dock_cpu": (ctypes.c_double, [vp, ctypes.c_int, ip, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ip,
-                                           ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)]),
+                                           ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong), dp]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -105,6 +109,31 @@ def philox(ctr, key):
     o = (ctypes.c_uint32 * 4)()
     L.orc_philox(c, k, o)
     return list(o)
+
+
+def bfgs_quadratic(A, b, x0, c=0.0, max_steps=20, prec=64, f0_zero=False):
+    """The oracle's BFGS on f(x) = x^T A x / 2 - b^T x + c with x moved by
+    x + alpha p.  Returns a dict: x, f (final), evals, steps and per step
+    alpha, f, trials, H (after the step's update), s = alpha p, y = g_new - g."""
+    L = load()
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    ev, st = ctypes.c_int(), ctypes.c_int()
+    al, fv = np.zeros(max_steps), np.zeros(max_steps)
+    tr = np.zeros(max_steps, dtype=np.int32)
+    H, s, y = np.zeros(max_steps * n * n), np.zeros(max_steps * n), np.zeros(max_steps * n)
+    pd = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    f = L.orc_bfgs_quadratic(prec, n, pd(A), _d(b), c, pd(x), max_steps, int(f0_zero), ctypes.byref(ev),
+                             ctypes.byref(st), pd(al), pd(fv), tr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                             pd(H), pd(s), pd(y))
+    k = st.value
+    return dict(x=x, f=f, evals=ev.value, steps=k, alpha=al[:k], fs=fv[:k], trials=tr[:k],
+                H=H[:k * n * n].reshape(k, n, n), s=s[:k * n].reshape(k, n), y=y[:k * n].reshape(k, n))
+
+
+def metropolis(old_f, new_f, temperature, u):
+    return bool(load().orc_metropolis(old_f, new_f, temperature, u))
 
 
 class Oracle:
@@ -304,11 +333,20 @@ class Oracle:
     def conf_independent(self, num_tors, x):
         return self.L.orc_conf_independent(self.h, num_tors, x)
 
-    def dock_cpu(self, ligs, exh, seed, global_steps, local_steps, n_threads):
+    def set_ls_f0_zero(self, on):
+        """First line search of every BFGS against f0 = 0 (the reference GPU's
+        bfgs_warp) instead of f(x) (bfgs.h)."""
+        self.L.orc_set_ls_f0_zero(self.h, int(bool(on)))
+
+    def dock_cpu(self, ligs, exh, seed, global_steps, local_steps, n_threads, poses=False):
+        """Every chain of the ligands: best energies and the evaluation count
+        (and the best poses, len(ligs) * exh x 39, with poses=True)."""
         ligs = np.ascontiguousarray(ligs, dtype=np.int32)
         ls = np.ascontiguousarray(local_steps, dtype=np.int32)
         be = np.zeros(len(ligs) * exh)
         ev = ctypes.c_longlong()
+        P = np.zeros((len(ligs) * exh, POSE)) if poses else None
         self.L.orc_dock_cpu(self.h, len(ligs), _i(ligs), exh, seed, global_steps, _i(ls), n_threads,
-                            be.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(ev))
-        return be, ev.value
+                            be.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(ev),
+                            P.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if poses else None)
+        return (be, ev.value, P) if poses else (be, ev.value)

@@ -352,6 +352,7 @@ void ud_default_search_params(ud_search_params* p) {
     p->temperature = 1.2f;
     p->mutation_amplitude = 2.f;
     p->chains_per_group = 0;
+    p->first_ls_f0_zero = 0;
 }
 
 int ud_create(int device_id, void* stream, ud_ctx** out) {
@@ -1287,6 +1288,7 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
     }
     sp.temperature = p->temperature;
     sp.amplitude = p->mutation_amplitude;
+    sp.ls_f0_zero = p->first_ls_f0_zero ? 1 : 0;
     // size buckets by atom capacity; each bucket is one launch, largest first
     std::vector<std::vector<int>> bl(UD_N_BUCKETS);
     if (subset) {

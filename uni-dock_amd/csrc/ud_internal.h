@@ -155,6 +155,7 @@ struct SearchDev {
     float temperature;
     float amplitude;
     float box_lo[3], box_hi[3];
+    int32_t ls_f0_zero;  // first line search of each BFGS against f0 = 0 (bfgs_warp) instead of f(x)
 };
 
 // Capacity of one launch (dynamic LDS is sized from it)

@@ -1372,7 +1372,9 @@ __device__ void mutate(float* pose, int T, float rgyr, const SearchDev& sp, uint
 enum { ST_RGYR = 0, ST_INIT = 1, ST_TRIAL = 2, ST_IDLE = 3 };
 
 // BFGS local search (bfgs() with line_search and bfgs_update, bfgs.h:30-147;
-// device twin bfgs_warp, warp_ops.cuh:1051-1119) and, when MC is set, the
+// the reference's GPU BFGS is bfgs_warp, warp_ops.cuh:1051-1119, whose first
+// line search compares with f0 = 0 where bfgs.h:107 uses f(x): SearchDev's
+// ls_f0_zero selects that rule, off by default) and, when MC is set, the
 // Monte Carlo chain around it (kernel<32, Config>, monte_carlo.cu:236-356):
 // mutate -> BFGS with hunt_cap -> Metropolis at T -> if the accepted pose
 // beats the chain's best, BFGS with authentic_v and keep it.
@@ -1482,7 +1484,9 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         }
         if (conv || (cont && step >= lsteps)) finished = true;
         if (is_init) {
-            f0 = f;
+            // the first line search compares with f(x) (bfgs.h:107) or, as
+            // the reference's GPU BFGS, with 0 (warp_ops.cuh:1062-1076)
+            f0 = sp.ls_f0_zero ? 0.f : f;
             f_orig = f;
             rg_x = rg;
             rg_o = rg;

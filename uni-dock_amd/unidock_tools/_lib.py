@@ -65,7 +65,7 @@ class UdSearchParams(ctypes.Structure):
                 ("seed", ctypes.c_uint64),
                 ("hunt_cap", ctypes.c_float * 3), ("authentic_v", ctypes.c_float * 3),
                 ("temperature", ctypes.c_float), ("mutation_amplitude", ctypes.c_float),
-                ("chains_per_group", ctypes.c_int32)]
+                ("chains_per_group", ctypes.c_int32), ("first_ls_f0_zero", ctypes.c_int32)]
 
 
 _LIB = None
@@ -566,7 +566,8 @@ class Context:
                                     _p(vv, ctypes.c_float), max_steps, _p(e, ctypes.c_float), _p(ev, ctypes.c_int32)))
         return P, e, ev
 
-    def _params(self, exhaustiveness, global_steps, seed, local_steps, lanes, chains_per_group=0):
+    def _params(self, exhaustiveness, global_steps, seed, local_steps, lanes, chains_per_group=0,
+                first_ls_f0_zero=False):
         p = UdSearchParams()
         self.L.ud_default_search_params(ctypes.byref(p))
         p.exhaustiveness = exhaustiveness
@@ -575,11 +576,14 @@ class Context:
         p.local_steps = local_steps
         p.lanes_per_chain = lanes
         p.chains_per_group = chains_per_group
+        p.first_ls_f0_zero = int(bool(first_ls_f0_zero))
         return p
 
     def dock(self, exhaustiveness=8, global_steps=20, seed=181129, local_steps=0, lanes=0, fetch=True,
-             chains_per_group=0):
-        p = self._params(exhaustiveness, global_steps, seed, local_steps, lanes, chains_per_group)
+             chains_per_group=0, first_ls_f0_zero=False):
+        """ud_dock; first_ls_f0_zero selects the reference GPU's first-line-search
+        rule (f0 = 0) instead of the CPU rule (f0 = f(x))."""
+        p = self._params(exhaustiveness, global_steps, seed, local_steps, lanes, chains_per_group, first_ls_f0_zero)
         self._ck(self.L.ud_dock(self.h, ctypes.byref(p), None, None, None))
         self._last_dock = (self.n_ligands, exhaustiveness)
         if fetch:
