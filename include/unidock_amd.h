@@ -220,6 +220,21 @@ int ud_score_grad(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const fl
 int ud_score_grad_device(ud_ctx* ctx, int n_poses, const int32_t* d_pose_ligand, const float* d_poses,
                          const float* v, float* d_energy, float* d_grad);
 
+/* ud_score_grad_device for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The same kernel and results
+ * as ud_score_grad_device; its pose permutation and work list live in a
+ * static device array of the library (UD_NOALLOC_SCRATCH_BYTES per device,
+ * held until the process ends) instead of context buffers, and the call
+ * makes no hipMalloc / hipFree.  At most UD_NOALLOC_MAX_POSES poses
+ * (UD_ERR_CAPACITY beyond).  Calls that overlap on one device (other
+ * contexts or streams of that device) share the array and race: serialise
+ * them.  Maps and ligands must be set as for ud_score_grad_device. */
+#define UD_NOALLOC_MAX_POSES 65536
+#define UD_NOALLOC_SCRATCH_BYTES (UD_NOALLOC_MAX_POSES * 20)
+int ud_score_grad_device_noalloc(ud_ctx* ctx, int n_poses, const int32_t* pose_ligand, const float* d_poses,
+                                 const float* v, float* d_energy, float* d_grad);
+
 /* Energy of poses with its gradient in the pose's own coordinates and in
  * the scoring-term weights, for differentiable use (torch.autograd).  The
  * energy is model::eval_deriv's (model.cpp:954-981) at the pose with its

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <mutex>
 #include <vector>
 
 #include "../../include/unidock_amd.h"
@@ -355,6 +356,39 @@ void ud_default_search_params(ud_search_params* p) {
     p->first_ls_f0_zero = 0;
 }
 
+}  // extern "C"
+
+// ---- scratch of the allocation-free scoring entry point
+// The pose permutation (4 B per pose) then the work list (16 B per item, at
+// most one item per pose) of ud_score_grad_device_noalloc.  Host code must
+// not pass `scratch` itself to a kernel or a copy: its device address comes
+// from hipGetSymbolAddress, once per device, kept in g_scratch.
+static_assert(sizeof(WorkItem) == 16, "work items are 16 B");
+static_assert(UD_NOALLOC_SCRATCH_BYTES == size_t(UD_NOALLOC_MAX_POSES) * (4 + sizeof(WorkItem)), "scratch size");
+static __device__ __align__(256) unsigned char scratch[UD_NOALLOC_SCRATCH_BYTES];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// Resolve this device's copy of `scratch` (the device must be current).
+int scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return UD_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return UD_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return UD_ERR_HIP;
+    g_scratch[device] = p;
+    return UD_OK;
+}
+void* scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch[device] : nullptr;
+}
+}  // namespace
+
+extern "C" {
+
 int ud_create(int device_id, void* stream, ud_ctx** out) {
     if (!out) return UD_ERR_INVALID;
     *out = nullptr;
@@ -362,6 +396,8 @@ int ud_create(int device_id, void* stream, ud_ctx** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UD_ERR_HIP;
     if (device_id < 0 || device_id >= ndev) return UD_ERR_INVALID;
     if (hipSetDevice(device_id) != hipSuccess) return UD_ERR_HIP;
+    if (device_id >= kMaxDevices) return UD_ERR_INVALID;
+    if (scratch_init(device_id) != UD_OK) return UD_ERR_HIP;
     ud_ctx* c = new ud_ctx();
     c->device = device_id;
     c->stream = reinterpret_cast<hipStream_t>(stream);
@@ -1087,6 +1123,39 @@ int ud_score_grad_device(ud_ctx* c, int n_poses, const int32_t* pose_ligand, con
     if (rc) return rc;
     UD_HIP(c, hipEventRecord(c->ev0, c->stream));
     UD_HIP(c, launch_score(lanes, grid_format(c), lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
+                           d_poses, v[0], v[1], d_energy, d_grad, c->stream));
+    UD_HIP(c, hipEventRecord(c->ev1, c->stream));
+    c->last_launches = 1;
+    return UD_OK;
+}
+
+int ud_score_grad_device_noalloc(ud_ctx* c, int n_poses, const int32_t* pose_ligand, const float* d_poses,
+                                 const float* v, float* d_energy, float* d_grad) {
+    if (!c || !v) return UD_ERR_INVALID;
+    if (n_poses > UD_NOALLOC_MAX_POSES)
+        return fail(c, UD_ERR_CAPACITY, "ud_score_grad_device_noalloc takes at most " +
+                                            std::to_string(UD_NOALLOC_MAX_POSES) + " poses");
+    int rc = check_pose_args(c, n_poses, pose_ligand);
+    if (rc) return rc;
+    if (n_poses == 0) return UD_OK;
+    if (!d_poses || !d_energy || !d_grad) return fail(c, UD_ERR_INVALID, "null device pointer");
+    UD_HIP(c, hipSetDevice(c->device));
+    unsigned char* scr = static_cast<unsigned char*>(scratch_of(c->device));
+    if (!scr) return fail(c, UD_ERR_STATE, "scratch of this device is not resolved");
+    rc = refresh_slots(c);
+    if (rc) return rc;
+    LaunchCap cap = caps_all(c);
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : 16, cap);
+    std::vector<int32_t> perm;
+    std::vector<WorkItem> work;
+    make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
+    // the permutation and the work list overwrite the array's earlier contents
+    int32_t* d_perm = reinterpret_cast<int32_t*>(scr);
+    WorkItem* d_work = reinterpret_cast<WorkItem*>(scr + size_t(UD_NOALLOC_MAX_POSES) * 4);
+    UD_HIP(c, hipMemcpyAsync(d_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, c->stream));
+    UD_HIP(c, hipMemcpyAsync(d_work, work.data(), work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, c->stream));
+    UD_HIP(c, hipEventRecord(c->ev0, c->stream));
+    UD_HIP(c, launch_score(lanes, grid_format(c), lig_dev(c), grid_dev(c), cap, d_work, int(work.size()), d_perm,
                            d_poses, v[0], v[1], d_energy, d_grad, c->stream));
     UD_HIP(c, hipEventRecord(c->ev1, c->stream));
     c->last_launches = 1;

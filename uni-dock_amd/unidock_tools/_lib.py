@@ -22,6 +22,7 @@ UD_POSE_FLOATS = 7 + UD_MAX_TORSIONS
 UD_GRAD_FLOATS = 6 + UD_MAX_TORSIONS
 UD_WEIGHT_FLOATS = 7  # dE/dw per pose, in the order of the scoring weights
 UD_N_BUCKETS = 6
+UD_NOALLOC_MAX_POSES = 65536
 BUCKET_ATOMS = (16, 32, 48, 64, 96, 128)
 
 VINA_WEIGHTS = (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 50.0, 0.05846)  # vina.h:129-132
@@ -107,6 +108,7 @@ def load() -> ctypes.CDLL:
         "ud_set_ligands": (ctypes.c_int, [vp, ctypes.POINTER(UdLigandBatch)]),
         "ud_score_grad": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, fp, fp]),
         "ud_score_grad_device": (ctypes.c_int, [vp, ctypes.c_int, ip, vp, fp, vp, vp]),
+        "ud_score_grad_device_noalloc": (ctypes.c_int, [vp, ctypes.c_int, ip, vp, fp, vp, vp]),
         "ud_energy_grad": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, fp, fp, fp]),
         "ud_energy_grad_device": (ctypes.c_int, [vp, ctypes.c_int, ip, vp, fp, vp, vp, vp]),
         "ud_optimize": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, fp, ctypes.c_int, fp, ip]),
@@ -555,6 +557,16 @@ class Context:
         self._ck(self.L.ud_score_grad_device(self.h, len(pl), _p(pl, ctypes.c_int32), ctypes.c_void_p(d_poses),
                                              _p(vv, ctypes.c_float), ctypes.c_void_p(d_energy),
                                              ctypes.c_void_p(d_grad)))
+
+    def score_grad_device_noalloc(self, pose_ligand, d_poses: int, d_energy: int, d_grad: int,
+                                  v=(1000.0, 1000.0, 1000.0)):
+        """ud_score_grad_device_noalloc: as score_grad_device, with no device
+        allocation (at most UD_NOALLOC_MAX_POSES poses)."""
+        pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
+        vv = np.array(v, dtype=np.float32)
+        self._ck(self.L.ud_score_grad_device_noalloc(self.h, len(pl), _p(pl, ctypes.c_int32), ctypes.c_void_p(d_poses),
+                                                     _p(vv, ctypes.c_float), ctypes.c_void_p(d_energy),
+                                                     ctypes.c_void_p(d_grad)))
 
     def optimize(self, pose_ligand, poses, v=(1000.0, 1000.0, 1000.0), max_steps=0):
         pl = np.ascontiguousarray(pose_ligand, dtype=np.int32)
