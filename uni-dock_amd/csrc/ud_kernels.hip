@@ -851,45 +851,23 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
     return e;
 }
 
-// Same result as atoms_forces_2s with every pair evaluated once: pass A
-// evaluates the grid term of the lane's atoms and the pairs gl, gl + L, ...,
-// storing each pair's force scale d (dor with the curl factor; the force on
-// the pair's second atom is d (c_b - c_a)) in the group's per-pair buffer;
-// pass B adds to each atom's grid gradient d (c_i - c_other) for its pairs
-// in pair order (forces[a] -= f, forces[b] += f, model.cpp:923-931; the
-// difference is formed from the same coordinates, so both ends see exact
-// negatives).  Lanes stay balanced whatever the atoms' pair counts, and the
-// per-atom accumulation order is the reference's.
+// Same result as atoms_forces_2s with every pair evaluated once, in two
+// passes.  Pass A (atoms_forces_pp_a) evaluates the grid term of the lane's
+// atoms and the pairs gl, gl + L, ..., storing each pair's force scale d (dor
+// with the curl factor; the force on the pair's second atom is d (c_b - c_a))
+// in the group's per-pair buffer; it gives the energy.  Pass B
+// (atoms_forces_pp_b), needed only for the gradient, adds to each atom's grid
+// gradient d (c_i - c_other) for its pairs in pair order (forces[a] -= f,
+// forces[b] += f, model.cpp:923-931; the difference is formed from the same
+// coordinates, so both ends see exact negatives).  Lanes stay balanced
+// whatever the atoms' pair counts, and the per-atom accumulation order is the
+// reference's.
 template <int L, int NM, int GF>
-__device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter) {
+__device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter) {
     const float4* coords = s.coords();
     float4* mf = s.mf();
     float* pf = s.pf();
     const int N = s.h.n_atoms;
-    const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
-    const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
-    constexpr int kRec = kPartnerPad;  // partner records fetched together (lists are padded to this)
-    // partner range of the lane's first atom (and, with UD_PREFETCH_RECORDS,
-    // its first record batch), requested before pass A so that the latency
-    // overlaps it
-    const uint32_t ap0 = (s.gl < N) ? apl[s.gl] : 0u;
-    // (each atom's list starts at a multiple of kPartnerPad records, so a
-    // batch is two aligned 16-byte loads)
-    static_assert(kPartnerPad == 8, "record batches are loaded as two uint4");
-    auto load_batch = [&](uint32_t first_rec, uint32_t (&r)[kRec]) {
-        const uint4 lo = ld_at<uint4>(part, first_rec * 4u), hi = ld_at<uint4>(part, first_rec * 4u + 16u);
-        r[0] = lo.x; r[1] = lo.y; r[2] = lo.z; r[3] = lo.w;
-        r[4] = hi.x; r[5] = hi.y; r[6] = hi.z; r[7] = hi.w;
-    };
-    uint32_t pre[UD_PREFETCH_RECORDS ? kRec : 1];
-    if constexpr (UD_PREFETCH_RECORDS) {
-        if (ap0 >> 20) {
-            load_batch(ap0 & 0xFFFFFu, pre);
-        } else {
-#pragma unroll
-            for (int u = 0; u < kRec; ++u) pre[u] = 0u;
-        }
-    }
     // pass A's pair batches: r, table index and fraction, then the sample
     // loads (issue), then energy and force scale into pf (finish); the first
     // batch is issued before the grid term so that its table loads and the
@@ -986,24 +964,39 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         if (q0 != s.gl) issue(q0);
         e += finish(q0);
     }
-    __syncthreads();
+    return e;
+}
+
+// Pass B of the pair-parallel forces (after a barrier that completes pass A):
+// each owned atom's grid gradient plus its pairs' forces.
+template <int L, int NM, int GF>
+__device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, NM>& s) {
+    const float4* coords = s.coords();
+    float4* mf = s.mf();
+    const float* pf = s.pf();
+    const int N = s.h.n_atoms;
+    const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
+    const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
+    constexpr int kRec = kPartnerPad;  // partner records fetched together (lists are padded to this)
+    // (each atom's list starts at a multiple of kPartnerPad records, so a
+    // batch is two aligned 16-byte loads)
+    static_assert(kPartnerPad == 8, "record batches are loaded as two uint4");
+    auto load_batch = [&](uint32_t first_rec, uint32_t (&r)[kRec]) {
+        const uint4 lo = ld_at<uint4>(part, first_rec * 4u), hi = ld_at<uint4>(part, first_rec * 4u + 16u);
+        r[0] = lo.x; r[1] = lo.y; r[2] = lo.z; r[3] = lo.w;
+        r[4] = hi.x; r[5] = hi.y; r[6] = hi.z; r[7] = hi.w;
+    };
     for (int i = s.gl; i < N; i += L) {
         float4 g = mf[i];
         const float4 ci = coords[i];
-        const bool first = i == s.gl;
-        const uint32_t ap = first ? ap0 : apl[i];
+        const uint32_t ap = apl[i];
         const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
         // sum d (c_i - c_o) = c_i sum d - sum d c_o over the atom's records,
         // in whole batches (padding records carry d = 0)
         float sd = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
         for (int k = 0; k < kn; k += kRec) {
             uint32_t rec[kRec];
-            if (UD_PREFETCH_RECORDS && first && k == 0) {
-#pragma unroll
-                for (int u = 0; u < kRec; ++u) rec[u] = pre[UD_PREFETCH_RECORDS ? u : 0];
-            } else {
-                load_batch(uint32_t(k0 + k), rec);
-            }
+            load_batch(uint32_t(k0 + k), rec);
 #pragma unroll
             for (int u = 0; u < kRec; ++u) {
                 const float d = pf[(rec[u] >> 8) & 0x1FFFu];
@@ -1019,12 +1012,34 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
         g.z += ci.z * sd - sz;
         mf[i] = g;
     }
+}
+
+// Energy of the group's pose (model::eval_deriv, model.cpp:954-981, first
+// half): forward kinematics, the grid term and the intramolecular pairs.
+// Leaves in LDS what eval_grad needs: coordinates, each atom's grid gradient
+// (or, without the pair buffer, its whole force) and the per-pair force
+// scales.  rgyr (if not null) receives the gyration radius.
+template <int L, int NM, int GF>
+__device__ float eval_energy(const KArgs& ka, const Grp<L, NM>& s, const float* pose, float v_intra, float v_inter,
+                             float* rgyr = nullptr) {
+    fk_frames(s, pose);
+    float rg2 = 0.f;
+    atoms_coords(s, rg2);
+    __syncthreads();
+    float e = ka.pair_forces ? atoms_forces_pp_a<L, NM, GF>(ka, s, v_intra, v_inter)
+                             : atoms_forces_2s<L, NM, GF>(ka, s, v_intra, v_inter);
+    __syncthreads();
+    e = gsum<L>(e);
+    if (rgyr) {
+        // model::gyration_radius (model.cpp:871-883): heavy atoms about the root origin
+        rg2 = gsum<L>(rg2);
+        *rgyr = s.h.n_heavy > 0 ? __builtin_amdgcn_sqrtf(rg2 * __builtin_amdgcn_rcpf(float(s.h.n_heavy))) : 0.f;
+    }
     return e;
 }
 
-// Energy (+ gradient) of the group's pose: model::eval_deriv (model.cpp:954-981)
-// = grid term and intramolecular pairs per atom (atoms_forces), then the
-// torsion-tree derivative
+// Gradient of the pose evaluated by the last eval_energy (model::eval_deriv,
+// second half): the pair forces of pass B, then the torsion-tree derivative
 // (tree.h:75-84, 152-193).  Each node's force and torque are sums over the
 // atoms of its subtree (a contiguous range in pre-order), which is the
 // reference's bottom-up accumulation written out:
@@ -1034,26 +1049,14 @@ __device__ __forceinline__ float atoms_forces_pp(const KArgs& ka, const Grp<L, N
 // ligands sum each subtree directly.
 // gout[j] receives gradient element gl + j * L.
 template <int L, int NM, int GF>
-__device__ float eval_group(const KArgs& ka, const Grp<L, NM>& s, const float* pose, float v_intra, float v_inter,
-                            float (&gout)[Grp<L, NM>::KV], bool want_grad, float* rgyr = nullptr) {
-    fk_frames(s, pose);
+__device__ void eval_grad(const KArgs& ka, const Grp<L, NM>& s, float (&gout)[Grp<L, NM>::KV]) {
+    if (ka.pair_forces) {
+        atoms_forces_pp_b<L, NM, GF>(ka, s);
+        __syncthreads();
+    }
     float4* coords = s.coords();
     float4* mf = s.mf();
     const float* fr = s.frames();
-    float rg2 = 0.f;
-    atoms_coords(s, rg2);
-    __syncthreads();
-    float e = ka.pair_forces ? atoms_forces_pp<L, NM, GF>(ka, s, v_intra, v_inter)
-                             : atoms_forces_2s<L, NM, GF>(ka, s, v_intra, v_inter);
-    __syncthreads();
-    e = gsum<L>(e);
-    if (rgyr) {
-        // model::gyration_radius (model.cpp:871-883): heavy atoms about the root origin
-        rg2 = gsum<L>(rg2);
-        *rgyr = s.h.n_heavy > 0 ? __builtin_amdgcn_sqrtf(rg2 * __builtin_amdgcn_rcpf(float(s.h.n_heavy))) : 0.f;
-    }
-    __syncthreads();
-    if (!want_grad) return e;
     float* gbuf = s.bc(2);
     const int nn = s.h.n_nodes;
 #if UD_TORQUE_SCAN
@@ -1143,6 +1146,15 @@ __device__ float eval_group(const KArgs& ka, const Grp<L, NM>& s, const float* p
         gout[j] = (idx < n) ? gbuf[idx] : 0.f;
     }
     __syncthreads();
+}
+
+// Energy and gradient of the group's pose (model::eval_deriv).  gout[j]
+// receives gradient element gl + j * L.
+template <int L, int NM, int GF>
+__device__ float eval_group(const KArgs& ka, const Grp<L, NM>& s, const float* pose, float v_intra, float v_inter,
+                            float (&gout)[Grp<L, NM>::KV], float* rgyr = nullptr) {
+    const float e = eval_energy<L, NM, GF>(ka, s, pose, v_intra, v_inter, rgyr);
+    eval_grad<L, NM, GF>(ka, s, gout);
     return e;
 }
 
@@ -1292,7 +1304,7 @@ __global__ __launch_bounds__(64) void score_kernel(KArgs ka, const WorkItem* wor
     for (int i = s.gl; i < np; i += L) x[i] = active ? poses[size_t(pi) * UD_POSE_FLOATS + i] : (i == 3 ? 1.f : 0.f);
     __syncthreads();
     float gout[Grp<L>::KV];
-    const float e = eval_group<L, kNMax, GF>(ka, s, x, v_intra, v_inter, gout, true);
+    const float e = eval_group<L, kNMax, GF>(ka, s, x, v_intra, v_inter, gout);
     if (active) {
         if (s.gl == 0) e_out[pi] = e;
 #pragma unroll
@@ -1458,13 +1470,27 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         // ---------------------------------------------------------------- evaluate
         const bool trialing = state == ST_TRIAL;
         float rg = 0.f;
-        const float f = eval_group<L, NM, GF>(ka, s, trialing ? XN : X, vI, vE, t, true, &rg);
+        const float f = eval_energy<L, NM, GF>(ka, s, trialing ? XN : X, vI, vE, &rg);
         if (state != ST_IDLE) ++evals;
-        // ---------------------------------------------------------------- line search / BFGS bookkeeping
-        const float gg = gdot<L>(g, g);
         const bool is_init = state == ST_INIT;
         const bool ok = trialing && (f - f0 < 0.0001f * alpha * pg);
         const bool ls_end = trialing && (ok || trial == 9);  // 10 trials (bfgs.h:70-83)
+        // Most evaluations are line-search trials that fail the Armijo test
+        // before the last one: their gradient is never used and their group's
+        // state only halves alpha.  When that holds for every group of the
+        // wavefront, take the next trial at once.
+        if (!wave_any(is_init || ls_end || state == ST_RGYR)) {
+            if (trialing) {
+                alpha *= 0.5f;
+                ++trial;
+            }
+            pose_increment(s, XN, X, pb, alpha, trialing);  // pb still holds each trialing group's direction
+            __syncthreads();
+            continue;
+        }
+        if (wave_any(is_init || ls_end)) eval_grad<L, NM, GF>(ka, s, t);
+        // ---------------------------------------------------------------- line search / BFGS bookkeeping
+        const float gg = gdot<L>(g, g);
         if (trialing && !ok) alpha *= 0.5f;
         if (trialing && !ls_end) ++trial;
         const int step_now = step;
@@ -1865,7 +1891,7 @@ __global__ __launch_bounds__(64) void energy_grad_kernel(KArgs ka, TermsDev td, 
         for (int j = 0; j < 4; ++j) x[3 + j] = qh[j];
     __syncthreads();
     float gout[Grp<L>::KV];
-    const float e = eval_group<L, kNMax, GF>(ka, s, x, v_intra, v_inter, gout, true);
+    const float e = eval_group<L, kNMax, GF>(ka, s, x, v_intra, v_inter, gout);
     const float* gb = s.bc(2);  // change-space gradient left by eval_group
     if (active) {
         const float tx = gb[3], ty = gb[4], tz = gb[5];
