@@ -6,13 +6,21 @@ synthetic ligands (10-40 heavy atoms, 0-10 torsions), search box 22.5 A at
 0.375 A (61^3 points per map), Vina scoring, exhaustiveness 128, max_step 20.
 
 A step is one ud_dock call over the whole batch with every input already
-resident on the device.  Two forms are timed:
-  batched  FP16 grid maps (per-map power-of-two scale, FP32 arithmetic),
-           lanes per chain chosen per size bucket           -> `value`
-  plain    FP32 grid maps, one chain per 64-lane wavefront
-The best pose of sampled ligands is re-scored by the FP64 oracle to report
-the FP16 form's energy error.  The CPU baseline is the FP32 C++ restatement
-of the reference's CPU global search on a bounded sample of the same batch.
+resident on the device.  Three forms are timed:
+  batched  FP16 grid maps (per-map power-of-two scale, FP32 arithmetic), one
+           launch per size bucket, lanes per chain autotuned per bucket,
+           bucket launches concurrent                         -> `value`
+  plain    the reference's launch shape: FP32 grid maps, one launch over the
+           whole batch sized for its largest ligand, 32 lanes per chain
+           (monte_carlo.cu:2877-2882 launches 32 threads per chain)
+  plain64  FP32 grid maps, size buckets, 64 lanes per chain
+Post-processing of the batched form (clustering, final score, and with
+refine_step 3 the refinement against the explicit receptor) is timed per
+batch after the docking steps.  The best pose of sampled ligands is
+re-scored by the FP64 oracle to report the FP16 form's energy error.  The
+CPU baseline is the FP32 C++ restatement of the reference's CPU global
+search, built -O3 -march=native on the measuring host, on a bounded sample
+of the same batch.
 
 --config 2 / 3 / 4 run the other benchmark configurations instead, each
 printing its own JSON line:
@@ -31,6 +39,7 @@ printing its own JSON line:
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -43,6 +52,8 @@ sys.path.insert(0, ROOT)
 from unidock_tools import _lib, synth  # noqa: E402
 
 HBM_PEAK = 8.0e12
+FP32_PEAK = 157.3e12        # MI355X published dense FP32 vector peak (packed FP32 instructions)
+FP32_PEAK_UNPACKED = 78.6e12  # without packed FP32 instructions, which this build does not emit
 # HBM bytes per search-kernel launch measured by FETCH_SIZE / WRITE_SIZE
 # counter runs of this benchmark (copy-kernel calibrated), one table row per
 # kernel and workgroup count
@@ -75,16 +86,18 @@ def setup_ctx(rec, ligs, precision):
     return ctx
 
 
-def time_form(ctx, n_lig, exh, gsteps, lanes, steps, warmup, seed):
+def time_form(ctx, n_lig, exh, gsteps, lanes, steps, warmup, seed, single_launch=False):
     for _ in range(warmup):
-        ctx.dock(exhaustiveness=exh, global_steps=gsteps, seed=seed, lanes=lanes, fetch=False)
+        ctx.dock(exhaustiveness=exh, global_steps=gsteps, seed=seed, lanes=lanes, fetch=False,
+                 single_launch=single_launch)
     if warmup:
         ctx.fetch_dock()
     kms = 0.0
     per_launch = None
     t0 = time.perf_counter()
     for _ in range(steps):
-        ctx.dock(exhaustiveness=exh, global_steps=gsteps, seed=seed, lanes=lanes, fetch=False)
+        ctx.dock(exhaustiveness=exh, global_steps=gsteps, seed=seed, lanes=lanes, fetch=False,
+                 single_launch=single_launch)
         P, e, ev = ctx.fetch_dock()
         ms, nl = ctx.last_kernel_time()
         kms += ms
@@ -99,6 +112,24 @@ def time_form(ctx, n_lig, exh, gsteps, lanes, steps, warmup, seed):
         a["ms"] /= steps
     return dict(wall=wall, ligs_per_s=n_lig * steps / wall, kernel_ms=kms / steps, launches=per_launch,
                 poses=P, energies=e, evals=ev)
+
+
+def energy_flop(lig):
+    """FP32 operations of the energy part of one evaluation in the search
+    kernel (eval_energy), which every evaluation performs, counted from the
+    arithmetic of each part: forward kinematics 31 for the root frame and 129
+    per torsion node (origin and axis from the parent frame 33, quaternion
+    product 31, normalisation 8, rotation matrix 31, half-angle sine/cosine
+    26); coordinates and gyration sum 27 per atom; the grid term 60 per typed
+    atom (cell coordinates 6, out-of-box penalty 6, trilinear value 21,
+    gradient 12, map scale 4, curl 8, grid factor 3); pair pass A 23 per pair
+    (difference 3, r^2 5, sample index 2, value and force-scale interpolation
+    6, curl 6, sum 1).  Evaluations whose line-search trial is accepted also
+    run the gradient part (pass B 14 per pair and 6 per atom, torque 12 per
+    atom and 26 per node), not counted here: the figure is a lower bound."""
+    n_typed = int(np.sum(np.asarray(lig.xs) >= 0))
+    n_pairs = len(np.asarray(lig.pairs).reshape(-1, 2))
+    return 31 + 129 * lig.n_tors + 27 * lig.n_atoms + 60 * n_typed + 23 * n_pairs
 
 
 def algorithmic_bytes(ligs, exh, evals, bytes_per_value, sel=None):
@@ -116,8 +147,24 @@ def bucket_mask(ligs, atom_cap):
     return np.array([lo < l.n_atoms <= atom_cap for l in ligs])
 
 
+def cpu_threads():
+    """Host threads of the CPU baseline: the cores this process may run on,
+    capped at the host share the GPU box gives one GPU (OMP_NUM_THREADS there,
+    16 by default)."""
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(len(os.sched_getaffinity(0)), share))
+
+
 def cpu_baseline(rec, ligs, exh, gsteps, n_sample, threads):
+    import tempfile
+    from oracle import oracle
     from oracle.oracle import Oracle
+    flags = "-O3 -march=native"
+    try:
+        oracle.load(oracle.build_native(tempfile.mkdtemp(prefix="oracle_native_")))
+    except (OSError, subprocess.CalledProcessError) as exc:  # no compiler on this host: the in-tree -O3 build
+        log(f"[bench] native oracle build failed ({exc}); timing the in-tree -O3 build")
+        flags = "-O3"
     o = Oracle(32, 0)
     o.set_grid(rec.cavity, np.array([22.5, 22.5, 22.5]), 0.375)
     o.build_maps(rec.xyz, rec.xs, LIGAND_TYPES)
@@ -130,8 +177,21 @@ def cpu_baseline(rec, ligs, exh, gsteps, n_sample, threads):
     wall = time.perf_counter() - t0
     return dict(value=n_sample / wall, unit="ligands/s", cores=threads, kind="port",
                 sample=f"{n_sample} ligands of the batch x exhaustiveness {exh} x {gsteps} MC steps, "
-                       f"FP32 C++ restatement of the reference CPU global search, {wall:.1f} s wall",
+                       f"FP32 C++ restatement of the reference CPU global search (g++ {flags}), {wall:.1f} s wall",
                 evals_per_s=ev / wall)
+
+
+def time_postprocess(ctx, refine_step, max_modes=9):
+    """Wall time of ud_postprocess over the last dock's batch (clustering,
+    refinement when refine_step > 0, final scores, RMSD bounds)."""
+    import ctypes
+    n = ctx.n_ligands
+    modes = (_lib.UdMode * (n * max_modes))()
+    cnt = np.zeros(n, dtype=np.int32)
+    t0 = time.perf_counter()
+    ctx._ck(ctx.L.ud_postprocess(ctx.h, max_modes, 1.0, refine_step, modes,
+                                 cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return time.perf_counter() - t0
 
 
 def fp64_error(rec, ligs, ctx16, res, exh, n_sample):
@@ -452,12 +512,19 @@ def main():
     b = time_form(ctx16, len(ligs), exh, gsteps, args.lanes, args.steps, args.warmup, 181129)
     log(f"[bench] batched: {b['ligs_per_s']:.1f} ligands/s, kernel {b['kernel_ms']:.1f} ms/step, "
         f"launches {b['launches']}")
-    plain = None
+    post = {}
+    for rs in (0, 3):
+        post[f"refine_step_{rs}"] = time_postprocess(ctx16, rs)
+    log(f"[bench] post-processing per batch: {post}")
+    plain = plain64 = None
     if not args.no_plain:
         ctx32 = setup_ctx(rec, ligs, _lib.UD_GRID_F32)
-        log("[bench] plain FP32 form")
-        plain = time_form(ctx32, len(ligs), exh, gsteps, 64, args.steps, args.warmup, 181129)
+        log("[bench] plain FP32 form (one launch per batch, 32 lanes per chain)")
+        plain = time_form(ctx32, len(ligs), exh, gsteps, 32, args.steps, args.warmup, 181129, single_launch=True)
         log(f"[bench] plain: {plain['ligs_per_s']:.1f} ligands/s, kernel {plain['kernel_ms']:.1f} ms/step")
+        log("[bench] plain64 FP32 form (size buckets, 64 lanes per chain)")
+        plain64 = time_form(ctx32, len(ligs), exh, gsteps, 64, args.steps, args.warmup, 181129)
+        log(f"[bench] plain64: {plain64['ligs_per_s']:.1f} ligands/s, kernel {plain64['kernel_ms']:.1f} ms/step")
         ctx32.close()
 
     evals = float(b["evals"].sum())
@@ -469,13 +536,21 @@ def main():
     dom_kernel = f"void ud::search_kernel<{dom['lanes']}, 1, true, 16>"
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
                 traffic=measured_traffic(dom_kernel, dom["n_workgroups"]))
+    # issue roofline of the same launch: FP32 operations of the evaluations'
+    # energy part (a lower bound of the work done) per second vs the FP32 peak
+    ev_dom = b["evals"].reshape(len(ligs), exh)[sel]
+    flop_dom = float(np.sum(ev_dom.sum(axis=1) * np.array([energy_flop(l) for l, s_ in zip(ligs, sel) if s_])))
+    fl = flop_dom / (dom["ms"] * 1e-3)
+    issue = dict(bound="valu", achieved=fl / 1e12, peak=FP32_PEAK / 1e12, unit="TFLOP/s", frac=fl / FP32_PEAK,
+                 frac_unpacked=fl / FP32_PEAK_UNPACKED, flop_per_eval_mean=flop_dom / float(ev_dom.sum()),
+                 counted="energy part of each evaluation (lower bound)")
     log(f"[bench] dominant launch: bucket <= {dom['atom_cap']} atoms, {dom['lanes']} lanes, "
         f"{dom['n_ligands']} ligands, {dom['ms']:.2f} ms, {bytes_dom / 1e6:.1f} MB algorithmic")
     err_max, err_mean = fp64_error(rec, ligs, ctx16, b, exh, 64)
     log(f"[bench] FP16 best-pose energy error vs FP64: max {err_max:.4f} mean {err_mean:.5f} kcal/mol")
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         log(f"[bench] CPU baseline on {threads} threads")
         cpu = cpu_baseline(rec, ligs, exh, gsteps, args.cpu_sample, threads)
         log(f"[bench] cpu: {cpu['value']:.3f} ligands/s")
@@ -500,6 +575,7 @@ def main():
             "lanes_per_chain": args.lanes or "autotuned per size bucket",
         },
         "roofline": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in roof.items()},
+        "issue_roofline": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in issue.items()},
         "cpu_baseline": cpu,
         "evals_per_s": evals / (b["kernel_ms"] * 1e-3),
         "kernel_ms_per_step": b["kernel_ms"],
@@ -509,7 +585,12 @@ def main():
                             "bucket_atoms": dom["atom_cap"], "avg_ms": dom["ms"],
                             "algorithmic_bytes": bytes_dom},
         "plain_fp32_ligands_per_s": plain["ligs_per_s"] if plain else None,
+        "plain_form": "FP32 maps, one launch over the batch sized for its largest ligand, 32 lanes per chain",
         "speedup_vs_plain": (b["ligs_per_s"] / plain["ligs_per_s"]) if plain else None,
+        "plain64_fp32_ligands_per_s": plain64["ligs_per_s"] if plain64 else None,
+        "speedup_vs_plain64": (b["ligs_per_s"] / plain64["ligs_per_s"]) if plain64 else None,
+        "postprocess_s_per_batch": post,
+        "end_to_end_ligands_per_s": {k: len(ligs) / (b["wall"] / args.steps + v) for k, v in post.items()},
         "fp16_best_pose_err_vs_fp64_kcal": {"max": err_max, "mean": err_mean, "sample": 64},
     }
     print(json.dumps(out))

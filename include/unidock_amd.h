@@ -151,6 +151,10 @@ typedef struct {
                                     f0 = 0, as the reference's GPU BFGS does (bfgs_warp,
                                     warp_ops.cuh:1062-1076); 0: with f0 = f(x), the CPU rule
                                     (bfgs.h:107) */
+    int32_t single_launch;       /* 1: one launch over the whole batch with capacities of its
+                                    largest ligand, as the reference launches one kernel per
+                                    batch (monte_carlo.cu:2877-2882); 0: one launch per size
+                                    bucket */
 } ud_search_params;
 
 /* fill *p with the reference defaults (exhaustiveness 8, global_steps 20) */

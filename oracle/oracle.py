@@ -26,16 +26,29 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def build_native(out_dir):
+    """An -O3 -march=native build of oracle.cpp for the machine it runs on (the
+    timed CPU baseline); returns the library path."""
+    out = os.path.join(out_dir, "liboracle_native.so")
+    subprocess.run(["g++", "-O3", "-march=native", "-std=c++17", "-fPIC", "-shared", "-o", out,
+                    os.path.join(HERE, "oracle.cpp"), "-lpthread"], check=True)
+    return out
+
+
 _L = None
+_L_PATH = None
 
 
-def load():
-    global _L
-    if _L is not None:
+def load(path=None):
+    """Load the oracle library (the in-tree build, or `path`); later Oracle
+    objects use the library loaded last."""
+    global _L, _L_PATH
+    path = path or _L_PATH or LIB
+    if _L is not None and path == _L_PATH:
         return _L
-    if not os.path.exists(LIB):
+    if path == LIB and not os.path.exists(LIB):
         build()
-    L = ctypes.CDLL(LIB)
+    L = ctypes.CDLL(path)
     vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
     u32p = ctypes.POINTER(ctypes.c_uint32)
     sig = {
@@ -84,6 +97,7 @@ def load():
         f.restype = r
         f.argtypes = a
     _L = L
+    _L_PATH = path
     return L
 
 

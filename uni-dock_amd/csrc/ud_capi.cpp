@@ -354,6 +354,7 @@ void ud_default_search_params(ud_search_params* p) {
     p->mutation_amplitude = 2.f;
     p->chains_per_group = 0;
     p->first_ls_f0_zero = 0;
+    p->single_launch = 0;
 }
 
 }  // extern "C"
@@ -1361,9 +1362,9 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
     // size buckets by atom capacity; each bucket is one launch, largest first
     std::vector<std::vector<int>> bl(UD_N_BUCKETS);
     if (subset) {
-        for (int l : *subset) bl[bucket_of(c->lig_nat[l])].push_back(l);
+        for (int l : *subset) bl[p->single_launch ? UD_N_BUCKETS - 1 : bucket_of(c->lig_nat[l])].push_back(l);
     } else {
-        for (int l = 0; l < c->n_lig; ++l) bl[bucket_of(c->lig_nat[l])].push_back(l);
+        for (int l = 0; l < c->n_lig; ++l) bl[p->single_launch ? UD_N_BUCKETS - 1 : bucket_of(c->lig_nat[l])].push_back(l);
     }
     std::vector<std::vector<WorkItem>> works;
     std::vector<int> lanes_v;

@@ -66,7 +66,8 @@ class UdSearchParams(ctypes.Structure):
                 ("seed", ctypes.c_uint64),
                 ("hunt_cap", ctypes.c_float * 3), ("authentic_v", ctypes.c_float * 3),
                 ("temperature", ctypes.c_float), ("mutation_amplitude", ctypes.c_float),
-                ("chains_per_group", ctypes.c_int32), ("first_ls_f0_zero", ctypes.c_int32)]
+                ("chains_per_group", ctypes.c_int32), ("first_ls_f0_zero", ctypes.c_int32),
+                ("single_launch", ctypes.c_int32)]
 
 
 _LIB = None
@@ -579,7 +580,7 @@ class Context:
         return P, e, ev
 
     def _params(self, exhaustiveness, global_steps, seed, local_steps, lanes, chains_per_group=0,
-                first_ls_f0_zero=False):
+                first_ls_f0_zero=False, single_launch=False):
         p = UdSearchParams()
         self.L.ud_default_search_params(ctypes.byref(p))
         p.exhaustiveness = exhaustiveness
@@ -589,13 +590,17 @@ class Context:
         p.lanes_per_chain = lanes
         p.chains_per_group = chains_per_group
         p.first_ls_f0_zero = int(bool(first_ls_f0_zero))
+        p.single_launch = int(bool(single_launch))
         return p
 
     def dock(self, exhaustiveness=8, global_steps=20, seed=181129, local_steps=0, lanes=0, fetch=True,
-             chains_per_group=0, first_ls_f0_zero=False):
+             chains_per_group=0, first_ls_f0_zero=False, single_launch=False):
         """ud_dock; first_ls_f0_zero selects the reference GPU's first-line-search
-        rule (f0 = 0) instead of the CPU rule (f0 = f(x))."""
-        p = self._params(exhaustiveness, global_steps, seed, local_steps, lanes, chains_per_group, first_ls_f0_zero)
+        rule (f0 = 0) instead of the CPU rule (f0 = f(x)); single_launch runs the
+        whole batch as one launch (the reference's launch shape) instead of one
+        launch per size bucket."""
+        p = self._params(exhaustiveness, global_steps, seed, local_steps, lanes, chains_per_group, first_ls_f0_zero,
+                         single_launch)
         self._ck(self.L.ud_dock(self.h, ctypes.byref(p), None, None, None))
         self._last_dock = (self.n_ligands, exhaustiveness)
         if fetch:
