@@ -5,7 +5,9 @@ check score ranges), so each oracle function is pinned by what the mathematics
 fixes: published Philox known-answer vectors, hand-computed potential values,
 affine fields that trilinear interpolation must reproduce exactly, geometric
 invariants of forward kinematics, and central finite differences of the
-energy.  Committed FP64 golden vectors (tests/golden/) freeze the result.
+energy.  (tests/golden/ holds the reference's own 1iep / 1a30 input fixtures,
+used by the ingestion and redock tests; the search functions are pinned in
+test_oracle_search.py.)
 """
 import math
 

@@ -83,7 +83,7 @@ def run_unidock(
     ("REMARK <file> <affinity>" per ligand) and returns the inputs with their
     scores; local_only optimises the input pose instead of searching."""
     if scoring.lower() not in SCORING:
-        raise NotImplementedError(f"scoring '{scoring}' is not built (vina and vinardo are)")
+        raise NotImplementedError(f"scoring '{scoring}' is not built (vina, vinardo and ad4 are)")
     if bias_file is not None or multi_bias:
         raise NotImplementedError("biased docking is not built")
     if search_mode:
