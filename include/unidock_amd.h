@@ -49,11 +49,12 @@ enum { UD_SF_VINA = 0, UD_SF_VINARDO = 1, UD_SF_AD4 = 2 };
  * interpolation and accumulation is FP32 either way */
 enum { UD_GRID_F32 = 0, UD_GRID_F16 = 1 };
 
-/* capacities (the reference's largest bucket is 300 atoms / 48 torsions,
- * unidock/src/cuda/kernel.h:242-277; exceeding ours returns UD_ERR_CAPACITY,
- * never undefined behaviour) */
-#define UD_MAX_ATOMS 128
-#define UD_MAX_TORSIONS 32
+/* capacities: the atoms and torsions of the reference's largest size class
+ * (MaxConfig: 300 atoms, 48 torsions, 4,096 pairs, unidock/src/cuda/kernel.h:
+ * 242-277), and more pairs; exceeding them returns UD_ERR_CAPACITY, never
+ * undefined behaviour */
+#define UD_MAX_ATOMS 300
+#define UD_MAX_TORSIONS 48
 #define UD_MAX_PAIRS 8128
 #define UD_MAX_MAPS 32
 #define UD_MAX_GRID_POINTS (1 << 24)
@@ -72,10 +73,10 @@ enum { UD_GRID_F32 = 0, UD_GRID_F16 = 1 };
 #define UD_WEIGHT_FLOATS 7
 
 /* ud_dock groups ligands into size buckets by heavy+hydrogen atom count
- * (<= 16, 32, 48, 64, 96, 128 atoms) and launches each bucket once; the
+ * (<= 16, 32, 48, 64, 96, 128, 192, 300 atoms) and launches each bucket once; the
  * reference groups ligands by size class too (SmallConfig / MediumConfig /
  * LargeConfig ..., kernel.h:242-277, chosen in main.cpp:267-275). */
-#define UD_N_BUCKETS 6
+#define UD_N_BUCKETS 8
 
 /* Timing of one launch of the last ud_dock (ud_last_launches). */
 typedef struct ud_launch_info {

@@ -48,6 +48,9 @@ namespace orc {
 static const double kEpsFl = 1.1920928955078125e-07;
 static const double kMaxFl = 3.4028234663852886e+38;
 static const double kPi = 3.1415926535897931;  // common.h:204 (stored as float there)
+// pose stride of the flat pose arrays exchanged with tests and the product's
+// C ABI: position 3, quaternion 4, torsions 48 (UD_POSE_FLOATS)
+static const int kPose = 7 + 48;
 
 enum { XS_C_H = 0, XS_C_P, XS_N_P, XS_N_D, XS_N_A, XS_N_DA, XS_O_P, XS_O_D, XS_O_A, XS_O_DA,
        XS_S_P, XS_P_P, XS_F_H, XS_Cl_H, XS_Br_H, XS_I_H, XS_Si, XS_At, XS_Met_D,
@@ -1659,7 +1662,7 @@ extern "C" double orc_conf_independent(OrcHandle* h, double num_tors, double x) 
 #include <atomic>
 #include <thread>
 // energies of many poses (FP64 or FP32 per the handle), n_threads threads;
-// poses[i * 39] in the pose layout, ligs[i] the ligand of pose i
+// poses[i * kPose] in the pose layout, ligs[i] the ligand of pose i
 extern "C" void orc_eval_batch(OrcHandle* h, int n, const int* ligs, const double* poses, const double* v,
                                int n_threads, double* e_out) {
     for (int i = 0; i < n && h->d.sf != SF_AD4; ++i) {  // build the lazily made type-pair tables first
@@ -1670,8 +1673,8 @@ extern "C" void orc_eval_batch(OrcHandle* h, int n, const int* ligs, const doubl
     auto work = [&] {
         for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) {
             std::vector<double> g(6 + 64);
-            e_out[i] = (h->prec == 64) ? eval_impl(h->d, ligs[i], poses + size_t(i) * 39, v, g.data(), nullptr, nullptr)
-                                       : eval_impl(h->f, ligs[i], poses + size_t(i) * 39, v, g.data(), nullptr, nullptr);
+            e_out[i] = (h->prec == 64) ? eval_impl(h->d, ligs[i], poses + size_t(i) * kPose, v, g.data(), nullptr, nullptr)
+                                       : eval_impl(h->f, ligs[i], poses + size_t(i) * kPose, v, g.data(), nullptr, nullptr);
         }
     };
     std::vector<std::thread> th;
@@ -1681,7 +1684,7 @@ extern "C" void orc_eval_batch(OrcHandle* h, int n, const int* ligs, const doubl
 
 // Every chain of n_lig ligands (chain ids ligand_position * exh + k), on
 // n_threads threads: best energies, evaluation count and (when best_pose_out
-// is not NULL) the best pose of each chain, 39 values per chain.
+// is not NULL) the best pose of each chain, kPose values per chain.
 extern "C" double orc_dock_cpu(OrcHandle* h, int n_lig, const int* ligs, int exh, uint64_t seed, int gsteps,
                                const int* lsteps, int n_threads, double* best_e_out, long long* evals_out,
                                double* best_pose_out) {
@@ -1702,7 +1705,7 @@ extern "C" double orc_dock_cpu(OrcHandle* h, int n_lig, const int* ligs, int exh
             else e = mc_impl(h->f, lig, seed, uint32_t(li * exh + ci), gsteps, lsteps[li], pose.data(), &ec, nullptr);
             be[t] = e;
             evals[tid] += ec;
-            if (best_pose_out) std::copy(pose.begin(), pose.begin() + 39, best_pose_out + size_t(t) * 39);
+            if (best_pose_out) std::copy(pose.begin(), pose.begin() + kPose, best_pose_out + size_t(t) * kPose);
         }
     };
     // tables are built lazily; build them for all pairs used before threading

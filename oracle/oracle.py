@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
-POSE = 7 + 32  # pose stride shared with the product's C ABI
+POSE = 7 + 48  # pose stride shared with the product's C ABI (UD_POSE_FLOATS)
 SF_VINA, SF_VINARDO, SF_AD4 = 0, 1, 2
 AD4_WEIGHTS = (0.1662, 0.1209, 0.1406, 0.1322, 50.0, 0.2983)  # vina.h:136-138
 AD4_MAP_ELEC, AD4_MAP_DSOLV = 35, 36  # map indices of the electrostatic / desolvation maps
@@ -253,7 +253,7 @@ class Oracle:
         n_t = self.ligs[lig].n_tors
         p = np.zeros(POSE)
         p[:len(pose)] = pose
-        g = np.zeros(6 + 32)
+        g = np.zeros(6 + 48)
         coords = np.zeros(3 * self.ligs[lig].n_atoms)
         terms = np.zeros(2)
         e = self.L.orc_eval_deriv(self.h, lig, _d(p), _d(v), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
@@ -303,7 +303,7 @@ class Oracle:
         return a.value, b.value
 
     def eval_batch(self, ligs, poses, v=(1000.0, 1000.0, 1000.0), n_threads=16):
-        """Energies of many poses (poses[i] in the 39-float layout) on n_threads threads."""
+        """Energies of many poses (poses[i] in the POSE-float layout) on n_threads threads."""
         lg = np.ascontiguousarray(ligs, dtype=np.int32)
         P = np.zeros((len(lg), POSE))
         P[:, :np.asarray(poses).shape[1]] = poses
@@ -340,7 +340,7 @@ class Oracle:
         n_t = self.ligs[lig].n_tors
         p = np.zeros(POSE)
         p[:len(pose)] = pose
-        g = np.zeros(6 + 32)
+        g = np.zeros(6 + 48)
         e = self.L.orc_eval_deriv_nc(self.h, lig, _d(p), _d(v), slope, g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return e, g[:6 + n_t]
 
@@ -354,7 +354,7 @@ class Oracle:
 
     def dock_cpu(self, ligs, exh, seed, global_steps, local_steps, n_threads, poses=False):
         """Every chain of the ligands: best energies and the evaluation count
-        (and the best poses, len(ligs) * exh x 39, with poses=True)."""
+        (and the best poses, len(ligs) * exh x POSE, with poses=True)."""
         ligs = np.ascontiguousarray(ligs, dtype=np.int32)
         ls = np.ascontiguousarray(local_steps, dtype=np.int32)
         be = np.zeros(len(ligs) * exh)

@@ -69,7 +69,7 @@ def oracle_scene(prec=64, sf=0, size=BOX, with_maps=True):
 
 def random_poses(rng, lig, center, spread, n):
     """Poses with the root inside the box, uniform orientation, torsions U(-pi, pi)."""
-    out = np.zeros((n, 7 + 32))
+    out = np.zeros((n, 7 + 48))  # UD_POSE_FLOATS
     for i in range(n):
         out[i, 0:3] = center + rng.uniform(-spread, spread, 3)
         q = rng.normal(size=4)

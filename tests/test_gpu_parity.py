@@ -355,7 +355,7 @@ def test_capacity_and_state_errors(scene):
     assert ei.value.code == _lib.UD_ERR_STATE
     big = synth.make_ligand(24, 5, seed=1)
     arr = synth.LigandBatch([big]).arrays()
-    arr["atom_offset"] = np.array([0, 200], dtype=np.int32)
+    arr["atom_offset"] = np.array([0, _lib.UD_MAX_ATOMS + 1], dtype=np.int32)
     with pytest.raises(_lib.UdError) as ei:
         ctx.set_ligands(arr)
     assert ei.value.code == _lib.UD_ERR_CAPACITY

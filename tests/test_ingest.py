@@ -105,7 +105,7 @@ def test_identity_pose_reproduces_input(lig):
     o = Oracle(64, 0)
     o.set_grid(lig.origin[0], np.array([30.0, 30.0, 30.0]), 0.375)
     o.add_ligand(lig)
-    pose = np.zeros(39)
+    pose = np.zeros(_lib.UD_POSE_FLOATS)
     pose[0:3] = lig.origin[0]
     pose[3] = 1.0
     _, _, coords, _ = o.eval_deriv(0, pose)
@@ -241,7 +241,7 @@ def test_sdf_identity_pose_reproduces_input(sdf_lig):
     o = Oracle(64, 0)
     o.set_grid(lig.origin[0], np.array([30.0, 30.0, 30.0]), 0.375)
     o.add_ligand(lig)
-    pose = np.zeros(39)
+    pose = np.zeros(_lib.UD_POSE_FLOATS)
     pose[0:3] = lig.origin[0]
     pose[3] = 1.0
     _, _, coords, _ = o.eval_deriv(0, pose)

@@ -157,7 +157,7 @@ def test_forward_kinematics_invariants():
     o, ligs, _ = oracle_scene(64, with_maps=False)
     rng = np.random.default_rng(2)
     for li, lig in enumerate(ligs):
-        zero = np.zeros(7 + 32)
+        zero = np.zeros(7 + 48)  # UD_POSE_FLOATS
         zero[0:3] = lig.origin[0]
         zero[3] = 1.0
         _, _, coords, _ = o.eval_deriv(li, zero)

@@ -176,7 +176,7 @@ struct ud_ctx {
     hipEvent_t lev[UD_N_BUCKETS] = {};
     ud_launch_info linfo[UD_N_BUCKETS] = {};
     // lanes per chain per size bucket (ud_autotune / ud_set_bucket_lanes; 0 = default)
-    int tuned_lanes[UD_N_BUCKETS] = {0, 0, 0, 0, 0, 0};
+    int tuned_lanes[UD_N_BUCKETS] = {};
     int forced_lanes = 0;  // ud_set_lanes
 };
 
@@ -926,9 +926,8 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
                 const float nrm = std::sqrt(r.ax[0] * r.ax[0] + r.ax[1] * r.ax[1] + r.ax[2] * r.ax[2]);
                 if (!(std::fabs(nrm - 1.f) < 0.005f)) return fail(c, UD_ERR_INVALID, tag + "node axis is not a unit vector");
             }
-            r.parent = (k == 0) ? 0 : par;
-            r.packed = uint32_t(b->node_abegin[k0 + k]) | (uint32_t(b->node_aend[k0 + k]) << 8) |
-                       (uint32_t(sub_end[k]) << 16) | (uint32_t(depth[k]) << 24);
+            r.link = node_link(uint32_t(k == 0 ? 0 : par), uint32_t(depth[k]));
+            r.packed = node_packed(uint32_t(b->node_abegin[k0 + k]), uint32_t(b->node_aend[k0 + k]), uint32_t(sub_end[k]));
             nodes.push_back(r);
         }
         int n_heavy = 0;
@@ -953,7 +952,7 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
             if (pa < 0 || pb < 0 || pa >= na || pb >= na || pa == pb) return fail(c, UD_ERR_INVALID, tag + "pair index out of range");
             const int ta = xs[l][pa], tb = xs[l][pb];
             if (ta < 0 || tb < 0) return fail(c, UD_ERR_INVALID, tag + "pair with an untyped atom");
-            pairs.push_back(uint32_t(pa) | (uint32_t(pb) << 8) | (uint32_t(tri_index(ta, tb)) << 16));
+            pairs.push_back(pair_rec(uint32_t(pa), uint32_t(pb), uint32_t(tri_index(ta, tb))));
             if (ad) {
                 // ad4_electrostatic's charge product and ad4_solvation's
                 // coefficient of the Gaussian (potentials.h:350-400)
@@ -967,9 +966,9 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
         std::vector<std::vector<PartnerRec>> part(na);
         for (int q = 0; q < np; ++q) {
             const int pa = b->pairs[2 * (p0 + q)], pb = b->pairs[2 * (p0 + q) + 1];
-            const uint32_t slot21 = uint32_t(tri_index(xs[l][pa], xs[l][pb])) << 21;
-            part[pa].push_back(uint32_t(pb) | (uint32_t(q) << 8) | slot21 | 0x80000000u);
-            part[pb].push_back(uint32_t(pa) | (uint32_t(q) << 8) | slot21);
+            const uint32_t slot = uint32_t(tri_index(xs[l][pa], xs[l][pb]));
+            part[pa].push_back(partner_rec(uint32_t(pb), uint32_t(q), slot));
+            part[pb].push_back(partner_rec(uint32_t(pa), uint32_t(q), slot));
         }
         const size_t pbeg = partners.size();
         // the search kernel loads record batches as aligned 16-byte pairs
@@ -977,7 +976,7 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
         for (int i = 0; i < na; ++i) {
             apar.push_back(uint32_t(partners.size() - pbeg) | (uint32_t(part[i].size()) << 20));
             partners.insert(partners.end(), part[i].begin(), part[i].end());
-            while ((partners.size() - pbeg) % kPartnerPad) partners.push_back(uint32_t(i) | (uint32_t(np) << 8));
+            while ((partners.size() - pbeg) % kPartnerPad) partners.push_back(partner_rec(uint32_t(i), uint32_t(np), 0));
         }
         LigHdr& h = hdr[l];
         std::memset(&h, 0, sizeof(h));
@@ -1291,7 +1290,7 @@ int ud_optimize(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses
 
 namespace {
 
-const int kBucketCaps[UD_N_BUCKETS] = {16, 32, 48, 64, 96, 128};
+const int kBucketCaps[UD_N_BUCKETS] = {16, 32, 48, 64, 96, 128, 192, 300};
 // one chain per lane group unless the caller asks for more: many small
 // workgroups balance the end of a launch (and of concurrent bucket launches)
 // better than fewer workgroups whose groups refill

@@ -30,6 +30,8 @@ static_assert(sizeof(LigHdr) == 48, "LigHdr layout");
 //   bits 8..15  grid map slot (0xFF = no grid term)
 //   bit  16     heavy atom
 //   bits 24..31 XS type (0xFF = untyped)
+// Atom indices in the node, pair and partner records take 9 or 10 bits
+// (UD_MAX_ATOMS = 300); nodes 8 bits (UD_MAX_TORSIONS + 1 = 49).
 struct AtomRec {
     float x, y, z;
     uint32_t meta;
@@ -37,26 +39,42 @@ struct AtomRec {
 
 // Node record (32 B): relative origin / axis in the parent frame
 // (segment::relative_origin / relative_axis, tree.h:121-131)
-//   packed bits 0..7 atom begin, 8..15 atom end, 16..23 subtree atom end,
-//   24..31 depth
+//   link   bits 0..15 parent node, 16..31 depth (root: parent 0, depth 0)
+//   packed bits 0..9 atom begin, 10..19 atom end, 20..29 subtree atom end
 struct NodeRec {
     float ro[3];
-    int32_t parent;
+    uint32_t link;
     float ax[3];
     uint32_t packed;
 };
+__host__ __device__ inline uint32_t node_link(uint32_t parent, uint32_t depth) { return parent | (depth << 16); }
+__host__ __device__ inline int node_parent(const NodeRec& n) { return int(n.link & 0xFFFFu); }
+__host__ __device__ inline int node_depth(const NodeRec& n) { return int(n.link >> 16); }
+__host__ __device__ inline uint32_t node_packed(uint32_t ab, uint32_t ae, uint32_t se) { return ab | (ae << 10) | (se << 20); }
+__host__ __device__ inline int node_abegin(const NodeRec& n) { return int(n.packed & 0x3FFu); }
+__host__ __device__ inline int node_aend(const NodeRec& n) { return int((n.packed >> 10) & 0x3FFu); }
+__host__ __device__ inline int node_subtree_end(const NodeRec& n) { return int((n.packed >> 20) & 0x3FFu); }
 
-// Pair record: a | b << 8 | table slot << 16
+// Pair record: a (bits 0..8) | b (9..17) | table slot (18..27)
 typedef uint32_t PairRec;
+__host__ __device__ inline uint32_t pair_rec(uint32_t a, uint32_t b, uint32_t slot) { return a | (b << 9) | (slot << 18); }
+__host__ __device__ inline uint32_t pair_a(uint32_t r) { return r & 0x1FFu; }
+__host__ __device__ inline uint32_t pair_b(uint32_t r) { return (r >> 9) & 0x1FFu; }
+__host__ __device__ inline uint32_t pair_slot(uint32_t r) { return r >> 18; }
 // Partner record (each pair listed once under each of its atoms, in pair
-// order): bits 0..7 other atom, 8..20 pair index, 21..30 table slot, bit 31
-// set when the atom is the pair's first atom.  Per atom, AtomPartners =
-// first partner (relative to the ligand's partner_begin) | count << 20.
-// Each atom's list is padded to a multiple of kPartnerPad records with
-// records naming the atom itself and pair n_pairs, whose force-scale slot
-// holds 0 (count excludes them), so that record batches of that size need no
-// per-record bound.
+// order): bits 0..8 other atom, 9..21 pair index, 22..31 table slot.  Per
+// atom, AtomPartners = first partner (relative to the ligand's
+// partner_begin) | count << 20.  Each atom's list is padded to a multiple of
+// kPartnerPad records with records naming the atom itself and pair n_pairs,
+// whose force-scale slot holds 0 (count excludes them), so that record
+// batches of that size need no per-record bound.
 typedef uint32_t PartnerRec;
+__host__ __device__ inline uint32_t partner_rec(uint32_t other, uint32_t pair, uint32_t slot) {
+    return other | (pair << 9) | (slot << 22);
+}
+__host__ __device__ inline uint32_t partner_other(uint32_t r) { return r & 0x1FFu; }
+__host__ __device__ inline uint32_t partner_pair(uint32_t r) { return (r >> 9) & 0x1FFFu; }
+__host__ __device__ inline uint32_t partner_slot(uint32_t r) { return r >> 22; }
 typedef uint32_t AtomPartners;
 static constexpr int kPartnerPad = 8;
 

@@ -419,7 +419,8 @@ __device__ void fk_frames(const Grp<L, NM>& s, const float* pose) {
     }
     // half-angle sines and cosines of the lane's torsions, which do not
     // depend on the parent frames: off the level-by-level critical path
-    constexpr int KN = (UD_MAX_TORSIONS + L - 1) / L;
+    // (the launch's vector capacity NM = 6 + torsions bounds the torsions)
+    constexpr int KN = (NM - 6 + L - 1) / L;
     float hc[KN], hs[KN];
 #pragma unroll
     for (int m = 0; m < KN; ++m) {
@@ -438,8 +439,8 @@ __device__ void fk_frames(const Grp<L, NM>& s, const float* pose) {
             const int k = 1 + s.gl + m * L;
             if (k >= s.h.n_nodes) continue;
             const NodeRec nd = (UD_FK_NODE_REGS && KN == 1) ? nd1 : s.nodes[k];
-            if (int(nd.packed >> 24) != d) continue;
-            const float* pf = fr + 20 * nd.parent;
+            if (node_depth(nd) != d) continue;
+            const float* pf = fr + 20 * node_parent(nd);
             const float* R = pf + 10;
             float* f = fr + 20 * k;
             const float ox = pf[0] + (R[0] * nd.ro[0] + R[1] * nd.ro[1] + R[2] * nd.ro[2]);
@@ -790,7 +791,7 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
                 other[u] = 0; idx[u] = 0; it[u] = 0; pq[u] = 0; rem[u] = 0.f; rx[u] = ry[u] = rz[u] = 0.f;
                 if (k + u < kn) {
                     const PartnerRec rec = ld_at<uint32_t>(part, uint32_t(k0 + k + u) * 4u);
-                    other[u] = int(rec & 0xff);
+                    other[u] = int(partner_other(rec));
                     const float4 cb = coords[other[u]];
                     rx[u] = cb.x - ci.x; ry[u] = cb.y - ci.y; rz[u] = cb.z - ci.z;
                     const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
@@ -798,9 +799,9 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
                         const float rf = 32.f * r2;
                         const int i1t = int(rf);
                         rem[u] = rf - float(i1t);
-                        idx[u] = int((rec >> 21) & 0x3FFu) * kTableStride + i1t;
+                        idx[u] = int(partner_slot(rec)) * kTableStride + i1t;
                         it[u] = i1t;
-                        pq[u] = int((rec >> 8) & 0x1FFFu);
+                        pq[u] = int(partner_pair(rec));
                         in[u] = true;
                     }
                 }
@@ -888,14 +889,14 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
             b.idx[u] = 0; b.it[u] = 0; b.rem[u] = 0.f;
             if (q < P) {
                 const PairRec rec = s.pairs[q];
-                const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
+                const float4 ca = coords[pair_a(rec)], cb = coords[pair_b(rec)];
                 const float rx = cb.x - ca.x, ry = cb.y - ca.y, rz = cb.z - ca.z;
                 const float r2 = rx * rx + ry * ry + rz * rz;
                 if (r2 < pair_cut2<GF>()) {
                     const float rf = 32.f * r2;
                     const int i1t = int(rf);
                     b.rem[u] = rf - float(i1t);
-                    b.idx[u] = int(rec >> 16) * kTableStride + i1t;
+                    b.idx[u] = int(pair_slot(rec)) * kTableStride + i1t;
                     b.it[u] = i1t;
                     b.in[u] = true;
                 }
@@ -999,8 +1000,8 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
             load_batch(uint32_t(k0 + k), rec);
 #pragma unroll
             for (int u = 0; u < kRec; ++u) {
-                const float d = pf[(rec[u] >> 8) & 0x1FFFu];
-                const float4 co = coords[rec[u] & 0xFFu];
+                const float d = pf[partner_pair(rec[u])];
+                const float4 co = coords[partner_other(rec[u])];
                 sd += d;
                 sx += d * co.x;
                 sy += d * co.y;
@@ -1080,7 +1081,7 @@ __device__ void eval_grad(const KArgs& ka, const Grp<L, NM>& s, float (&gout)[Gr
         for (int k0 = 0; k0 < nn; k0 += L) {
             const int k = k0 + s.gl;
             const NodeRec nd = s.nodes[min(k, nn - 1)];
-            const int ab = nd.packed & 0xff, se = (nd.packed >> 16) & 0xff;
+            const int ab = node_abegin(nd), se = node_subtree_end(nd);
             float d[6];
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
@@ -1116,7 +1117,7 @@ __device__ void eval_grad(const KArgs& ka, const Grp<L, NM>& s, float (&gout)[Gr
         float F0 = 0.f, F1 = 0.f, F2 = 0.f, T0 = 0.f, T1 = 0.f, T2 = 0.f;
         if (k < nn) {
             const NodeRec nd = s.nodes[k];
-            const int ab = nd.packed & 0xff, se = (nd.packed >> 16) & 0xff;
+            const int ab = node_abegin(nd), se = node_subtree_end(nd);
             for (int i = ab + jj; i < se; i += S) {
                 const float4 fi = mf[i], ci = coords[i];
                 const float dx = ci.x - f[0], dy = ci.y - f[1], dz = ci.z - f[2];
@@ -1813,14 +1814,14 @@ __global__ __launch_bounds__(64) void terms_kernel(KArgs ka, const WorkItem* wor
     const PairRec* prs = s.pairs;
     for (int q = s.gl; q < s.h.n_pairs; q += L) {
         const PairRec rec = prs[q];
-        const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
+        const float4 ca = coords[pair_a(rec)], cb = coords[pair_b(rec)];
         const float rx = cb.x - ca.x, ry = cb.y - ca.y, rz = cb.z - ca.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         if (r2 < pair_cut2<GF>()) {
             // precalculate_element::eval_fast (precalculate.h:40-47); AD4.2:
             // type-pair part plus the charge terms of the pair
             const int k = int(32.f * r2);
-            double t = (k < kTableStride) ? double(ka.ls.table_fast[size_t(rec >> 16) * kTableStride + k]) : 0.0;
+            double t = (k < kTableStride) ? double(ka.ls.table_fast[size_t(pair_slot(rec)) * kTableStride + k]) : 0.0;
             if constexpr (Fmt<GF>::ad4) {
                 const float2 cf = ka.ls.pair_coef[s.h.pair_begin + q];
                 const float2 uf = ka.ls.ad4_fast[k];
@@ -1945,14 +1946,14 @@ __global__ __launch_bounds__(64) void energy_grad_kernel(KArgs ka, TermsDev td, 
     const size_t tab_stride = size_t(kTypePairs) * kTableStride;
     for (int q = s.gl; q < s.h.n_pairs; q += L) {
         const PairRec rec = s.pairs[q];
-        const float4 ca = coords[rec & 0xff], cb = coords[(rec >> 8) & 0xff];
+        const float4 ca = coords[pair_a(rec)], cb = coords[pair_b(rec)];
         const float rx = cb.x - ca.x, ry = cb.y - ca.y, rz = cb.z - ca.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         if (!(r2 < 64.f)) continue;
         const float rf = 32.f * r2;
         const int it = int(rf);
         const float rem = rf - float(it);
-        const size_t idx = size_t(rec >> 16) * kTableStride + it;
+        const size_t idx = size_t(pair_slot(rec)) * kTableStride + it;
         const float2 t0 = ka.ls.table_deriv[idx], t1 = ka.ls.table_deriv[idx + 1];
         const float c = curl_slope(t0.x + rem * (t1.x - t0.x), v_intra);
 #pragma unroll

@@ -16,14 +16,15 @@ UD_OK, UD_ERR_INVALID, UD_ERR_CAPACITY, UD_ERR_HIP, UD_ERR_STATE, UD_ERR_NOMEM =
 UD_SF_VINA, UD_SF_VINARDO, UD_SF_AD4 = 0, 1, 2
 UD_AD4_MAP_ELEC, UD_AD4_MAP_DSOLV = 35, 36  # AD4.2 electrostatic / desolvation map indices
 UD_GRID_F32, UD_GRID_F16 = 0, 1
-UD_MAX_ATOMS = 128
-UD_MAX_TORSIONS = 32
+UD_MAX_ATOMS = 300
+UD_MAX_TORSIONS = 48
+UD_MAX_PAIRS = 8128
 UD_POSE_FLOATS = 7 + UD_MAX_TORSIONS
 UD_GRAD_FLOATS = 6 + UD_MAX_TORSIONS
 UD_WEIGHT_FLOATS = 7  # dE/dw per pose, in the order of the scoring weights
-UD_N_BUCKETS = 6
+UD_N_BUCKETS = 8
 UD_NOALLOC_MAX_POSES = 65536
-BUCKET_ATOMS = (16, 32, 48, 64, 96, 128)
+BUCKET_ATOMS = (16, 32, 48, 64, 96, 128, 192, 300)
 
 VINA_WEIGHTS = (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 50.0, 0.05846)  # vina.h:129-132
 VINARDO_WEIGHTS = (-0.045, 0.8, -0.035, -0.600, 50.0, 0.05846)                      # vina.h:133-135
