@@ -57,6 +57,7 @@ enum { UD_GRID_F32 = 0, UD_GRID_F16 = 1 };
 #define UD_MAX_TORSIONS 48
 #define UD_MAX_PAIRS 8128
 #define UD_MAX_MAPS 32
+#define UD_MAX_MODES 64  /* output modes per ligand of ud_postprocess */
 #define UD_MAX_GRID_POINTS (1 << 24)
 
 /* A pose is UD_POSE_FLOATS floats: position[3], orientation quaternion

@@ -159,6 +159,25 @@ struct ud_ctx {
     bool slots_dirty = true;
     int32_t* d_i1 = nullptr;
     size_t i_cap = 0;
+    // post-processing on the device: compact chain coordinates, cluster
+    // lists, per-atom element classes, mode coordinates, ligands, reference
+    // modes and RMSD bounds
+    float* d_cc = nullptr;
+    size_t cc_cap = 0;
+    int32_t* d_kept = nullptr;
+    size_t kept_cap = 0;
+    int32_t* d_nkept = nullptr;
+    size_t nkept_cap = 0;
+    int32_t* d_elem = nullptr;
+    size_t elem_cap = 0;
+    float* d_mc = nullptr;
+    size_t mc_cap = 0;
+    int32_t* d_mlig = nullptr;
+    int32_t* d_mref = nullptr;
+    size_t mlig_cap = 0, mref_cap = 0;
+    float* d_ub = nullptr;
+    float* d_lb = nullptr;
+    size_t ub_cap = 0, lb_cap = 0;
     float* d_out_pose = nullptr;
     float* d_out_e = nullptr;
     long long* d_out_evals = nullptr;
@@ -424,7 +443,8 @@ int ud_destroy(ud_ctx* c) {
                     c->d_cell_start,
                     c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs, c->d_partners, c->d_apar,
                     c->d_work, c->d_perm, c->d_poses, c->d_f1, c->d_f2, c->d_f3, c->d_i1, c->d_out_pose, c->d_out_e,
-                    c->d_out_evals, c->d_ad4_deriv, c->d_ad4_fast, c->d_charge, c->d_pair_coef};
+                    c->d_out_evals, c->d_ad4_deriv, c->d_ad4_fast, c->d_charge, c->d_pair_coef, c->d_cc, c->d_kept,
+                    c->d_nkept, c->d_elem, c->d_mc, c->d_mlig, c->d_mref, c->d_ub, c->d_lb};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1041,6 +1061,9 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     }
     c->lig_ad4 = ad;
     c->h_elem = elem;
+    rc = grow(c, c->d_elem, c->elem_cap, std::max<size_t>(elem.size(), 1));
+    if (rc) return rc;
+    UD_HIP(c, hipMemcpy(c->d_elem, elem.data(), elem.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     c->hdr = hdr;
     c->n_lig = L;
     c->lig_nat = nat;
@@ -1757,78 +1780,51 @@ int ud_cluster_modes(int n_cand, int n_heavy, const float* heavy_xyz, const floa
 int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, ud_mode* modes_out,
                    int32_t* n_modes_out) {
     if (!c || max_modes < 1 || !(min_rmsd >= 0.0) || !modes_out || !n_modes_out) return UD_ERR_INVALID;
+    if (max_modes > UD_MAX_MODES) return fail(c, UD_ERR_INVALID, "max_modes must be at most UD_MAX_MODES");
     if (refine_step < 0 || refine_step > 8) return fail(c, UD_ERR_INVALID, "refine_step must be in 0..8");
     if (c->out_n == 0 || c->last_exh <= 0) return fail(c, UD_ERR_STATE, "no docking results");
     const int exh = c->last_exh, L = c->n_lig;
     const size_t nch = size_t(L) * exh;
-    std::vector<float> P(nch * UD_POSE_FLOATS), E(nch);
-    int rc = ud_fetch_dock_results(c, P.data(), E.data(), nullptr);
+    UD_HIP(c, hipSetDevice(c->device));
+    // every chain's best-pose coordinates, packed per ligand, then the
+    // clustering of each ligand's chains, on the device
+    size_t n_atoms_all = 0;
+    for (int l = 0; l < L; ++l) n_atoms_all += size_t(c->lig_nat[l]);
+    int rc = grow(c, c->d_cc, c->cc_cap, n_atoms_all * exh * 3);
     if (rc) return rc;
-    // heavy atoms of each ligand and their element classes
-    std::vector<std::vector<int>> heavy(L), elem(L);
-    std::vector<size_t> hoff(L + 1, 0);
-    for (int l = 0; l < L; ++l) {
-        const LigHdr& h = c->hdr[l];
-        for (int i = 0; i < h.n_atoms; ++i) {
-            const uint32_t meta = c->h_atoms[size_t(h.atom_begin) + i].meta;
-            if ((meta >> 16) & 1) {
-                heavy[l].push_back(i);
-                elem[l].push_back(c->h_elem[size_t(h.atom_begin) + i]);
-            }
-        }
-        hoff[l + 1] = hoff[l] + size_t(exh) * heavy[l].size() * 3;
-    }
-    // heavy-atom coordinates of every chain's best pose, in chunks through the coords kernel
-    std::vector<float> hx(hoff[L]);
-    const int chunk = 4096;
-    std::vector<float> tmp(size_t(chunk) * UD_MAX_ATOMS * 3);
-    std::vector<int32_t> pl(chunk);
-    for (size_t c0 = 0; c0 < nch; c0 += chunk) {
-        const int n = int(std::min<size_t>(chunk, nch - c0));
-        for (int k = 0; k < n; ++k) pl[k] = int32_t((c0 + k) / exh);
-        rc = ud_pose_coords(c, n, pl.data(), P.data() + c0 * UD_POSE_FLOATS, tmp.data());
-        if (rc) return rc;
-        for (int k = 0; k < n; ++k) {
-            const int l = pl[k], j = int((c0 + k) % exh);
-            const std::vector<int>& hv = heavy[l];
-            float* dst = hx.data() + hoff[l] + size_t(j) * hv.size() * 3;
-            for (size_t a = 0; a < hv.size(); ++a)
-                for (int d = 0; d < 3; ++d) dst[3 * a + d] = tmp[(size_t(k) * UD_MAX_ATOMS + hv[a]) * 3 + d];
-        }
-    }
-    // per ligand: merge the chains in order, then remove_redundant over the result
-    std::vector<std::vector<int>> kept(L);
-    auto cluster = [&](int l) {
-        const int nh = int(heavy[l].size());
-        const float* xyz = hx.data() + hoff[l];
-        std::vector<int> first;
-        cluster_modes(exh, nh, xyz, E.data() + size_t(l) * exh, float(min_rmsd), max_modes, first);
-        std::vector<float> sx(first.size() * nh * 3), se(first.size());
-        for (size_t k = 0; k < first.size(); ++k) {
-            std::copy(xyz + size_t(first[k]) * nh * 3, xyz + size_t(first[k] + 1) * nh * 3, sx.begin() + k * nh * 3);
-            se[k] = E[size_t(l) * exh + first[k]];
-        }
-        std::vector<int> second;
-        cluster_modes(int(first.size()), nh, sx.data(), se.data(), float(min_rmsd), int(first.size()), second);
-        for (int k : second) kept[l].push_back(first[k]);
-    };
-    const int nt = std::max(1, std::min<int>({16, L, int(std::thread::hardware_concurrency())}));
+    rc = grow(c, c->d_kept, c->kept_cap, size_t(L) * max_modes);
+    if (rc) return rc;
+    rc = grow(c, c->d_nkept, c->nkept_cap, size_t(L));
+    if (rc) return rc;
     {
-        std::vector<std::thread> pool;
-        for (int t = 0; t < nt; ++t)
-            pool.emplace_back([&, t] {
-                for (int l = t; l < L; l += nt) cluster(l);
-            });
-        for (auto& th : pool) th.join();
+        LaunchCap cap = caps_all(c);
+        const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : default_lanes(cap.n_atoms), cap);
+        const int G = 64 / lanes;
+        std::vector<WorkItem> work;
+        for (int l = 0; l < L; ++l)
+            for (int c0 = 0; c0 < exh; c0 += G) work.push_back(WorkItem{l, l * exh + c0, std::min(G, exh - c0), 0});
+        rc = work_upload(c, std::vector<int32_t>(), work);
+        if (rc) return rc;
+        UD_HIP(c, launch_coords(lanes, lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), nullptr,
+                                c->d_out_pose, c->d_cc, c->stream, exh));
+        UD_HIP(c, launch_cluster(lig_dev(c), L, c->d_cc, c->d_out_e, exh, max_modes, float(min_rmsd), c->d_kept,
+                                 c->d_nkept, c->stream));
     }
+    std::vector<int32_t> kept_all(size_t(L) * max_modes), nkept(L);
+    UD_HIP(c, hipMemcpyAsync(kept_all.data(), c->d_kept, kept_all.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipMemcpyAsync(nkept.data(), c->d_nkept, nkept.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<float> P(nch * UD_POSE_FLOATS), E(nch);
+    rc = ud_fetch_dock_results(c, P.data(), E.data(), nullptr);  // synchronises the stream
+    if (rc) return rc;
     // kept modes of every ligand in cluster order
     std::vector<int32_t> kl, kchain;
     std::vector<float> kp;
     for (int l = 0; l < L; ++l)
-        for (int k : kept[l]) {
+        for (int k = 0; k < nkept[l]; ++k) {
+            const int ch = kept_all[size_t(l) * max_modes + k];
             kl.push_back(l);
-            kchain.push_back(k);
-            const float* src = P.data() + (size_t(l) * exh + k) * UD_POSE_FLOATS;
+            kchain.push_back(ch);
+            const float* src = P.data() + (size_t(l) * exh + ch) * UD_POSE_FLOATS;
             kp.insert(kp.end(), src, src + UD_POSE_FLOATS);
         }
     const int nk_all = int(kl.size());
@@ -1847,25 +1843,12 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, u
         }
         if (rc) return rc;
     }
-    // heavy-atom coordinates of the kept modes (refined poses have moved)
-    std::vector<float> mx;
-    std::vector<size_t> moff(nk_all + 1, 0);
-    for (int q = 0; q < nk_all; ++q) moff[q + 1] = moff[q] + heavy[kl[q]].size() * 3;
-    mx.resize(moff[nk_all]);
-    for (int q0 = 0; q0 < nk_all; q0 += chunk) {
-        const int n = std::min(chunk, nk_all - q0);
-        rc = ud_pose_coords(c, n, kl.data() + q0, kp.data() + size_t(q0) * UD_POSE_FLOATS, tmp.data());
-        if (rc) return rc;
-        for (int k = 0; k < n; ++k) {
-            const std::vector<int>& hv = heavy[kl[q0 + k]];
-            float* dst = mx.data() + moff[q0 + k];
-            for (size_t a = 0; a < hv.size(); ++a)
-                for (int d = 0; d < 3; ++d) dst[3 * a + d] = tmp[(size_t(k) * UD_MAX_ATOMS + hv[a]) * 3 + d];
-        }
-    }
-    size_t q = 0;
+    // final scores and order per ligand
+    std::vector<float> mpose(size_t(nk_all) * UD_POSE_FLOATS);
+    std::vector<int32_t> mlig(nk_all), mref(nk_all);
+    size_t q = 0, m0 = 0;
     for (int l = 0; l < L; ++l) {
-        const int nk = int(kept[l].size());
+        const int nk = nkept[l];
         ud_mode* out = modes_out + size_t(l) * max_modes;
         n_modes_out[l] = nk;
         if (nk == 0) continue;
@@ -1876,7 +1859,6 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, u
         if (refine)
             std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return rkey[q + a] < rkey[q + b]; });
         const float intra_best = sf_ad4(c) ? 0.f : intra[q + ord[0]];
-        std::vector<int> slot_of(nk);  // mode k of `out` -> index among the ligand's kept modes
         for (int k = 0; k < nk; ++k) {
             ud_mode& m = out[k];
             const size_t i = q + ord[k];
@@ -1899,18 +1881,52 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, u
             m.chain = int32_t(size_t(l) * exh + kchain[i]);
         }
         std::stable_sort(out, out + nk, [](const ud_mode& a, const ud_mode& b) { return a.energy < b.energy; });
-        for (int k = 0; k < nk; ++k)
-            for (int j = 0; j < nk; ++j)
-                if (size_t(l) * exh + kchain[q + j] == size_t(out[k].chain)) slot_of[k] = j;
-        // RMSD bounds from the best mode
-        const int nh = int(heavy[l].size());
-        const float* best = mx.data() + moff[q + slot_of[0]];
         for (int k = 0; k < nk; ++k) {
-            const float* cur = mx.data() + moff[q + slot_of[k]];
-            out[k].rmsd_ub = rmsd_upper_bound(cur, best, nh);
-            out[k].rmsd_lb = rmsd_lower_bound(cur, best, elem[l].data(), nh);
+            std::copy(out[k].pose, out[k].pose + UD_POSE_FLOATS, mpose.begin() + (m0 + k) * UD_POSE_FLOATS);
+            mlig[m0 + k] = l;
+            mref[m0 + k] = int32_t(m0);  // RMSD bounds from the ligand's best mode
         }
         q += nk;
+        m0 += nk;
+    }
+    if (nk_all == 0) return UD_OK;
+    // RMSD bounds of every mode against its ligand's best mode, on the device
+    LaunchCap cap = caps_all(c);
+    const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : default_lanes(cap.n_atoms), cap);
+    std::vector<int32_t> perm;
+    std::vector<WorkItem> work;
+    make_pose_work(nk_all, mlig.data(), 64 / lanes, perm, work);
+    rc = pose_upload(c, nk_all, mpose.data());
+    if (rc) return rc;
+    rc = work_upload(c, perm, work);
+    if (rc) return rc;
+    rc = grow(c, c->d_mc, c->mc_cap, size_t(nk_all) * UD_MAX_ATOMS * 3);
+    if (rc) return rc;
+    rc = grow(c, c->d_mlig, c->mlig_cap, size_t(nk_all));
+    if (rc) return rc;
+    rc = grow(c, c->d_mref, c->mref_cap, size_t(nk_all));
+    if (rc) return rc;
+    rc = grow(c, c->d_ub, c->ub_cap, size_t(nk_all));
+    if (rc) return rc;
+    rc = grow(c, c->d_lb, c->lb_cap, size_t(nk_all));
+    if (rc) return rc;
+    UD_HIP(c, hipMemcpyAsync(c->d_mlig, mlig.data(), mlig.size() * 4, hipMemcpyHostToDevice, c->stream));
+    UD_HIP(c, hipMemcpyAsync(c->d_mref, mref.data(), mref.size() * 4, hipMemcpyHostToDevice, c->stream));
+    UD_HIP(c, launch_coords(lanes, lig_dev(c), grid_dev(c), cap, c->d_work, int(work.size()), c->d_perm, c->d_poses,
+                            c->d_mc, c->stream));
+    UD_HIP(c, launch_bounds(lig_dev(c), c->d_elem, nk_all, c->d_mc, c->d_mlig, c->d_mref, c->d_ub, c->d_lb, c->stream));
+    std::vector<float> ub(nk_all), lb(nk_all);
+    UD_HIP(c, hipMemcpyAsync(ub.data(), c->d_ub, ub.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipMemcpyAsync(lb.data(), c->d_lb, lb.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    m0 = 0;
+    for (int l = 0; l < L; ++l) {
+        ud_mode* out = modes_out + size_t(l) * max_modes;
+        for (int k = 0; k < nkept[l]; ++k) {
+            out[k].rmsd_ub = ub[m0 + k];
+            out[k].rmsd_lb = lb[m0 + k];
+        }
+        m0 += nkept[l];
     }
     return UD_OK;
 }

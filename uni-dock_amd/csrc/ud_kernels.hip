@@ -1970,22 +1970,36 @@ __global__ __launch_bounds__(64) void energy_grad_kernel(KArgs ka, TermsDev td, 
         for (int k = 0; k < UD_WEIGHT_FLOATS; ++k) gw_out[size_t(pi) * UD_WEIGHT_FLOATS + k] = dw[k];
 }
 
+// Atom coordinates of poses.  With compact_exh == 0: pose perm[w.first + g]
+// of the work item, UD_MAX_ATOMS atoms per pose.  With compact_exh > 0 the
+// poses are docking chains (chain w.first + g of ligand w.lig, perm unused)
+// and chain k of ligand l writes its n_atoms atoms at atom
+// atom_begin(l) * compact_exh + k * n_atoms, so that all chains of all
+// ligands take sum(n_atoms) * exh entries.
 template <int L>
 __global__ __launch_bounds__(64) void coords_kernel(KArgs ka, const WorkItem* work, const int32_t* perm,
-                                                    const float* poses, float* coords_out) {
+                                                    const float* poses, float* coords_out, int compact_exh) {
     extern __shared__ float4 lds_raw[];
     float* lds = reinterpret_cast<float*>(lds_raw);
     const WorkItem w = work[blockIdx.x];
     Grp<L> s = setup_group<L, kNMax>(ka, w.lig, lds);
     const bool active = s.g < w.count;
-    const int pi = active ? perm[w.first + s.g] : 0;
+    const int pi = active ? (compact_exh > 0 ? w.first + s.g : perm[w.first + s.g]) : 0;
     float* x = s.pose(PX);
     const int np = 7 + s.ntors();
     for (int i = s.gl; i < np; i += L) x[i] = active ? poses[size_t(pi) * UD_POSE_FLOATS + i] : (i == 3 ? 1.f : 0.f);
     __syncthreads();
     fk_frames(s, x);
     const float* fr = s.frames();
-    if (active) {
+    if (active && compact_exh > 0) {
+        const size_t first = size_t(s.h.atom_begin) * compact_exh + size_t(pi - w.lig * compact_exh) * s.h.n_atoms;
+        for (int i = s.gl; i < s.h.n_atoms; i += L) {
+            const AtomRec a = s.atoms[i];
+            const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
+            float* o = coords_out + (first + i) * 3;
+            o[0] = p.x; o[1] = p.y; o[2] = p.z;
+        }
+    } else if (active) {
         for (int i = s.gl; i < UD_MAX_ATOMS; i += L) {
             float3 p = make_float3(0.f, 0.f, 0.f);
             if (i < s.h.n_atoms) {
@@ -1995,6 +2009,142 @@ __global__ __launch_bounds__(64) void coords_kernel(KArgs ka, const WorkItem* wo
             float* o = coords_out + (size_t(pi) * UD_MAX_ATOMS + i) * 3;
             o[0] = p.x; o[1] = p.y; o[2] = p.z;
         }
+    }
+}
+
+// ====================================================================== clustering and RMSD bounds
+static constexpr int kMaxModes = UD_MAX_MODES;  // output modes per ligand of ud_postprocess
+// rmsd_upper_bound (coords.cpp:21-30) over the heavy atoms `heavy[0..nh)` of
+// two coordinate sets, in atom order with every product and sum rounded to
+// FP32 (no contraction), as the reference's fl accumulation.
+__device__ float rmsd_ub_ordered(const float* a, const float* b, const int* heavy, int nh) {
+    float acc = 0.f;
+    for (int k = 0; k < nh; ++k) {
+        const int i = 3 * heavy[k];
+        const float dx = __fsub_rn(a[i], b[i]), dy = __fsub_rn(a[i + 1], b[i + 1]), dz = __fsub_rn(a[i + 2], b[i + 2]);
+        acc = __fadd_rn(acc, __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz)));
+    }
+    return nh > 0 ? __fsqrt_rn(__fdiv_rn(acc, float(nh))) : 0.f;
+}
+
+// One add_to_output_container step (coords.cpp:42-68) with candidate t of
+// energy e on the container kept[0..nk) (sorted by energy, stably): the
+// closest kept pose (the first of equal RMSDs) absorbs the candidate if it
+// is within min_rmsd, else a full container replaces its worst entry if the
+// candidate is better.  Lanes compute the RMSDs; lane 0 updates.
+__device__ void container_add(int t, float e, const float* base, int na, const int* heavy, int nh, const float* en,
+                              float min_rmsd, int max_size, int* kept, int& nk, float* rr) {
+    if (e < kEpsFl && e > -kEpsFl) return;  // an energy of exactly 0 marks a failed chain
+    for (int k = threadIdx.x; k < nk; k += kWG)
+        rr[k] = rmsd_ub_ordered(base + size_t(t) * na * 3, base + size_t(kept[k]) * na * 3, heavy, nh);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int closest = -1;
+        float cr = 0.f;
+        for (int k = 0; k < nk; ++k)
+            if (k == 0 || rr[k] < cr) { closest = k; cr = rr[k]; }
+        int pos = -1;
+        if (closest >= 0 && cr < min_rmsd) {
+            if (e < en[kept[closest]]) { kept[closest] = t; pos = closest; }
+        } else if (nk < max_size) {
+            kept[nk] = t;
+            pos = nk++;
+        } else if (nk > 0 && e < en[kept[nk - 1]]) {
+            kept[nk - 1] = t;
+            pos = nk - 1;
+        }
+        // stable sort by energy: only the changed entry is out of place
+        if (pos >= 0) {
+            while (pos > 0 && en[kept[pos]] < en[kept[pos - 1]]) { const int x = kept[pos]; kept[pos] = kept[pos - 1]; kept[--pos] = x; }
+            while (pos + 1 < nk && en[kept[pos + 1]] < en[kept[pos]]) { const int x = kept[pos]; kept[pos] = kept[pos + 1]; kept[++pos] = x; }
+        }
+    }
+    __syncthreads();
+}
+
+// Per ligand (one workgroup): the chains in chain order through
+// add_to_output_container (as monte_carlo.cu:2940-2943 merges the chains'
+// results), then remove_redundant (vina.cpp:1465-1470): the kept poses, in
+// their order, through the same step into an empty container.  coords: the
+// compact chain coordinates of coords_kernel; energy: the chains' best
+// energies.  kept_out[lig * max_size + k] = chain index within the ligand,
+// sorted by energy; n_out[lig] their number.
+__global__ __launch_bounds__(64) void cluster_kernel(const LigHdr* hdr, const AtomRec* atoms, const float* coords,
+                                                     const float* energy, int exh, int max_size, float min_rmsd,
+                                                     int32_t* kept_out, int32_t* n_out) {
+    __shared__ int heavy[UD_MAX_ATOMS];
+    __shared__ int kept[kMaxModes], kept2[kMaxModes];
+    __shared__ float rr[kMaxModes];
+    __shared__ int nh_s;
+    const int lig = blockIdx.x;
+    const LigHdr h = hdr[lig];
+    if (threadIdx.x == 0) {
+        int nh = 0;
+        for (int i = 0; i < h.n_atoms; ++i)
+            if ((atoms[h.atom_begin + i].meta >> 16) & 1) heavy[nh++] = i;
+        nh_s = nh;
+    }
+    __syncthreads();
+    const int nh = nh_s, na = h.n_atoms;
+    const float* base = coords + size_t(h.atom_begin) * exh * 3;
+    const float* en = energy + size_t(lig) * exh;
+    int nk = 0;
+    for (int t = 0; t < exh; ++t) {
+        container_add(t, en[t], base, na, heavy, nh, en, min_rmsd, max_size, kept, nk, rr);
+        nk = __shfl(nk, 0);
+    }
+    int nk2 = 0;
+    for (int j = 0; j < nk; ++j) {
+        container_add(kept[j], en[kept[j]], base, na, heavy, nh, en, min_rmsd, nk, kept2, nk2, rr);
+        nk2 = __shfl(nk2, 0);
+    }
+    if (threadIdx.x == 0) n_out[lig] = nk2;
+    for (int k = threadIdx.x; k < nk2; k += kWG) kept_out[size_t(lig) * max_size + k] = kept2[k];
+}
+
+// RMSD bounds of mode m against mode ref[m] (model::rmsd_upper_bound /
+// rmsd_lower_bound, model.cpp:1021-1064) over the heavy atoms of ligand
+// lig[m]: the upper bound pairs atoms by index; the lower bound takes for
+// every atom the nearest atom of the same element in the other pose, both
+// ways, the larger of the two.  coords: UD_MAX_ATOMS atoms per mode.
+__global__ __launch_bounds__(64) void bounds_kernel(const LigHdr* hdr, const AtomRec* atoms, const int32_t* elem,
+                                                    const float* coords, const int32_t* mode_lig,
+                                                    const int32_t* mode_ref, float* ub_out, float* lb_out) {
+    __shared__ int heavy[UD_MAX_ATOMS];
+    __shared__ int nh_s;
+    const int m = blockIdx.x;
+    const LigHdr h = hdr[mode_lig[m]];
+    if (threadIdx.x == 0) {
+        int nh = 0;
+        for (int i = 0; i < h.n_atoms; ++i)
+            if ((atoms[h.atom_begin + i].meta >> 16) & 1) heavy[nh++] = i;
+        nh_s = nh;
+    }
+    __syncthreads();
+    const int nh = nh_s;
+    const float* a = coords + size_t(m) * UD_MAX_ATOMS * 3;
+    const float* b = coords + size_t(mode_ref[m]) * UD_MAX_ATOMS * 3;
+    const int32_t* el = elem + h.atom_begin;
+    auto one_way = [&](const float* x, const float* y) {
+        float acc = 0.f;
+        for (int k = threadIdx.x; k < nh; k += kWG) {
+            const int i = heavy[k];
+            float best = kMaxFl;
+            for (int q = 0; q < nh; ++q) {
+                const int j = heavy[q];
+                if (el[j] != el[i]) continue;
+                const float dx = x[3 * i] - y[3 * j], dy = x[3 * i + 1] - y[3 * j + 1], dz = x[3 * i + 2] - y[3 * j + 2];
+                best = fminf(best, dx * dx + dy * dy + dz * dz);
+            }
+            acc += best;
+        }
+        acc = gsum<kWG>(acc);
+        return nh > 0 ? sqrtf(acc / float(nh)) : 0.f;
+    };
+    const float lb = fmaxf(one_way(a, b), one_way(b, a));
+    if (threadIdx.x == 0) {
+        ub_out[m] = rmsd_ub_ordered(a, b, heavy, nh);
+        lb_out[m] = lb;
     }
 }
 
@@ -2214,12 +2364,28 @@ hipError_t launch_energy_grad(int lanes, int gf, const LigandSetDev& ls, const G
 
 hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                          const WorkItem* work, int n_work, const int32_t* perm, const float* poses, float* coords_out,
-                         hipStream_t st) {
+                         hipStream_t st, int compact_exh) {
     const KArgs ka = make_kargs(ls, g, cap);
     const size_t shm = lds_bytes(lanes, cap);
     UD_DISPATCH_L(lanes, {
-        hipLaunchKernelGGL((coords_kernel<L>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses, coords_out);
+        hipLaunchKernelGGL((coords_kernel<L>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses, coords_out,
+                           compact_exh);
     });
+    return hipGetLastError();
+}
+
+hipError_t launch_cluster(const LigandSetDev& ls, int n_lig, const float* coords, const float* energy, int exh,
+                          int max_size, float min_rmsd, int32_t* kept_out, int32_t* n_out, hipStream_t st) {
+    hipLaunchKernelGGL(cluster_kernel, dim3(n_lig), dim3(kWG), 0, st, ls.hdr, ls.atoms, coords, energy, exh, max_size,
+                       min_rmsd, kept_out, n_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bounds(const LigandSetDev& ls, const int32_t* elem, int n_modes, const float* coords,
+                         const int32_t* mode_lig, const int32_t* mode_ref, float* ub_out, float* lb_out,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(bounds_kernel, dim3(n_modes), dim3(kWG), 0, st, ls.hdr, ls.atoms, elem, coords, mode_lig,
+                       mode_ref, ub_out, lb_out);
     return hipGetLastError();
 }
 

@@ -59,7 +59,17 @@ hipError_t launch_score_explicit(int lanes, const LigandSetDev& ls, const GridDe
 hipError_t launch_terms_explicit(int lanes, const LigandSetDev& ls, const GridDev& g, const RecDev& rec,
                                  const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
                                  const float* poses, float v, float* inter_out, float* intra_out, hipStream_t st);
+// coords_out: UD_MAX_ATOMS atoms per pose, or with compact_exh > 0 the
+// chains' atoms packed per ligand (coords_kernel)
 hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap, const WorkItem* work,
-                         int n_work, const int32_t* perm, const float* poses, float* coords_out, hipStream_t st);
+                         int n_work, const int32_t* perm, const float* poses, float* coords_out, hipStream_t st,
+                         int compact_exh = 0);
+// add_to_output_container + remove_redundant per ligand (cluster_kernel)
+hipError_t launch_cluster(const LigandSetDev& ls, int n_lig, const float* coords, const float* energy, int exh,
+                          int max_size, float min_rmsd, int32_t* kept_out, int32_t* n_out, hipStream_t st);
+// RMSD bounds of modes against reference modes (bounds_kernel)
+hipError_t launch_bounds(const LigandSetDev& ls, const int32_t* elem, int n_modes, const float* coords,
+                         const int32_t* mode_lig, const int32_t* mode_ref, float* ub_out, float* lb_out,
+                         hipStream_t st);
 
 }  // namespace ud
