@@ -473,7 +473,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="lanes per chain of the batched form (0 = per bucket)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4),
                     help="benchmark configuration (1 = the metric's; 2-4 print their own line)")
@@ -510,6 +510,9 @@ def main():
             f"in {time.perf_counter() - t1:.1f}s (untimed set-up)")
     log("[bench] batched FP16 form")
     b = time_form(ctx16, len(ligs), exh, gsteps, args.lanes, args.steps, args.warmup, 181129)
+    ev_lig = b["evals"].reshape(len(ligs), exh).sum(axis=1)
+    for a in b["launches"]:
+        a["evals"] = int(ev_lig[bucket_mask(ligs, a["atom_cap"])].sum())
     log(f"[bench] batched: {b['ligs_per_s']:.1f} ligands/s, kernel {b['kernel_ms']:.1f} ms/step, "
         f"launches {b['launches']}")
     post = {}

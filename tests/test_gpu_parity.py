@@ -455,12 +455,24 @@ def test_postprocess_matches_oracle(scene):
     """ud_postprocess (clustering, rescoring, RMSD bounds) against the
     restatement in oracle/post.py fed with the same chains' coordinates and
     energy terms."""
-    from oracle import post
     _say("[gpu] postprocess")
+    _check_postprocess(scene["ctx"], scene["ligs"], exh=24, max_modes=9, min_rmsd=1.0, seed=12)
+
+
+def test_postprocess_mode_capacity(scene):
+    """The same check with the container at UD_MAX_MODES (many chains, a
+    small clustering radius), and max_modes above it rejected."""
+    _say("[gpu] postprocess at UD_MAX_MODES")
     ctx, ligs = scene["ctx"], scene["ligs"]
-    exh = 24
-    P, E, _ = ctx.dock(exhaustiveness=exh, global_steps=4, seed=12)
-    max_modes, min_rmsd = 9, 1.0
+    n = _check_postprocess(ctx, ligs, exh=160, max_modes=_lib.UD_MAX_MODES, min_rmsd=0.25, seed=5)
+    assert n >= 8 * len(ligs)
+    with pytest.raises(_lib.UdError):
+        ctx.postprocess(max_modes=_lib.UD_MAX_MODES + 1, min_rmsd=1.0, refine_step=0)
+
+
+def _check_postprocess(ctx, ligs, exh, max_modes, min_rmsd, seed):
+    from oracle import post
+    P, E, _ = ctx.dock(exhaustiveness=exh, global_steps=4, seed=seed)
     modes = ctx.postprocess(max_modes=max_modes, min_rmsd=min_rmsd, refine_step=0)
     arr = synth.LigandBatch(list(ligs)).arrays()
     rot_internal = 5 * _lib.VINA_WEIGHTS[6] / 0.1 - 1
@@ -493,6 +505,7 @@ def test_postprocess_matches_oracle(scene):
                 assert post.rmsd_upper_bound(ci, cj) >= min_rmsd
         n_checked += len(got)
     assert n_checked >= 2 * len(ligs)
+    return n_checked
 
 
 def test_vinardo_maps_energy_and_gradient():

@@ -23,6 +23,7 @@ UD_POSE_FLOATS = 7 + UD_MAX_TORSIONS
 UD_GRAD_FLOATS = 6 + UD_MAX_TORSIONS
 UD_WEIGHT_FLOATS = 7  # dE/dw per pose, in the order of the scoring weights
 UD_N_BUCKETS = 8
+UD_MAX_MODES = 64  # output modes per ligand of ud_postprocess
 UD_NOALLOC_MAX_POSES = 65536
 BUCKET_ATOMS = (16, 32, 48, 64, 96, 128, 192, 300)
 
