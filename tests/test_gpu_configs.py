@@ -217,8 +217,13 @@ def test_config1_batch_of_1024():
     # chain ids differ between the subset and the batch, so compare distributions
     g_e = e.reshape(len(ligs), exh)[sel]
     _search_parity(g_e.ravel(), ec, "config 1 search (32 ligands x 128 chains)")
+    # per-ligand medians, paired: the oracle against itself with six seeds
+    # (181129, 7..11) gives |median d| up to 0.76 over the 15 seed pairs
+    # (spread about 0.4), so 1.0 bounds a shift beyond the search's own
+    # noise; the signed-rank test checks the pairs' symmetry about 0
     d = np.median(g_e, axis=1) - np.median(ec.reshape(32, exh), axis=1)
-    assert abs(np.median(d)) < 0.25, d
+    assert abs(np.median(d)) < 1.0, d
+    assert stats.wilcoxon(d).pvalue > 1e-3, d
     ctx.close()
 
 

@@ -142,6 +142,7 @@ struct ud_ctx {
     AtomRec* d_atoms = nullptr;
     NodeRec* d_nodes = nullptr;
     PairRec* d_pairs = nullptr;
+    uint2* d_pair_ofs = nullptr;
     PartnerRec* d_partners = nullptr;
     AtomPartners* d_apar = nullptr;
     // scratch / results
@@ -265,6 +266,7 @@ LigandSetDev lig_dev(const ud_ctx* c) {
     l.atoms = c->d_atoms;
     l.nodes = c->d_nodes;
     l.pairs = c->d_pairs;
+    l.pair_ofs = c->d_pair_ofs;
     l.partners = c->d_partners;
     l.atom_partners = c->d_apar;
     l.table_deriv = c->d_table_deriv;
@@ -441,7 +443,7 @@ int ud_destroy(ud_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_table_deriv, c->d_table_fast, c->d_maps, c->d_rec, c->d_term_tab, c->d_term_maps, c->d_cells,
                     c->d_cell_start,
-                    c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs, c->d_partners, c->d_apar,
+                    c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs, c->d_pair_ofs, c->d_partners, c->d_apar,
                     c->d_work, c->d_perm, c->d_poses, c->d_f1, c->d_f2, c->d_f3, c->d_i1, c->d_out_pose, c->d_out_e,
                     c->d_out_evals, c->d_ad4_deriv, c->d_ad4_fast, c->d_charge, c->d_pair_coef, c->d_cc, c->d_kept,
                     c->d_nkept, c->d_elem, c->d_mc, c->d_mlig, c->d_mref, c->d_ub, c->d_lb};
@@ -887,6 +889,7 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     std::vector<AtomRec> atoms;
     std::vector<NodeRec> nodes;
     std::vector<PairRec> pairs;
+    std::vector<uint2> pofs;
     std::vector<PartnerRec> partners;
     std::vector<AtomPartners> apar;
     std::vector<int> nat(L), nnd(L);
@@ -973,6 +976,7 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
             const int ta = xs[l][pa], tb = xs[l][pb];
             if (ta < 0 || tb < 0) return fail(c, UD_ERR_INVALID, tag + "pair with an untyped atom");
             pairs.push_back(pair_rec(uint32_t(pa), uint32_t(pb), uint32_t(tri_index(ta, tb))));
+            pofs.push_back(pair_ofs(uint32_t(pa), uint32_t(pb), uint32_t(tri_index(ta, tb)), uint32_t(kTableStride)));
             if (ad) {
                 // ad4_electrostatic's charge product and ad4_solvation's
                 // coefficient of the Gaussian (potentials.h:350-400)
@@ -1016,11 +1020,12 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     }
     // commit
     for (void* p : {static_cast<void*>(c->d_hdr), static_cast<void*>(c->d_nodes), static_cast<void*>(c->d_pairs),
-                    static_cast<void*>(c->d_partners), static_cast<void*>(c->d_apar)})
+                    static_cast<void*>(c->d_pair_ofs), static_cast<void*>(c->d_partners), static_cast<void*>(c->d_apar)})
         if (p) (void)hipFree(p);
     c->d_hdr = nullptr;
     c->d_nodes = nullptr;
     c->d_pairs = nullptr;
+    c->d_pair_ofs = nullptr;
     c->d_partners = nullptr;
     c->d_apar = nullptr;
     size_t cap = 0;
@@ -1033,6 +1038,9 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     rc = grow(c, c->d_pairs, cap, std::max<size_t>(pairs.size(), 1));
     if (rc) return rc;
     cap = 0;
+    rc = grow(c, c->d_pair_ofs, cap, std::max<size_t>(pofs.size(), 1));
+    if (rc) return rc;
+    cap = 0;
     rc = grow(c, c->d_partners, cap, std::max<size_t>(partners.size(), 1));
     if (rc) return rc;
     cap = 0;
@@ -1043,7 +1051,10 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     UD_HIP(c, hipMemcpy(c->d_apar, apar.data(), apar.size() * sizeof(AtomPartners), hipMemcpyHostToDevice));
     UD_HIP(c, hipMemcpy(c->d_hdr, hdr.data(), hdr.size() * sizeof(LigHdr), hipMemcpyHostToDevice));
     UD_HIP(c, hipMemcpy(c->d_nodes, nodes.data(), nodes.size() * sizeof(NodeRec), hipMemcpyHostToDevice));
-    if (!pairs.empty()) UD_HIP(c, hipMemcpy(c->d_pairs, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
+    if (!pairs.empty()) {
+        UD_HIP(c, hipMemcpy(c->d_pairs, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
+        UD_HIP(c, hipMemcpy(c->d_pair_ofs, pofs.data(), pofs.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    }
     for (void* p : {static_cast<void*>(c->d_charge), static_cast<void*>(c->d_pair_coef)})
         if (p) (void)hipFree(p);
     c->d_charge = nullptr;

@@ -61,6 +61,12 @@ __host__ __device__ inline uint32_t pair_rec(uint32_t a, uint32_t b, uint32_t sl
 __host__ __device__ inline uint32_t pair_a(uint32_t r) { return r & 0x1FFu; }
 __host__ __device__ inline uint32_t pair_b(uint32_t r) { return (r >> 9) & 0x1FFu; }
 __host__ __device__ inline uint32_t pair_slot(uint32_t r) { return r >> 18; }
+// The same pair pre-decoded for the search's pair pass: x = byte offsets of
+// both atoms' coordinates in the group's float4 coordinate array (a * 16 |
+// b * 16 << 16), y = first sample of the pair's table (slot * kTableStride).
+__host__ __device__ inline uint2 pair_ofs(uint32_t a, uint32_t b, uint32_t slot, uint32_t stride) {
+    return make_uint2((a * 16u) | ((b * 16u) << 16), slot * stride);
+}
 // Partner record (each pair listed once under each of its atoms, in pair
 // order): bits 0..8 other atom, 9..21 pair index, 22..31 table slot.  Per
 // atom, AtomPartners = first partner (relative to the ligand's
@@ -122,6 +128,7 @@ struct LigandSetDev {
     const AtomRec* atoms;
     const NodeRec* nodes;
     const PairRec* pairs;
+    const uint2* pair_ofs;  // pre-decoded pairs, indexed like pairs
     const PartnerRec* partners;
     const AtomPartners* atom_partners;  // per atom, indexed like atoms
     const float2* table_deriv;  // (e, dor) per sample, slot * kTableStride

@@ -484,8 +484,37 @@ struct GridPt {
     int reg[3];
 };
 
+// grid coordinates of p (in units of the spacing, from the box's first sample)
+__device__ __forceinline__ void grid_scale(const GridDev& g, float3 p, float (&s)[3]) {
+    s[0] = (p.x - g.init[0]) * g.factor[0];
+    s[1] = (p.y - g.init[1]) * g.factor[1];
+    s[2] = (p.z - g.init[2]) * g.factor[2];
+}
+// inside the box on every axis (0 <= s < n - 1), or NaN
+__device__ __forceinline__ bool grid_inside(const GridDev& g, const float (&s)[3]) {
+    return !(s[0] < 0.f || s[0] >= g.dimm1[0] || s[1] < 0.f || s[1] >= g.dimm1[1] || s[2] < 0.f || s[2] >= g.dimm1[2]);
+}
+// grid_locate for a point inside the box: the same cell and fractions, no
+// penalty (a NaN coordinate converts to cell 0 and keeps a NaN fraction)
+__device__ __forceinline__ GridPt grid_locate_in(const GridDev& g, int slot, const float (&s)[3]) {
+    GridPt q;
+    int a[3];
+    float f[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        a[i] = min(max(int(s[i]), 0), g.n[i] - 2);
+        f[i] = s[i] - float(a[i]);
+        q.reg[i] = 0;
+    }
+    q.penalty = 0.f;
+    q.base = uint32_t(slot) * uint32_t(g.n_points) + uint32_t(a[0] + g.n[0] * (a[1] + g.n[1] * a[2]));
+    q.x = f[0]; q.y = f[1]; q.z = f[2];
+    return q;
+}
+
 __device__ __forceinline__ GridPt grid_locate(const GridDev& g, int slot, float3 p) {
-    float s[3] = {(p.x - g.init[0]) * g.factor[0], (p.y - g.init[1]) * g.factor[1], (p.z - g.init[2]) * g.factor[2]};
+    float s[3];
+    grid_scale(g, p, s);
     float miss[3];
     int a[3];
     GridPt q;
@@ -571,7 +600,7 @@ __device__ __forceinline__ void grid_gather(const GridDev& g, uint32_t base, flo
 // linear interpolations (x, then y, then z) so that value and gradient share
 // their partial sums; FP16 maps apply the power-of-two map scale to value and
 // gradient before the curl
-template <int GF>
+template <int GF, bool INBOX = false>
 __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const GridPt& q, const float (&f)[8], float v,
                                              float3* d) {
     const float x = q.x, y = q.y, z = q.z;
@@ -597,13 +626,19 @@ __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const G
             const float t2 = tmp * tmp;
             gx *= t2; gy *= t2; gz *= t2;
         }
-        d->x = g.factor[0] * (q.reg[0] == 0 ? gx : 0.f) + g.slope * float(q.reg[0]);
-        d->y = g.factor[1] * (q.reg[1] == 0 ? gy : 0.f) + g.slope * float(q.reg[1]);
-        d->z = g.factor[2] * (q.reg[2] == 0 ? gz : 0.f) + g.slope * float(q.reg[2]);
+        if (INBOX) {
+            d->x = g.factor[0] * gx;
+            d->y = g.factor[1] * gy;
+            d->z = g.factor[2] * gz;
+        } else {
+            d->x = g.factor[0] * (q.reg[0] == 0 ? gx : 0.f) + g.slope * float(q.reg[0]);
+            d->y = g.factor[1] * (q.reg[1] == 0 ? gy : 0.f) + g.slope * float(q.reg[1]);
+            d->z = g.factor[2] * (q.reg[2] == 0 ? gz : 0.f) + g.slope * float(q.reg[2]);
+        }
     } else if (e > 0.f && v < 0.1f * kMaxFl) {
         e *= curl_factor(v, e);
     }
-    return e + q.penalty;
+    return INBOX ? e : e + q.penalty;
 }
 
 // ad4cache::eval_deriv (ad4cache.cpp:223-262) for an atom of charge q whose
@@ -722,14 +757,14 @@ static constexpr int kPairBatch = UD_PAIR_BATCH;  // pairs per lane whose table 
 // gl handling atoms gl, gl + L, ...; also the heavy-atom sum of squared
 // distances to the root origin (model::gyration_radius, model.cpp:871-883).
 template <int L, int NM>
-__device__ __forceinline__ void atoms_coords(const Grp<L, NM>& s, float& rg2) {
+__device__ __forceinline__ void atoms_coords(const Grp<L, NM>& s, float& rg2, bool want_rg = true) {
     float4* coords = s.coords();
     const float* fr = s.frames();
     for (int i = s.gl; i < s.h.n_atoms; i += L) {
         const AtomRec a = s.atoms[i];
         const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
         coords[i] = make_float4(p.x, p.y, p.z, 0.f);
-        if ((a.meta >> 16) & 1) {
+        if (want_rg && ((a.meta >> 16) & 1)) {
             const float dx = p.x - fr[0], dy = p.y - fr[1], dz = p.z - fr[2];
             rg2 += dx * dx + dy * dy + dz * dz;
         }
@@ -881,22 +916,38 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
         float2 t0[kPairBatch], t1[kPairBatch], cf[kPairBatch];
         float4 u0[kPairBatch], u1[kPairBatch];
     } b;
+    // pre-decoded pair records (coordinate byte offsets, table base), each
+    // batch's loaded one batch ahead
+    const uint2* pofs = ka.ls.pair_ofs + s.h.pair_begin;
+    const char* cbase = reinterpret_cast<const char*>(coords);
+    uint2 nrec[kPairBatch];
+    auto fetch = [&](int q0) {
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u) {
+            const int q = q0 + u * L;
+            nrec[u] = q < P ? ld_at<uint2>(pofs, uint32_t(q) * 8u) : make_uint2(0u, 0u);
+        }
+    };
     auto issue = [&](int q0) {
+        uint2 rec[kPairBatch];
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u) rec[u] = nrec[u];
+        fetch(q0 + kPairBatch * L);
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u) {
             const int q = q0 + u * L;
             b.in[u] = false;
             b.idx[u] = 0; b.it[u] = 0; b.rem[u] = 0.f;
             if (q < P) {
-                const PairRec rec = s.pairs[q];
-                const float4 ca = coords[pair_a(rec)], cb = coords[pair_b(rec)];
+                const float4 ca = *reinterpret_cast<const float4*>(cbase + (rec[u].x & 0xFFFFu));
+                const float4 cb = *reinterpret_cast<const float4*>(cbase + (rec[u].x >> 16));
                 const float rx = cb.x - ca.x, ry = cb.y - ca.y, rz = cb.z - ca.z;
                 const float r2 = rx * rx + ry * ry + rz * rz;
                 if (r2 < pair_cut2<GF>()) {
                     const float rf = 32.f * r2;
                     const int i1t = int(rf);
                     b.rem[u] = rf - float(i1t);
-                    b.idx[u] = int(pair_slot(rec)) * kTableStride + i1t;
+                    b.idx[u] = int(rec[u].y) + i1t;
                     b.it[u] = i1t;
                     b.in[u] = true;
                 }
@@ -939,6 +990,7 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
         }
         return ep;
     };
+    fetch(s.gl);
     issue(s.gl);
     float e = 0.f;
     for (int i = s.gl; i < N; i += L) {
@@ -953,10 +1005,20 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
                 e += ad4_grid_term<GF>(ka.grid, slot, ka.ls.charge[s.h.atom_begin + i], make_float3(ci.x, ci.y, ci.z),
                                        v_inter, &d);
             } else {
-                const GridPt gq = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
+                // with every atom of the wavefront's pass inside the box, the
+                // clamping and out-of-box terms (zero there) are left out
+                float sg[3];
+                grid_scale(ka.grid, make_float3(ci.x, ci.y, ci.z), sg);
                 float gf[8];
-                grid_gather<GF>(ka.grid, gq.base, gf);
-                e += grid_finish<GF>(ka.grid, slot, gq, gf, v_inter, &d);
+                if (!wave_any(!grid_inside(ka.grid, sg))) {
+                    const GridPt gq = grid_locate_in(ka.grid, slot, sg);
+                    grid_gather<GF>(ka.grid, gq.base, gf);
+                    e += grid_finish<GF, true>(ka.grid, slot, gq, gf, v_inter, &d);
+                } else {
+                    const GridPt gq = grid_locate(ka.grid, slot, make_float3(ci.x, ci.y, ci.z));
+                    grid_gather<GF>(ka.grid, gq.base, gf);
+                    e += grid_finish<GF>(ka.grid, slot, gq, gf, v_inter, &d);
+                }
             }
         }
         mf[i] = make_float4(d.x, d.y, d.z, 0.f);
@@ -1025,7 +1087,7 @@ __device__ float eval_energy(const KArgs& ka, const Grp<L, NM>& s, const float* 
                              float* rgyr = nullptr) {
     fk_frames(s, pose);
     float rg2 = 0.f;
-    atoms_coords(s, rg2);
+    atoms_coords(s, rg2, rgyr != nullptr);
     __syncthreads();
     float e = ka.pair_forces ? atoms_forces_pp_a<L, NM, GF>(ka, s, v_intra, v_inter)
                              : atoms_forces_2s<L, NM, GF>(ka, s, v_intra, v_inter);
@@ -1037,6 +1099,25 @@ __device__ float eval_energy(const KArgs& ka, const Grp<L, NM>& s, const float* 
         *rgyr = s.h.n_heavy > 0 ? __builtin_amdgcn_sqrtf(rg2 * __builtin_amdgcn_rcpf(float(s.h.n_heavy))) : 0.f;
     }
     return e;
+}
+
+// Gyration radius of the pose evaluated by the last eval_energy, from its
+// coordinates and root frame still in LDS: the same sum, in the same order,
+// as eval_energy's (model::gyration_radius, model.cpp:871-883).
+template <int L, int NM>
+__device__ __forceinline__ float gyration(const Grp<L, NM>& s) {
+    const float4* coords = s.coords();
+    const float* fr = s.frames();
+    float rg2 = 0.f;
+    for (int i = s.gl; i < s.h.n_atoms; i += L) {
+        if ((s.atoms[i].meta >> 16) & 1) {
+            const float4 p = coords[i];
+            const float dx = p.x - fr[0], dy = p.y - fr[1], dz = p.z - fr[2];
+            rg2 += dx * dx + dy * dy + dz * dz;
+        }
+    }
+    rg2 = gsum<L>(rg2);
+    return s.h.n_heavy > 0 ? __builtin_amdgcn_sqrtf(rg2 * __builtin_amdgcn_rcpf(float(s.h.n_heavy))) : 0.f;
 }
 
 // Gradient of the pose evaluated by the last eval_energy (model::eval_deriv,
@@ -1470,8 +1551,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         if (!wave_any(state != ST_IDLE)) break;
         // ---------------------------------------------------------------- evaluate
         const bool trialing = state == ST_TRIAL;
-        float rg = 0.f;
-        const float f = eval_energy<L, NM, GF>(ka, s, trialing ? XN : X, vI, vE, &rg);
+        const float f = eval_energy<L, NM, GF>(ka, s, trialing ? XN : X, vI, vE);
         if (state != ST_IDLE) ++evals;
         const bool is_init = state == ST_INIT;
         const bool ok = trialing && (f - f0 < 0.0001f * alpha * pg);
@@ -1489,9 +1569,13 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
             __syncthreads();
             continue;
         }
+        // the gyration radius is used only at the ends of line searches and
+        // of the start's measurement, all of which take this path
+        const float rg = gyration<L, NM>(s);
         if (wave_any(is_init || ls_end)) eval_grad<L, NM, GF>(ka, s, t);
         // ---------------------------------------------------------------- line search / BFGS bookkeeping
-        const float gg = gdot<L>(g, g);
+        // (each product below runs only when some group of the wavefront uses it)
+        const float gg = wave_any(ls_end) ? gdot<L>(g, g) : 0.f;
         if (trialing && !ok) alpha *= 0.5f;
         if (trialing && !ls_end) ++trial;
         const int step_now = step;
@@ -1526,12 +1610,14 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         copy_pose(s, X, XN, ls_end);
         copy_pose(s, XO, X, is_init);
         if constexpr (HREG) {
+            if (wave_any(is_init)) {
 #pragma unroll
-            for (int j = 0; j < KV; ++j)
+                for (int j = 0; j < KV; ++j)
 #pragma unroll
-                for (int c = 0; c < NM / HS; ++c)
-                    Hr[j][c] = is_init ? ((hess_col0<L, NM, HS>(s.gl) + c == hess_row<L, NM, HS>(s.gl, j)) ? 1.f : 0.f)
-                                       : Hr[j][c];
+                    for (int c = 0; c < NM / HS; ++c)
+                        Hr[j][c] = is_init ? ((hess_col0<L, NM, HS>(s.gl) + c == hess_row<L, NM, HS>(s.gl, j)) ? 1.f : 0.f)
+                                           : Hr[j][c];
+            }
         } else if (is_init) {
             for (int idx = s.gl; idx < nh; idx += L) H[idx] = 0.f;
             for (int i = s.gl; i < n; i += L) H[i * hs + i] = 1.f;
@@ -1546,12 +1632,14 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         }
         const bool scale0 = cont && step_now == 0 && fabsf(yy) > kEpsFl;
         if constexpr (HREG) {
-            const float dv = alpha * yp0 * __builtin_amdgcn_rcpf(yy);
+            if (wave_any(scale0)) {
+                const float dv = alpha * yp0 * __builtin_amdgcn_rcpf(yy);
 #pragma unroll
-            for (int j = 0; j < KV; ++j)
+                for (int j = 0; j < KV; ++j)
 #pragma unroll
-                for (int c = 0; c < NM / HS; ++c)
-                    Hr[j][c] = (scale0 && hess_col0<L, NM, HS>(s.gl) + c == hess_row<L, NM, HS>(s.gl, j)) ? dv : Hr[j][c];
+                    for (int c = 0; c < NM / HS; ++c)
+                        Hr[j][c] = (scale0 && hess_col0<L, NM, HS>(s.gl) + c == hess_row<L, NM, HS>(s.gl, j)) ? dv : Hr[j][c];
+            }
         } else if (scale0) {
             const float dv = alpha * yp0 * __builtin_amdgcn_rcpf(yy);
             for (int i = s.gl; i < n; i += L) H[i * hs + i] = dv;
@@ -1560,12 +1648,12 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         // rank-2 update (bfgs_update, bfgs.h:46-63); skipped on the last step,
         // where the reference's update is never used
         const bool upd_pre = cont && !finished;
-        const float yp = float(gdot_d<L>(y, p));
+        const float yp = wave_any(upd_pre) ? float(gdot_d<L>(y, p)) : 0.f;
         const bool do_upd = upd_pre && !(alpha * yp < kEpsFl);
-        if constexpr (HREG) minus_hmatvec_reg<L, NM, KV, HS>(s, Hr, y, mhy, n);
-        else minus_hmatvec<L, NM>(s, y, mhy, n);
-        const float yhy = -gdot<L>(y, mhy);
-        {
+        if (wave_any(do_upd)) {
+            if constexpr (HREG) minus_hmatvec_reg<L, NM, KV, HS>(s, Hr, y, mhy, n);
+            else minus_hmatvec<L, NM>(s, y, mhy, n);
+            const float yhy = -gdot<L>(y, mhy);
             float* ma = s.bc(0);
 #pragma unroll
             for (int j = 0; j < KV; ++j) {
@@ -1575,11 +1663,9 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
             }
             __syncthreads();
             if constexpr (HREG) {
-                if (wave_any(do_upd)) {
-                    const float r = __builtin_amdgcn_rcpf(alpha * yp);
-                    const float c1 = alpha * r, c2 = alpha * alpha * (r * r * yhy + r);
-                    hess_update_reg<L, NM, KV, HS>(s, Hr, mhy, p, do_upd, c1, c2, ma, pb);
-                }
+                const float r = __builtin_amdgcn_rcpf(alpha * yp);
+                const float c1 = alpha * r, c2 = alpha * alpha * (r * r * yhy + r);
+                hess_update_reg<L, NM, KV, HS>(s, Hr, mhy, p, do_upd, c1, c2, ma, pb);
             } else if (do_upd) {
                 const float r = __builtin_amdgcn_rcpf(alpha * yp);
                 const float c1 = alpha * r, c2 = alpha * alpha * (r * r * yhy + r);
@@ -1597,18 +1683,20 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         }
         // new search direction p = -H g (minus_mat_vec_product, bfgs.h:30-38)
         const bool new_dir = !finished && (is_init || cont);
-        if constexpr (HREG) minus_hmatvec_reg<L, NM, KV, HS>(s, Hr, g, t, n);
-        else minus_hmatvec<L, NM>(s, g, t, n);
-        if (new_dir) {
+        if (wave_any(new_dir)) {
+            if constexpr (HREG) minus_hmatvec_reg<L, NM, KV, HS>(s, Hr, g, t, n);
+            else minus_hmatvec<L, NM>(s, g, t, n);
+            if (new_dir) {
 #pragma unroll
-            for (int j = 0; j < KV; ++j) p[j] = t[j];
-        }
-        const float pg_new = gdot<L>(p, g);
-        if (new_dir) {
-            pg = pg_new;
-            alpha = 1.f;
-            trial = 0;
-            state = ST_TRIAL;
+                for (int j = 0; j < KV; ++j) p[j] = t[j];
+            }
+            const float pg_new = gdot<L>(p, g);
+            if (new_dir) {
+                pg = pg_new;
+                alpha = 1.f;
+                trial = 0;
+                state = ST_TRIAL;
+            }
         }
         // ---------------------------------------------------------------- BFGS end
         bool revert = false;
