@@ -15,13 +15,17 @@ Tolerances
                 the FP32 oracle reaches the FP64 minimum, less 10 %
   post-process  final energy = inter + w_rot TORSDOF, unbound = -intra, no
                 refinement, components equal to the rescoring of the pose
+  weight grad   dE/d(w_vdw, w_hb, w_elec, w_dsolv) vs central differences of
+                the FP64 oracle with maps and tables rebuilt per weight set
+                (h = 1e-3): 2e-3 * max(1, |dE/dw|_inf); with curl off and the
+                atoms in the box, sum_k w_k dE/dw_k = E (1e-4 relative)
 """
 import numpy as np
 import pytest
 
 from helpers import (ad4_map_types, box_of, oracle_scene_ad4, random_poses, small_ligands_ad4,
                      small_receptor_ad4)
-from oracle.oracle import AD4_MAP_DSOLV, AD4_MAP_ELEC
+from oracle.oracle import AD4_MAP_DSOLV, AD4_MAP_ELEC, AD4_WEIGHTS, Oracle
 from unidock_tools import _lib, synth
 
 pytestmark = pytest.mark.gpu
@@ -196,11 +200,118 @@ def test_dock_and_postprocess(sc):
             assert abs(m["inter"] - a) <= 1e-5 * abs(a) + 1e-4 + allow, (li, m["inter"], a)
 
 
+def _unit(p):
+    q = p.copy()
+    q[3:7] /= np.linalg.norm(q[3:7])
+    return q
+
+
+def _oracle_w(weights):
+    xyz, ad, q, cav = small_receptor_ad4()
+    ligs = small_ligands_ad4()
+    o = Oracle(prec=64, sf=2, weights=weights)
+    c, s = box_of(cav, EDGE)
+    o.set_grid(c, s, 0.375)
+    o.build_maps_ad4(xyz, ad, q, ad4_map_types(ligs))
+    for l in ligs:
+        o.add_ligand(l)
+    return o
+
+
+def test_weight_gradient_central_differences(sc):
+    _say("[gpu] ad4 energy_grad weight gradient vs FP64 central differences")
+    ctx = sc["ctx"]
+    pl, P = _poses(sc, 3, 21, 1.0)
+    P = np.array([_unit(p) for p in P])
+    e, _, gw = ctx.energy_grad(pl, P)
+    # the same evaluation as the search's
+    es, _ = ctx.score_grad(pl, P)
+    np.testing.assert_allclose(e, es, rtol=1e-6, atol=1e-5)
+    w0 = np.array(AD4_WEIGHTS, dtype=np.float64)
+    h = 1e-3
+    fd = np.zeros((len(pl), 4))
+    for k in range(4):
+        wp, wm = w0.copy(), w0.copy()
+        wp[k] += h
+        wm[k] -= h
+        op, om = _oracle_w(wp), _oracle_w(wm)
+        for i, li in enumerate(pl):
+            fd[i, k] = (op.eval_deriv(li, P[i])[0] - om.eval_deriv(li, P[i])[0]) / (2 * h)
+    _say(f"[gpu] ad4 |dE/dw|_inf per term: {np.abs(fd).max(axis=0).round(3).tolist()}")
+    for i in range(len(pl)):
+        s_ = max(1.0, np.abs(fd[i]).max())
+        np.testing.assert_allclose(gw[i, :4], fd[i], rtol=0, atol=2e-3 * s_, err_msg=f"pose {i}")
+    # glue joins no ligand pairs; the torsional weight is conformation independent
+    assert np.all(gw[:, 4:] == 0)
+    # vdW, electrostatic and desolvation are exercised on these poses
+    assert np.all(np.abs(fd[:, [0, 2, 3]]).max(axis=0) > 0.1)
+
+
+def test_weight_gradient_euler_identity(sc):
+    _say("[gpu] ad4 energy_grad Euler identity (curl off)")
+    ctx, o64 = sc["ctx"], sc["o64"]
+    lo = o64.geo[:3]
+    hi = lo + o64.geo[6:9] * (np.array(o64.n) - 1)
+    pl, P = _poses(sc, 4, 22, 1.0)
+    P = np.array([_unit(p) for p in P])
+    v = (3.4028234663852886e38,) * 3
+    e, _, gw = ctx.energy_grad(pl, P, v)
+    coords = ctx.pose_coords(pl, P)
+    w = np.array(AD4_WEIGHTS[:4], dtype=np.float64)
+    n_in = 0
+    for i, li in enumerate(pl):
+        c = coords[i, :sc["ligs"][li].n_atoms]
+        if np.all(c >= lo) and np.all(c <= hi):
+            n_in += 1
+            assert float(np.dot(w, gw[i, :4])) == pytest.approx(float(e[i]), rel=1e-4, abs=1e-4)
+    assert n_in >= 6
+
+
+def test_torch_autograd_ad4(sc):
+    """DockingEnergy under AD4.2: forward and backward equal ud_energy_grad on
+    the same maps; new weights rebuild the maps along the weight gradient."""
+    _say("[gpu] ad4 torch autograd energy")
+    import torch
+    from unidock_tools.torch_energy import DockingEnergy
+    xyz, ad, q, cav = small_receptor_ad4()
+    c, s = box_of(cav, EDGE)
+    model = DockingEnergy(xyz, ad, list(sc["ligs"]), c, s, sf=_lib.UD_SF_AD4, receptor_charge=q)
+    pl, P = _poses(sc, 3, 23, 1.0)
+    P = np.array([_unit(p) for p in P])
+    e_ref, gp_ref, gw_ref = sc["ctx"].energy_grad(pl, P)
+    k = 7 + max(l.n_tors for l in sc["ligs"])
+    poses = torch.tensor(P[:, :k], dtype=torch.float64, device="cuda", requires_grad=True)
+    w = torch.tensor(AD4_WEIGHTS, dtype=torch.float64, requires_grad=True)
+    e = model(poses, pl, w)
+    np.testing.assert_allclose(e.detach().cpu().numpy(), e_ref, rtol=1e-6, atol=1e-6)
+    r = torch.linspace(-1.0, 2.0, len(pl), dtype=torch.float64, device="cuda")
+    (e * r).sum().backward()
+    rn = r.cpu().numpy()[:, None]
+    np.testing.assert_allclose(poses.grad.cpu().numpy(), rn * gp_ref[:, :k], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(w.grad.numpy(), (rn * gw_ref[:, :len(AD4_WEIGHTS)]).sum(0), rtol=1e-4, atol=1e-4)
+    w2 = w.detach().clone()
+    w2[2] *= 1.01
+    e2 = model(poses.detach(), pl, w2).cpu().numpy()
+    pred = e_ref + gw_ref[:, 2] * float(w2[2] - w.detach()[2])
+    np.testing.assert_allclose(e2, pred, rtol=0, atol=1e-3 * max(1.0, np.abs(e_ref).max()))
+
+
 def test_state_errors(sc):
     ctx = sc["ctx"]
-    with pytest.raises(_lib.UdError) as ei:
-        ctx.energy_grad(np.zeros(1, np.int32), np.zeros((1, _lib.UD_POSE_FLOATS), np.float32))
-    assert ei.value.code == _lib.UD_ERR_STATE
+    # term-weight gradients need the receptor the maps were built from
+    c3 = _lib.Context(0)
+    try:
+        c3.set_scoring(_lib.UD_SF_AD4)
+        c, s = box_of(sc["cav"], EDGE)
+        n = c3.set_box(c, s, 0.375)
+        types = sc["types"] + [AD4_MAP_ELEC, AD4_MAP_DSOLV]
+        c3.set_maps(types, np.zeros(len(types) * int(np.prod(n)), dtype=np.float32))
+        c3.set_ligands(synth.LigandBatch(list(sc["ligs"])).arrays())
+        with pytest.raises(_lib.UdError) as ei:
+            c3.energy_grad(np.zeros(1, np.int32), np.zeros((1, _lib.UD_POSE_FLOATS), np.float32))
+        assert ei.value.code == _lib.UD_ERR_STATE
+    finally:
+        c3.close()
     with pytest.raises(_lib.UdError) as ei:
         ctx.refine(np.zeros(1, np.int32), np.zeros((1, _lib.UD_POSE_FLOATS), np.float32))
     assert ei.value.code == _lib.UD_ERR_STATE

@@ -124,10 +124,19 @@ struct ud_ctx {
     int32_t* d_cell_start = nullptr;
     size_t cells_cap = 0, cstart_cap = 0;
     int cell_dim[3] = {1, 1, 1};
-    // per-term maps and tables for term-weight gradients (built on demand)
+    // receptor atoms (x, y, z, AD type bits) and charges of the last
+    // ud_compute_maps_ad4: the input of the AD4.2 per-term maps
+    float4* d_rec_ad4 = nullptr;
+    float* d_rec_q = nullptr;
+    size_t rec_ad4_cap = 0, rec_q_cap = 0;
+    int n_rec_ad4 = 0;
+    bool have_rec_ad4 = false;
+    // per-term maps and tables for term-weight gradients (built on demand);
+    // AD4.2 adds the electrostatic / desolvation radial tables at unit weights
     float2* d_term_tab = nullptr;
     float* d_term_maps = nullptr;
-    size_t term_tab_cap = 0, term_maps_cap = 0;
+    float4* d_term_ad4 = nullptr;
+    size_t term_tab_cap = 0, term_maps_cap = 0, term_ad4_cap = 0;
     bool terms_ok = false;
     // ligands
     int n_lig = 0;
@@ -442,6 +451,7 @@ int ud_destroy(ud_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_table_deriv, c->d_table_fast, c->d_maps, c->d_rec, c->d_term_tab, c->d_term_maps, c->d_cells,
+                    c->d_rec_ad4, c->d_rec_q, c->d_term_ad4,
                     c->d_cell_start,
                     c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs, c->d_pair_ofs, c->d_partners, c->d_apar,
                     c->d_work, c->d_perm, c->d_poses, c->d_f1, c->d_f2, c->d_f3, c->d_i1, c->d_out_pose, c->d_out_e,
@@ -529,6 +539,7 @@ int ud_set_box(ud_ctx* c, const double* center, const double* size, double granu
     c->n_points = int(pts);
     c->box_set = true;
     c->have_rec = false;
+    c->have_rec_ad4 = false;
     c->terms_ok = false;
     if (c->d_maps) {
         (void)hipFree(c->d_maps);
@@ -671,12 +682,55 @@ static GridBuildArgs build_args(const ud_ctx* c, int n_types, const int32_t* typ
     return a;
 }
 
+// AD4.2 terms alone (vdW, H-bond, electrostatic, desolvation; ad4cache.cpp
+// and potentials.h:350-480): the type-pair tables, the affinity /
+// electrostatic / desolvation maps from the receptor of the last
+// ud_compute_maps_ad4, and the electrostatic / desolvation radial tables,
+// each at unit weight.  The builders are linear in the weights, so these are
+// the derivatives of the weighted maps and tables.
+static int ensure_terms_ad4(ud_ctx* c) {
+    if (!c->have_rec_ad4 || c->n_maps < 2)
+        return fail(c, UD_ERR_STATE, "term-weight gradients need maps built by ud_compute_maps_ad4");
+    const int nt = 4;
+    const size_t tab = size_t(kTypePairs) * kTableStride;
+    const size_t mp = size_t(c->n_maps) * c->n_points;
+    int rc = grow(c, c->d_term_tab, c->term_tab_cap, nt * tab);
+    if (rc) return rc;
+    rc = grow(c, c->d_term_maps, c->term_maps_cap, nt * mp);
+    if (rc) return rc;
+    rc = grow(c, c->d_term_ad4, c->term_ad4_cap, size_t(kAd4TableN));
+    if (rc) return rc;
+    float* d_fast = nullptr;
+    if (hipMalloc(&d_fast, sizeof(float) * tab) != hipSuccess) return fail(c, UD_ERR_NOMEM, "table allocation failed");
+    // the affinity maps in slot order, then the electrostatic and desolvation
+    // maps, as ud_compute_maps_ad4 stored them
+    const GridBuildArgs a = build_args(c, c->n_maps - 2, c->map_types);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < nt && e == hipSuccess; ++k) {
+        float w1[7] = {0, 0, 0, 0, 0, 0, 0};
+        w1[k] = 1.f;
+        e = launch_tables(UD_SF_AD4, w1, c->d_term_tab + k * tab, d_fast, c->stream);
+        if (e == hipSuccess)
+            e = launch_grid_build_ad4(c->d_rec_ad4, c->d_rec_q, c->n_rec_ad4, a, w1, c->d_term_maps + k * mp, c->stream);
+    }
+    std::vector<float4> deriv;
+    std::vector<float2> fast;
+    ad4_radial_tables(1.f, 1.f, deriv, fast);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_term_ad4, deriv.data(), sizeof(float4) * kAd4TableN, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_fast);
+    if (e != hipSuccess) return fail(c, UD_ERR_HIP, hipGetErrorString(e));
+    c->terms_ok = true;
+    return UD_OK;
+}
+
 // Maps and tables of each scoring term alone (its weight 1, the others 0),
 // from the receptor of the last ud_compute_maps: the inputs of the
 // term-weight gradients.  Built once per scoring / box / map set.
 static int ensure_terms(ud_ctx* c) {
-    if (sf_ad4(c)) return fail(c, UD_ERR_STATE, "term-weight gradients are built for Vina and Vinardo");
     if (c->terms_ok) return UD_OK;
+    if (sf_ad4(c)) return ensure_terms_ad4(c);
     if (!c->have_rec || c->n_maps == 0)
         return fail(c, UD_ERR_STATE, "term-weight gradients need maps built by ud_compute_maps");
     const int nt = (c->sf == UD_SF_VINA) ? 6 : 5;
@@ -766,6 +820,7 @@ int ud_compute_maps(ud_ctx* c, int n_rec, const float* rec_xyz, const int32_t* r
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
     c->last_ms = ms;
     c->last_launches = 1;
+    c->have_rec_ad4 = false;
     rc = store_maps(c, d_f32, n_types, types, precision);
     if (rc == UD_OK) c->have_rec = true;
     return rc;
@@ -785,6 +840,7 @@ int ud_set_maps(ud_ctx* c, int n_types, const int32_t* types, const float* value
         return fail(c, UD_ERR_NOMEM, "map allocation failed");
     UD_HIP(c, hipMemcpy(d_f32, values, sizeof(float) * size_t(n_types) * c->n_points, hipMemcpyHostToDevice));
     c->have_rec = false;
+    c->have_rec_ad4 = false;
     return store_maps(c, d_f32, n_types, types, precision);
 }
 
@@ -811,21 +867,23 @@ int ud_compute_maps_ad4(ud_ctx* c, int n_rec, const float* rec_xyz, const int32_
         q.push_back(rec_charge[i]);
     }
     c->have_rec = false;
-    float4* d_r = nullptr;
-    float* d_q = nullptr;
-    float* d_f32 = nullptr;
+    c->have_rec_ad4 = false;
     const size_t nr = std::max<size_t>(rec.size(), 1);
-    if (hipMalloc(&d_r, sizeof(float4) * nr) != hipSuccess || hipMalloc(&d_q, sizeof(float) * nr) != hipSuccess ||
-        hipMalloc(&d_f32, sizeof(float) * size_t(n_types + 2) * c->n_points) != hipSuccess) {
-        (void)hipFree(d_r);
-        (void)hipFree(d_q);
-        (void)hipFree(d_f32);
+    rc = grow(c, c->d_rec_ad4, c->rec_ad4_cap, nr);
+    if (rc) return rc;
+    rc = grow(c, c->d_rec_q, c->rec_q_cap, nr);
+    if (rc) return rc;
+    float* d_f32 = nullptr;
+    if (hipMalloc(&d_f32, sizeof(float) * size_t(n_types + 2) * c->n_points) != hipSuccess)
         return fail(c, UD_ERR_NOMEM, "map allocation failed");
-    }
     if (!rec.empty()) {
-        UD_HIP(c, hipMemcpyAsync(d_r, rec.data(), sizeof(float4) * rec.size(), hipMemcpyHostToDevice, c->stream));
-        UD_HIP(c, hipMemcpyAsync(d_q, q.data(), sizeof(float) * q.size(), hipMemcpyHostToDevice, c->stream));
+        UD_HIP(c, hipMemcpyAsync(c->d_rec_ad4, rec.data(), sizeof(float4) * rec.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        UD_HIP(c, hipMemcpyAsync(c->d_rec_q, q.data(), sizeof(float) * q.size(), hipMemcpyHostToDevice, c->stream));
     }
+    c->n_rec_ad4 = int(rec.size());
+    float4* d_r = c->d_rec_ad4;
+    float* d_q = c->d_rec_q;
     const GridBuildArgs a = build_args(c, n_types, types);
     UD_HIP(c, hipEventRecord(c->ev0, c->stream));
     UD_HIP(c, launch_grid_build_ad4(d_r, d_q, int(rec.size()), a, c->w, d_f32, c->stream));
@@ -835,12 +893,12 @@ int ud_compute_maps_ad4(ud_ctx* c, int n_rec, const float* rec_xyz, const int32_
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
     c->last_ms = ms;
     c->last_launches = 1;
-    (void)hipFree(d_r);
-    (void)hipFree(d_q);
     std::vector<int32_t> all(types, types + n_types);
     all.push_back(kAd4MapElec);
     all.push_back(kAd4MapDsolv);
-    return store_maps(c, d_f32, n_types + 2, all.data(), precision);
+    rc = store_maps(c, d_f32, n_types + 2, all.data(), precision);
+    if (rc == UD_OK) c->have_rec_ad4 = true;
+    return rc;
 }
 
 int ud_get_map(ud_ctx* c, int xs_type, float* out) {
@@ -1244,10 +1302,11 @@ int ud_energy_grad_device(ud_ctx* c, int n_poses, const int32_t* pose_ligand, co
     TermsDev td;
     td.maps = c->d_term_maps;
     td.tables = c->d_term_tab;
-    td.n_terms = (c->sf == UD_SF_VINA) ? 6 : 5;
+    td.ad4_unit = c->d_term_ad4;
+    td.n_terms = sf_ad4(c) ? 4 : (c->sf == UD_SF_VINA) ? 6 : 5;
     td.n_maps = c->n_maps;
     UD_HIP(c, hipEventRecord(c->ev0, c->stream));
-    UD_HIP(c, launch_energy_grad(lanes, c->precision, lig_dev(c), grid_dev(c), td, cap, c->d_work, int(work.size()),
+    UD_HIP(c, launch_energy_grad(lanes, grid_format(c), lig_dev(c), grid_dev(c), td, cap, c->d_work, int(work.size()),
                                  c->d_perm, d_poses, v[0], v[1], d_energy, d_pose_grad, d_weight_grad, c->stream));
     UD_HIP(c, hipEventRecord(c->ev1, c->stream));
     c->last_launches = 1;

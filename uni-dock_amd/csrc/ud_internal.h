@@ -158,7 +158,8 @@ struct RecDev {
 struct TermsDev {
     const float* maps;     // [term][slot][n_points], FP32, same slots as GridDev
     const float2* tables;  // [term][type pair][kTableStride] (e, dor); e is read
-    int32_t n_terms;       // 6 (Vina) or 5 (Vinardo), glue included
+    const float4* ad4_unit;  // AD4.2: electrostatic / desolvation radial tables at unit weights
+    int32_t n_terms;       // 6 (Vina) or 5 (Vinardo), glue included; 4 (AD4.2: vdW, H-bond, elec, desolvation)
     int32_t n_maps;
 };
 

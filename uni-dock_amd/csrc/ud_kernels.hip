@@ -1956,6 +1956,89 @@ __device__ __forceinline__ float trilinear(const float (&f)[8], const GridPt& q)
 //     energy's.  Out-of-box penalties do not depend on the weights.
 // gw_out holds UD_WEIGHT_FLOATS entries per pose in the order of the weights
 // array; the conf-independent (rot) entry is 0: the energy does not contain it.
+// dE/dw of the AD4.2 terms (vdW, H-bond, electrostatic, desolvation) for the
+// lane's atoms and pairs, at the coordinates in LDS.  Each atom's energy is
+// curl(A) + q curl(El) + |q| curl(D) over its affinity map A and the
+// electrostatic / desolvation maps El, D (ad4cache::eval_deriv,
+// ad4cache.cpp:223-262); each pair's is curl(type-pair table + q_a q_b E(r)
+// + c_ab G(r)) (precalculate_element::eval_deriv).  Maps and tables are
+// linear in the weights, so dE/dw_k is the curl slope of each weighted
+// value times its unit-weight counterpart.
+template <int L, int NM, int GF>
+__device__ void ad4_weight_grad(const KArgs& ka, const TermsDev& td, const Grp<L, NM>& s, float v_intra,
+                                float v_inter, float (&dw)[UD_WEIGHT_FLOATS]) {
+    const float4* coords = s.coords();
+    const GridDev& g = ka.grid;
+    const size_t map_stride = size_t(td.n_maps) * g.n_points;
+    const size_t sy = size_t(g.n[0]), sz = sy * g.n[1];
+    for (int i = s.gl; i < s.h.n_atoms; i += L) {
+        const AtomRec a = s.atoms[i];
+        const int slot = (a.meta >> 8) & 0xff;
+        if (slot == 0xff) continue;
+        const float4 ci = coords[i];
+        const float q = ka.ls.charge[s.h.atom_begin + i];
+        const GridPt gq = grid_locate(g, slot, make_float3(ci.x, ci.y, ci.z));
+        const uint32_t cell = gq.base - uint32_t(slot) * uint32_t(g.n_points);
+        const int sl[3] = {slot, g.elec_slot, g.dsolv_slot};
+        const float fac[3] = {1.f, q, fabsf(q)};
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            float f[8];
+            grid_gather<GF>(g, cell + uint32_t(sl[m]) * uint32_t(g.n_points), f);
+            float er = trilinear(f, gq);
+            if (Fmt<GF>::store == UD_GRID_F16) er *= g.scale[sl[m]];
+            const float c = fac[m] * curl_slope(er, v_inter);
+            for (int k = 0; k < td.n_terms; ++k) {
+                const float* M = td.maps + size_t(k) * map_stride + size_t(sl[m]) * g.n_points + cell;
+                const float t[8] = {M[0], M[1], M[sy], M[sy + 1], M[sz], M[sz + 1], M[sy + sz], M[sy + sz + 1]};
+                dw[k] += c * trilinear(t, gq);
+            }
+        }
+    }
+    const size_t tab_stride = size_t(kTypePairs) * kTableStride;
+    for (int q = s.gl; q < s.h.n_pairs; q += L) {
+        const PairRec rec = s.pairs[q];
+        const float4 ca = coords[pair_a(rec)], cb = coords[pair_b(rec)];
+        const float rx = cb.x - ca.x, ry = cb.y - ca.y, rz = cb.z - ca.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (!(r2 < pair_cut2<GF>())) continue;
+        const float rf = 32.f * r2;
+        const int it = int(rf);
+        const float rem = rf - float(it);
+        const size_t idx = size_t(pair_slot(rec)) * kTableStride + it;
+        // the weighted pair energy, as the search evaluates it
+        float pe = 0.f, dor = 0.f;
+        float2 t0 = make_float2(0.f, 0.f), t1 = t0;
+        if (it < kTableStride - 1) {
+            t0 = ka.ls.table_deriv[idx];
+            t1 = ka.ls.table_deriv[idx + 1];
+        }
+        pe = t0.x + rem * (t1.x - t0.x);
+        const float4 u0 = ka.ls.ad4_deriv[it], u1 = ka.ls.ad4_deriv[it + 1];
+        const float2 cf = ka.ls.pair_coef[s.h.pair_begin + q];
+        ad4_charge_terms(cf, u0, u1, rem, pe, dor);
+        const float c = curl_slope(pe, v_intra);
+        // unit-weight parts: type-pair tables of vdW and H-bond, and the
+        // electrostatic / desolvation radial samples times the pair's coefficients
+        const float4 w0 = td.ad4_unit[it], w1 = td.ad4_unit[it + 1];
+        float part[4];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            float e0 = 0.f, e1 = 0.f;
+            if (it < kTableStride - 1) {
+                const float2* T = td.tables + size_t(k) * tab_stride + idx;
+                e0 = T[0].x;
+                e1 = T[1].x;
+            }
+            part[k] = e0 + rem * (e1 - e0);
+        }
+        part[2] = cf.x * (w0.x + rem * (w1.x - w0.x));
+        part[3] = cf.y * (w0.z + rem * (w1.z - w0.z));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) dw[k] += c * part[k];
+    }
+}
+
 template <int L, int GF>
 __global__ __launch_bounds__(64) void energy_grad_kernel(KArgs ka, TermsDev td, const WorkItem* work,
                                                          const int32_t* perm, const float* poses, float v_intra,
@@ -2011,6 +2094,9 @@ __global__ __launch_bounds__(64) void energy_grad_kernel(KArgs ka, TermsDev td, 
     for (int k = 0; k < UD_WEIGHT_FLOATS; ++k) dw[k] = 0.f;
     const float4* coords = s.coords();
     const size_t map_stride = size_t(td.n_maps) * ka.grid.n_points;
+    if constexpr (Fmt<GF>::ad4) {
+        ad4_weight_grad<L, kNMax, GF>(ka, td, s, v_intra, v_inter, dw);
+    } else {
     for (int i = s.gl; i < s.h.n_atoms; i += L) {
         const AtomRec a = s.atoms[i];
         const int slot = (a.meta >> 8) & 0xff;
@@ -2051,6 +2137,7 @@ __global__ __launch_bounds__(64) void energy_grad_kernel(KArgs ka, TermsDev td, 
             const float e0 = T[0].x, e1 = T[1].x;
             dw[k] += c * (e0 + rem * (e1 - e0));
         }
+    }
     }
 #pragma unroll
     for (int k = 0; k < UD_WEIGHT_FLOATS; ++k) dw[k] = gsum<L>(dw[k]);
@@ -2442,6 +2529,12 @@ hipError_t launch_energy_grad(int lanes, int gf, const LigandSetDev& ls, const G
     UD_DISPATCH_L(lanes, {
         if (gf == UD_GRID_F16)
             hipLaunchKernelGGL((energy_grad_kernel<L, UD_GRID_F16>), dim3(n_work), dim3(kWG), shm, st, ka, td, work,
+                               perm, poses, v_intra, v_inter, e_out, gpose_out, gw_out);
+        else if (gf == kGridAd4F32)
+            hipLaunchKernelGGL((energy_grad_kernel<L, kGridAd4F32>), dim3(n_work), dim3(kWG), shm, st, ka, td, work,
+                               perm, poses, v_intra, v_inter, e_out, gpose_out, gw_out);
+        else if (gf == kGridAd4F16)
+            hipLaunchKernelGGL((energy_grad_kernel<L, kGridAd4F16>), dim3(n_work), dim3(kWG), shm, st, ka, td, work,
                                perm, poses, v_intra, v_inter, e_out, gpose_out, gw_out);
         else
             hipLaunchKernelGGL((energy_grad_kernel<L, UD_GRID_F32>), dim3(n_work), dim3(kWG), shm, st, ka, td, work,

@@ -19,6 +19,11 @@ Usage::
 
 Weights that differ from the ones the maps were built with rebuild the
 tables and maps (set_scoring + compute_maps) before the evaluation.
+
+With sf = UD_SF_AD4 the receptor types are AutoDock types, receptor_charge
+gives the receptor's partial charges, the ligands carry AutoDock types and
+charges, and the weights are AD4.2's (vdW, H-bond, electrostatic,
+desolvation, glue, torsional; the last two have zero gradient).
 """
 from __future__ import annotations
 
@@ -61,14 +66,15 @@ class _EnergyFn(torch.autograd.Function):
 class DockingEnergy(torch.nn.Module):
     """Energy of poses of a ligand batch against one receptor, on one GPU.
 
-    receptor_xyz (n, 3) and receptor_xs (n,) X-Score types; ligands: a list of
-    ligand objects (unidock_tools.synth.Ligand or ParsedLigand) or an arrays
-    dict of LigandBatch / LigandSet; center and size of the search box (A).
+    receptor_xyz (n, 3) and receptor_xs (n,) X-Score types (AutoDock types
+    with sf = UD_SF_AD4, plus receptor_charge); ligands: a list of ligand
+    objects (unidock_tools.synth.Ligand or ParsedLigand) or an arrays dict of
+    LigandBatch / LigandSet; center and size of the search box (A).
     """
 
     def __init__(self, receptor_xyz, receptor_xs, ligands, center, size, granularity=0.375,
                  sf: int = _lib.UD_SF_VINA, weights=None, precision: int = _lib.UD_GRID_F32, device: int = 0,
-                 v=(1000.0, 1000.0, 1000.0)):
+                 v=(1000.0, 1000.0, 1000.0), receptor_charge=None):
         super().__init__()
         from . import synth
         self.device = torch.device("cuda", device)
@@ -80,10 +86,15 @@ class DockingEnergy(torch.nn.Module):
         self._rec = (np.ascontiguousarray(receptor_xyz, dtype=np.float32),
                      np.ascontiguousarray(receptor_xs, dtype=np.int32))
         arrays = ligands if isinstance(ligands, dict) else synth.LigandBatch(list(ligands)).arrays()
-        xs = np.asarray(arrays["atom_xs"])
-        self.types = sorted({_lib.grid_type(int(t)) for t in xs} - {-1})
+        if sf == _lib.UD_SF_AD4:
+            if receptor_charge is None:
+                raise ValueError("AD4.2 scoring needs receptor_charge")
+            self._rec_q = np.ascontiguousarray(receptor_charge, dtype=np.float32)
+            self.types = sorted({_lib.ad4_grid_type(int(t)) for t in np.asarray(arrays["atom_ad"])} - {-1})
+        else:
+            self.types = sorted({_lib.grid_type(int(t)) for t in np.asarray(arrays["atom_xs"])} - {-1})
         self.ctx.set_box(center, size, granularity)
-        w0 = _lib.VINA_WEIGHTS if sf == _lib.UD_SF_VINA else _lib.VINARDO_WEIGHTS
+        w0 = _lib.DEFAULT_WEIGHTS[sf]
         self._w = None
         self._use_weights(torch.tensor(w0 if weights is None else weights, dtype=torch.float64))
         self.ctx.set_ligands(arrays)
@@ -93,7 +104,10 @@ class DockingEnergy(torch.nn.Module):
         if w == self._w:
             return
         self.ctx.set_scoring(self.sf, w)
-        self.ctx.compute_maps(self._rec[0], self._rec[1], self.types, self.precision)
+        if self.sf == _lib.UD_SF_AD4:
+            self.ctx.compute_maps_ad4(self._rec[0], self._rec[1], self._rec_q, self.types, self.precision)
+        else:
+            self.ctx.compute_maps(self._rec[0], self._rec[1], self.types, self.precision)
         self._w = w
 
     def _check_stream(self):
