@@ -500,14 +500,15 @@ def main():
     exh = args.exhaustiveness
 
     ctx16 = setup_ctx(rec, ligs, _lib.UD_GRID_F16)
-    tuned = None
+    tuned = tuned_pb = None
     if args.lanes == 0:
         t1 = time.perf_counter()
         # whole buckets, 2 MC steps: a sample must fill the GPU as the real
         # launch does, or lane counts with more workgroups win by occupancy
         tuned = [int(x) for x in ctx16.autotune(exhaustiveness=exh, global_steps=2, sample_ligands=len(ligs))]
-        log(f"[bench] autotuned lanes per bucket {dict(zip(_lib.BUCKET_ATOMS, tuned))} "
-            f"in {time.perf_counter() - t1:.1f}s (untimed set-up)")
+        tuned_pb = [int(x) for x in ctx16.bucket_layout()[1]]
+        log(f"[bench] autotuned (lanes, per-pair LDS buffer) per bucket "
+            f"{dict(zip(_lib.BUCKET_ATOMS, zip(tuned, tuned_pb)))} in {time.perf_counter() - t1:.1f}s (untimed set-up)")
     log("[bench] batched FP16 form")
     b = time_form(ctx16, len(ligs), exh, gsteps, args.lanes, args.steps, args.warmup, 181129)
     ev_lig = b["evals"].reshape(len(ligs), exh).sum(axis=1)
@@ -584,6 +585,7 @@ def main():
         "kernel_ms_per_step": b["kernel_ms"],
         "launches": b["launches"],
         "tuned_lanes": dict(zip([str(c) for c in _lib.BUCKET_ATOMS], tuned)) if tuned else None,
+        "tuned_pair_buffer": dict(zip([str(c) for c in _lib.BUCKET_ATOMS], tuned_pb)) if tuned else None,
         "dominant_kernel": {"name": f"search_kernel<{dom['lanes']}, fp16 maps, MC>",
                             "bucket_atoms": dom["atom_cap"], "avg_ms": dom["ms"],
                             "algorithmic_bytes": bytes_dom},

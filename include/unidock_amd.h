@@ -85,6 +85,8 @@ typedef struct ud_launch_info {
     int32_t lanes;         /* lanes per chain */
     int32_t n_ligands;
     int32_t n_workgroups;  /* one wavefront each */
+    int32_t pair_buffer;   /* 1: pair-parallel forces through a per-pair LDS buffer; 0: per-atom pair loops */
+    int32_t lds_bytes;     /* LDS per workgroup */
     double ms;             /* kernel time, HIP events around the launch on its stream */
 } ud_launch_info;
 
@@ -316,13 +318,27 @@ int ud_last_launches(const ud_ctx* ctx, int max_launches, ud_launch_info* out, i
  * (monte_carlo.cu:117-118, 2877-2882). */
 int ud_set_bucket_lanes(ud_ctx* ctx, const int32_t* lanes /* [UD_N_BUCKETS] */);
 
+/* LDS layout of each size bucket for ud_dock: 1 = intramolecular pair
+ * forces evaluated once per pair through a per-pair buffer in LDS (when it
+ * fits), 0 = per-atom pair loops without that buffer (each pair evaluated by
+ * both of its atoms; less LDS per workgroup, more resident workgroups for
+ * large ligands), -1 = built-in default (the buffer when it fits).  The
+ * reference keeps its per-thread pair work in registers and global memory
+ * (kernel.h:98-277 sizes its shared arrays per size class). */
+int ud_set_bucket_pair_buffer(ud_ctx* ctx, const int32_t* pair_buffer /* [UD_N_BUCKETS] */);
+
+/* The current per-bucket settings: lanes per chain (0 = default) and LDS
+ * layout (-1, 0, 1 as ud_set_bucket_pair_buffer).  Either pointer may be NULL. */
+int ud_get_bucket_layout(const ud_ctx* ctx, int32_t* lanes_out, int32_t* pair_buffer_out);
+
 /* Autotuner: for every size bucket of the current ligand set, docks an
  * evenly spaced sample of up to sample_ligands ligands with params at 8, 16,
- * 32 and 64 lanes per chain (those that fit the LDS), keeps the lane count
- * with the most energy+gradient evaluations per second and stores it as the
- * bucket's setting (also written to lanes_out[UD_N_BUCKETS] if non-NULL).
- * Discards earlier docking results.  No counterpart in the reference (fixed
- * 32 threads per chain). */
+ * 32 and 64 lanes per chain, each with and without the per-pair LDS buffer
+ * (the layouts that fit the LDS), keeps the combination with the most
+ * energy+gradient evaluations per second and stores it as the bucket's
+ * setting (lanes also written to lanes_out[UD_N_BUCKETS] if non-NULL;
+ * ud_get_bucket_layout reads both).  Discards earlier docking results.  No
+ * counterpart in the reference (fixed 32 threads per chain). */
 int ud_autotune(ud_ctx* ctx, const ud_search_params* params, int sample_ligands, int32_t* lanes_out);
 
 /* One output mode of a ligand after ud_postprocess. */

@@ -363,16 +363,19 @@ def test_capacity_and_state_errors(scene):
 
 
 def test_autotune_and_bucket_lanes(scene):
-    """The autotuner stores one supported lane count per occupied size bucket;
-    docking with the tuned setting equals docking with the same lanes set by
-    hand, and the per-launch timings cover the whole dock."""
+    """The autotuner stores one supported lane count and LDS layout (per-pair
+    force buffer or not) per occupied size bucket; docking with the tuned
+    setting equals docking with the same lanes set by hand, and the
+    per-launch timings cover the whole dock."""
     _say("[gpu] autotune")
     ctx = scene["ctx"]
     nat = [l.n_atoms for l in scene["ligs"]]
     occupied = {min(b for b in _lib.BUCKET_ATOMS if n <= b) for n in nat}
     tuned = ctx.autotune(exhaustiveness=8, global_steps=1, sample_ligands=4)
-    for cap, lanes in zip(_lib.BUCKET_ATOMS, tuned):
-        assert (lanes in (8, 16, 32, 64)) if cap in occupied else lanes == 0
+    lanes_l, pb_l = ctx.bucket_layout()
+    assert list(lanes_l) == list(tuned)
+    for cap, lanes, pb in zip(_lib.BUCKET_ATOMS, tuned, pb_l):
+        assert (lanes in (8, 16, 32, 64) and pb in (0, 1)) if cap in occupied else (lanes == 0 and pb == -1)
     with pytest.raises(_lib.UdError) as ei:
         ctx.fetch_dock()
     assert ei.value.code == _lib.UD_ERR_STATE
@@ -382,7 +385,8 @@ def test_autotune_and_bucket_lanes(scene):
     assert n == len(li) == len(occupied)
     assert [a["atom_cap"] for a in li] == sorted(occupied, reverse=True)
     for a in li:
-        assert a["lanes"] == tuned[_lib.BUCKET_ATOMS.index(a["atom_cap"])] and a["ms"] > 0
+        b = _lib.BUCKET_ATOMS.index(a["atom_cap"])
+        assert a["lanes"] == tuned[b] and a["pair_buffer"] == pb_l[b] and a["ms"] > 0 and a["lds_bytes"] > 0
     # bucket launches overlap on their own streams: each lies inside the dock's span
     assert max(a["ms"] for a in li) <= total_ms * (1 + 1e-3) + 1e-2
     assert total_ms <= sum(a["ms"] for a in li) * (1 + 1e-3) + 0.5
@@ -394,6 +398,22 @@ def test_autotune_and_bucket_lanes(scene):
         ctx.set_bucket_lanes([12, 0, 0, 0, 0, 0])
     assert ei.value.code == _lib.UD_ERR_INVALID
     ctx.set_bucket_lanes([0] * _lib.UD_N_BUCKETS)
+    # both LDS layouts on demand: the per-atom pair loops use less LDS and
+    # search alike (pooled chain-best energies, KS test)
+    res = {}
+    for pb in (1, 0):
+        ctx.set_bucket_pair_buffer([pb] * _lib.UD_N_BUCKETS)
+        _, e, _ = ctx.dock(exhaustiveness=32, global_steps=4, seed=8)
+        li = ctx.last_launches()
+        assert all(a["pair_buffer"] == pb for a in li)
+        res[pb] = (e, sum(a["lds_bytes"] for a in li))
+    assert res[0][1] < res[1][1]
+    from scipy.stats import ks_2samp
+    assert ks_2samp(res[0][0], res[1][0]).pvalue > 1e-3
+    with pytest.raises(_lib.UdError) as ei:
+        ctx.set_bucket_pair_buffer([2] * _lib.UD_N_BUCKETS)
+    assert ei.value.code == _lib.UD_ERR_INVALID
+    ctx.set_bucket_pair_buffer([-1] * _lib.UD_N_BUCKETS)
 
 
 def test_large_ligand_without_pair_buffer(scene):

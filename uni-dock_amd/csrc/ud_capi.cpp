@@ -206,6 +206,7 @@ struct ud_ctx {
     ud_launch_info linfo[UD_N_BUCKETS] = {};
     // lanes per chain per size bucket (ud_autotune / ud_set_bucket_lanes; 0 = default)
     int tuned_lanes[UD_N_BUCKETS] = {};
+    int tuned_pb[UD_N_BUCKETS] = {-1, -1, -1, -1, -1, -1, -1, -1};  // per-pair LDS buffer: -1 default, 0 off, 1 on
     int forced_lanes = 0;  // ud_set_lanes
 };
 
@@ -287,14 +288,15 @@ LigandSetDev lig_dev(const ud_ctx* c) {
     return l;
 }
 
-// largest lanes-per-group whose LDS footprint fits one workgroup (64 KiB)
-int fit_lanes(int lanes, LaunchCap& cap) {
-    // the per-pair force buffer is used when the launch still fits with it
+// largest lanes-per-group whose LDS footprint fits one workgroup (64 KiB);
+// the per-pair force buffer is used when the launch still fits with it,
+// unless pair_buffer is 0
+int fit_lanes(int lanes, LaunchCap& cap, int pair_buffer = -1) {
     cap.pair_forces = 0;
     while (lanes < 64 && lds_bytes(lanes, cap) > 65536) lanes *= 2;
     LaunchCap with = cap;
     with.pair_forces = 1;
-    if (lds_bytes(lanes, with) <= 65536) cap.pair_forces = 1;
+    if (pair_buffer != 0 && lds_bytes(lanes, with) <= 65536) cap.pair_forces = 1;
     return lanes;
 }
 
@@ -1411,7 +1413,9 @@ int check_search_params(ud_ctx* c, const ud_search_params* p) {
 // ligand * exhaustiveness + chain of the context's output buffers, so a
 // subset run leaves the other ligands' slots untouched.  lanes_all != 0
 // forces the lanes per chain of every bucket.
-int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subset, int lanes_all) {
+// lanes_all / pb_all: one lane count and LDS layout for every bucket (the
+// autotuner's trials), else each bucket's setting (lanes_all 0, pb_all -2)
+int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subset, int lanes_all, int pb_all = -2) {
     UD_HIP(c, hipSetDevice(c->device));
     int rc = refresh_slots(c);
     if (rc) return rc;
@@ -1469,7 +1473,7 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
         int lanes = lanes_all ? lanes_all
                               : (p->lanes_per_chain ? p->lanes_per_chain
                                                     : (c->tuned_lanes[bi] ? c->tuned_lanes[bi] : default_lanes(cap.n_atoms)));
-        lanes = fit_lanes(lanes, cap);
+        lanes = fit_lanes(lanes, cap, pb_all != -2 ? pb_all : (p->single_launch ? -1 : c->tuned_pb[bi]));
         if (lds_bytes(lanes, cap) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
         const int G = 64 / lanes;
         // larger ligands first inside the bucket (tail balance)
@@ -1490,6 +1494,8 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
         li.lanes = lanes;
         li.n_ligands = int(order.size());
         li.n_workgroups = int(w.size());
+        li.pair_buffer = cap.pair_forces;
+        li.lds_bytes = int(lds_bytes(lanes, cap));
         li.ms = 0.0;
         works.push_back(std::move(w));
         lanes_v.push_back(lanes);
@@ -1573,6 +1579,23 @@ int ud_set_bucket_lanes(ud_ctx* c, const int32_t* lanes) {
     return UD_OK;
 }
 
+int ud_set_bucket_pair_buffer(ud_ctx* c, const int32_t* pair_buffer) {
+    if (!c || !pair_buffer) return UD_ERR_INVALID;
+    for (int b = 0; b < UD_N_BUCKETS; ++b)
+        if (pair_buffer[b] < -1 || pair_buffer[b] > 1) return fail(c, UD_ERR_INVALID, "pair buffer must be -1, 0 or 1");
+    for (int b = 0; b < UD_N_BUCKETS; ++b) c->tuned_pb[b] = pair_buffer[b];
+    return UD_OK;
+}
+
+int ud_get_bucket_layout(const ud_ctx* c, int32_t* lanes_out, int32_t* pair_buffer_out) {
+    if (!c) return UD_ERR_INVALID;
+    for (int b = 0; b < UD_N_BUCKETS; ++b) {
+        if (lanes_out) lanes_out[b] = c->tuned_lanes[b];
+        if (pair_buffer_out) pair_buffer_out[b] = c->tuned_pb[b];
+    }
+    return UD_OK;
+}
+
 int ud_autotune(ud_ctx* c, const ud_search_params* p, int sample_ligands, int32_t* lanes_out) {
     if (!c || !p || sample_ligands < 1) return UD_ERR_INVALID;
     int rc = check_search_params(c, p);
@@ -1590,30 +1613,37 @@ int ud_autotune(ud_ctx* c, const ud_search_params* p, int sample_ligands, int32_
         const int m = std::min<int>(sample_ligands, int(bl[bi].size()));
         for (int k = 0; k < m; ++k) sub.push_back(bl[bi][size_t(k) * bl[bi].size() / m]);
         LaunchCap cap = caps_of(c, &sub);
-        int best = 0;
+        int best = 0, best_pb = -1;
         double best_rate = -1.0;
         for (int lanes : {8, 16, 32, 64}) {
             if (fit_lanes(lanes, cap) != lanes) continue;
-            double ms_min = 1e300;
-            long long tot = 0;
-            for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and code
-                rc = dock_run(c, p, &sub, lanes);
-                if (rc) return rc;
-                UD_HIP(c, hipStreamSynchronize(c->stream));
-                dock_times(c);
-                ms_min = std::min(ms_min, c->last_ms);
-            }
-            ev.resize(c->out_n);
-            UD_HIP(c, hipMemcpy(ev.data(), c->d_out_evals, ev.size() * sizeof(long long), hipMemcpyDeviceToHost));
-            for (int l : sub)
-                for (int k = 0; k < exh; ++k) tot += ev[size_t(l) * exh + k];
-            const double rate = double(tot) / std::max(ms_min, 1e-6);  // evaluations per ms
-            if (rate > best_rate) {
-                best_rate = rate;
-                best = lanes;
+            for (int pb : {1, 0}) {
+                LaunchCap trial = cap;
+                (void)fit_lanes(lanes, trial, pb);
+                if (trial.pair_forces != pb) continue;  // the buffer does not fit at this lane count
+                double ms_min = 1e300;
+                long long tot = 0;
+                for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and code
+                    rc = dock_run(c, p, &sub, lanes, pb);
+                    if (rc) return rc;
+                    UD_HIP(c, hipStreamSynchronize(c->stream));
+                    dock_times(c);
+                    ms_min = std::min(ms_min, c->last_ms);
+                }
+                ev.resize(c->out_n);
+                UD_HIP(c, hipMemcpy(ev.data(), c->d_out_evals, ev.size() * sizeof(long long), hipMemcpyDeviceToHost));
+                for (int l : sub)
+                    for (int k = 0; k < exh; ++k) tot += ev[size_t(l) * exh + k];
+                const double rate = double(tot) / std::max(ms_min, 1e-6);  // evaluations per ms
+                if (rate > best_rate) {
+                    best_rate = rate;
+                    best = lanes;
+                    best_pb = pb;
+                }
             }
         }
         c->tuned_lanes[bi] = best;
+        c->tuned_pb[bi] = best_pb;
     }
     if (lanes_out)
         for (int b = 0; b < UD_N_BUCKETS; ++b) lanes_out[b] = c->tuned_lanes[b];

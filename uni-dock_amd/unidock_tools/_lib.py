@@ -52,7 +52,8 @@ class UdLigandBatch(ctypes.Structure):
 
 class UdLaunchInfo(ctypes.Structure):
     _fields_ = [("atom_cap", ctypes.c_int32), ("lanes", ctypes.c_int32), ("n_ligands", ctypes.c_int32),
-                ("n_workgroups", ctypes.c_int32), ("ms", ctypes.c_double)]
+                ("n_workgroups", ctypes.c_int32), ("pair_buffer", ctypes.c_int32), ("lds_bytes", ctypes.c_int32),
+                ("ms", ctypes.c_double)]
 
 
 class UdMode(ctypes.Structure):
@@ -123,6 +124,8 @@ def load() -> ctypes.CDLL:
         "ud_set_lanes": (ctypes.c_int, [vp, ctypes.c_int]),
         "ud_last_launches": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(UdLaunchInfo), ip]),
         "ud_set_bucket_lanes": (ctypes.c_int, [vp, ip]),
+        "ud_set_bucket_pair_buffer": (ctypes.c_int, [vp, ip]),
+        "ud_get_bucket_layout": (ctypes.c_int, [vp, ip, ip]),
         "ud_autotune": (ctypes.c_int, [vp, ctypes.POINTER(UdSearchParams), ctypes.c_int, ip]),
         "ud_postprocess": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(UdMode), ip]),
         "ud_refine": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, ctypes.c_int, fp, ip]),
@@ -701,12 +704,25 @@ class Context:
         n = ctypes.c_int32()
         self._ck(self.L.ud_last_launches(self.h, UD_N_BUCKETS, info, ctypes.byref(n)))
         return [dict(atom_cap=i.atom_cap, lanes=i.lanes, n_ligands=i.n_ligands, n_workgroups=i.n_workgroups,
-                     ms=i.ms) for i in info[:n.value]]
+                     pair_buffer=i.pair_buffer, lds_bytes=i.lds_bytes, ms=i.ms) for i in info[:n.value]]
 
     def set_bucket_lanes(self, lanes):
         a = np.zeros(UD_N_BUCKETS, dtype=np.int32)
         a[:len(lanes)] = lanes
         self._ck(self.L.ud_set_bucket_lanes(self.h, _p(a, ctypes.c_int32)))
+
+    def set_bucket_pair_buffer(self, pair_buffer):
+        """Per-bucket LDS layout: 1 per-pair force buffer, 0 per-atom pair loops, -1 default."""
+        a = np.full(UD_N_BUCKETS, -1, dtype=np.int32)
+        a[:len(pair_buffer)] = pair_buffer
+        self._ck(self.L.ud_set_bucket_pair_buffer(self.h, _p(a, ctypes.c_int32)))
+
+    def bucket_layout(self):
+        """(lanes per chain, pair-buffer setting) per size bucket."""
+        lanes = np.zeros(UD_N_BUCKETS, dtype=np.int32)
+        pb = np.zeros(UD_N_BUCKETS, dtype=np.int32)
+        self._ck(self.L.ud_get_bucket_layout(self.h, _p(lanes, ctypes.c_int32), _p(pb, ctypes.c_int32)))
+        return lanes, pb
 
     def autotune(self, exhaustiveness=8, global_steps=2, seed=181129, local_steps=0, sample_ligands=1 << 20):
         """Pick lanes per chain per size bucket by timing short docks; returns the chosen lanes."""
