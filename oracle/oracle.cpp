@@ -1,8 +1,8 @@
 // oracle.cpp -- CPU restatement of Uni-Dock's batched pose-search path.
 //
 // TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
-// cpu_baseline leg of bench.py may load this library, and only as the checker
-// (or as the timed CPU baseline).  The product (uni-dock_amd/) never links,
+// cpu_baseline leg of bench.py may load this library, and only to check the
+// GPU's results (or as the timed CPU baseline).  The product (uni-dock_amd/) never links,
 // loads or calls it.
 //
 // Every function is templated on the real type R and instantiated twice:

@@ -1,8 +1,8 @@
 """ctypes wrapper of the CPU oracle (liboracle.so built from oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker / the timed CPU baseline,
-never by the product.
+cpu_baseline leg of bench.py, always to check the GPU's results or as the timed
+CPU baseline, never by the product.
 """
 from __future__ import annotations
 
