@@ -1,6 +1,6 @@
 """CPU restatement of the docking post-processing (numpy).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the library's
+TEST INFRASTRUCTURE ONLY: imported by tests/ to check the library's
 post-processing (ud_cluster_modes / ud_postprocess), never by the product.
 
 Follows Vina::global_search_gpu after do_docking (unidock/src/lib/vina.h:232-345)
