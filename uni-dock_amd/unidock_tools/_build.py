@@ -49,7 +49,30 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_host(verbose)
     return LIB
+
+
+HOST_DIR = os.path.join(os.path.dirname(CSRC), "host")
+HOST_LIB = os.path.join(HOST_DIR, "libunidock_amd_vina.so")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The Vina-style C++ host (host/unidock_vina.cpp) as a shared library
+    linked against libunidock_amd.so: host code, built with g++."""
+    include = os.path.join(os.path.dirname(os.path.dirname(CSRC)), "include")
+    libdir = os.path.dirname(LIB)
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Werror", "-I", include, "-I", HOST_DIR,
+           os.path.join(HOST_DIR, "unidock_vina.cpp"), "-o", HOST_LIB + ".tmp", "-L", libdir, "-lunidock_amd",
+           "-Wl,-rpath,$ORIGIN/../unidock_tools"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build of libunidock_amd_vina.so failed")
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    if verbose:
+        print("built", HOST_LIB)
+    return HOST_LIB
 
 
 if __name__ == "__main__":
