@@ -178,6 +178,9 @@ struct ud_ctx {
     size_t kept_cap = 0;
     int32_t* d_nkept = nullptr;
     size_t nkept_cap = 0;
+    float* d_kp = nullptr;  // kept chains' poses and energies, one slot per (ligand, mode)
+    float* d_ke = nullptr;
+    size_t kp_cap = 0, ke_cap = 0;
     int32_t* d_elem = nullptr;
     size_t elem_cap = 0;
     float* d_mc = nullptr;
@@ -458,7 +461,7 @@ int ud_destroy(ud_ctx* c) {
                     c->d_hdr, c->d_atoms, c->d_nodes, c->d_pairs, c->d_pair_ofs, c->d_partners, c->d_apar,
                     c->d_work, c->d_perm, c->d_poses, c->d_f1, c->d_f2, c->d_f3, c->d_i1, c->d_out_pose, c->d_out_e,
                     c->d_out_evals, c->d_ad4_deriv, c->d_ad4_fast, c->d_charge, c->d_pair_coef, c->d_cc, c->d_kept,
-                    c->d_nkept, c->d_elem, c->d_mc, c->d_mlig, c->d_mref, c->d_ub, c->d_lb};
+                    c->d_nkept, c->d_kp, c->d_ke, c->d_elem, c->d_mc, c->d_mlig, c->d_mref, c->d_ub, c->d_lb};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1884,7 +1887,6 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, u
     if (refine_step < 0 || refine_step > 8) return fail(c, UD_ERR_INVALID, "refine_step must be in 0..8");
     if (c->out_n == 0 || c->last_exh <= 0) return fail(c, UD_ERR_STATE, "no docking results");
     const int exh = c->last_exh, L = c->n_lig;
-    const size_t nch = size_t(L) * exh;
     UD_HIP(c, hipSetDevice(c->device));
     // every chain's best-pose coordinates, packed per ligand, then the
     // clustering of each ligand's chains, on the device
@@ -1910,22 +1912,32 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, u
         UD_HIP(c, launch_cluster(lig_dev(c), L, c->d_cc, c->d_out_e, exh, max_modes, float(min_rmsd), c->d_kept,
                                  c->d_nkept, c->stream));
     }
-    std::vector<int32_t> kept_all(size_t(L) * max_modes), nkept(L);
+    // only the kept chains' poses and energies come back
+    const size_t slots = size_t(L) * max_modes;
+    rc = grow(c, c->d_kp, c->kp_cap, slots * UD_POSE_FLOATS);
+    if (rc) return rc;
+    rc = grow(c, c->d_ke, c->ke_cap, slots);
+    if (rc) return rc;
+    UD_HIP(c, launch_gather_kept(c->d_out_pose, c->d_out_e, c->d_kept, c->d_nkept, L, exh, max_modes, c->d_kp, c->d_ke,
+                                 c->stream));
+    std::vector<int32_t> kept_all(slots), nkept(L);
+    std::vector<float> KP(slots * UD_POSE_FLOATS), KE(slots);
     UD_HIP(c, hipMemcpyAsync(kept_all.data(), c->d_kept, kept_all.size() * 4, hipMemcpyDeviceToHost, c->stream));
     UD_HIP(c, hipMemcpyAsync(nkept.data(), c->d_nkept, nkept.size() * 4, hipMemcpyDeviceToHost, c->stream));
-    std::vector<float> P(nch * UD_POSE_FLOATS), E(nch);
-    rc = ud_fetch_dock_results(c, P.data(), E.data(), nullptr);  // synchronises the stream
-    if (rc) return rc;
+    UD_HIP(c, hipMemcpyAsync(KP.data(), c->d_kp, KP.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipMemcpyAsync(KE.data(), c->d_ke, KE.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    dock_times(c);
     // kept modes of every ligand in cluster order
     std::vector<int32_t> kl, kchain;
-    std::vector<float> kp;
+    std::vector<float> kp, ke;
     for (int l = 0; l < L; ++l)
         for (int k = 0; k < nkept[l]; ++k) {
-            const int ch = kept_all[size_t(l) * max_modes + k];
+            const size_t slot = size_t(l) * max_modes + k;
             kl.push_back(l);
-            kchain.push_back(ch);
-            const float* src = P.data() + (size_t(l) * exh + ch) * UD_POSE_FLOATS;
-            kp.insert(kp.end(), src, src + UD_POSE_FLOATS);
+            kchain.push_back(kept_all[slot]);
+            kp.insert(kp.end(), KP.begin() + slot * UD_POSE_FLOATS, KP.begin() + (slot + 1) * UD_POSE_FLOATS);
+            ke.push_back(KE[slot]);
         }
     const int nk_all = int(kl.size());
     // refinement against the explicit receptor when it is known (vina.h:248-272),
@@ -1977,7 +1989,7 @@ int ud_postprocess(ud_ctx* c, int max_modes, double min_rmsd, int refine_step, u
                 m.conf_independent = m.energy - x;
                 m.unbound = intra_best;
             }
-            m.search_energy = E[size_t(l) * exh + kchain[i]];
+            m.search_energy = ke[i];
             m.chain = int32_t(size_t(l) * exh + kchain[i]);
         }
         std::stable_sort(out, out + nk, [](const ud_mode& a, const ud_mode& b) { return a.energy < b.energy; });

@@ -2323,6 +2323,24 @@ __global__ __launch_bounds__(64) void bounds_kernel(const LigHdr* hdr, const Ato
     }
 }
 
+// The kept chains' best poses and energies, one slot per (ligand, mode):
+// slot l * max_size + k holds chain kept[slot] of ligand l for k < n_kept[l].
+__global__ __launch_bounds__(256) void gather_kept_kernel(const float* poses, const float* energy,
+                                                          const int32_t* kept, const int32_t* n_kept, int n_lig,
+                                                          int exh, int max_size, float* poses_out, float* e_out) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t slot = t / UD_POSE_FLOATS;
+    const int f = int(t % UD_POSE_FLOATS);
+    if (slot >= size_t(n_lig) * max_size) return;
+    const int l = int(slot / max_size), k = int(slot % max_size);
+    if (k >= n_kept[l]) return;
+    const int ch = kept[slot];
+    if (ch < 0 || ch >= exh) return;
+    const size_t src = size_t(l) * exh + ch;
+    poses_out[slot * UD_POSE_FLOATS + f] = poses[src * UD_POSE_FLOATS + f];
+    if (f == 0) e_out[slot] = energy[src];
+}
+
 // ====================================================================== launchers
 size_t lds_bytes(int lanes, const LaunchCap& cap) {
     Layout l;
@@ -2559,6 +2577,15 @@ hipError_t launch_cluster(const LigandSetDev& ls, int n_lig, const float* coords
                           int max_size, float min_rmsd, int32_t* kept_out, int32_t* n_out, hipStream_t st) {
     hipLaunchKernelGGL(cluster_kernel, dim3(n_lig), dim3(kWG), 0, st, ls.hdr, ls.atoms, coords, energy, exh, max_size,
                        min_rmsd, kept_out, n_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_kept(const float* poses, const float* energy, const int32_t* kept, const int32_t* n_kept,
+                              int n_lig, int exh, int max_size, float* poses_out, float* e_out, hipStream_t st) {
+    const size_t n = size_t(n_lig) * max_size * UD_POSE_FLOATS;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_kept_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, poses, energy, kept,
+                       n_kept, n_lig, exh, max_size, poses_out, e_out);
     return hipGetLastError();
 }
 

@@ -68,6 +68,8 @@ hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, co
 hipError_t launch_cluster(const LigandSetDev& ls, int n_lig, const float* coords, const float* energy, int exh,
                           int max_size, float min_rmsd, int32_t* kept_out, int32_t* n_out, hipStream_t st);
 // RMSD bounds of modes against reference modes (bounds_kernel)
+hipError_t launch_gather_kept(const float* poses, const float* energy, const int32_t* kept, const int32_t* n_kept,
+                              int n_lig, int exh, int max_size, float* poses_out, float* e_out, hipStream_t st);
 hipError_t launch_bounds(const LigandSetDev& ls, const int32_t* elem, int n_modes, const float* coords,
                          const int32_t* mode_lig, const int32_t* mode_ref, float* ub_out, float* lb_out,
                          hipStream_t st);
