@@ -267,6 +267,36 @@ def test_weight_gradient_euler_identity(sc):
     assert n_in >= 6
 
 
+def test_ad4_fp16_maps_energy_grad(sc):
+    """AD4.2 over FP16 maps: energy_grad equals the search's evaluation on the
+    same maps, and stays within the FP16 maps' error of the FP32-map results
+    (weight gradients too: the per-term maps are FP32 in both)."""
+    _say("[gpu] ad4 energy_grad FP16 maps")
+    xyz, ad, q, cav = small_receptor_ad4()
+    pl, P = _poses(sc, 3, 24, 1.0)
+    P = np.array([_unit(p) for p in P])
+    e32, g32, w32 = sc["ctx"].energy_grad(pl, P)
+    ctx = _lib.Context(0)
+    try:
+        ctx.set_scoring(_lib.UD_SF_AD4)
+        c, s = box_of(cav, EDGE)
+        ctx.set_box(c, s, 0.375)
+        ctx.compute_maps_ad4(xyz, ad, q, sc["types"], _lib.UD_GRID_F16)
+        ctx.set_ligands(synth.LigandBatch(list(sc["ligs"])).arrays())
+        e16, g16, w16 = ctx.energy_grad(pl, P)
+        es16, gs16 = ctx.score_grad(pl, P)
+    finally:
+        ctx.close()
+    # the search's evaluation on the same maps, up to the renormalised
+    # quaternion (as test_matches_score_grad_at_unit_quaternions bounds it)
+    for i in range(len(pl)):
+        assert abs(e16[i] - es16[i]) <= 1e-6 * abs(es16[i]) + 2e-6 * max(1.0, np.abs(gs16[i]).max()), i
+    for i in range(len(pl)):
+        assert abs(e16[i] - e32[i]) <= 2e-3 * max(1.0, abs(float(e32[i]))), (i, e16[i], e32[i])
+        np.testing.assert_allclose(g16[i], g32[i], rtol=0, atol=5e-3 * max(1.0, np.abs(g32[i]).max()))
+        np.testing.assert_allclose(w16[i], w32[i], rtol=0, atol=2e-3 * max(1.0, np.abs(w32[i]).max()))
+
+
 def test_torch_autograd_ad4(sc):
     """DockingEnergy under AD4.2: forward and backward equal ud_energy_grad on
     the same maps; new weights rebuild the maps along the weight gradient."""

@@ -207,3 +207,83 @@ def test_torch_autograd_function(eg):
     e2 = model(poses.detach(), pl, w2).cpu().numpy()
     pred = e_ref + gw_ref[:, 0] * float(w2[0] - w.detach()[0])
     np.testing.assert_allclose(e2, pred, rtol=0, atol=1e-3 * max(1.0, np.abs(e_ref).max()))
+
+
+def _vinardo_ctx(precision):
+    xyz, xs, cav = small_receptor()
+    ctx = _lib.Context(0)
+    ctx.set_scoring(_lib.UD_SF_VINARDO)
+    c, s = box_of(cav, EDGE)
+    ctx.set_box(c, s, 0.375)
+    ctx.compute_maps(xyz, xs, LIGAND_TYPES, precision)
+    ctx.set_ligands(synth.LigandBatch(list(small_ligands())).arrays())
+    return ctx
+
+
+def test_vinardo_weight_gradient(eg):
+    """Vinardo (gauss1, repulsion, hydrophobic, hydrogen, glue): dE/dw against
+    central differences of the FP64 oracle with maps and tables rebuilt per
+    weight set, and Euler's identity with curl off for poses in the box."""
+    _say("[gpu] energy_grad Vinardo weight gradient")
+    pl, P = eg["pl"], np.array([_unit(p) for p in eg["P"]])
+    ctx = _vinardo_ctx(_lib.UD_GRID_F32)
+    try:
+        _, _, gw = ctx.energy_grad(pl, P)
+        ev, _, gwv = ctx.energy_grad(pl, P, (FLT_MAX,) * 3)
+    finally:
+        ctx.close()
+    w0 = np.array(_lib.VINARDO_WEIGHTS, dtype=np.float64)
+    h = 1e-3
+    fd = np.zeros((len(pl), 5))
+    for k in range(5):
+        wp, wm = w0.copy(), w0.copy()
+        wp[k] += h
+        wm[k] -= h
+        op, om = _oracle(wp, sf=1), _oracle(wm, sf=1)
+        for i, li in enumerate(pl):
+            fd[i, k] = (op.eval_deriv(li, P[i])[0] - om.eval_deriv(li, P[i])[0]) / (2 * h)
+    for i in range(len(pl)):
+        s_ = max(1.0, np.abs(fd[i]).max())
+        np.testing.assert_allclose(gw[i, :5], fd[i], rtol=0, atol=2e-3 * s_, err_msg=f"pose {i}")
+    assert np.all(gw[:, 5:] == 0)
+    assert np.all(np.abs(fd[:, :3]).max(axis=0) > 0.1)
+    n_in = 0
+    for i, li in enumerate(pl):
+        c = eg["ctx"].pose_coords(pl[i:i + 1], P[i:i + 1])[0, :eg["ligs"][li].n_atoms]
+        if np.all(c >= eg["lo"]) and np.all(c <= eg["hi"]):
+            n_in += 1
+            assert float(np.dot(w0[:5], gwv[i, :5])) == pytest.approx(float(ev[i]), rel=1e-4, abs=1e-4)
+    assert n_in >= 6
+
+
+def test_fp16_maps_energy_grad(eg):
+    """The FP16-map instantiation: energy and pose gradient within the FP16
+    maps' own error of the FP32-map results (the maps differ by at most half
+    an FP16 step times the map scale), weight gradients likewise (the per-term
+    maps are FP32 in both; only the curl slope reads the FP16 map)."""
+    _say("[gpu] energy_grad FP16 maps")
+    pl, P = eg["pl"], np.array([_unit(p) for p in eg["P"]])
+    e32, g32, w32 = eg["ctx"].energy_grad(pl, P)
+    xyz, xs, cav = small_receptor()
+    ctx = _lib.Context(0)
+    try:
+        ctx.set_scoring(_lib.UD_SF_VINA)
+        c, s = box_of(cav, EDGE)
+        ctx.set_box(c, s, 0.375)
+        ctx.compute_maps(xyz, xs, LIGAND_TYPES, _lib.UD_GRID_F16)
+        ctx.set_ligands(synth.LigandBatch(list(eg["ligs"])).arrays())
+        e16, g16, w16 = ctx.energy_grad(pl, P)
+        es16, gs16 = ctx.score_grad(pl, P)
+    finally:
+        ctx.close()
+    # the search's evaluation on the same maps, up to the renormalised
+    # quaternion (as test_matches_score_grad_at_unit_quaternions bounds it)
+    for i in range(len(pl)):
+        assert abs(e16[i] - es16[i]) <= 1e-6 * abs(es16[i]) + 2e-6 * max(1.0, np.abs(gs16[i]).max()), i
+    for i in range(len(pl)):
+        sc = max(1.0, abs(float(e32[i])))
+        assert abs(e16[i] - e32[i]) <= 2e-3 * sc, (i, e16[i], e32[i])
+        gs = max(1.0, np.abs(g32[i]).max())
+        np.testing.assert_allclose(g16[i], g32[i], rtol=0, atol=5e-3 * gs)
+        ws = max(1.0, np.abs(w32[i]).max())
+        np.testing.assert_allclose(w16[i], w32[i], rtol=0, atol=2e-3 * ws)
