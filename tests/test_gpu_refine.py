@@ -19,12 +19,17 @@ Tolerances
   post-process  same clustered chains as without refinement; each mode's
                 components are those of its refined pose; the pipeline after
                 refinement (reorder by refined energy, intramolecular
-                reference, rescoring, final score) matches the FP32 oracle
-                fed with the GPU-refined poses to 1e-4 relative.  Started from
-                short-search poses, which are often strained, whole oracle
-                pipelines part from the GPU's more often than from each
-                other; at least half the modes must still agree within
-                0.02 kcal/mol
+                reference, rescoring, final score) matches the FP32 or the
+                FP64 oracle fed with the GPU-refined poses to 1e-4 relative
+                (a receptor pair on a bin edge of the piecewise-constant fast
+                table can read adjacent bins in the FP32 restatement and on
+                the GPU: one such mode was 0.0023 kcal/mol from the FP32
+                restatement, whose own value moves by that much under a
+                1e-6 A shift of the pose, and 3e-6 from the FP64 one).
+                Started from short-search poses, which are often strained,
+                whole oracle pipelines part from the GPU's more often than
+                from each other; at least half the modes must still agree
+                within 0.02 kcal/mol
 """
 import numpy as np
 import pytest
@@ -165,15 +170,20 @@ def test_postprocess_with_refinement(rs):
         # the intramolecular reference is the mode with the lowest refined energy
         _, er, _ = ctx.refine(pl, P[[m["chain"] for m in got]], refine_step=3)
         assert got[0]["unbound"] == pytest.approx(intra[int(np.argmin(er))], abs=0)
-        # the pipeline after refinement, on the GPU-refined poses, by the oracle
+        # the pipeline after refinement, on the GPU-refined poses, by the
+        # oracle: each mode agrees with the FP32 or the FP64 restatement
         Pr, _, _ = ctx.refine(pl, P[[m["chain"] for m in got]], refine_step=3)
-        ti = [o32.score_terms_nc(li, Pr[k].astype(np.float64)) for k in range(len(got))]
-        ref_best = ti[int(np.argmin(er))][1]
+        ib = int(np.argmin(er))
         for k, m in enumerate(got):
-            assert m["inter"] == pytest.approx(ti[k][0], rel=1e-5, abs=1e-4)
-            assert m["intra"] == pytest.approx(ti[k][1], rel=1e-5, abs=1e-4)
-            final = o32.conf_independent(float(lig.num_tors), ti[k][0] + ti[k][1] - ref_best)
-            assert m["energy"] == pytest.approx(final, rel=1e-4, abs=1e-4)
+            ok = False
+            for o in (o32, rs["o64"]):
+                a, b = o.score_terms_nc(li, Pr[k].astype(np.float64))
+                ref_best = o.score_terms_nc(li, Pr[ib].astype(np.float64))[1]
+                final = o.conf_independent(float(lig.num_tors), a + b - ref_best)
+                ok = ok or (m["inter"] == pytest.approx(a, rel=1e-5, abs=1e-4) and
+                            m["intra"] == pytest.approx(b, rel=1e-5, abs=1e-4) and
+                            m["energy"] == pytest.approx(final, rel=1e-4, abs=1e-4))
+            assert ok, (li, k, m["inter"], m["intra"], m["energy"])
         # whole oracle pipeline (its own refinement) on the same kept chains
         ref_e, ref_inter, ref_intra = {}, {}, {}
         for m in got:

@@ -706,24 +706,47 @@ __device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p,
     const int cell = c[0] + r.dim[0] * (c[1] + r.dim[1] * c[2]);
     const int k0 = r.cell_start[cell], k1 = r.cell_start[cell + 1];
     float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-    for (int k = k0; k < k1; ++k) {
-        const float4 b = r.atoms[k];
-        const float rx = a[0] - b.x, ry = a[1] - b.y, rz = a[2] - b.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < 64.f) {
-            const int t2 = __float_as_int(b.w);
-            const size_t base = size_t(t <= t2 ? t + t2 * (t2 + 1) / 2 : t2 + t * (t + 1) / 2) * kTableStride;
-            const float rf = 32.f * r2;
-            const int it = int(rf);
-            if (d) {
-                const float rem = rf - float(it);
-                const float2 t0 = ka.ls.table_deriv[base + it], t1 = ka.ls.table_deriv[base + it + 1];
-                e += t0.x + rem * (t1.x - t0.x);
-                const float dor = t0.y + rem * (t1.y - t0.y);
-                gx += dor * rx; gy += dor * ry; gz += dor * rz;
-            } else {
-                e += ka.ls.table_fast[base + it];
-            }
+    // receptor atoms in batches of kRecBatch: their loads, then their table
+    // loads, are in flight together; the sums run in list order
+    constexpr int kRecBatch = 4;
+    for (int k = k0; k < k1; k += kRecBatch) {
+        float4 b[kRecBatch];
+        bool in[kRecBatch];
+        float rx[kRecBatch], ry[kRecBatch], rz[kRecBatch], rf[kRecBatch];
+        size_t idx[kRecBatch];
+#pragma unroll
+        for (int u = 0; u < kRecBatch; ++u) b[u] = (k + u < k1) ? r.atoms[k + u] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+#pragma unroll
+        for (int u = 0; u < kRecBatch; ++u) {
+            rx[u] = a[0] - b[u].x; ry[u] = a[1] - b[u].y; rz[u] = a[2] - b[u].z;
+            const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+            in[u] = (k + u < k1) && r2 < 64.f;
+            const int t2 = __float_as_int(b[u].w);
+            rf[u] = 32.f * r2;
+            idx[u] = in[u] ? size_t(t <= t2 ? t + t2 * (t2 + 1) / 2 : t2 + t * (t + 1) / 2) * kTableStride + size_t(int(rf[u]))
+                           : 0;
+        }
+        if (d) {
+            float2 t0[kRecBatch], t1[kRecBatch];
+#pragma unroll
+            for (int u = 0; u < kRecBatch; ++u)
+                if (in[u]) { t0[u] = ka.ls.table_deriv[idx[u]]; t1[u] = ka.ls.table_deriv[idx[u] + 1]; }
+#pragma unroll
+            for (int u = 0; u < kRecBatch; ++u)
+                if (in[u]) {
+                    const float rem = rf[u] - float(int(rf[u]));
+                    e += t0[u].x + rem * (t1[u].x - t0[u].x);
+                    const float dor = t0[u].y + rem * (t1[u].y - t0[u].y);
+                    gx += dor * rx[u]; gy += dor * ry[u]; gz += dor * rz[u];
+                }
+        } else {
+            float tf[kRecBatch];
+#pragma unroll
+            for (int u = 0; u < kRecBatch; ++u)
+                if (in[u]) tf[u] = ka.ls.table_fast[idx[u]];
+#pragma unroll
+            for (int u = 0; u < kRecBatch; ++u)
+                if (in[u]) e += tf[u];
         }
     }
     if (e > 0.f && v < 0.1f * kMaxFl) {
