@@ -152,7 +152,7 @@ struct ud_ctx {
     NodeRec* d_nodes = nullptr;
     PairRec* d_pairs = nullptr;
     uint2* d_pair_ofs = nullptr;
-    PartnerRec* d_partners = nullptr;
+    PartnerRec* d_partners = nullptr;  // records, then the same pre-decoded
     AtomPartners* d_apar = nullptr;
     // scratch / results
     WorkItem* d_work = nullptr;
@@ -1104,6 +1104,11 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     rc = grow(c, c->d_pair_ofs, cap, std::max<size_t>(pofs.size(), 1));
     if (rc) return rc;
     cap = 0;
+    // the records, then the same records pre-decoded
+    const size_t n_part = partners.size();
+    if (2 * n_part > size_t(INT32_MAX)) return fail(c, UD_ERR_CAPACITY, "too many intramolecular pairs in one ligand set");
+    for (size_t k = 0; k < n_part; ++k) partners.push_back(partner_ofs(partners[k]));
+    for (LigHdr& h : hdr) h.partner_ofs_begin = int32_t(n_part) + h.partner_begin;
     rc = grow(c, c->d_partners, cap, std::max<size_t>(partners.size(), 1));
     if (rc) return rc;
     cap = 0;

@@ -21,7 +21,7 @@ struct LigHdr {
     float num_tors;       // conf_independent torsion count
     int32_t n_heavy;
     int32_t partner_begin;  // into the partner array (2 * n_pairs records)
-    int32_t pad;
+    int32_t partner_ofs_begin;  // the same list pre-decoded (partner_ofs), in the same array
 };
 static_assert(sizeof(LigHdr) == 48, "LigHdr layout");
 
@@ -83,6 +83,15 @@ __host__ __device__ inline uint32_t partner_pair(uint32_t r) { return (r >> 9) &
 __host__ __device__ inline uint32_t partner_slot(uint32_t r) { return r >> 22; }
 typedef uint32_t AtomPartners;
 static constexpr int kPartnerPad = 8;
+// The same partner record pre-decoded for the search's second pair pass:
+// byte offset of the other atom's coordinates in the group's float4 array
+// (other * 16, bits 0..15) | byte offset of the pair's force scale in the
+// group's float array (pair * 4, bits 16..31).  Both fit 16 bits for 300
+// atoms and 8,128 + 1 pairs.
+__host__ __device__ inline uint32_t partner_ofs(uint32_t r) {
+    return (partner_other(r) * 16u) | ((partner_pair(r) * 4u) << 16);
+}
+static_assert(UD_MAX_ATOMS * 16 <= 0x10000 && (UD_MAX_PAIRS + 1) * 4 <= 0x10000, "partner offsets fit 16 bits");
 
 // Pair-table entry for the derivative path: e_k, dor_k, e_{k+1} - e_k,
 // dor_{k+1} - dor_k (precalculate_element::eval_deriv, precalculate.h:48-65)
@@ -129,6 +138,8 @@ struct LigandSetDev {
     const NodeRec* nodes;
     const PairRec* pairs;
     const uint2* pair_ofs;  // pre-decoded pairs, indexed like pairs
+    // partner records of every ligand, then the same records pre-decoded
+    // (partner_ofs); LigHdr::partner_begin and partner_ofs_begin index both
     const PartnerRec* partners;
     const AtomPartners* atom_partners;  // per atom, indexed like atoms
     const float2* table_deriv;  // (e, dor) per sample, slot * kTableStride

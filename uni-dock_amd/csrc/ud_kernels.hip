@@ -620,7 +620,17 @@ __device__ __forceinline__ float grid_finish(const GridDev& g, int slot, const G
             const float sc = g.scale[slot];
             gx *= sc; gy *= sc; gz *= sc;
         }
-        if (e > 0.f && v < 0.1f * kMaxFl) {
+        if constexpr (INBOX) {
+            // curl as a factor of 1 where it does not apply (exact), without
+            // a branch
+            const float tmp = (e > 0.f && v < 0.1f * kMaxFl) ? curl_factor(v, e) : 1.f;
+            e = __fmul_rn(e, tmp);
+            const float t2 = tmp * tmp;
+            gx *= t2; gy *= t2; gz *= t2;
+        } else if (e > 0.f && v < 0.1f * kMaxFl) {
+            // (conditional here: the out-of-box penalty is added to this
+            // value, and AD4.2 adds charge-scaled copies of it whose sums
+            // cancel, so the rounding of that sum is kept as it is)
             const float tmp = curl_factor(v, e);
             e *= tmp;
             const float t2 = tmp * tmp;
@@ -941,15 +951,22 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
     } b;
     // pre-decoded pair records (coordinate byte offsets, table base), each
     // batch's loaded one batch ahead
-    const uint2* pofs = ka.ls.pair_ofs + s.h.pair_begin;
+    // The lane's next record is read through a per-lane address in vector
+    // registers, advanced by one batch per fetch: the uniform base would
+    // otherwise compete for scalar registers with the search state.
+    typedef const uint64_t __attribute__((address_space(1)))* GlobalRecPtr;  // global memory (not flat) loads
+    GlobalRecPtr rp = (GlobalRecPtr)(ka.ls.pair_ofs + s.h.pair_begin + s.gl);
+    asm volatile("" : "+v"(rp));
     const char* cbase = reinterpret_cast<const char*>(coords);
     uint2 nrec[kPairBatch];
     auto fetch = [&](int q0) {
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u) {
             const int q = q0 + u * L;
-            nrec[u] = q < P ? ld_at<uint2>(pofs, uint32_t(q) * 8u) : make_uint2(0u, 0u);
+            const uint64_t r = q < P ? rp[u * L] : 0ull;
+            nrec[u] = make_uint2(uint32_t(r), uint32_t(r >> 32));
         }
+        rp += kPairBatch * L;
     };
     auto issue = [&](int q0) {
         uint2 rec[kPairBatch];
@@ -959,8 +976,7 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u) {
             const int q = q0 + u * L;
-            b.in[u] = false;
-            b.idx[u] = 0; b.it[u] = 0; b.rem[u] = 0.f;
+            b.in[u] = false;  // rem, idx and it are read only for pairs within the cutoff
             if (q < P) {
                 const float4 ca = *reinterpret_cast<const float4*>(cbase + (rec[u].x & 0xFFFFu));
                 const float4 cb = *reinterpret_cast<const float4*>(cbase + (rec[u].x >> 16));
@@ -992,7 +1008,7 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
             }
     };
     auto finish = [&](int q0) {
-        float ep = 0.f;
+        float ep = -0.f;  // -0 + pe is pe: no addition for the first pair
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u) {
             const int q = q0 + u * L;
@@ -1002,11 +1018,12 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
                 float pe = b.t0[u].x + b.rem[u] * (b.t1[u].x - b.t0[u].x);
                 d = b.t0[u].y + b.rem[u] * (b.t1[u].y - b.t0[u].y);
                 if constexpr (Fmt<GF>::ad4) ad4_charge_terms(b.cf[u], b.u0[u], b.u1[u], b.rem[u], pe, d);
-                if (pe > 0.f && v_intra < 0.1f * kMaxFl) {
-                    const float tmp = curl_factor(v_intra, pe);
-                    pe *= tmp;
-                    d *= tmp * tmp;
-                }
+                // curl, as a factor of 1 where it does not apply (exact); the
+                // product is rounded before it is summed, as a conditional
+                // multiply would be (no fused multiply-add into the sum)
+                const float tmp = (pe > 0.f && v_intra < 0.1f * kMaxFl) ? curl_factor(v_intra, pe) : 1.f;
+                pe = __fmul_rn(pe, tmp);
+                d *= tmp * tmp;
                 ep += pe;
             }
             pf[q] = d;
@@ -1061,7 +1078,11 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
     float4* mf = s.mf();
     const float* pf = s.pf();
     const int N = s.h.n_atoms;
-    const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
+    // pre-decoded partner records: byte offsets of the other atom's
+    // coordinates and of the pair's force scale
+    const uint32_t* part = ka.ls.partners + s.h.partner_ofs_begin;
+    const char* cbase = reinterpret_cast<const char*>(coords);
+    const char* pbase = reinterpret_cast<const char*>(pf);
     const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
     constexpr int kRec = kPartnerPad;  // partner records fetched together (lists are padded to this)
     // (each atom's list starts at a multiple of kPartnerPad records, so a
@@ -1085,8 +1106,8 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
             load_batch(uint32_t(k0 + k), rec);
 #pragma unroll
             for (int u = 0; u < kRec; ++u) {
-                const float d = pf[partner_pair(rec[u])];
-                const float4 co = coords[partner_other(rec[u])];
+                const float d = *reinterpret_cast<const float*>(pbase + (rec[u] >> 16));
+                const float4 co = *reinterpret_cast<const float4*>(cbase + (rec[u] & 0xFFFFu));
                 sd += d;
                 sx += d * co.x;
                 sy += d * co.y;
