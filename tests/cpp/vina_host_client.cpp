@@ -2,6 +2,7 @@
 //   dock RECEPTOR LIGAND CX CY CZ SIZE EXH MAX_STEP SEED GRANULARITY OUT SF N_POSES REFINE
 //   ad4 LIGAND MAP_PREFIX EXH MAX_STEP SEED N_POSES OUT   (AutoGrid maps)
 //   errors RECEPTOR LIGAND
+//   modes MODE...   (--search_mode presets: applied?, exhaustiveness, max_step)
 // prints one JSON line (energies of every mode of ligand 0, or the error
 // messages caught).
 #include <cstdio>
@@ -49,6 +50,16 @@ int main(int argc, char** argv) {
                                 std::strtoull(argv[6], nullptr, 10), 0);
             v.write_poses_gpu({argv[8]}, std::atoi(argv[7]), 1e30);
             print_modes(v, std::atoi(argv[7]));
+            return 0;
+        }
+        if (cmd == "modes") {
+            std::printf("{\"modes\": [");
+            for (int k = 2; k < argc; ++k) {
+                int exh = 8, max_step = 0;  // the values before --search_mode is read
+                const bool applied = unidock_amd::search_mode_preset(argv[k], exh, max_step);
+                std::printf("%s[%d, %d, %d]", k > 2 ? ", " : "", applied ? 1 : 0, exh, max_step);
+            }
+            std::printf("]}\n");
             return 0;
         }
         if (cmd == "errors" && argc == 4) {

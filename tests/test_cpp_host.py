@@ -13,6 +13,8 @@ ABI, compiled with g++ together with tests/cpp/vina_host_client.cpp.
            read back by load_maps: the modes equal the Python binding's
            load_ad4_maps + dock
   errors   the reference's error messages (vina.h:156-166, vina.cpp:491-510)
+  modes    the --search_mode presets (main.cpp:644-656), unknown names
+           leaving exhaustiveness and max_step unchanged (CPU)
 """
 import json
 import os
@@ -56,6 +58,15 @@ def _terms(modes):
 
 def test_build(tmp_path):
     _build(tmp_path)
+
+
+def test_search_mode_presets(tmp_path):
+    exe = _build(tmp_path)
+    got = _run(exe, "modes", "fast", "balance", "balanced", "detail", "detailed", "Fast", "")["modes"]
+    assert got == [[1, 128, 20], [1, 384, 40], [1, 384, 40], [1, 512, 40], [1, 512, 40], [0, 8, 0], [0, 8, 0]]
+    # the Python binding's presets are the same table
+    assert {k: list(v) for k, v in docking.SEARCH_MODES.items()} == {
+        "fast": [128, 20], "balance": [384, 40], "balanced": [384, 40], "detail": [512, 40], "detailed": [512, 40]}
 
 
 @pytest.mark.gpu

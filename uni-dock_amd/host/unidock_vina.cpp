@@ -83,6 +83,22 @@ AutoGridMap read_autogrid_map(const std::string& path) {
 
 }  // namespace
 
+bool search_mode_preset(const std::string& search_mode, int& exhaustiveness, int& max_step) {
+    struct Preset {
+        const char* name;
+        int exhaustiveness, max_step;
+    };
+    static const Preset presets[] = {{"fast", 128, 20},   {"balance", 384, 40},  {"balanced", 384, 40},
+                                     {"detail", 512, 40}, {"detailed", 512, 40}};
+    for (const Preset& p : presets)
+        if (search_mode == p.name) {
+            exhaustiveness = p.exhaustiveness;
+            max_step = p.max_step;
+            return true;
+        }
+    return false;
+}
+
 Vina::Vina(const std::string& sf_name, int seed, int verbosity, bool no_refine, int device, int map_precision)
     : m_verbosity(verbosity), m_no_refine(no_refine), m_precision(map_precision) {
     // generate_seed (vina.h:431): 0 asks for a random seed

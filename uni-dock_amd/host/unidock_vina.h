@@ -22,6 +22,12 @@
 
 namespace unidock_amd {
 
+// --search_mode presets (main.cpp:644-656): "fast" sets exhaustiveness 128
+// and max_step 20, "balance" / "balanced" 384 and 40, "detail" / "detailed"
+// 512 and 40.  Any other name leaves both as given, as the reference does;
+// the return value says whether a preset applied.
+bool search_mode_preset(const std::string& search_mode, int& exhaustiveness, int& max_step);
+
 class Vina {
 public:
     // sf_name: "vina", "vinardo" or "ad4" (vina.h:75-110).  seed 0 draws a
