@@ -53,7 +53,7 @@ from unidock_tools import _lib, synth  # noqa: E402
 
 HBM_PEAK = 8.0e12
 FP32_PEAK = 157.3e12        # MI355X published dense FP32 vector peak (packed FP32 instructions)
-FP32_PEAK_UNPACKED = 78.6e12  # without packed FP32 instructions, which this build does not emit
+FP32_PEAK_UNPACKED = 78.6e12  # without packed FP32 instructions (the HIP sources are compiled without them)
 # HBM bytes per search-kernel launch measured by FETCH_SIZE / WRITE_SIZE
 # counter runs of this benchmark (copy-kernel calibrated), one table row per
 # kernel and workgroup count
@@ -149,8 +149,8 @@ def bucket_mask(ligs, atom_cap):
 
 def cpu_threads():
     """Host threads of the CPU baseline: the cores this process may run on,
-    capped at the host share the GPU box gives one GPU (OMP_NUM_THREADS there,
-    16 by default)."""
+    capped at the host cores allotted to one GPU (OMP_NUM_THREADS, 16 when
+    unset)."""
     share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
     return max(1, min(len(os.sched_getaffinity(0)), share))
 
