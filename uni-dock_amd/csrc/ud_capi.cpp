@@ -1101,7 +1101,8 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     rc = grow(c, c->d_pairs, cap, std::max<size_t>(pairs.size(), 1));
     if (rc) return rc;
     cap = 0;
-    rc = grow(c, c->d_pair_ofs, cap, std::max<size_t>(pofs.size(), 1));
+    pofs.insert(pofs.end(), kPairOfsPad, make_uint2(0u, 0u));
+    rc = grow(c, c->d_pair_ofs, cap, pofs.size());
     if (rc) return rc;
     cap = 0;
     // the records, then the same records pre-decoded
@@ -1119,10 +1120,9 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     UD_HIP(c, hipMemcpy(c->d_apar, apar.data(), apar.size() * sizeof(AtomPartners), hipMemcpyHostToDevice));
     UD_HIP(c, hipMemcpy(c->d_hdr, hdr.data(), hdr.size() * sizeof(LigHdr), hipMemcpyHostToDevice));
     UD_HIP(c, hipMemcpy(c->d_nodes, nodes.data(), nodes.size() * sizeof(NodeRec), hipMemcpyHostToDevice));
-    if (!pairs.empty()) {
+    if (!pairs.empty())
         UD_HIP(c, hipMemcpy(c->d_pairs, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
-        UD_HIP(c, hipMemcpy(c->d_pair_ofs, pofs.data(), pofs.size() * sizeof(uint2), hipMemcpyHostToDevice));
-    }
+    UD_HIP(c, hipMemcpy(c->d_pair_ofs, pofs.data(), pofs.size() * sizeof(uint2), hipMemcpyHostToDevice));
     for (void* p : {static_cast<void*>(c->d_charge), static_cast<void*>(c->d_pair_coef)})
         if (p) (void)hipFree(p);
     c->d_charge = nullptr;

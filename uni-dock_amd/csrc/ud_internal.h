@@ -67,6 +67,10 @@ __host__ __device__ inline uint32_t pair_slot(uint32_t r) { return r >> 18; }
 __host__ __device__ inline uint2 pair_ofs(uint32_t a, uint32_t b, uint32_t slot, uint32_t stride) {
     return make_uint2((a * 16u) | ((b * 16u) << 16), slot * stride);
 }
+// Zero records after the last ligand's pre-decoded pairs: the pair pass
+// prefetches records up to two batches of 64 lanes past a ligand's last
+// pair without a bound check (those records are never used).
+static constexpr int kPairOfsPad = 256;
 // Partner record (each pair listed once under each of its atoms, in pair
 // order): bits 0..8 other atom, 9..21 pair index, 22..31 table slot.  Per
 // atom, AtomPartners = first partner (relative to the ligand's

@@ -777,6 +777,9 @@ __device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p,
 #ifndef UD_ATOM_BATCH
 #define UD_ATOM_BATCH 1
 #endif
+#ifndef UD_PAIR_PIPELINE
+#define UD_PAIR_PIPELINE 1  // pass A's table loads one batch ahead of its math
+#endif
 #ifndef UD_PAIR_BATCH
 #define UD_PAIR_BATCH 1
 #endif
@@ -948,7 +951,7 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
         bool in[kPairBatch];
         float2 t0[kPairBatch], t1[kPairBatch], cf[kPairBatch];
         float4 u0[kPairBatch], u1[kPairBatch];
-    } b;
+    } b, b2;
     // pre-decoded pair records (coordinate byte offsets, table base), each
     // batch's loaded one batch ahead
     // The lane's next record is read through a per-lane address in vector
@@ -962,13 +965,14 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
     auto fetch = [&](int q0) {
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u) {
-            const int q = q0 + u * L;
-            const uint64_t r = q < P ? rp[u * L] : 0ull;
+            // (past the ligand's last pair this reads the next ligand's
+            // records or the array's zero padding; they are not used)
+            const uint64_t r = rp[u * L];
             nrec[u] = make_uint2(uint32_t(r), uint32_t(r >> 32));
         }
         rp += kPairBatch * L;
     };
-    auto issue = [&](int q0) {
+    auto issue = [&](Batch& b, int q0) {
         uint2 rec[kPairBatch];
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u) rec[u] = nrec[u];
@@ -976,7 +980,8 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u) {
             const int q = q0 + u * L;
-            b.in[u] = false;  // rem, idx and it are read only for pairs within the cutoff
+            b.in[u] = false;  // rem and it are read only for pairs within the cutoff
+            b.idx[u] = 0;
             if (q < P) {
                 const float4 ca = *reinterpret_cast<const float4*>(cbase + (rec[u].x & 0xFFFFu));
                 const float4 cb = *reinterpret_cast<const float4*>(cbase + (rec[u].x >> 16));
@@ -994,8 +999,13 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
         }
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u)
-            if (b.in[u]) {
-                if (!Fmt<GF>::ad4 || b.it[u] < kTableStride - 1) {
+            if constexpr (!Fmt<GF>::ad4) {
+                // every lane loads (sample 0 of table 0 outside the cutoff),
+                // so that no load sits in a branch and each batch waits
+                // only for its own loads
+                ld_samples(ka.ls.table_deriv, uint32_t(b.idx[u]), b.t0[u], b.t1[u]);
+            } else if (b.in[u]) {
+                if (b.it[u] < kTableStride - 1) {
                     ld_samples(ka.ls.table_deriv, uint32_t(b.idx[u]), b.t0[u], b.t1[u]);
                 } else {
                     b.t0[u] = b.t1[u] = make_float2(0.f, 0.f);
@@ -1007,7 +1017,7 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
                 }
             }
     };
-    auto finish = [&](int q0) {
+    auto finish = [&](const Batch& b, int q0) {
         float ep = -0.f;  // -0 + pe is pe: no addition for the first pair
 #pragma unroll
         for (int u = 0; u < kPairBatch; ++u) {
@@ -1031,7 +1041,7 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
         return ep;
     };
     fetch(s.gl);
-    issue(s.gl);
+    issue(b, s.gl);
     float e = 0.f;
     for (int i = s.gl; i < N; i += L) {
         const AtomRec a = s.atoms[i];
@@ -1063,10 +1073,23 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
         }
         mf[i] = make_float4(d.x, d.y, d.z, 0.f);
     }
-    for (int q0 = s.gl; q0 < P; q0 += kPairBatch * L) {
-        if (q0 != s.gl) issue(q0);
-        e += finish(q0);
+#if UD_PAIR_PIPELINE
+    // software-pipelined: each batch's table loads are issued before the
+    // previous batch is finished, so they are in flight during its math
+    constexpr int QB = kPairBatch * L;
+    for (int q0 = s.gl; q0 < P; q0 += 2 * QB) {
+        issue(b2, q0 + QB);
+        e += finish(b, q0);
+        if (q0 + QB >= P) break;
+        issue(b, q0 + 2 * QB);
+        e += finish(b2, q0 + QB);
     }
+#else
+    for (int q0 = s.gl; q0 < P; q0 += kPairBatch * L) {
+        if (q0 != s.gl) issue(b, q0);
+        e += finish(b, q0);
+    }
+#endif
     return e;
 }
 
