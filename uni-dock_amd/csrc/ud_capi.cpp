@@ -1109,6 +1109,9 @@ int ud_set_ligands(ud_ctx* c, const ud_ligand_batch* b) {
     const size_t n_part = partners.size();
     if (2 * n_part > size_t(INT32_MAX)) return fail(c, UD_ERR_CAPACITY, "too many intramolecular pairs in one ligand set");
     for (size_t k = 0; k < n_part; ++k) partners.push_back(partner_ofs(partners[k]));
+    // zero records after the pre-decoded lists: the second pair pass loads
+    // one record batch past an atom's list without a bound check
+    partners.insert(partners.end(), 2 * kPartnerPad, 0u);
     for (LigHdr& h : hdr) h.partner_ofs_begin = int32_t(n_part) + h.partner_begin;
     rc = grow(c, c->d_partners, cap, std::max<size_t>(partners.size(), 1));
     if (rc) return rc;

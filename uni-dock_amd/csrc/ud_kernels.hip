@@ -1117,16 +1117,20 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
         r[4] = hi.x; r[5] = hi.y; r[6] = hi.z; r[7] = hi.w;
     };
     for (int i = s.gl; i < N; i += L) {
-        float4 g = mf[i];
-        const float4 ci = coords[i];
         const uint32_t ap = apl[i];
         const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
+        // each next record batch is loaded (unconditionally) while the
+        // current one is summed; the load after the last batch reads the
+        // next atom's list or the array's zero padding and is not used
+        uint32_t rec[kRec], nrec[kRec];
+        load_batch(uint32_t(k0), rec);
+        float4 g = mf[i];
+        const float4 ci = coords[i];
         // sum d (c_i - c_o) = c_i sum d - sum d c_o over the atom's records,
         // in whole batches (padding records carry d = 0)
         float sd = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
         for (int k = 0; k < kn; k += kRec) {
-            uint32_t rec[kRec];
-            load_batch(uint32_t(k0 + k), rec);
+            load_batch(uint32_t(k0 + k + kRec), nrec);
 #pragma unroll
             for (int u = 0; u < kRec; ++u) {
                 const float d = *reinterpret_cast<const float*>(pbase + (rec[u] >> 16));
@@ -1136,6 +1140,8 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
                 sy += d * co.y;
                 sz += d * co.z;
             }
+#pragma unroll
+            for (int u = 0; u < kRec; ++u) rec[u] = nrec[u];
         }
         g.x += ci.x * sd - sx;
         g.y += ci.y * sd - sy;
