@@ -325,7 +325,7 @@ static constexpr int kNSmall = 16;
 // many chains fit a compute unit.  Pair records stay in global memory.
 struct Layout {
     int na, nn, nc, nh, ps;
-    int o_coords, o_mf, o_frames, o_poses, o_bc, o_H, o_pf, group_floats, shared_floats;
+    int o_coords, o_mf, o_frames, o_poses, o_bc, o_H, o_pf, group_floats, shared_floats, o_apl;
     // pf: per-pair buffer (one float per pair, then a zero) for pair-parallel forces
     __host__ __device__ void init(int na_, int nn_, int np_, bool pf, int lanes) {
         na = na_;
@@ -346,7 +346,10 @@ struct Layout {
         // group blocks start 4 floats apart modulo 32 banks, so that the
         // same offset in up to 8 groups falls into different LDS banks
         group_floats = ((end + 31) & ~31) + 4;
-        shared_floats = 4 * na + 8 * nn;  // atoms, nodes
+        // shared: atoms, nodes, and with the pair buffer each atom's partner
+        // list descriptor (read by the second pair pass)
+        o_apl = 4 * na + 8 * nn;
+        shared_floats = o_apl + (pf ? (na + 3) & ~3 : 0);
     }
     __host__ __device__ size_t bytes(int groups) const { return size_t(shared_floats + groups * group_floats) * 4; }
 };
@@ -367,6 +370,7 @@ template <int L, int NM = kNMax> struct Grp {
     static constexpr int KV = (NM + L - 1) / L;
     const AtomRec* atoms;  // LDS (shared)
     const NodeRec* nodes;  // LDS (shared)
+    const AtomPartners* apl;  // LDS (shared; with the pair buffer only)
     const PairRec* pairs;  // global memory
     float* gb;             // LDS group block
     Layout lay;
@@ -393,8 +397,12 @@ __device__ Grp<L, NM> setup_group(const KArgs& ka, int lig, float* lds) {
     NodeRec* nodes = reinterpret_cast<NodeRec*>(lds + 4 * s.lay.na);
     for (int i = threadIdx.x; i < s.h.n_atoms; i += kWG) atoms[i] = ka.ls.atoms[s.h.atom_begin + i];
     for (int i = threadIdx.x; i < s.h.n_nodes; i += kWG) nodes[i] = ka.ls.nodes[s.h.node_begin + i];
+    AtomPartners* apl = reinterpret_cast<AtomPartners*>(lds + s.lay.o_apl);
+    if (ka.pair_forces)
+        for (int i = threadIdx.x; i < s.h.n_atoms; i += kWG) apl[i] = ka.ls.atom_partners[s.h.atom_begin + i];
     s.atoms = atoms;
     s.nodes = nodes;
+    s.apl = apl;
     s.pairs = ka.ls.pairs + s.h.pair_begin;
     s.g = threadIdx.x / L;
     s.gl = threadIdx.x % L;
@@ -1106,7 +1114,6 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
     const uint32_t* part = ka.ls.partners + s.h.partner_ofs_begin;
     const char* cbase = reinterpret_cast<const char*>(coords);
     const char* pbase = reinterpret_cast<const char*>(pf);
-    const AtomPartners* apl = ka.ls.atom_partners + s.h.atom_begin;
     constexpr int kRec = kPartnerPad;  // partner records fetched together (lists are padded to this)
     // (each atom's list starts at a multiple of kPartnerPad records, so a
     // batch is two aligned 16-byte loads)
@@ -1117,7 +1124,7 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
         r[4] = hi.x; r[5] = hi.y; r[6] = hi.z; r[7] = hi.w;
     };
     for (int i = s.gl; i < N; i += L) {
-        const uint32_t ap = apl[i];
+        const uint32_t ap = s.apl[i];  // staged in LDS with the ligand
         const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
         // each next record batch is loaded (unconditionally) while the
         // current one is summed; the load after the last batch reads the
