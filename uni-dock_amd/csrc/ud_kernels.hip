@@ -943,7 +943,8 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
 // whatever the atoms' pair counts, and the per-atom accumulation order is the
 // reference's.
 template <int L, int NM, int GF>
-__device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter) {
+__device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter,
+                                                   const uint64_t (&rec0)[kPairBatch]) {
     const float4* coords = s.coords();
     float4* mf = s.mf();
     float* pf = s.pf();
@@ -1048,7 +1049,10 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
         }
         return ep;
     };
-    fetch(s.gl);
+    // the first batch's records, loaded before the forward kinematics
+#pragma unroll
+    for (int u = 0; u < kPairBatch; ++u) nrec[u] = make_uint2(uint32_t(rec0[u]), uint32_t(rec0[u] >> 32));
+    rp += kPairBatch * L;
     issue(b, s.gl);
     float e = 0.f;
     for (int i = s.gl; i < N; i += L) {
@@ -1165,11 +1169,20 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
 template <int L, int NM, int GF>
 __device__ float eval_energy(const KArgs& ka, const Grp<L, NM>& s, const float* pose, float v_intra, float v_inter,
                              float* rgyr = nullptr) {
+    // the pair pass's first record batch does not depend on the pose: its
+    // loads are issued here and complete during the forward kinematics (the
+    // record array is padded, so lanes past the last pair read zeros)
+    uint64_t rec0[kPairBatch];
+    if (ka.pair_forces) {
+#pragma unroll
+        for (int u = 0; u < kPairBatch; ++u)
+            rec0[u] = ld_at<uint64_t>(ka.ls.pair_ofs + s.h.pair_begin, uint32_t(s.gl + u * L) * 8u);
+    }
     fk_frames(s, pose);
     float rg2 = 0.f;
     atoms_coords(s, rg2, rgyr != nullptr);
     __syncthreads();
-    float e = ka.pair_forces ? atoms_forces_pp_a<L, NM, GF>(ka, s, v_intra, v_inter)
+    float e = ka.pair_forces ? atoms_forces_pp_a<L, NM, GF>(ka, s, v_intra, v_inter, rec0)
                              : atoms_forces_2s<L, NM, GF>(ka, s, v_intra, v_inter);
     __syncthreads();
     e = gsum<L>(e);
