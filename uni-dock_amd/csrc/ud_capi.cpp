@@ -649,12 +649,15 @@ static int build_cells(ud_ctx* c, const std::vector<float4>& rec) {
                     if (brick_distance_sqr(lo, hi, rec[i]) < cut2) cells.push_back(rec[i]);
                 start[size_t(x) + size_t(nx) * (y + size_t(ny) * z) + 1] = int32_t(cells.size());
             }
-    int rc = grow(c, c->d_cells, c->cells_cap, std::max<size_t>(cells.size(), 1));
+    // far-away atoms after the last cell: the refinement loads one batch of
+    // atoms past a cell's list without a bound check (beyond the cutoff of
+    // any point, and masked by the list bound anyway)
+    cells.insert(cells.end(), kCellPad, make_float4(1e30f, 1e30f, 1e30f, 0.f));
+    int rc = grow(c, c->d_cells, c->cells_cap, cells.size());
     if (rc) return rc;
     rc = grow(c, c->d_cell_start, c->cstart_cap, start.size());
     if (rc) return rc;
-    if (!cells.empty())
-        UD_HIP(c, hipMemcpyAsync(c->d_cells, cells.data(), sizeof(float4) * cells.size(), hipMemcpyHostToDevice, c->stream));
+    UD_HIP(c, hipMemcpyAsync(c->d_cells, cells.data(), sizeof(float4) * cells.size(), hipMemcpyHostToDevice, c->stream));
     UD_HIP(c, hipMemcpyAsync(c->d_cell_start, start.data(), sizeof(int32_t) * start.size(), hipMemcpyHostToDevice, c->stream));
     UD_HIP(c, hipStreamSynchronize(c->stream));
     return UD_OK;
@@ -1798,8 +1801,8 @@ int ud_refine(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses, 
     for (int i = 0; i < n_poses; ++i) active[i] = i;
     std::vector<float> e(n_poses, kMaxFloat);
     std::vector<char> inside(n_poses, 0);
-    std::vector<float> sub, se, xyz;
-    std::vector<int32_t> spl;
+    std::vector<float> sub, se;
+    std::vector<int32_t> spl, win;
     const float margin = 1e-4f;  // non_cache::within default (non_cache.h:41)
     double ms_total = 0;
     for (int pass = 0; pass < refine_step && !active.empty(); ++pass) {
@@ -1828,34 +1831,33 @@ int ud_refine(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses, 
         UD_HIP(c, launch_refine(lanes, lig_dev(c), g, rec_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
                                 c->d_poses, 1000.f, 1000.f, 0, c->d_f1, c->d_i1, c->stream));
         UD_HIP(c, hipEventRecord(c->ev1, c->stream));
+        // non_cache::within (non_cache.cpp:408-418): heavy atoms inside the
+        // box with a margin, decided on the device for the refined poses
+        // (the refinement's evaluation counts in d_i1 are not needed)
+        BoxLimits lim;
+        for (int d = 0; d < 3; ++d) {
+            lim.lo[d] = c->init[d] - margin;
+            lim.hi[d] = c->gend[d] + margin;
+        }
+        UD_HIP(c, launch_coords(lanes, lig_dev(c), g, cap, c->d_work, int(work.size()), c->d_perm, c->d_poses, nullptr,
+                                c->stream, 0, &lim, c->d_i1));
         se.resize(na);
+        win.resize(na);
         UD_HIP(c, hipMemcpyAsync(sub.data(), c->d_poses, sizeof(float) * size_t(na) * UD_POSE_FLOATS,
                                  hipMemcpyDeviceToHost, c->stream));
         UD_HIP(c, hipMemcpyAsync(se.data(), c->d_f1, sizeof(float) * na, hipMemcpyDeviceToHost, c->stream));
+        UD_HIP(c, hipMemcpyAsync(win.data(), c->d_i1, sizeof(int32_t) * na, hipMemcpyDeviceToHost, c->stream));
         UD_HIP(c, hipStreamSynchronize(c->stream));
         float ms = 0;
         (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
         ms_total += ms;
-        // non_cache::within (non_cache.cpp:408-418): heavy atoms inside the box with a margin
-        xyz.resize(size_t(na) * UD_MAX_ATOMS * 3);
-        rc = ud_pose_coords(c, na, spl.data(), sub.data(), xyz.data());
-        if (rc) return rc;
         std::vector<int> still;
         for (int k = 0; k < na; ++k) {
             const int i = active[k];
             std::copy(sub.begin() + size_t(k) * UD_POSE_FLOATS, sub.begin() + size_t(k + 1) * UD_POSE_FLOATS,
                       poses + size_t(i) * UD_POSE_FLOATS);
             e[i] = se[k];
-            const LigHdr& h = c->hdr[spl[k]];
-            bool in = true;
-            for (int a = 0; a < h.n_atoms && in; ++a) {
-                if (!((c->h_atoms[size_t(h.atom_begin) + a].meta >> 16) & 1)) continue;
-                for (int d = 0; d < 3; ++d) {
-                    const float x = xyz[(size_t(k) * UD_MAX_ATOMS + a) * 3 + d];
-                    if (x < c->init[d] - margin || x > c->gend[d] + margin) in = false;
-                }
-            }
-            if (in) inside[i] = 1;
+            if (win[k]) inside[i] = 1;
             else still.push_back(i);
         }
         active.swap(still);

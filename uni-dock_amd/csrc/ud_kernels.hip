@@ -724,16 +724,22 @@ __device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p,
     const int cell = c[0] + r.dim[0] * (c[1] + r.dim[1] * c[2]);
     const int k0 = r.cell_start[cell], k1 = r.cell_start[cell + 1];
     float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-    // receptor atoms in batches of kRecBatch: their loads, then their table
-    // loads, are in flight together; the sums run in list order
+    // receptor atoms in batches of kRecBatch, each batch loaded while the
+    // previous one is summed and its table samples loaded together, all
+    // unconditionally (atoms past the cell's list are the next cell's or the
+    // array's far padding and are masked; sample 0 outside the cutoff), so
+    // that each batch waits only for its own loads; the sums run in list order
     constexpr int kRecBatch = 4;
+    float4 b[kRecBatch];
+#pragma unroll
+    for (int u = 0; u < kRecBatch; ++u) b[u] = r.atoms[k0 + u];
     for (int k = k0; k < k1; k += kRecBatch) {
-        float4 b[kRecBatch];
+        float4 nb[kRecBatch];
+#pragma unroll
+        for (int u = 0; u < kRecBatch; ++u) nb[u] = r.atoms[k + kRecBatch + u];
         bool in[kRecBatch];
         float rx[kRecBatch], ry[kRecBatch], rz[kRecBatch], rf[kRecBatch];
-        size_t idx[kRecBatch];
-#pragma unroll
-        for (int u = 0; u < kRecBatch; ++u) b[u] = (k + u < k1) ? r.atoms[k + u] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+        uint32_t idx[kRecBatch];
 #pragma unroll
         for (int u = 0; u < kRecBatch; ++u) {
             rx[u] = a[0] - b[u].x; ry[u] = a[1] - b[u].y; rz[u] = a[2] - b[u].z;
@@ -741,14 +747,14 @@ __device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p,
             in[u] = (k + u < k1) && r2 < 64.f;
             const int t2 = __float_as_int(b[u].w);
             rf[u] = 32.f * r2;
-            idx[u] = in[u] ? size_t(t <= t2 ? t + t2 * (t2 + 1) / 2 : t2 + t * (t + 1) / 2) * kTableStride + size_t(int(rf[u]))
-                           : 0;
+            idx[u] = in[u] ? uint32_t(t <= t2 ? t + t2 * (t2 + 1) / 2 : t2 + t * (t + 1) / 2) * uint32_t(kTableStride) +
+                                 uint32_t(int(rf[u]))
+                           : 0u;
         }
         if (d) {
             float2 t0[kRecBatch], t1[kRecBatch];
 #pragma unroll
-            for (int u = 0; u < kRecBatch; ++u)
-                if (in[u]) { t0[u] = ka.ls.table_deriv[idx[u]]; t1[u] = ka.ls.table_deriv[idx[u] + 1]; }
+            for (int u = 0; u < kRecBatch; ++u) { t0[u] = ka.ls.table_deriv[idx[u]]; t1[u] = ka.ls.table_deriv[idx[u] + 1]; }
 #pragma unroll
             for (int u = 0; u < kRecBatch; ++u)
                 if (in[u]) {
@@ -760,12 +766,13 @@ __device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p,
         } else {
             float tf[kRecBatch];
 #pragma unroll
-            for (int u = 0; u < kRecBatch; ++u)
-                if (in[u]) tf[u] = ka.ls.table_fast[idx[u]];
+            for (int u = 0; u < kRecBatch; ++u) tf[u] = ka.ls.table_fast[idx[u]];
 #pragma unroll
             for (int u = 0; u < kRecBatch; ++u)
                 if (in[u]) e += tf[u];
         }
+#pragma unroll
+        for (int u = 0; u < kRecBatch; ++u) b[u] = nb[u];
     }
     if (e > 0.f && v < 0.1f * kMaxFl) {
         const float tmp = curl_factor(v, e);
@@ -2246,7 +2253,8 @@ __global__ __launch_bounds__(64) void energy_grad_kernel(KArgs ka, TermsDev td, 
 // ligands take sum(n_atoms) * exh entries.
 template <int L>
 __global__ __launch_bounds__(64) void coords_kernel(KArgs ka, const WorkItem* work, const int32_t* perm,
-                                                    const float* poses, float* coords_out, int compact_exh) {
+                                                    const float* poses, float* coords_out, int compact_exh,
+                                                    BoxLimits lim, int32_t* within_out) {
     extern __shared__ float4 lds_raw[];
     float* lds = reinterpret_cast<float*>(lds_raw);
     const WorkItem w = work[blockIdx.x];
@@ -2259,6 +2267,23 @@ __global__ __launch_bounds__(64) void coords_kernel(KArgs ka, const WorkItem* wo
     __syncthreads();
     fk_frames(s, x);
     const float* fr = s.frames();
+    if (within_out) {
+        // non_cache::within (non_cache.cpp:408-418): heavy atoms in the box
+        // with its margin (a NaN coordinate does not count as outside)
+        float outside = 0.f;
+        for (int i = s.gl; i < s.h.n_atoms; i += L) {
+            const AtomRec a = s.atoms[i];
+            if (!((a.meta >> 16) & 1)) continue;
+            const float3 p = frame_apply(fr + 20 * (a.meta & 0xff), a);
+            const float c[3] = {p.x, p.y, p.z};
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (c[d] < lim.lo[d] || c[d] > lim.hi[d]) outside = 1.f;
+        }
+        outside = gsum<L>(outside);
+        if (active && s.gl == 0) within_out[pi] = outside == 0.f ? 1 : 0;
+        return;
+    }
     if (active && compact_exh > 0) {
         const size_t first = size_t(s.h.atom_begin) * compact_exh + size_t(pi - w.lig * compact_exh) * s.h.n_atoms;
         for (int i = s.gl; i < s.h.n_atoms; i += L) {
@@ -2656,12 +2681,13 @@ hipError_t launch_energy_grad(int lanes, int gf, const LigandSetDev& ls, const G
 
 hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap,
                          const WorkItem* work, int n_work, const int32_t* perm, const float* poses, float* coords_out,
-                         hipStream_t st, int compact_exh) {
+                         hipStream_t st, int compact_exh, const BoxLimits* lim, int32_t* within_out) {
     const KArgs ka = make_kargs(ls, g, cap);
     const size_t shm = lds_bytes(lanes, cap);
+    const BoxLimits bl = lim ? *lim : BoxLimits{};
     UD_DISPATCH_L(lanes, {
         hipLaunchKernelGGL((coords_kernel<L>), dim3(n_work), dim3(kWG), shm, st, ka, work, perm, poses, coords_out,
-                           compact_exh);
+                           compact_exh, bl, within_out);
     });
     return hipGetLastError();
 }

@@ -60,10 +60,15 @@ hipError_t launch_terms_explicit(int lanes, const LigandSetDev& ls, const GridDe
                                  const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
                                  const float* poses, float v, float* inter_out, float* intra_out, hipStream_t st);
 // coords_out: UD_MAX_ATOMS atoms per pose, or with compact_exh > 0 the
-// chains' atoms packed per ligand (coords_kernel)
+// chains' atoms packed per ligand (coords_kernel).  With within_out set, no
+// coordinates are written: within_out[pose] = 1 if every heavy atom lies in
+// [lo - margin, hi + margin] on every axis (non_cache::within), else 0.
+struct BoxLimits {
+    float lo[3], hi[3];
+};
 hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap, const WorkItem* work,
                          int n_work, const int32_t* perm, const float* poses, float* coords_out, hipStream_t st,
-                         int compact_exh = 0);
+                         int compact_exh = 0, const BoxLimits* lim = nullptr, int32_t* within_out = nullptr);
 // add_to_output_container + remove_redundant per ligand (cluster_kernel)
 hipError_t launch_cluster(const LigandSetDev& ls, int n_lig, const float* coords, const float* energy, int exh,
                           int max_size, float min_rmsd, int32_t* kept_out, int32_t* n_out, hipStream_t st);
