@@ -161,7 +161,7 @@ struct LigandSetDev {
 // non_cache.cpp) in the refinement: cells of about 3 A over the box
 // (szv_grid_dims, szv_grid.cpp:86-96), each listing in receptor order the
 // typed atoms within the cutoff of the cell (szv_grid.cpp:24-55).
-static constexpr int kCellPad = 16;  // far-away atoms after the cell lists (see build_cells)
+static constexpr int kCellPad = 64;  // far-away atoms after the cell lists (see build_cells)
 struct RecDev {
     const float4* atoms;        // x, y, z, XS type bits; the cell lists concatenated, then kCellPad far atoms
     const int32_t* cell_start;  // n_cells + 1 offsets into atoms

@@ -700,51 +700,66 @@ template <int GF> __device__ __forceinline__ constexpr float pair_cut2() {
 // type-pair table value (interpolated e and dor with a gradient, the fast
 // table without), then curl; the pair gradient is kept in clamped directions
 // as in the reference.  The type pair uses t itself (get_type_pair_index).
-__device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p, float v, float3* d) {
+// The atom's clamped position, out-of-box penalty and receptor cell list.
+struct ExplicitAtom {
+    float a[3], oob[3], pen;
+    int k0, k1;
+};
+__device__ __forceinline__ ExplicitAtom explicit_locate(const KArgs& ka, float3 p) {
     const RecDev& r = ka.rec;
-    const float slope = ka.grid.slope;
-    float a[3] = {p.x, p.y, p.z}, oob[3] = {0.f, 0.f, 0.f};
-    float pen = 0.f;
+    ExplicitAtom x;
+    x.a[0] = p.x; x.a[1] = p.y; x.a[2] = p.z;
+    x.pen = 0.f;
     int c[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        if (a[i] < r.init[i]) {
-            pen += r.init[i] - a[i];
-            a[i] = r.init[i];
-            oob[i] = -1.f;
-        } else if (a[i] > r.end[i]) {
-            pen += a[i] - r.end[i];
-            a[i] = r.end[i];
-            oob[i] = 1.f;
+        x.oob[i] = 0.f;
+        if (x.a[i] < r.init[i]) {
+            x.pen += r.init[i] - x.a[i];
+            x.a[i] = r.init[i];
+            x.oob[i] = -1.f;
+        } else if (x.a[i] > r.end[i]) {
+            x.pen += x.a[i] - r.end[i];
+            x.a[i] = r.end[i];
+            x.oob[i] = 1.f;
         }
         // szv_grid::possibilities (szv_grid.cpp:65-74) with fl_to_sz (common.h:205-211)
-        const float x = (a[i] - r.init[i]) * float(r.dim[i]) / r.range[i];
-        c[i] = (x <= 0.f) ? 0 : (x >= float(r.dim[i] - 1)) ? r.dim[i] - 1 : max(0, min(int(x), r.dim[i] - 1));
+        const float xs = (x.a[i] - r.init[i]) * float(r.dim[i]) / r.range[i];
+        c[i] = (xs <= 0.f) ? 0 : (xs >= float(r.dim[i] - 1)) ? r.dim[i] - 1 : max(0, min(int(xs), r.dim[i] - 1));
     }
     const int cell = c[0] + r.dim[0] * (c[1] + r.dim[1] * c[2]);
-    const int k0 = r.cell_start[cell], k1 = r.cell_start[cell + 1];
-    float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-    // receptor atoms in batches of kRecBatch, each batch loaded while the
-    // previous one is summed and its table samples loaded together, all
-    // unconditionally (atoms past the cell's list are the next cell's or the
-    // array's far padding and are masked; sample 0 outside the cutoff), so
-    // that each batch waits only for its own loads; the sums run in list order
-    constexpr int kRecBatch = 4;
+    x.k0 = r.cell_start[cell];
+    x.k1 = r.cell_start[cell + 1];
+    return x;
+}
+// Sums over the receptor atoms of batches part, part + parts, ... of the
+// atom's cell list (kRecBatch atoms each): energy and the gradient's sum of
+// dor * (a - b) (with d == false the fast table's energy only).  Each batch
+// is loaded while the previous one is summed and its table samples loaded
+// together, all unconditionally (atoms past the cell's list are the next
+// cell's or the array's far padding and are masked; sample 0 outside the
+// cutoff), so that each batch waits only for its own loads.
+static constexpr int kRecBatch = 4;
+static_assert(kCellPad >= 9 * kRecBatch, "look-ahead of up to 8 interleaved parts");
+__device__ __forceinline__ void explicit_sum(const KArgs& ka, int t, const ExplicitAtom& x, int part, int parts, bool d,
+                                             float& e, float& gx, float& gy, float& gz) {
+    const RecDev& r = ka.rec;
+    const int step = parts * kRecBatch;
     float4 b[kRecBatch];
 #pragma unroll
-    for (int u = 0; u < kRecBatch; ++u) b[u] = r.atoms[k0 + u];
-    for (int k = k0; k < k1; k += kRecBatch) {
+    for (int u = 0; u < kRecBatch; ++u) b[u] = r.atoms[x.k0 + part * kRecBatch + u];
+    for (int k = x.k0 + part * kRecBatch; k < x.k1; k += step) {
         float4 nb[kRecBatch];
 #pragma unroll
-        for (int u = 0; u < kRecBatch; ++u) nb[u] = r.atoms[k + kRecBatch + u];
+        for (int u = 0; u < kRecBatch; ++u) nb[u] = r.atoms[k + step + u];
         bool in[kRecBatch];
         float rx[kRecBatch], ry[kRecBatch], rz[kRecBatch], rf[kRecBatch];
         uint32_t idx[kRecBatch];
 #pragma unroll
         for (int u = 0; u < kRecBatch; ++u) {
-            rx[u] = a[0] - b[u].x; ry[u] = a[1] - b[u].y; rz[u] = a[2] - b[u].z;
+            rx[u] = x.a[0] - b[u].x; ry[u] = x.a[1] - b[u].y; rz[u] = x.a[2] - b[u].z;
             const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
-            in[u] = (k + u < k1) && r2 < 64.f;
+            in[u] = (k + u < x.k1) && r2 < 64.f;
             const int t2 = __float_as_int(b[u].w);
             rf[u] = 32.f * r2;
             idx[u] = in[u] ? uint32_t(t <= t2 ? t + t2 * (t2 + 1) / 2 : t2 + t * (t + 1) / 2) * uint32_t(kTableStride) +
@@ -774,14 +789,25 @@ __device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p,
 #pragma unroll
         for (int u = 0; u < kRecBatch; ++u) b[u] = nb[u];
     }
+}
+// Curl of the summed energy, then the out-of-box penalty and its gradient.
+__device__ __forceinline__ float explicit_finish(const KArgs& ka, const ExplicitAtom& x, float v, float e, float gx,
+                                                 float gy, float gz, float3* d) {
+    const float slope = ka.grid.slope;
     if (e > 0.f && v < 0.1f * kMaxFl) {
         const float tmp = curl_factor(v, e);
         e *= tmp;
         const float t2 = tmp * tmp;
         gx *= t2; gy *= t2; gz *= t2;
     }
-    if (d) *d = make_float3(gx + slope * oob[0], gy + slope * oob[1], gz + slope * oob[2]);
-    return e + slope * pen;
+    if (d) *d = make_float3(gx + slope * x.oob[0], gy + slope * x.oob[1], gz + slope * x.oob[2]);
+    return e + slope * x.pen;
+}
+__device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p, float v, float3* d) {
+    const ExplicitAtom x = explicit_locate(ka, p);
+    float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    explicit_sum(ka, t, x, 0, 1, d != nullptr, e, gx, gy, gz);
+    return explicit_finish(ka, x, v, e, gx, gy, gz, d);
 }
 
 // Compile-time tuning of the search kernel: load batching and the occupancy
@@ -1062,14 +1088,38 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
     rp += kPairBatch * L;
     issue(b, s.gl);
     float e = 0.f;
+    if constexpr (GF == kGridExplicit) {
+        // the explicit receptor: S lanes per atom (the largest power of two
+        // up to 8 with S * N <= L) share its receptor list, batch by batch in
+        // turn, and add their sums before the curl
+        int S = 1;
+        while (S < 8 && 2 * S * N <= L) S *= 2;
+        const int per = L / S, ia = s.gl / S, jj = s.gl % S;
+        for (int i0 = 0; i0 < N; i0 += per) {
+            const int i = i0 + ia;
+            const bool have = i < N;
+            const AtomRec a = s.atoms[have ? i : 0];
+            const bool typed = have && ((a.meta >> 8) & 0xff) != 0xff;
+            const float4 ci = coords[have ? i : 0];
+            const ExplicitAtom x = explicit_locate(ka, make_float3(ci.x, ci.y, ci.z));
+            float ep = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+            if (typed) explicit_sum(ka, int(a.meta >> 24), x, jj, S, true, ep, gx, gy, gz);
+            ep = sub_sum(ep, S); gx = sub_sum(gx, S); gy = sub_sum(gy, S); gz = sub_sum(gz, S);
+            if (have && jj == 0) {
+                float3 d = make_float3(0.f, 0.f, 0.f);
+                if (typed) e += explicit_finish(ka, x, v_inter, ep, gx, gy, gz, &d);
+                mf[i] = make_float4(d.x, d.y, d.z, 0.f);
+            }
+        }
+    }
     for (int i = s.gl; i < N; i += L) {
+        if constexpr (GF == kGridExplicit) break;  // done above
         const AtomRec a = s.atoms[i];
         const int slot = (a.meta >> 8) & 0xff;
         float3 d = make_float3(0.f, 0.f, 0.f);
         if (slot != 0xff) {
             const float4 ci = coords[i];
             if constexpr (GF == kGridExplicit) {
-                e += explicit_term(ka, int(a.meta >> 24), make_float3(ci.x, ci.y, ci.z), v_inter, &d);
             } else if constexpr (Fmt<GF>::ad4) {
                 e += ad4_grid_term<GF>(ka.grid, slot, ka.ls.charge[s.h.atom_begin + i], make_float3(ci.x, ci.y, ci.z),
                                        v_inter, &d);
