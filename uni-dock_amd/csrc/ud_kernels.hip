@@ -1142,23 +1142,24 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
         }
         mf[i] = make_float4(d.x, d.y, d.z, 0.f);
     }
-#if UD_PAIR_PIPELINE
-    // software-pipelined: each batch's table loads are issued before the
-    // previous batch is finished, so they are in flight during its math
-    constexpr int QB = kPairBatch * L;
-    for (int q0 = s.gl; q0 < P; q0 += 2 * QB) {
-        issue(b2, q0 + QB);
-        e += finish(b, q0);
-        if (q0 + QB >= P) break;
-        issue(b, q0 + 2 * QB);
-        e += finish(b2, q0 + QB);
+    if constexpr (UD_PAIR_PIPELINE && !Fmt<GF>::ad4) {
+        // software-pipelined: each batch's table loads are issued before the
+        // previous batch is finished, so they are in flight during its math
+        // (not with AD4.2's larger batches, whose second copy would spill)
+        constexpr int QB = kPairBatch * L;
+        for (int q0 = s.gl; q0 < P; q0 += 2 * QB) {
+            issue(b2, q0 + QB);
+            e += finish(b, q0);
+            if (q0 + QB >= P) break;
+            issue(b, q0 + 2 * QB);
+            e += finish(b2, q0 + QB);
+        }
+    } else {
+        for (int q0 = s.gl; q0 < P; q0 += kPairBatch * L) {
+            if (q0 != s.gl) issue(b, q0);
+            e += finish(b, q0);
+        }
     }
-#else
-    for (int q0 = s.gl; q0 < P; q0 += kPairBatch * L) {
-        if (q0 != s.gl) issue(b, q0);
-        e += finish(b, q0);
-    }
-#endif
     return e;
 }
 
@@ -1190,15 +1191,19 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
         // each next record batch is loaded (unconditionally) while the
         // current one is summed; the load after the last batch reads the
         // next atom's list or the array's zero padding and is not used
-        uint32_t rec[kRec], nrec[kRec];
-        load_batch(uint32_t(k0), rec);
+        // (AD4.2 kernels, with fewer registers to spare, load each batch
+        // when it is needed)
+        constexpr bool kAhead = !Fmt<GF>::ad4;
+        uint32_t rec[kRec], nrec[kAhead ? kRec : 1];
+        if constexpr (kAhead) load_batch(uint32_t(k0), rec);
         float4 g = mf[i];
         const float4 ci = coords[i];
         // sum d (c_i - c_o) = c_i sum d - sum d c_o over the atom's records,
         // in whole batches (padding records carry d = 0)
         float sd = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
         for (int k = 0; k < kn; k += kRec) {
-            load_batch(uint32_t(k0 + k + kRec), nrec);
+            if constexpr (kAhead) load_batch(uint32_t(k0 + k + kRec), nrec);
+            else load_batch(uint32_t(k0 + k), rec);
 #pragma unroll
             for (int u = 0; u < kRec; ++u) {
                 const float d = *reinterpret_cast<const float*>(pbase + (rec[u] >> 16));
@@ -1208,8 +1213,10 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
                 sy += d * co.y;
                 sz += d * co.z;
             }
+            if constexpr (kAhead) {
 #pragma unroll
-            for (int u = 0; u < kRec; ++u) rec[u] = nrec[u];
+                for (int u = 0; u < kRec; ++u) rec[u] = nrec[u];
+            }
         }
         g.x += ci.x * sd - sx;
         g.y += ci.y * sd - sy;
