@@ -78,6 +78,10 @@ enum { UD_GRID_F32 = 0, UD_GRID_F16 = 1 };
  * reference groups ligands by size class too (SmallConfig / MediumConfig /
  * LargeConfig ..., kernel.h:242-277, chosen in main.cpp:267-275). */
 #define UD_N_BUCKETS 8
+/* Launches of one ud_dock: each size bucket's ligands with at most 10
+ * torsions (whose search keeps the inverse Hessian in registers) and those
+ * with more are launched separately. */
+#define UD_MAX_LAUNCHES (2 * UD_N_BUCKETS)
 
 /* Timing of one launch of the last ud_dock (ud_last_launches). */
 typedef struct ud_launch_info {
@@ -87,6 +91,7 @@ typedef struct ud_launch_info {
     int32_t n_workgroups;  /* one wavefront each */
     int32_t pair_buffer;   /* 1: pair-parallel forces through a per-pair LDS buffer; 0: per-atom pair loops */
     int32_t lds_bytes;     /* LDS per workgroup */
+    int32_t max_torsions;  /* most torsions of a ligand in the launch */
     double ms;             /* kernel time, HIP events around the launch on its stream */
 } ud_launch_info;
 
@@ -308,8 +313,9 @@ int ud_set_lanes(ud_ctx* ctx, int lanes);
 int ud_last_kernel_time(const ud_ctx* ctx, double* ms, int32_t* launches);
 
 /* Per-launch timing of the last ud_dock (resolved by ud_fetch_dock_results,
- * or by ud_dock when it fetches): up to max_launches entries, launch order
- * (largest bucket first); n_out receives the number of launches.  The
+ * or by ud_dock when it fetches): up to max_launches entries (at most
+ * UD_MAX_LAUNCHES launches), launch order (largest bucket first); n_out
+ * receives the number of launches.  The
  * reference times its docking loop on the host only (main.cpp:272-276). */
 int ud_last_launches(const ud_ctx* ctx, int max_launches, ud_launch_info* out, int32_t* n_out);
 

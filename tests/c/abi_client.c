@@ -38,10 +38,12 @@ static void layout(void) {
     FIELD(ud_ligand_batch, atom_charge); FIELD(ud_ligand_batch, torsdof);
     FIELD(ud_launch_info, atom_cap); FIELD(ud_launch_info, lanes); FIELD(ud_launch_info, n_ligands);
     FIELD(ud_launch_info, n_workgroups); FIELD(ud_launch_info, pair_buffer); FIELD(ud_launch_info, lds_bytes);
-    FIELD(ud_launch_info, ms);
+    FIELD(ud_launch_info, max_torsions); FIELD(ud_launch_info, ms);
     printf("\"UD_POSE_FLOATS\": %d, \"UD_GRAD_FLOATS\": %d, \"UD_MAX_ATOMS\": %d, \"UD_MAX_TORSIONS\": %d, "
-           "\"UD_MAX_PAIRS\": %d, \"UD_N_BUCKETS\": %d, \"UD_MAX_MODES\": %d, \"UD_NOALLOC_MAX_POSES\": %d}\n",
-           UD_POSE_FLOATS, UD_GRAD_FLOATS, UD_MAX_ATOMS, UD_MAX_TORSIONS, UD_MAX_PAIRS, UD_N_BUCKETS, UD_MAX_MODES,
+           "\"UD_MAX_PAIRS\": %d, \"UD_N_BUCKETS\": %d, \"UD_MAX_LAUNCHES\": %d, \"UD_MAX_MODES\": %d, "
+           "\"UD_NOALLOC_MAX_POSES\": %d}\n",
+           UD_POSE_FLOATS, UD_GRAD_FLOATS, UD_MAX_ATOMS, UD_MAX_TORSIONS, UD_MAX_PAIRS, UD_N_BUCKETS, UD_MAX_LAUNCHES,
+           UD_MAX_MODES,
            UD_NOALLOC_MAX_POSES);
 }
 

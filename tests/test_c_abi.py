@@ -51,7 +51,8 @@ def test_struct_layouts_match_ctypes(tmp_path):
     assert sorted(k for k in c if "." in k and not k.startswith("sizeof")) == \
         sorted(k for k in py if not k.startswith("sizeof"))
     assert c["UD_POSE_FLOATS"] == _lib.UD_POSE_FLOATS and c["UD_GRAD_FLOATS"] == _lib.UD_GRAD_FLOATS
-    for k in ("UD_MAX_ATOMS", "UD_MAX_TORSIONS", "UD_MAX_PAIRS", "UD_N_BUCKETS", "UD_MAX_MODES", "UD_NOALLOC_MAX_POSES"):
+    for k in ("UD_MAX_ATOMS", "UD_MAX_TORSIONS", "UD_MAX_PAIRS", "UD_N_BUCKETS", "UD_MAX_LAUNCHES", "UD_MAX_MODES",
+              "UD_NOALLOC_MAX_POSES"):
         assert c[k] == getattr(_lib, k), k
 
 

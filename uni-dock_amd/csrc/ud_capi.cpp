@@ -203,10 +203,10 @@ struct ud_ctx {
     // ud_dock runs its size-bucket launches concurrently, launch k on
     // bstream[k] (forked from and joined back to `stream` with events);
     // lev0[k] / lev[k] bracket launch k
-    hipStream_t bstream[UD_N_BUCKETS] = {};
-    hipEvent_t lev0[UD_N_BUCKETS] = {};
-    hipEvent_t lev[UD_N_BUCKETS] = {};
-    ud_launch_info linfo[UD_N_BUCKETS] = {};
+    hipStream_t bstream[UD_MAX_LAUNCHES] = {};
+    hipEvent_t lev0[UD_MAX_LAUNCHES] = {};
+    hipEvent_t lev[UD_MAX_LAUNCHES] = {};
+    ud_launch_info linfo[UD_MAX_LAUNCHES] = {};
     // lanes per chain per size bucket (ud_autotune / ud_set_bucket_lanes; 0 = default)
     int tuned_lanes[UD_N_BUCKETS] = {};
     int tuned_pb[UD_N_BUCKETS] = {-1, -1, -1, -1, -1, -1, -1, -1};  // per-pair LDS buffer: -1 default, 0 off, 1 on
@@ -440,7 +440,7 @@ int ud_create(int device_id, void* stream, ud_ctx** out) {
     for (int i = 0; i < 36; ++i) c->slot_of_type[i] = -1;
     for (int i = 0; i < UD_MAX_MAPS; ++i) c->scale[i] = 1.f;
     bool ev_ok = hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
-    for (int k = 0; k < UD_N_BUCKETS && ev_ok; ++k)
+    for (int k = 0; k < UD_MAX_LAUNCHES && ev_ok; ++k)
         ev_ok = hipEventCreate(&c->lev[k]) == hipSuccess && hipEventCreate(&c->lev0[k]) == hipSuccess &&
                 hipStreamCreateWithFlags(&c->bstream[k], hipStreamNonBlocking) == hipSuccess;
     if (!ev_ok) {
@@ -466,7 +466,7 @@ int ud_destroy(ud_ctx* c) {
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    for (int k = 0; k < UD_N_BUCKETS; ++k) {
+    for (int k = 0; k < UD_MAX_LAUNCHES; ++k) {
         if (c->lev[k]) (void)hipEventDestroy(c->lev[k]);
         if (c->lev0[k]) (void)hipEventDestroy(c->lev0[k]);
         if (c->bstream[k]) (void)hipStreamDestroy(c->bstream[k]);
@@ -1481,9 +1481,26 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
     std::vector<LaunchCap> cap_v;
     int nl = 0;
     size_t total_work = 0;
+    // each bucket's ligands with at most kNSmall - 6 torsions (the search
+    // keeps their inverse Hessian in registers) and those with more are
+    // launched separately, so that a few flexible ligands do not move the
+    // whole bucket to the larger-vector kernel
+    std::vector<std::pair<int, std::vector<int>>> parts;
     for (int bi = UD_N_BUCKETS - 1; bi >= 0; --bi) {
         if (bl[bi].empty()) continue;
-        LaunchCap cap = caps_of(c, &bl[bi]);
+        std::vector<int> few, many;
+        for (int l : bl[bi]) (c->lig_nn[l] - 1 <= kNSmall - 6 ? few : many).push_back(l);
+        if (p->single_launch) {
+            parts.emplace_back(bi, bl[bi]);
+            continue;
+        }
+        if (!many.empty()) parts.emplace_back(bi, std::move(many));
+        if (!few.empty()) parts.emplace_back(bi, std::move(few));
+    }
+    for (auto& part : parts) {
+        const int bi = part.first;
+        const std::vector<int>& members = part.second;
+        LaunchCap cap = caps_of(c, &members);
         int lanes = lanes_all ? lanes_all
                               : (p->lanes_per_chain ? p->lanes_per_chain
                                                     : (c->tuned_lanes[bi] ? c->tuned_lanes[bi] : default_lanes(cap.n_atoms)));
@@ -1491,7 +1508,7 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
         if (lds_bytes(lanes, cap) > 65536) return fail(c, UD_ERR_CAPACITY, "ligand too large for one workgroup's LDS");
         const int G = 64 / lanes;
         // larger ligands first inside the bucket (tail balance)
-        std::vector<int> order = bl[bi];
+        std::vector<int> order = members;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b2) {
             return c->lig_nat[a] * c->lig_nn[a] > c->lig_nat[b2] * c->lig_nn[b2];
         });
@@ -1510,6 +1527,7 @@ int dock_run(ud_ctx* c, const ud_search_params* p, const std::vector<int>* subse
         li.n_workgroups = int(w.size());
         li.pair_buffer = cap.pair_forces;
         li.lds_bytes = int(lds_bytes(lanes, cap));
+        li.max_torsions = cap.n_nodes - 1;
         li.ms = 0.0;
         works.push_back(std::move(w));
         lanes_v.push_back(lanes);

@@ -87,6 +87,9 @@ __host__ __device__ inline uint32_t partner_pair(uint32_t r) { return (r >> 9) &
 __host__ __device__ inline uint32_t partner_slot(uint32_t r) { return r >> 22; }
 typedef uint32_t AtomPartners;
 static constexpr int kPartnerPad = 8;
+// Gradient-vector capacity of the small search variant (6 + at most 10
+// torsions), whose inverse Hessian fits in registers.
+static constexpr int kNSmall = 16;
 // The same partner record pre-decoded for the search's second pair pass:
 // byte offset of the other atom's coordinates in the group's float4 array
 // (other * 16, bits 0..15) | byte offset of the pair's force scale in the

@@ -288,7 +288,6 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 // (kNSmall); others use the full capacity.  With kNSmall vectors and groups
 // of at most 32 lanes the inverse Hessian lives in registers, otherwise in LDS
 // (full-wavefront groups are register-bound already).
-static constexpr int kNSmall = 16;
 // Compile-time switches of the search kernel (defaults measured fastest on
 // MI355X): the occupancy targets of 32- and 16-lane groups (waves per SIMD), the
 // register form of the small inverse Hessian, subtree sums from prefix

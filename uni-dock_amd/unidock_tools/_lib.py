@@ -23,6 +23,7 @@ UD_POSE_FLOATS = 7 + UD_MAX_TORSIONS
 UD_GRAD_FLOATS = 6 + UD_MAX_TORSIONS
 UD_WEIGHT_FLOATS = 7  # dE/dw per pose, in the order of the scoring weights
 UD_N_BUCKETS = 8
+UD_MAX_LAUNCHES = 2 * UD_N_BUCKETS
 UD_MAX_MODES = 64  # output modes per ligand of ud_postprocess
 UD_NOALLOC_MAX_POSES = 65536
 BUCKET_ATOMS = (16, 32, 48, 64, 96, 128, 192, 300)
@@ -53,7 +54,7 @@ class UdLigandBatch(ctypes.Structure):
 class UdLaunchInfo(ctypes.Structure):
     _fields_ = [("atom_cap", ctypes.c_int32), ("lanes", ctypes.c_int32), ("n_ligands", ctypes.c_int32),
                 ("n_workgroups", ctypes.c_int32), ("pair_buffer", ctypes.c_int32), ("lds_bytes", ctypes.c_int32),
-                ("ms", ctypes.c_double)]
+                ("max_torsions", ctypes.c_int32), ("ms", ctypes.c_double)]
 
 
 class UdMode(ctypes.Structure):
@@ -700,11 +701,12 @@ class Context:
 
     def last_launches(self):
         """Per-launch (bucket) timing of the last dock: list of dicts."""
-        info = (UdLaunchInfo * UD_N_BUCKETS)()
+        info = (UdLaunchInfo * UD_MAX_LAUNCHES)()
         n = ctypes.c_int32()
-        self._ck(self.L.ud_last_launches(self.h, UD_N_BUCKETS, info, ctypes.byref(n)))
+        self._ck(self.L.ud_last_launches(self.h, UD_MAX_LAUNCHES, info, ctypes.byref(n)))
         return [dict(atom_cap=i.atom_cap, lanes=i.lanes, n_ligands=i.n_ligands, n_workgroups=i.n_workgroups,
-                     pair_buffer=i.pair_buffer, lds_bytes=i.lds_bytes, ms=i.ms) for i in info[:n.value]]
+                     pair_buffer=i.pair_buffer, lds_bytes=i.lds_bytes, max_torsions=i.max_torsions, ms=i.ms)
+                for i in info[:n.value]]
 
     def set_bucket_lanes(self, lanes):
         a = np.zeros(UD_N_BUCKETS, dtype=np.int32)
