@@ -60,21 +60,32 @@ def test_first_mutation_and_bfgs_step_match_oracle(f0_zero):
     restatement's up to rounding."""
     _say(f"[gpu] first mutation + BFGS step vs oracle, f0_zero={f0_zero}")
     ctx, o32, ligs = _search_scene(f0_zero)
+    o64, _, _ = oracle_scene(64)
+    for t in LIGAND_TYPES:
+        o64.set_map(t, ctx.get_map(t).astype(np.float64))
+    o64.set_ls_f0_zero(f0_zero)
     exh, seed = 24, 4242
     P, e, ev = ctx.dock(exhaustiveness=exh, global_steps=1, local_steps=1, seed=seed, first_ls_f0_zero=f0_zero)
-    close, total, worst = 0, 0, 0.0
+    close, total, worst, flips_ref = 0, 0, 0.0, 0
     for li, lig in enumerate(ligs):
         for k in range(exh):
             ch = li * exh + k
             eo, po, evo, _ = o32.mc_chain(li, seed, ch, 1, local_steps=1)
+            e64 = o64.mc_chain(li, seed, ch, 1, local_steps=1)[0]
+            flips_ref += abs(eo - e64) / max(1.0, abs(e64)) > 1e-4
             total += 1
             d = abs(e[ch] - eo) / max(1.0, abs(eo))
             worst = max(worst, d)
             if d <= 1e-4:
                 close += 1
                 np.testing.assert_allclose(P[ch, :7 + lig.n_tors], po[:7 + lig.n_tors], atol=2e-3)
-    _say(f"[gpu] first step: {close}/{total} chains within 1e-4 relative, worst {worst:.2e}")
-    assert close >= 0.95 * total
+    _say(f"[gpu] first step: {close}/{total} chains within 1e-4 relative, worst {worst:.2e}; "
+         f"FP32 and FP64 restatements differ on {flips_ref}")
+    # with f0 = f(x) the first Armijo test compares f(x + a p) - f(x) at
+    # clash starts, where rounding flips the branch: the GPU may part from
+    # the FP32 restatement on at most 5 % of the chains, or twice as many as
+    # the FP32 and FP64 restatements part on
+    assert total - close <= max(0.05 * total, 2 * flips_ref)
     ctx.close()
 
 

@@ -166,22 +166,53 @@ __device__ __forceinline__ void sincos_half(float x, float* s, float* c) {
     pc = fmaf(pc, z, -0.5f);
     *c = fmaf(pc, z, 1.f);
 }
-// Hamilton product p * q, (w, x, y, z)
+// Hamilton product p * q, components (w, x, y, z) stored as float4 (x, y, z, w)
 __device__ __forceinline__ float4 qmul(float4 p, float4 q) {
     return make_float4(p.x * q.x - p.y * q.y - p.z * q.z - p.w * q.w,
                        p.x * q.y + p.y * q.x + p.z * q.w - p.w * q.z,
                        p.x * q.z - p.y * q.w + p.z * q.x + p.w * q.y,
                        p.x * q.w + p.y * q.z - p.z * q.y + p.w * q.x);
 }
-// components stored as float4 (x=w, y=x, z=y, w=z)
-// quaternion_to_r3 (quaternion.cpp:77-114), row-major m[9]
-__device__ __forceinline__ void q_to_mat(float4 q, float* m) {
-    const float a = q.x, b = q.y, c = q.z, d = q.w;
-    const float aa = a * a, ab = a * b, ac = a * c, ad = a * d, bb = b * b, bc = b * c, bd = b * d,
-                cc = c * c, cd = c * d, dd = d * d;
-    m[0] = aa + bb - cc - dd; m[1] = 2.f * (-ad + bc); m[2] = 2.f * (ac + bd);
-    m[3] = 2.f * (ad + bc); m[4] = aa - bb + cc - dd; m[5] = 2.f * (-ab + cd);
-    m[6] = 2.f * (-ac + bd); m[7] = 2.f * (ab + cd); m[8] = aa - bb - cc + dd;
+// Quaternions split over the four lanes of a quad: lane c holds component c
+// (w, x, y, z).  quad_perm<S0, S1, S2, S3>(v): lane i of each quad receives
+// v of quad lane Si (all four lanes of the quad must be active).
+template <int S0, int S1, int S2, int S3> __device__ __forceinline__ float quad_perm(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), S0 | (S1 << 2) | (S2 << 4) | (S3 << 6), 0xF, 0xF, true));
+}
+// x with its sign bit flipped where the mask's is set
+__device__ __forceinline__ float sign_xor(float x, uint32_t m) { return __int_as_float(__float_as_int(x) ^ m); }
+// Per-lane sign masks of the quad forms below (lane c = gl % 4): the Hamilton
+// product's terms 1..3 of component c, and the b^2, c^2, d^2 terms of the
+// diagonal of matrix row min(c, 2).  Computed arithmetically from c.
+struct QuadSigns {
+    uint32_t e1, e2, e3, db, dc, dd;
+    __device__ __forceinline__ explicit QuadSigns(uint32_t c) {
+        e1 = ((c & 1u) ^ 1u) << 31;                  // + for c = 1, 3
+        e2 = (((c ^ (c >> 1)) & 1u) ^ 1u) << 31;     // + for c = 1, 2
+        e3 = (((c >> 1) & 1u) ^ 1u) << 31;           // + for c = 2, 3
+        db = ((c | (c >> 1)) & 1u) << 31;            // + for c = 0
+        dc = (((c ^ 1u) | ((c ^ 1u) >> 1)) & 1u) << 31;  // + for c = 1
+        dd = e3;                                     // + for c = 2, 3
+    }
+};
+// Row r = min(c, 2) of the rotation matrix of the quad's quaternion
+// (quaternion_to_r3, quaternion.cpp:77-114): with (a, b, c, d) the components,
+// v = (b, c, d) and (r, s, t) = (r, r + 1, r + 2) mod 3,
+//   m[r][r] = ((a^2 + -+b^2) + -+c^2) + -+d^2   (+ for the row's own axis),
+//   m[r][s] = 2 (v_r v_s - a v_t),   m[r][t] = 2 (a v_s + v_r v_t),
+// the reference's terms, every square and product rounded on its own.
+// Lane 3 computes a copy of row 2.
+__device__ __forceinline__ void quad_mat_row(float q, const QuadSigns& sg, float& mrr, float& mrs, float& mrt) {
+#pragma clang fp contract(off)
+    const float sq = q * q;
+    const float aa = quad_perm<0, 0, 0, 0>(sq), bb = quad_perm<1, 1, 1, 1>(sq);
+    const float cc = quad_perm<2, 2, 2, 2>(sq), dd = quad_perm<3, 3, 3, 3>(sq);
+    const float a = quad_perm<0, 0, 0, 0>(q), vr = quad_perm<1, 2, 3, 3>(q);
+    const float vs = quad_perm<2, 3, 1, 1>(q), vt = quad_perm<3, 1, 2, 2>(q);
+    mrr = ((aa + sign_xor(bb, sg.db)) + sign_xor(cc, sg.dc)) + sign_xor(dd, sg.dd);
+    const float p1 = vr * vs, p2 = vr * vt;
+    mrs = 2.f * fmaf(-a, vt, p1);
+    mrt = 2.f * fmaf(a, vs, p2);
 }
 // quaternion_normalize_approx (quaternion.h:105-117)
 __device__ __forceinline__ float4 q_normalize_approx(float4 q) {

@@ -416,54 +416,97 @@ __device__ Grp<L, NM> setup_group(const KArgs& ka, int lig, float* lds) {
 // land in LDS; node frames (origin, axis, quaternion, rotation matrix) too.
 template <int L, int NM>
 __device__ void fk_frames(const Grp<L, NM>& s, const float* pose) {
+    // Each node's frame is computed by the four lanes of a quad: lane c
+    // (component c) does coordinate c of the origin and of the axis (c < 3),
+    // quaternion component c and matrix row c (c < 3), and the lanes
+    // exchange components through DPP.  Every product and sum is rounded on
+    // its own in a fixed order (explicit fused multiply-adds), so that all
+    // kernels compute the same frames.
+#pragma clang fp contract(off)
     float* fr = s.frames();
-    if (s.gl == 0) {
-        const float4 q = make_float4(pose[3], pose[4], pose[5], pose[6]);
-        fr[0] = pose[0]; fr[1] = pose[1]; fr[2] = pose[2];
-        fr[3] = 0.f; fr[4] = 0.f; fr[5] = 0.f;
-        fr[6] = q.x; fr[7] = q.y; fr[8] = q.z; fr[9] = q.w;
-        q_to_mat(q, fr + 10);
+    // the lane's quad position and what derives from it are recomputed per
+    // call (the empty asm keeps them from being hoisted out of the search
+    // loop, where they would hold registers for the whole kernel)
+    int c = s.gl & 3;
+    asm volatile("" : "+v"(c));
+    const int r = c < 3 ? c : 2;
+    const QuadSigns sg{uint32_t(c)};
+    if (s.gl < 4) {
+        // root: origin and orientation of the pose, rotation matrix
+        const float q = pose[3 + c];
+        float mrr, mrs, mrt;
+        quad_mat_row(q, sg, mrr, mrs, mrt);
+        fr[6 + c] = q;
+        if (c < 3) {
+            fr[c] = pose[c];
+            fr[3 + c] = 0.f;
+            float* m = fr + 10 + 3 * r;
+            m[r] = mrr;
+            m[r == 2 ? 0 : r + 1] = mrs;
+            m[r == 0 ? 2 : r - 1] = mrt;
+        }
     }
-    // half-angle sines and cosines of the lane's torsions, which do not
-    // depend on the parent frames: off the level-by-level critical path
-    // (the launch's vector capacity NM = 6 + torsions bounds the torsions)
-    constexpr int KN = (NM - 6 + L - 1) / L;
-    float hc[KN], hs[KN];
+    // half-angle sine and cosine of each node's torsion (one node per lane),
+    // which do not depend on the parent frames; they wait in the last two
+    // entries of the node's frame (matrix entry 8, not yet written) until
+    // its level
+    constexpr int KT = (NM - 6 + L - 1) / L;
 #pragma unroll
-    for (int m = 0; m < KN; ++m) {
+    for (int m = 0; m < KT; ++m) {
         const int k = 1 + s.gl + m * L;
-        hc[m] = 1.f;
-        hs[m] = 0.f;
-        if (k < s.h.n_nodes) sincos_half(normalize_angle(pose[7 + k - 1]) * 0.5f, &hs[m], &hc[m]);
+        if (k < s.h.n_nodes) {
+            float sn, cs;
+            sincos_half(normalize_angle(pose[7 + k - 1]) * 0.5f, &sn, &cs);
+            fr[20 * k + 18] = cs;
+            fr[20 * k + 19] = sn;
+        }
     }
-    // with one node per lane its record is read once, not once per level
+    // the quad of lanes 4j..4j+3 handles nodes 1 + j + m * Q
+    constexpr int Q = L / 4;
+    constexpr int KN = (NM - 6 + Q - 1) / Q;
     NodeRec nd1;
-    if (UD_FK_NODE_REGS && KN == 1) nd1 = s.nodes[min(1 + s.gl, s.h.n_nodes - 1)];
+    if (UD_FK_NODE_REGS) nd1 = s.nodes[min(1 + (s.gl >> 2), s.h.n_nodes - 1)];
     __syncthreads();
     for (int d = 1; d <= s.h.max_depth; ++d) {
 #pragma unroll
         for (int m = 0; m < KN; ++m) {
-            const int k = 1 + s.gl + m * L;
+            if (1 + m * Q >= s.h.n_nodes) break;
+            const int k = 1 + (s.gl >> 2) + m * Q;
             if (k >= s.h.n_nodes) continue;
-            const NodeRec nd = (UD_FK_NODE_REGS && KN == 1) ? nd1 : s.nodes[k];
+            const NodeRec nd = (UD_FK_NODE_REGS && m == 0) ? nd1 : s.nodes[k];
             if (node_depth(nd) != d) continue;
+            // the quad's four lanes share k and so take this branch together
             const float* pf = fr + 20 * node_parent(nd);
-            const float* R = pf + 10;
+            const float* R = pf + 10 + 3 * r;
             float* f = fr + 20 * k;
-            const float ox = pf[0] + (R[0] * nd.ro[0] + R[1] * nd.ro[1] + R[2] * nd.ro[2]);
-            const float oy = pf[1] + (R[3] * nd.ro[0] + R[4] * nd.ro[1] + R[5] * nd.ro[2]);
-            const float oz = pf[2] + (R[6] * nd.ro[0] + R[7] * nd.ro[1] + R[8] * nd.ro[2]);
-            const float axx = R[0] * nd.ax[0] + R[1] * nd.ax[1] + R[2] * nd.ax[2];
-            const float axy = R[3] * nd.ax[0] + R[4] * nd.ax[1] + R[5] * nd.ax[2];
-            const float axz = R[6] * nd.ax[0] + R[7] * nd.ax[1] + R[8] * nd.ax[2];
-            const float4 qp = make_float4(pf[6], pf[7], pf[8], pf[9]);
-            // angle_to_quaternion(axis, angle) (quaternion.cpp:37-47) times the parent's
-            const float4 qa = make_float4(hc[m], hs[m] * axx, hs[m] * axy, hs[m] * axz);
-            const float4 q = q_normalize_approx(qmul(qa, qp));
-            f[0] = ox; f[1] = oy; f[2] = oz;
-            f[3] = axx; f[4] = axy; f[5] = axz;
-            f[6] = q.x; f[7] = q.y; f[8] = q.z; f[9] = q.w;
-            q_to_mat(q, f + 10);
+            const float hc = f[18], hs = f[19];
+            // origin and axis coordinate r: parent origin + row r of R times
+            // the relative origin, row r of R times the relative axis
+            const float o = pf[r] + fmaf(R[2], nd.ro[2], fmaf(R[1], nd.ro[1], R[0] * nd.ro[0]));
+            const float ac = fmaf(R[2], nd.ax[2], fmaf(R[1], nd.ax[1], R[0] * nd.ax[0]));
+            const float ax = quad_perm<0, 0, 0, 0>(ac), ay = quad_perm<1, 1, 1, 1>(ac), az = quad_perm<2, 2, 2, 2>(ac);
+            // component c of angle_to_quaternion(axis, angle) (quaternion.cpp:37-47)
+            // times the parent's: sum over j of +-qa_j qp_(c xor j)
+            const float qp = pf[6 + c];
+            const float q1 = quad_perm<1, 0, 3, 2>(qp), q2 = quad_perm<2, 3, 0, 1>(qp), q3 = quad_perm<3, 2, 1, 0>(qp);
+            const float p1 = sign_xor(hs, sg.e1) * ax, p2 = sign_xor(hs, sg.e2) * ay, p3 = sign_xor(hs, sg.e3) * az;
+            float q = fmaf(p3, q3, fmaf(p2, q2, fmaf(p1, q1, hc * qp)));
+            // quaternion_normalize_approx (quaternion.h:105-117)
+            const float sq = q * q;
+            float n2 = sq + quad_perm<1, 0, 3, 2>(sq);
+            n2 = n2 + quad_perm<2, 3, 0, 1>(n2);
+            if (!(fabsf(n2 - 1.f) < 1e-6f)) q = q * __builtin_amdgcn_rsqf(n2);
+            float mrr, mrs, mrt;
+            quad_mat_row(q, sg, mrr, mrs, mrt);
+            f[6 + c] = q;
+            if (c < 3) {
+                f[r] = o;
+                f[3 + r] = ac;
+                float* mm = f + 10 + 3 * r;
+                mm[r] = mrr;
+                mm[r == 2 ? 0 : r + 1] = mrs;
+                mm[r == 0 ? 2 : r - 1] = mrt;
+            }
         }
         __syncthreads();
     }
