@@ -317,14 +317,19 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 #define UD_FK_NODE_REGS 1
 #endif
 
+// Line-search trials per BFGS step (bfgs.h:70-83): trial k takes step
+// 2^-k along the search direction.
+static constexpr int kTrials = 10;
+
 // LDS layout of one workgroup: the ligand's atoms and nodes, then one block
 // per lane group (coordinates, minus-forces, node frames, five pose buffers,
-// three broadcast vectors, inverse Hessian).  Pose and vector strides follow
+// three broadcast vectors, the orientations of the line search's trials,
+// inverse Hessian).  Pose and vector strides follow
 // the launch's node capacity, so small ligands take little LDS per chain and
 // many chains fit a compute unit.  Pair records stay in global memory.
 struct Layout {
     int na, nn, nc, nh, ps;
-    int o_coords, o_mf, o_frames, o_poses, o_bc, o_H, o_pf, group_floats, shared_floats, o_apl;
+    int o_coords, o_mf, o_frames, o_poses, o_bc, o_qt, o_H, o_pf, group_floats, shared_floats, o_apl;
     // pf: per-pair buffer (one float per pair, then a zero) for pair-parallel forces
     __host__ __device__ void init(int na_, int nn_, int np_, bool pf, int lanes) {
         na = na_;
@@ -339,7 +344,8 @@ struct Layout {
         o_frames = 8 * na;
         o_poses = o_frames + 20 * nn;
         o_bc = o_poses + 5 * ps;
-        o_H = o_bc + 3 * ps;
+        o_qt = o_bc + 3 * ps;
+        o_H = o_qt + 4 * kTrials;
         o_pf = pf ? o_H + nh : -1;
         const int end = pf ? o_pf + ((np_ + 1 + 3) & ~3) : o_H + nh;  // + a zero entry after the pairs
         // group blocks start 4 floats apart modulo 32 banks, so that the
@@ -381,6 +387,7 @@ template <int L, int NM = kNMax> struct Grp {
     __device__ float* pose(int k) const { return gb + lay.o_poses + k * lay.ps; }
     __device__ float* bc(int k) const { return gb + lay.o_bc + k * lay.ps; }
     __device__ float* H() const { return gb + lay.o_H; }
+    __device__ float* qt() const { return gb + lay.o_qt; }
     __device__ float* pf() const { return gb + lay.o_pf; }
     __device__ int ntors() const { return h.n_nodes - 1; }
     __device__ int ndof() const { return 6 + h.n_nodes - 1; }
@@ -1545,22 +1552,36 @@ __device__ __forceinline__ void hess_update_reg(const Grp<L, NM>& s, float (&Hr)
     }
 }
 
-// conf::increment (conf.h:98-104, 50-56, 169-172): position += a p,
-// orientation <- exp(a rot) * q (normalised), torsions wrapped.
+// Orientations of the line search's trials: entry k = the increment of
+// src's orientation by 2^-k times the direction's rotation part
+// (quaternion_increment, conf.h:50-56), the value trial k's pose_increment
+// would form, computed once per line search with one trial per lane.
 template <int L, int NM>
-__device__ __forceinline__ void pose_increment(const Grp<L, NM>& s, float* dst, const float* src, const float* p, float alpha, bool pred) {
+__device__ __forceinline__ void trial_orientations(const Grp<L, NM>& s, const float* src, const float* p, bool pred) {
+    if (!pred) return;
+    float* qt = s.qt();
+    for (int k = s.gl; k < kTrials; k += L) {
+        const float a = __builtin_ldexpf(1.f, -k);  // the trial's alpha, exactly
+        const float4 q = q_increment(make_float4(src[3], src[4], src[5], src[6]), a * p[3], a * p[4], a * p[5]);
+        qt[4 * k] = q.x; qt[4 * k + 1] = q.y; qt[4 * k + 2] = q.z; qt[4 * k + 3] = q.w;
+    }
+}
+
+// conf::increment (conf.h:98-104, 50-56, 169-172) for line-search trial
+// `trial` (alpha = 2^-trial): position += alpha p, orientation from the
+// trial table, torsions wrapped.
+template <int L, int NM>
+__device__ __forceinline__ void pose_increment(const Grp<L, NM>& s, float* dst, const float* src, const float* p, float alpha,
+                                               int trial, bool pred) {
     if (!pred) return;
     const int T = s.ntors();
-    for (int idx = s.gl; idx < 4 + T; idx += L) {
-        if (idx < 3) {
-            dst[idx] = src[idx] + alpha * p[idx];
-        } else if (idx == 3) {
-            const float4 q = q_increment(make_float4(src[3], src[4], src[5], src[6]), alpha * p[3], alpha * p[4], alpha * p[5]);
-            dst[3] = q.x; dst[4] = q.y; dst[5] = q.z; dst[6] = q.w;
-        } else {
-            const int k = idx - 4;
-            dst[7 + k] = normalize_angle(src[7 + k] + normalize_angle(alpha * p[6 + k]));
-        }
+    const float* qt = s.qt() + 4 * trial;
+    for (int idx = s.gl; idx < 7 + T; idx += L) {
+        float v;
+        if (idx < 3) v = src[idx] + alpha * p[idx];
+        else if (idx < 7) v = qt[idx - 3];
+        else v = normalize_angle(src[idx] + normalize_angle(alpha * p[idx - 1]));
+        dst[idx] = v;
     }
 }
 
@@ -1754,7 +1775,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         if (state != ST_IDLE) ++evals;
         const bool is_init = state == ST_INIT;
         const bool ok = trialing && (f - f0 < 0.0001f * alpha * pg);
-        const bool ls_end = trialing && (ok || trial == 9);  // 10 trials (bfgs.h:70-83)
+        const bool ls_end = trialing && (ok || trial == kTrials - 1);  // 10 trials (bfgs.h:70-83)
         // Most evaluations are line-search trials that fail the Armijo test
         // before the last one: their gradient is never used and their group's
         // state only halves alpha.  When that holds for every group of the
@@ -1764,7 +1785,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
                 alpha *= 0.5f;
                 ++trial;
             }
-            pose_increment(s, XN, X, pb, alpha, trialing);  // pb still holds each trialing group's direction
+            pose_increment(s, XN, X, pb, alpha, trial, trialing);  // pb still holds each trialing group's direction
             __syncthreads();
             continue;
         }
@@ -1997,7 +2018,13 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
             }
         }
         __syncthreads();
-        pose_increment(s, XN, X, pb, alpha, state == ST_TRIAL);
+        // a new line search: its trials' orientations
+        const bool ls_start = state == ST_TRIAL && trial == 0;
+        if (wave_any(ls_start)) {
+            trial_orientations(s, X, pb, ls_start);
+            __syncthreads();
+        }
+        pose_increment(s, XN, X, pb, alpha, trial, state == ST_TRIAL);
         __syncthreads();
     }
 
