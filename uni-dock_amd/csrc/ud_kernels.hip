@@ -1239,10 +1239,11 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
         const int k0 = int(ap & 0xFFFFFu), kn = int(ap >> 20);
         // each next record batch is loaded (unconditionally) while the
         // current one is summed; the load after the last batch reads the
-        // next atom's list or the array's zero padding and is not used
-        // (AD4.2 kernels, with fewer registers to spare, load each batch
-        // when it is needed)
-        constexpr bool kAhead = !Fmt<GF>::ad4;
+        // next atom's list or the array's zero padding and is not used.
+        // AD4.2 kernels and 32-lane groups, with fewer registers to spare,
+        // load each batch when it is needed (at 32 lanes the look-ahead's 8
+        // registers cost 14 spilled VGPRs and 6 % of the launch's time)
+        constexpr bool kAhead = !Fmt<GF>::ad4 && L != 32;
         uint32_t rec[kRec], nrec[kAhead ? kRec : 1];
         if constexpr (kAhead) load_batch(uint32_t(k0), rec);
         float4 g = mf[i];
