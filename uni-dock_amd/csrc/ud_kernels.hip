@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void grid_to_half_kernel(const float* maps, in
 // register form of the small inverse Hessian, subtree sums from prefix
 // sums, whether pass B's first record batch is loaded before pass A (off:
 // the 8 registers it holds through pass A cost more than the latency saved),
-// and the lane's torsion-tree node record kept in registers across the
+// and each lane quad's torsion-tree node record kept in registers across the
 // forward-kinematics levels.
 #ifndef UD_WAVES_L32
 #define UD_WAVES_L32 4
@@ -419,8 +419,9 @@ __device__ Grp<L, NM> setup_group(const KArgs& ka, int lig, float* lds) {
 }
 
 // Forward kinematics (model::set, tree.h:121-144, 176-214): root frame from
-// the pose, then tree levels parent-first, then atom coordinates.  Coordinates
-// land in LDS; node frames (origin, axis, quaternion, rotation matrix) too.
+// the pose, then tree levels parent-first.  The node frames (origin, axis,
+// quaternion, rotation matrix) land in LDS, from which atoms_coords places
+// the atoms.
 template <int L, int NM>
 __device__ void fk_frames(const Grp<L, NM>& s, const float* pose) {
     // Each node's frame is computed by the four lanes of a quad: lane c
