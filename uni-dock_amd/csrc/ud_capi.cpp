@@ -1814,72 +1814,48 @@ int ud_refine(ud_ctx* c, int n_poses, const int32_t* pose_ligand, float* poses, 
     if (rc) return rc;
     LaunchCap cap = caps_all(c);
     const int lanes = fit_lanes(c->forced_lanes ? c->forced_lanes : default_lanes(cap.n_atoms), cap);
-    // poses still refining (the reference stops a pose's passes once it is within the box)
-    std::vector<int> active(n_poses);
-    for (int i = 0; i < n_poses; ++i) active[i] = i;
-    std::vector<float> e(n_poses, kMaxFloat);
-    std::vector<char> inside(n_poses, 0);
-    std::vector<float> sub, se;
-    std::vector<int32_t> spl, win;
+    // all passes in one launch: a pose runs pass k + 1 (slope 100 * 10^2k,
+    // vina.h:261) only while its heavy atoms are outside the box
+    // (non_cache::within, decided on the device after each pass), as the
+    // reference stops a pose's passes once it is within the box
+    std::vector<float> slopes(refine_step);
+    for (int k = 0; k < refine_step; ++k) slopes[k] = float(100.0 * std::pow(10.0, 2.0 * k));
     const float margin = 1e-4f;  // non_cache::within default (non_cache.h:41)
-    double ms_total = 0;
-    for (int pass = 0; pass < refine_step && !active.empty(); ++pass) {
-        const int na = int(active.size());
-        sub.resize(size_t(na) * UD_POSE_FLOATS);
-        spl.resize(na);
-        for (int k = 0; k < na; ++k) {
-            std::copy(poses + size_t(active[k]) * UD_POSE_FLOATS, poses + size_t(active[k] + 1) * UD_POSE_FLOATS,
-                      sub.begin() + size_t(k) * UD_POSE_FLOATS);
-            spl[k] = pose_ligand[active[k]];
-        }
-        rc = pose_upload(c, na, sub.data());
-        if (rc) return rc;
-        rc = grow(c, c->d_f1, c->f_cap, size_t(na));
-        if (rc) return rc;
-        rc = grow(c, c->d_i1, c->i_cap, size_t(na));
-        if (rc) return rc;
-        std::vector<int32_t> perm;
-        std::vector<WorkItem> work;
-        make_pose_work(na, spl.data(), 64 / lanes, perm, work);
-        rc = work_upload(c, perm, work);
-        if (rc) return rc;
-        GridDev g = grid_dev(c);
-        g.slope = float(100.0 * std::pow(10.0, 2.0 * pass));  // vina.h:261
-        UD_HIP(c, hipEventRecord(c->ev0, c->stream));
-        UD_HIP(c, launch_refine(lanes, lig_dev(c), g, rec_dev(c), cap, c->d_work, int(work.size()), c->d_perm,
-                                c->d_poses, 1000.f, 1000.f, 0, c->d_f1, c->d_i1, c->stream));
-        UD_HIP(c, hipEventRecord(c->ev1, c->stream));
-        // non_cache::within (non_cache.cpp:408-418): heavy atoms inside the
-        // box with a margin, decided on the device for the refined poses
-        // (the refinement's evaluation counts in d_i1 are not needed)
-        BoxLimits lim;
-        for (int d = 0; d < 3; ++d) {
-            lim.lo[d] = c->init[d] - margin;
-            lim.hi[d] = c->gend[d] + margin;
-        }
-        UD_HIP(c, launch_coords(lanes, lig_dev(c), g, cap, c->d_work, int(work.size()), c->d_perm, c->d_poses, nullptr,
-                                c->stream, 0, &lim, c->d_i1));
-        se.resize(na);
-        win.resize(na);
-        UD_HIP(c, hipMemcpyAsync(sub.data(), c->d_poses, sizeof(float) * size_t(na) * UD_POSE_FLOATS,
-                                 hipMemcpyDeviceToHost, c->stream));
-        UD_HIP(c, hipMemcpyAsync(se.data(), c->d_f1, sizeof(float) * na, hipMemcpyDeviceToHost, c->stream));
-        UD_HIP(c, hipMemcpyAsync(win.data(), c->d_i1, sizeof(int32_t) * na, hipMemcpyDeviceToHost, c->stream));
-        UD_HIP(c, hipStreamSynchronize(c->stream));
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-        ms_total += ms;
-        std::vector<int> still;
-        for (int k = 0; k < na; ++k) {
-            const int i = active[k];
-            std::copy(sub.begin() + size_t(k) * UD_POSE_FLOATS, sub.begin() + size_t(k + 1) * UD_POSE_FLOATS,
-                      poses + size_t(i) * UD_POSE_FLOATS);
-            e[i] = se[k];
-            if (win[k]) inside[i] = 1;
-            else still.push_back(i);
-        }
-        active.swap(still);
+    BoxLimits lim;
+    for (int d = 0; d < 3; ++d) {
+        lim.lo[d] = c->init[d] - margin;
+        lim.hi[d] = c->gend[d] + margin;
     }
+    rc = pose_upload(c, n_poses, poses);
+    if (rc) return rc;
+    rc = grow(c, c->d_f1, c->f_cap, size_t(n_poses));
+    if (rc) return rc;
+    rc = grow(c, c->d_i1, c->i_cap, size_t(n_poses));
+    if (rc) return rc;
+    std::vector<int32_t> perm;
+    std::vector<WorkItem> work;
+    make_pose_work(n_poses, pose_ligand, 64 / lanes, perm, work);
+    rc = work_upload(c, perm, work);
+    if (rc) return rc;
+    GridDev g = grid_dev(c);
+    g.slope = slopes[0];
+    UD_HIP(c, hipEventRecord(c->ev0, c->stream));
+    UD_HIP(c, launch_refine(lanes, lig_dev(c), g, rec_dev(c), cap, c->d_work, int(work.size()), c->d_perm, c->d_poses,
+                            1000.f, 1000.f, 0, c->d_f1, c->d_i1, c->stream, refine_step, slopes.data(), lim));
+    UD_HIP(c, hipEventRecord(c->ev1, c->stream));
+    // the within flags of the refined poses (the refinement's evaluation
+    // counts in d_i1 are not needed)
+    UD_HIP(c, launch_coords(lanes, lig_dev(c), g, cap, c->d_work, int(work.size()), c->d_perm, c->d_poses, nullptr,
+                            c->stream, 0, &lim, c->d_i1));
+    std::vector<float> e(n_poses);
+    std::vector<int32_t> inside(n_poses);
+    UD_HIP(c, hipMemcpyAsync(poses, c->d_poses, sizeof(float) * size_t(n_poses) * UD_POSE_FLOATS,
+                             hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipMemcpyAsync(e.data(), c->d_f1, sizeof(float) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipMemcpyAsync(inside.data(), c->d_i1, sizeof(int32_t) * n_poses, hipMemcpyDeviceToHost, c->stream));
+    UD_HIP(c, hipStreamSynchronize(c->stream));
+    float ms_total = 0;
+    (void)hipEventElapsedTime(&ms_total, c->ev0, c->ev1);
     for (int i = 0; i < n_poses; ++i) {
         energy_out[i] = inside[i] ? e[i] : kMaxFloat;  // vina.h:268
         if (within_out) within_out[i] = inside[i];

@@ -190,6 +190,7 @@ struct WorkItem {
     int32_t pad;
 };
 
+static constexpr int kMaxRefinePasses = 8;
 struct SearchDev {
     int32_t exh;
     int32_t global_steps;
@@ -201,6 +202,11 @@ struct SearchDev {
     float amplitude;
     float box_lo[3], box_hi[3];
     int32_t ls_f0_zero;  // first line search of each BFGS against f0 = 0 (bfgs_warp) instead of f(x)
+    // refinement against the explicit receptor: passes (> 1: a pose whose
+    // heavy atoms leave [box_lo, box_hi] after a pass runs the next one) and
+    // each pass's out-of-box penalty slope
+    int32_t refine_passes;
+    float refine_slope[kMaxRefinePasses];
 };
 
 // Capacity of one launch (dynamic LDS is sized from it)

@@ -841,9 +841,10 @@ __device__ __forceinline__ void explicit_sum(const KArgs& ka, int t, const Expli
     }
 }
 // Curl of the summed energy, then the out-of-box penalty and its gradient.
+// slope < 0: the launch's (ka.grid.slope).
 __device__ __forceinline__ float explicit_finish(const KArgs& ka, const ExplicitAtom& x, float v, float e, float gx,
-                                                 float gy, float gz, float3* d) {
-    const float slope = ka.grid.slope;
+                                                 float gy, float gz, float3* d, float slope_in = -1.f) {
+    const float slope = slope_in < 0.f ? ka.grid.slope : slope_in;
     if (e > 0.f && v < 0.1f * kMaxFl) {
         const float tmp = curl_factor(v, e);
         e *= tmp;
@@ -853,11 +854,12 @@ __device__ __forceinline__ float explicit_finish(const KArgs& ka, const Explicit
     if (d) *d = make_float3(gx + slope * x.oob[0], gy + slope * x.oob[1], gz + slope * x.oob[2]);
     return e + slope * x.pen;
 }
-__device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p, float v, float3* d) {
+__device__ __forceinline__ float explicit_term(const KArgs& ka, int t, float3 p, float v, float3* d,
+                                               float slope = -1.f) {
     const ExplicitAtom x = explicit_locate(ka, p);
     float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
     explicit_sum(ka, t, x, 0, 1, d != nullptr, e, gx, gy, gz);
-    return explicit_finish(ka, x, v, e, gx, gy, gz, d);
+    return explicit_finish(ka, x, v, e, gx, gy, gz, d, slope);
 }
 
 // Compile-time tuning of the search kernel: load batching and the occupancy
@@ -920,7 +922,8 @@ __device__ __forceinline__ void ad4_charge_terms(float2 cf, float4 u0, float4 u1
 // and no atomics are needed.  A pair's energy is counted on the side of its
 // lower atom index.
 template <int L, int NM, int GF>
-__device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter) {
+__device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter,
+                                                 float slope = -1.f) {
     const float4* coords = s.coords();
     float4* mf = s.mf();
     const PartnerRec* part = ka.ls.partners + s.h.partner_begin;
@@ -1001,7 +1004,7 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
         }
         float3 d = make_float3(0.f, 0.f, 0.f);
         if constexpr (GF == kGridExplicit) {
-            if (slot != 0xff) e += explicit_term(ka, int(a.meta >> 24), make_float3(ci.x, ci.y, ci.z), v_inter, &d);
+            if (slot != 0xff) e += explicit_term(ka, int(a.meta >> 24), make_float3(ci.x, ci.y, ci.z), v_inter, &d, slope);
         } else if constexpr (Fmt<GF>::ad4) {
             if (slot != 0xff)
                 e += ad4_grid_term<GF>(ka.grid, slot, ka.ls.charge[s.h.atom_begin + i], make_float3(ci.x, ci.y, ci.z),
@@ -1027,7 +1030,7 @@ __device__ __forceinline__ float atoms_forces_2s(const KArgs& ka, const Grp<L, N
 // reference's.
 template <int L, int NM, int GF>
 __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L, NM>& s, float v_intra, float v_inter,
-                                                   const uint64_t (&rec0)[kPairBatch]) {
+                                                   const uint64_t (&rec0)[kPairBatch], float slope = -1.f) {
     const float4* coords = s.coords();
     float4* mf = s.mf();
     float* pf = s.pf();
@@ -1157,7 +1160,7 @@ __device__ __forceinline__ float atoms_forces_pp_a(const KArgs& ka, const Grp<L,
             ep = sub_sum(ep, S); gx = sub_sum(gx, S); gy = sub_sum(gy, S); gz = sub_sum(gz, S);
             if (have && jj == 0) {
                 float3 d = make_float3(0.f, 0.f, 0.f);
-                if (typed) e += explicit_finish(ka, x, v_inter, ep, gx, gy, gz, &d);
+                if (typed) e += explicit_finish(ka, x, v_inter, ep, gx, gy, gz, &d, slope);
                 mf[i] = make_float4(d.x, d.y, d.z, 0.f);
             }
         }
@@ -1283,7 +1286,7 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
 // scales.  rgyr (if not null) receives the gyration radius.
 template <int L, int NM, int GF>
 __device__ float eval_energy(const KArgs& ka, const Grp<L, NM>& s, const float* pose, float v_intra, float v_inter,
-                             float* rgyr = nullptr) {
+                             float* rgyr = nullptr, float slope = -1.f) {
     // the pair pass's first record batch does not depend on the pose: its
     // loads are issued here and complete during the forward kinematics (the
     // record array is padded, so lanes past the last pair read zeros)
@@ -1297,8 +1300,8 @@ __device__ float eval_energy(const KArgs& ka, const Grp<L, NM>& s, const float* 
     float rg2 = 0.f;
     atoms_coords(s, rg2, rgyr != nullptr);
     __syncthreads();
-    float e = ka.pair_forces ? atoms_forces_pp_a<L, NM, GF>(ka, s, v_intra, v_inter, rec0)
-                             : atoms_forces_2s<L, NM, GF>(ka, s, v_intra, v_inter);
+    float e = ka.pair_forces ? atoms_forces_pp_a<L, NM, GF>(ka, s, v_intra, v_inter, rec0, slope)
+                             : atoms_forces_2s<L, NM, GF>(ka, s, v_intra, v_inter, slope);
     __syncthreads();
     e = gsum<L>(e);
     if (rgyr) {
@@ -1738,6 +1741,9 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
     float vI = v_intra_opt, vE = v_inter_opt;
     float f0 = 0.f, f_orig = 0.f, alpha = 1.f, pg = 0.f;
     float tmp_e = 0.f, best_e = __builtin_huge_valf(), rg_tmp = 0.f, rg_x = 0.f, rg_o = 0.f;
+    // refinement pass and its penalty slope (explicit receptor only)
+    int pass = 0;
+    float slope = (!MC && GF == kGridExplicit && sp.refine_passes > 1) ? sp.refine_slope[0] : -1.f;
     float g[KV], p[KV], t[KV], y[KV], mhy[KV];
 #pragma unroll
     for (int j = 0; j < KV; ++j) { g[j] = 0.f; p[j] = 0.f; y[j] = 0.f; }
@@ -1773,7 +1779,7 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         if (!wave_any(state != ST_IDLE)) break;
         // ---------------------------------------------------------------- evaluate
         const bool trialing = state == ST_TRIAL;
-        const float f = eval_energy<L, NM, GF>(ka, s, trialing ? XN : X, vI, vE);
+        const float f = eval_energy<L, NM, GF>(ka, s, trialing ? XN : X, vI, vE, nullptr, slope);
         if (state != ST_IDLE) ++evals;
         const bool is_init = state == ST_INIT;
         const bool ok = trialing && (f - f0 < 0.0001f * alpha * pg);
@@ -1934,6 +1940,35 @@ void search_kernel(KArgs ka, SearchDev sp, const WorkItem* work, const int32_t* 
         bool start_bfgs = false, next_step = false;
         if (!MC) {
             if (finished) state = ST_IDLE;
+            if constexpr (GF == kGridExplicit) {
+                // refinement passes (vina.h:249-272): a pose whose heavy
+                // atoms are not all inside the box (non_cache::within,
+                // non_cache.cpp:408-418, with its margin) after a pass runs
+                // the next pass's BFGS from its refined pose with the next,
+                // steeper penalty slope
+                const bool check = finished && pass + 1 < sp.refine_passes;
+                if (wave_any(check)) {
+                    fk_frames(s, X);  // every group's (the others' frames are not used again)
+                    const float* fr = s.frames();
+                    float outside = 0.f;
+                    for (int i = s.gl; i < s.h.n_atoms; i += L) {
+                        const AtomRec a = s.atoms[i];
+                        if (!((a.meta >> 16) & 1)) continue;
+                        const float3 q = frame_apply(fr + 20 * (a.meta & 0xff), a);
+                        const float c[3] = {q.x, q.y, q.z};
+#pragma unroll
+                        for (int d = 0; d < 3; ++d)
+                            if (c[d] < sp.box_lo[d] || c[d] > sp.box_hi[d]) outside = 1.f;
+                    }
+                    outside = gsum<L>(outside);
+                    if (check && outside != 0.f) {
+                        ++pass;
+                        slope = sp.refine_slope[pass];
+                        state = ST_INIT;
+                    }
+                    __syncthreads();
+                }
+            }
         } else {
             if (state == ST_RGYR) {
                 rg_tmp = rg;
@@ -2714,13 +2749,22 @@ hipError_t launch_optimize(int lanes, int gf, const LigandSetDev& ls, const Grid
 }
 
 // BFGS of given poses against the explicit receptor (the refinement of
-// Vina::global_search_gpu, vina.h:249-272); ka.grid.slope is the pass's slope
+// Vina::global_search_gpu, vina.h:249-272), all passes in one launch; with
+// one pass ka.grid.slope is its slope
 hipError_t launch_refine(int lanes, const LigandSetDev& ls, const GridDev& g, const RecDev& rec, const LaunchCap& cap,
                          const WorkItem* work, int n_work, const int32_t* perm, float* poses, float v_intra,
-                         float v_inter, int max_steps, float* e_out, int32_t* evals_out, hipStream_t st) {
+                         float v_inter, int max_steps, float* e_out, int32_t* evals_out, hipStream_t st,
+                         int passes, const float* slopes, const BoxLimits& lim) {
+    if (passes < 1 || passes > kMaxRefinePasses) return hipErrorInvalidValue;
     const KArgs ka = make_kargs(ls, g, cap, &rec);
     const size_t shm = lds_bytes(lanes, cap);
     SearchDev sp = {};
+    sp.refine_passes = passes;
+    for (int k = 0; k < passes; ++k) sp.refine_slope[k] = slopes[k];
+    for (int d = 0; d < 3; ++d) {
+        sp.box_lo[d] = lim.lo[d];
+        sp.box_hi[d] = lim.hi[d];
+    }
     // the vector capacity must follow the LDS layout's choice (Layout::init):
     // small launches keep the inverse Hessian in registers at <= 32 lanes
     const bool small = 6 + cap.n_nodes - 1 <= kNSmall;

@@ -49,9 +49,17 @@ hipError_t launch_energy_grad(int lanes, int gf, const LigandSetDev& ls, const G
                               const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
                               const float* poses, float v_intra, float v_inter, float* e_out, float* gpose_out,
                               float* gw_out, hipStream_t st);
+// Box limits of a within test: [lo, hi] per axis (the box with its margin).
+struct BoxLimits {
+    float lo[3], hi[3];
+};
+// Refinement BFGS against the explicit receptor: `passes` passes, pass k
+// with penalty slope slopes[k]; a pose whose heavy atoms stay inside lim
+// after a pass stops there.
 hipError_t launch_refine(int lanes, const LigandSetDev& ls, const GridDev& g, const RecDev& rec, const LaunchCap& cap,
                          const WorkItem* work, int n_work, const int32_t* perm, float* poses, float v_intra,
-                         float v_inter, int max_steps, float* e_out, int32_t* evals_out, hipStream_t st);
+                         float v_inter, int max_steps, float* e_out, int32_t* evals_out, hipStream_t st,
+                         int passes, const float* slopes, const BoxLimits& lim);
 hipError_t launch_score_explicit(int lanes, const LigandSetDev& ls, const GridDev& g, const RecDev& rec,
                                  const LaunchCap& cap, const WorkItem* work, int n_work, const int32_t* perm,
                                  const float* poses, float v_intra, float v_inter, float* e_out, float* g_out,
@@ -63,9 +71,6 @@ hipError_t launch_terms_explicit(int lanes, const LigandSetDev& ls, const GridDe
 // chains' atoms packed per ligand (coords_kernel).  With within_out set, no
 // coordinates are written: within_out[pose] = 1 if every heavy atom lies in
 // [lo - margin, hi + margin] on every axis (non_cache::within), else 0.
-struct BoxLimits {
-    float lo[3], hi[3];
-};
 hipError_t launch_coords(int lanes, const LigandSetDev& ls, const GridDev& g, const LaunchCap& cap, const WorkItem* work,
                          int n_work, const int32_t* perm, const float* poses, float* coords_out, hipStream_t st,
                          int compact_exh = 0, const BoxLimits* lim = nullptr, int32_t* within_out = nullptr);
