@@ -1229,14 +1229,19 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
     const uint32_t* part = ka.ls.partners + s.h.partner_ofs_begin;
     const char* cbase = reinterpret_cast<const char*>(coords);
     const char* pbase = reinterpret_cast<const char*>(pf);
-    constexpr int kRec = kPartnerPad;  // partner records fetched together (lists are padded to this)
-    // (each atom's list starts at a multiple of kPartnerPad records, so a
-    // batch is two aligned 16-byte loads)
-    static_assert(kPartnerPad == 8, "record batches are loaded as two uint4");
+    // partner records fetched together: each atom's list is padded to a
+    // multiple of kPartnerPad (8) records and starts at one, so a batch is
+    // one or two aligned 16-byte loads; 32-lane groups, with fewer registers
+    // to spare, take half batches
+    constexpr int kRec = (L == 32 && !Fmt<GF>::ad4) ? 4 : kPartnerPad;
+    static_assert(kPartnerPad == 8 && kPartnerPad % kRec == 0, "record batches are one or two uint4");
     auto load_batch = [&](uint32_t first_rec, uint32_t (&r)[kRec]) {
-        const uint4 lo = ld_at<uint4>(part, first_rec * 4u), hi = ld_at<uint4>(part, first_rec * 4u + 16u);
+        const uint4 lo = ld_at<uint4>(part, first_rec * 4u);
         r[0] = lo.x; r[1] = lo.y; r[2] = lo.z; r[3] = lo.w;
-        r[4] = hi.x; r[5] = hi.y; r[6] = hi.z; r[7] = hi.w;
+        if constexpr (kRec == 8) {
+            const uint4 hi = ld_at<uint4>(part, first_rec * 4u + 16u);
+            r[4] = hi.x; r[5] = hi.y; r[6] = hi.z; r[7] = hi.w;
+        }
     };
     for (int i = s.gl; i < N; i += L) {
         const uint32_t ap = s.apl[i];  // staged in LDS with the ligand
@@ -1244,10 +1249,11 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
         // each next record batch is loaded (unconditionally) while the
         // current one is summed; the load after the last batch reads the
         // next atom's list or the array's zero padding and is not used.
-        // AD4.2 kernels and 32-lane groups, with fewer registers to spare,
-        // load each batch when it is needed (at 32 lanes the look-ahead's 8
-        // registers cost 14 spilled VGPRs and 6 % of the launch's time)
-        constexpr bool kAhead = !Fmt<GF>::ad4 && L != 32;
+        // AD4.2 kernels, with fewer registers to spare, load each batch when
+        // it is needed (at 32 lanes a look-ahead of whole 8-record batches
+        // cost 14 spilled VGPRs and 6 % of the launch's time, hence the half
+        // batches there)
+        constexpr bool kAhead = !Fmt<GF>::ad4;
         uint32_t rec[kRec], nrec[kAhead ? kRec : 1];
         if constexpr (kAhead) load_batch(uint32_t(k0), rec);
         float4 g = mf[i];
