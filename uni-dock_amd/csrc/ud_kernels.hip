@@ -1231,9 +1231,9 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
     const char* pbase = reinterpret_cast<const char*>(pf);
     // partner records fetched together: each atom's list is padded to a
     // multiple of kPartnerPad (8) records and starts at one, so a batch is
-    // one or two aligned 16-byte loads; 32-lane groups, with fewer registers
-    // to spare, take half batches
-    constexpr int kRec = (L == 32 && !Fmt<GF>::ad4) ? 4 : kPartnerPad;
+    // one or two aligned 16-byte loads; the search's 32- and 64-lane groups,
+    // with few registers to spare, take half batches
+    constexpr int kRec = ((L == 32 || L == 64) && !Fmt<GF>::ad4) ? 4 : kPartnerPad;
     static_assert(kPartnerPad == 8 && kPartnerPad % kRec == 0, "record batches are one or two uint4");
     auto load_batch = [&](uint32_t first_rec, uint32_t (&r)[kRec]) {
         const uint4 lo = ld_at<uint4>(part, first_rec * 4u);
@@ -1251,8 +1251,8 @@ __device__ __forceinline__ void atoms_forces_pp_b(const KArgs& ka, const Grp<L, 
         // next atom's list or the array's zero padding and is not used.
         // AD4.2 kernels, with fewer registers to spare, load each batch when
         // it is needed (at 32 lanes a look-ahead of whole 8-record batches
-        // cost 14 spilled VGPRs and 6 % of the launch's time, hence the half
-        // batches there)
+        // cost 14 spilled VGPRs and 6 % of the launch's time, at 64 lanes 7
+        // spilled VGPRs and 1 %, hence the half batches there)
         constexpr bool kAhead = !Fmt<GF>::ad4;
         uint32_t rec[kRec], nrec[kAhead ? kRec : 1];
         if constexpr (kAhead) load_batch(uint32_t(k0), rec);
